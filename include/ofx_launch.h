@@ -1,0 +1,104 @@
+/*
+ * ofx_launch.h - thin C-ABI between the host plan layer (ops-fft_amd/lib) and the HIP kernels
+ * (ops-fft_amd/csrc) of libcann_ops_fft.so.
+ *
+ * This layer replaces the reference's per-operator host entries, which take
+ * (float* x, float* y, uint32_t n, int32_t norm, uint32_t batches, int isForward, void* stream)
+ * and allocate, stage, launch and synchronise inside every call:
+ *   aclfftFft1DC2C / aclfftFft1DC2CMix   (src/fft1_d/fft1_d.h:37-58),
+ *   aclfftRfft1DFft                       (src/rfft1_d/rfft1_d.h:47-74),
+ *   aclfftIrfft1DFft                      (src/irfft1_d/irfft1_d.h:32-39),
+ *   aclfftFft2DDd                         (src/fft2_d/fft2_d.h:27-28).
+ * Here the plan is built once (kernel choice, device twiddle tables, workspace) and each call is a
+ * pure asynchronous launch on a HIP stream with device pointers; nothing allocates or blocks.
+ *
+ * Kernels are identified by an index into a registry of compiled configurations; the host queries
+ * the registry (ofx_kernel_count / ofx_kernel_info) to choose one.  All functions return a
+ * hipError_t value (0 = success).
+ */
+#ifndef OFX_LAUNCH_H
+#define OFX_LAUNCH_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+struct ihipStream_t;
+typedef struct ihipStream_t* hipStream_t;
+
+#if defined(__GNUC__)
+#define OFX_API __attribute__((visibility("default")))
+#else
+#define OFX_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OFX_MAX_PASSES 12
+
+/* Kernel families. */
+enum ofx_kind {
+    OFX_KIND_C2C = 0,      /* batch of contiguous length-n complex transforms             */
+    OFX_KIND_R2C = 1,      /* even n: n/2-point complex FFT + fused Hermitian unpack       */
+    OFX_KIND_C2R = 2,      /* even n: fused Hermitian pre-pack + n/2-point complex FFT     */
+    OFX_KIND_COL = 3,      /* strided "column" transforms (2D second pass, four-step)      */
+    OFX_KIND_R2C_ODD = 4,  /* odd n: full complex FFT of real input, first n/2+1 bins      */
+    OFX_KIND_C2R_ODD = 5   /* odd n: Hermitian expansion + full complex FFT, real part     */
+};
+
+/* Description of one compiled kernel configuration. */
+typedef struct ofx_kernel_info {
+    int kind;                       /* enum ofx_kind                                        */
+    int n;                          /* complex FFT length the kernel computes               */
+    int threads_per_transform;      /* T                                                    */
+    int transforms_per_block;       /* TPB (for COL: columns per block)                     */
+    int block_threads;              /* T * TPB                                              */
+    int npass;                      /* number of Stockham passes                            */
+    int radices[OFX_MAX_PASSES];    /* radix of each pass, first to last                    */
+    int staged;                     /* 1: HBM<->LDS tile copies with 16-byte accesses       */
+    int lds_bytes;                  /* static LDS per workgroup                             */
+    int tw_len;                     /* complex64 entries of the pass-twiddle table          */
+} ofx_kernel_info;
+
+/* Arguments of every 1D launch.  Element distances are in complex64 units for complex data and
+ * in float units for real data. */
+typedef struct ofx_args {
+    const void* in;
+    void* out;
+    const void* tw;        /* pass-twiddle table (ofx_gen_pass_twiddles)                  */
+    const void* tw_aux;    /* R2C/C2R: W_n^k, k in [0, n/2]; four-step: unused            */
+    long long batch;       /* number of transforms                                        */
+    long long idist;       /* distance between consecutive transforms in `in`             */
+    long long odist;       /* distance between consecutive transforms in `out`            */
+    long long istride;     /* COL: distance between consecutive elements of a transform   */
+    long long ostride;
+    float in_sign;         /* -1 conjugates the input on load (backward = conj o fwd o conj) */
+    float out_sign;        /* -1 conjugates the output on store                            */
+    float scale;           /* multiplies every output value                                */
+    int flags;             /* OFX_FLAG_*                                                   */
+} ofx_args;
+
+/* R2C: multiply bins 1..n/2-1 by 2 (adjoint of C2R).  C2R: multiply input bins 1..n/2-1 by 1/2
+ * (adjoint of R2C).  Used by the hand-written adjoint kernels. */
+#define OFX_FLAG_HERMITIAN_WEIGHT 1
+
+OFX_API int ofx_kernel_count(void);
+OFX_API int ofx_kernel_info_get(int id, ofx_kernel_info* info);
+
+/* Launch registry kernel `id` on `stream`.  Returns 0 or a hipError_t. */
+OFX_API int ofx_launch(int id, const ofx_args* args, hipStream_t stream);
+
+/* Fill `tw` (device, tw_len complex64) with the pass-twiddle table of a radix sequence:
+ * FP64 sincospi on the GPU, rounded once to FP32. */
+OFX_API int ofx_gen_pass_twiddles(void* tw, int npass, const int* radices, hipStream_t stream);
+
+/* Fill `tw` (device, count complex64) with W_len^k = e^{-2 pi i k/len}, k = 0..count-1 (FP64
+ * sincospi on the GPU, rounded to FP32). */
+OFX_API int ofx_gen_roots(void* tw, long long len, long long count, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OFX_LAUNCH_H */

@@ -1,0 +1,116 @@
+"""Python mirror of the aclfft plan interface.
+
+`Plan` wraps an aclfftHandle and exposes the reference's operations with the reference's names and
+argument meaning (plan1d(nx, type, batch, dimType), plan2d(batch, nx, ny, type), exec_c2c(idata,
+odata, direction), exec_r2c, exec_c2r, set_stream, destroy).  Errors raise AclfftError carrying
+the aclfftResult code.
+
+Buffers: torch tensors on a GPU (passed as device pointers; the call is asynchronous on the
+plan's stream, which defaults to torch's current stream) or host numpy arrays (the library
+stages them and returns after copying the result back, the reference's synchronous semantics).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib, check, C2C, R2C, C2R, FORWARD, BACKWARD, HORIZONTAL, VERTICAL
+
+
+def _ptr(a):
+    """Device or host address of a torch tensor / numpy array (None -> NULL)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data
+    # torch.Tensor
+    if not a.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return a.data_ptr()
+
+
+def _is_device(a):
+    return a is not None and not isinstance(a, np.ndarray) and a.is_cuda
+
+
+class Plan:
+    def __init__(self, handle, kind, dims, batch, vertical=False):
+        self._h = handle
+        self.type = kind
+        self.dims = dims
+        self.batch = batch
+        self.vertical = vertical
+        self.work_size = 0
+        self._stream = None
+
+    # ---- construction (reference: aclfftPlan1d / aclfftPlan2d / Create + MakePlan*) ----------
+    @classmethod
+    def plan1d(cls, nx, type=C2C, batch=1, dim_type=HORIZONTAL):
+        h = ctypes.c_void_p()
+        check(lib.aclfftPlan1d(ctypes.byref(h), nx, type, batch, dim_type), "aclfftPlan1d")
+        return cls(h, type, (nx,), batch, dim_type == VERTICAL)
+
+    @classmethod
+    def plan2d(cls, batch, nx, ny, type=C2C):
+        h = ctypes.c_void_p()
+        check(lib.aclfftPlan2d(ctypes.byref(h), batch, nx, ny, type), "aclfftPlan2d")
+        return cls(h, type, (nx, ny), batch)
+
+    @classmethod
+    def make_plan1d(cls, nx, type=C2C, batch=1, dim_type=HORIZONTAL):
+        h = ctypes.c_void_p()
+        check(lib.aclfftCreate(ctypes.byref(h)), "aclfftCreate")
+        ws = ctypes.c_size_t(0)
+        r = lib.aclfftMakePlan1d(h, nx, type, batch, dim_type, ctypes.byref(ws))
+        if r != _lib.SUCCESS:
+            lib.aclfftDestroy(h)
+            check(r, "aclfftMakePlan1d")
+        p = cls(h, type, (nx,), batch, dim_type == VERTICAL)
+        p.work_size = ws.value
+        return p
+
+    # ---- management ---------------------------------------------------------------------------
+    def set_stream(self, stream):
+        """stream: torch.cuda.Stream, an integer hipStream_t, or None (null stream)."""
+        s = getattr(stream, "cuda_stream", stream)
+        check(lib.aclfftSetStream(self._h, s), "aclfftSetStream")
+        self._stream = s
+
+    def _bind_stream(self, a):
+        if _is_device(a) and self._stream is None:
+            import torch
+            s = torch.cuda.current_stream(a.device).cuda_stream
+            check(lib.aclfftSetStream(self._h, s), "aclfftSetStream")
+
+    def destroy(self):
+        if self._h is not None and self._h.value:
+            check(lib.aclfftDestroy(self._h), "aclfftDestroy")
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ---- execution ----------------------------------------------------------------------------
+    def exec_c2c(self, idata, odata, direction=FORWARD):
+        self._bind_stream(idata)
+        check(lib.aclfftExecC2C(self._h, _ptr(idata), _ptr(odata), direction), "aclfftExecC2C")
+        return odata
+
+    def exec_r2c(self, idata, odata):
+        self._bind_stream(idata)
+        check(lib.aclfftExecR2C(self._h, _ptr(idata), _ptr(odata)), "aclfftExecR2C")
+        return odata
+
+    def exec_c2r(self, idata, odata):
+        self._bind_stream(idata)
+        check(lib.aclfftExecC2R(self._h, _ptr(idata), _ptr(odata)), "aclfftExecC2R")
+        return odata

@@ -1,0 +1,95 @@
+#!/usr/bin/env python3
+"""Generate registry_1d.inc: the compiled Stockham configurations of kernels_1d.hip.
+
+Each line instantiates one kernel (kind, complex length N, threads per transform T, transforms per
+block TPB, LDS pad shift, staged flag, radix sequence).  The host chooser picks among the entries
+for a length at plan time (heuristic order = order in this file, or autotuned).
+
+Run: python3 gen_registry.py  (rewrites registry_1d.inc next to this script)
+"""
+import os
+
+KINDS = {"C2C": "OFX_KIND_C2C", "R2C": "OFX_KIND_R2C", "C2R": "OFX_KIND_C2R",
+         "R2C_ODD": "OFX_KIND_R2C_ODD", "C2R_ODD": "OFX_KIND_C2R_ODD"}
+
+# Radix plans for the lengths that get a dedicated (fully unrolled) kernel.  Everything else runs
+# on the runtime-radix kernel (kernels_generic.hip).
+PLANS = {
+    2: [(2,)], 4: [(4,)], 8: [(8,)], 16: [(16,)],
+    32: [(4, 8)], 64: [(8, 8)], 128: [(8, 16)], 256: [(16, 16)],
+    512: [(8, 8, 8), (16, 16, 2)],
+    1024: [(16, 16, 4), (4, 16, 16), (32, 32), (8, 8, 16), (16, 4, 16)],
+    2048: [(16, 16, 8), (8, 16, 16)],
+    4096: [(16, 16, 16)],
+    8192: [(16, 16, 16, 2)],
+    # mixed lengths of the uneven-batch workload and the real-transform halves
+    60: [(4, 3, 5)], 243: [(27, 9)], 1000: [(10, 10, 10)], 6561: [(27, 27, 9)],
+    1500: [(12, 5, 25), (10, 10, 15)],
+    3000: [(8, 15, 25)],
+}
+
+
+def elems(n, t, rad):
+    return max(((n // r + t - 1) // t) * r for r in rad)
+
+
+def choose_t(n, rad):
+    """Threads per transform: about 16 elements per thread, wave-contained when possible."""
+    best = None
+    for t in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024):
+        e = elems(n, t, rad)
+        if e <= 32 and (best is None):
+            best = t
+        if e <= 16:
+            return t
+    return best or 1024
+
+
+def tpb_for(n, t):
+    """Transforms per block: aim at 256 threads and at most ~64 KiB of LDS."""
+    tpb = max(1, 256 // t)
+    while tpb > 1 and tpb * (n + n // 16) * 8 > 65536:
+        tpb //= 2
+    return tpb
+
+
+def lines():
+    out = []
+    for n, plans in sorted(PLANS.items()):
+        for pi, rad in enumerate(plans):
+            t = choose_t(n, rad)
+            if n == 1024 and rad == (32, 32):
+                t = 32
+            tpb = tpb_for(n, t)
+            for staged in ((False, True) if pi == 0 else (False,)):
+                out.append(("C2C", n, t, tpb, 4, staged, rad))
+    # real transforms: even n = 2N uses the N-point complex kernel with fused pack/unpack
+    for n in (1024, 2048, 1500, 4096, 512, 256, 128, 64, 32, 16, 8, 4, 2, 8192, 3000, 1000):
+        rad = PLANS[n][0]
+        t = choose_t(n, rad)
+        tpb = tpb_for(n, t)
+        out.append(("R2C", n, t, tpb, 4, False, rad))
+        out.append(("C2R", n, t, tpb, 4, False, rad))
+        if n in (2048, 1500, 4096):
+            out.append(("R2C", n, t, tpb, 4, True, rad))
+            out.append(("C2R", n, t, tpb, 4, True, rad))
+    return out
+
+
+def main():
+    here = os.path.dirname(os.path.abspath(__file__))
+    rows = lines()
+    with open(os.path.join(here, "registry_1d.inc"), "w") as f:
+        f.write("// generated by gen_registry.py - do not edit\n")
+        f.write("static const Entry kRegistry[] = {\n")
+        for kind, n, t, tpb, padsh, staged, rad in rows:
+            assert t * tpb <= 1024
+            f.write("    OFX_E(%s, %d, %d, %d, %d, %s, %s),\n" % (
+                KINDS[kind], n, t, tpb, padsh, "true" if staged else "false",
+                ", ".join(str(r) for r in rad)))
+        f.write("};\n")
+    print("wrote %d registry entries" % len(rows))
+
+
+if __name__ == "__main__":
+    main()

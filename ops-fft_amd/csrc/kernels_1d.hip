@@ -1,0 +1,443 @@
+// kernels_1d.hip - 1D FP32 Stockham kernels for gfx950 (C2C, R2C, C2R) and the kernel registry.
+//
+// Data layout in HBM: transform t of a batch starts at in + t*idist (complex64 units for complex
+// data, float units for real data); elements are contiguous.  One workgroup owns TPB whole
+// transforms, one launch covers the whole batch, HBM is read once and written once.
+//
+// R2C of even n = 2M (reference behaviour: src/rfft1_d/arch35/fft/rfft1_d_fft_kernel.h:39-228):
+//   z[m] = x[2m] + i x[2m+1] is the real input reinterpreted as M complex values (no pack pass),
+//   Z = FFT_M(z) stays in LDS, and the Hermitian unpack
+//     X[k] = (Z[k] + conj Z[M-k])/2 - (i/2) W_n^k (Z[k] - conj Z[M-k]),   k = 0..M
+//   is fused into the store (DC and Nyquist imaginary parts written as exact zeros).
+// C2R of even n (reference: src/irfft1_d/arch35/fft/irfft1_d_fft_kernel.h:40-180 expands to n and
+//   runs an n-point inverse; here the pre-pack below halves the work):
+//     Z'[k] = (X[k] + conj X[M-k]) + i W_n^{-k} (X[k] - conj X[M-k]),      k = 0..M-1
+//   with Im X[0] = Im X[M] = 0, followed by an M-point backward FFT whose complex output is the
+//   interleaved real result (unnormalised, like the reference).
+// Odd n: R2C runs the full n-point complex FFT of (x, 0) and keeps bins 0..n/2; C2R expands the
+//   Hermitian half-spectrum to n points on load and keeps the real part.
+#include <hip/hip_runtime.h>
+#include "ofx_launch.h"
+#include "stockham.h"
+
+namespace ofx {
+
+// ---------------------------------------------------------------------------------------------
+// HBM functors
+// ---------------------------------------------------------------------------------------------
+struct GLoadC {  // complex input, conjugated for the backward direction
+    const float2* __restrict__ p;
+    float sg;
+    __device__ __forceinline__ float2 operator()(int i) const {
+        float2 v = p[i];
+        return make_float2(v.x, v.y * sg);
+    }
+};
+struct GStoreC {
+    float2* __restrict__ p;
+    float sg, scale;
+    bool ok;
+    __device__ __forceinline__ void operator()(int i, float2 v) const {
+        if (ok) p[i] = make_float2(v.x * scale, v.y * (sg * scale));
+    }
+};
+
+// C2R (even n) pre-pack, computed while loading pass 0.  Output is conjugated (backward via the
+// forward kernel).
+struct GLoadC2RPre {
+    const float2* __restrict__ X;   // M+1 bins of this transform
+    const float2* __restrict__ W;   // W_n^k, k = 0..M
+    int M;
+    float half_weight;              // 0.5 for the adjoint of R2C, 1 otherwise (bins 1..M-1)
+    __device__ __forceinline__ float2 operator()(int k) const {
+        float2 a = X[k];
+        float2 b = X[M - k];
+        if (k == 0) {
+            a.y = 0.f;
+            b.y = 0.f;
+        } else {
+            a = cscale(a, half_weight);
+            b = cscale(b, half_weight);
+        }
+        float2 bc = cconj(b);
+        float2 s = cadd(a, bc);
+        float2 d = csub(a, bc);
+        float2 w = W[k];
+        // i * conj(w) * d
+        float2 t = mul_posi(cmulc(d, w));
+        float2 z = cadd(s, t);
+        return cconj(z);
+    }
+};
+
+// odd n R2C: real input as complex with zero imaginary part; keep bins 0..n/2.
+struct GLoadRealAsC {
+    const float* __restrict__ p;
+    __device__ __forceinline__ float2 operator()(int i) const { return make_float2(p[i], 0.f); }
+};
+struct GStoreHalfSpectrum {
+    float2* __restrict__ p;
+    int half;      // n/2
+    float scale, weight;
+    bool ok;
+    __device__ __forceinline__ void operator()(int i, float2 v) const {
+        if (ok && i <= half) {
+            // odd n has no Nyquist bin: every bin but DC has a distinct conjugate partner
+            const float s = (i == 0) ? scale : scale * weight;
+            float2 r = make_float2(v.x * s, v.y * s);
+            if (i == 0) r.y = 0.f;
+            p[i] = r;
+        }
+    }
+};
+// odd n C2R: Hermitian expansion on load (conjugated for the backward direction), real part out.
+struct GLoadHermExpand {
+    const float2* __restrict__ X;
+    int n;
+    float half_weight;
+    __device__ __forceinline__ float2 operator()(int i) const {
+        const int h = n / 2;
+        float2 v;
+        if (i == 0) {
+            v = make_float2(X[0].x, 0.f);
+        } else if (i <= h) {
+            v = cscale(X[i], half_weight);
+        } else {
+            v = cconj(cscale(X[n - i], half_weight));
+        }
+        return cconj(v);
+    }
+};
+struct GStoreReal {
+    float* __restrict__ p;
+    float scale;
+    bool ok;
+    __device__ __forceinline__ void operator()(int i, float2 v) const {
+        if (ok) p[i] = v.x * scale;
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// staged tile copies (HBM <-> LDS, 16-byte accesses): tile = `nvalid` contiguous transforms
+// ---------------------------------------------------------------------------------------------
+template <class C>
+__device__ __forceinline__ void stage_in(float2* lds, const float2* __restrict__ src, int nvalid, float sg) {
+    const int total = nvalid * C::N;
+    if constexpr ((C::N & 1) == 0) {
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+        for (int l = threadIdx.x; l < total / 2; l += C::BLOCK) {
+            float4 v = s4[l];
+            const int e = 2 * l;
+            const int t = e / C::N, i = e - t * C::N;
+            float2* d = lds + t * C::LDS_N;
+            d[C::pad(i)] = make_float2(v.x, v.y * sg);
+            d[C::pad(i + 1)] = make_float2(v.z, v.w * sg);
+        }
+    } else {
+        for (int l = threadIdx.x; l < total; l += C::BLOCK) {
+            float2 v = src[l];
+            const int t = l / C::N, i = l - t * C::N;
+            lds[t * C::LDS_N + C::pad(i)] = make_float2(v.x, v.y * sg);
+        }
+    }
+}
+template <class C>
+__device__ __forceinline__ void stage_out(const float2* lds, float2* __restrict__ dst, int nvalid, float sg,
+                                          float scale) {
+    const int total = nvalid * C::N;
+    const float si = sg * scale;
+    if constexpr ((C::N & 1) == 0) {
+        float4* d4 = reinterpret_cast<float4*>(dst);
+        for (int l = threadIdx.x; l < total / 2; l += C::BLOCK) {
+            const int e = 2 * l;
+            const int t = e / C::N, i = e - t * C::N;
+            const float2* s = lds + t * C::LDS_N;
+            float2 a = s[C::pad(i)], b = s[C::pad(i + 1)];
+            d4[l] = make_float4(a.x * scale, a.y * si, b.x * scale, b.y * si);
+        }
+    } else {
+        for (int l = threadIdx.x; l < total; l += C::BLOCK) {
+            const int t = l / C::N, i = l - t * C::N;
+            float2 a = lds[t * C::LDS_N + C::pad(i)];
+            dst[l] = make_float2(a.x * scale, a.y * si);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------------------------
+template <class C>
+__global__ __launch_bounds__(C::BLOCK) void k_c2c(ofx_args a) {
+    __shared__ float2 lds[C::TPB * C::LDS_N];
+    const int tl = threadIdx.x / C::T;
+    const int tid = threadIdx.x - tl * C::T;
+    const long long t0 = (long long)blockIdx.x * C::TPB;
+    const long long t = t0 + tl;
+    const float2* __restrict__ tw = static_cast<const float2*>(a.tw);
+    float2* lt = lds + tl * C::LDS_N;
+    if constexpr (C::STAGED) {
+        // staged tiles require contiguous transforms (idist == odist == N): checked by the launcher
+        const int nvalid = (int)min((long long)C::TPB, a.batch - t0);
+        const float2* src = static_cast<const float2*>(a.in) + t0 * C::N;
+        float2* dst = static_cast<float2*>(a.out) + t0 * C::N;
+        stage_in<C>(lds, src, nvalid, a.in_sign);
+        __syncthreads();
+        LdsRead<C> gin{lt};
+        LdsWrite<C> gout{lt};
+        stockham_passes<C, true, true>(lt, tid, tw, gin, gout);
+        __syncthreads();
+        stage_out<C>(lds, dst, nvalid, a.out_sign, a.scale);
+    } else {
+        const bool ok = t < a.batch;
+        const long long tt = ok ? t : 0;
+        GLoadC gin{static_cast<const float2*>(a.in) + tt * a.idist, a.in_sign};
+        GStoreC gout{static_cast<float2*>(a.out) + tt * a.odist, a.out_sign, a.scale, ok};
+        if constexpr (C::WAVE_SYNC) {
+            if (!ok) return;  // whole transforms per wave: no block-wide barrier follows
+        }
+        stockham_passes<C, false, false>(lt, tid, tw, gin, gout);
+    }
+}
+
+// R2C, even n = 2*C::N.  in: real (idist floats), out: complex (odist complex, >= C::N+1).
+template <class C>
+__global__ __launch_bounds__(C::BLOCK) void k_r2c(ofx_args a) {
+    __shared__ float2 lds[C::TPB * C::LDS_N];
+    const int tl = threadIdx.x / C::T;
+    const int tid = threadIdx.x - tl * C::T;
+    const long long t0 = (long long)blockIdx.x * C::TPB;
+    const long long t = t0 + tl;
+    const bool ok = t < a.batch;
+    const long long tt = ok ? t : 0;
+    const float2* __restrict__ tw = static_cast<const float2*>(a.tw);
+    float2* lt = lds + tl * C::LDS_N;
+    LdsWrite<C> lw{lt};
+    if constexpr (C::STAGED) {
+        const int nvalid = (int)min((long long)C::TPB, a.batch - t0);
+        const float2* src = reinterpret_cast<const float2*>(static_cast<const float*>(a.in)) + t0 * C::N;
+        stage_in<C>(lds, src, nvalid, 1.f);
+        __syncthreads();
+        LdsRead<C> gin{lt};
+        stockham_passes<C, true, true>(lt, tid, tw, gin, lw);
+    } else {
+        if constexpr (C::WAVE_SYNC) {
+            if (!ok) return;
+        }
+        GLoadC gin{reinterpret_cast<const float2*>(static_cast<const float*>(a.in) + tt * a.idist), 1.f};
+        stockham_passes<C, false, true>(lt, tid, tw, gin, lw);
+    }
+    if constexpr (!C::WAVE_SYNC || C::STAGED) __syncthreads();
+    if (!ok) return;
+    // fused Hermitian unpack
+    constexpr int M = C::N;
+    const float2* __restrict__ W = static_cast<const float2*>(a.tw_aux);
+    float2* __restrict__ out = static_cast<float2*>(a.out) + tt * a.odist;
+    const float wgt = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 2.f : 1.f;
+    for (int k = tid; k <= M; k += C::T) {
+        const int k1 = (k == M) ? 0 : k;
+        const int k2 = (k == 0) ? 0 : M - k;
+        float2 zk = lt[C::pad(k1)];
+        float2 zc = cconj(lt[C::pad(k2)]);
+        float2 e = cscale(cadd(zk, zc), 0.5f);
+        float2 o = cscale(csub(zk, zc), 0.5f);  // (Z[k] - conj Z[M-k]) / 2
+        float2 x = cadd(e, mul_negi(cmul(W[k], o)));
+        float s = a.scale;
+        if (k == 0 || k == M) {
+            x.y = 0.f;
+        } else {
+            s *= wgt;
+        }
+        out[k] = make_float2(x.x * s, x.y * s);
+    }
+}
+
+// C2R, even n = 2*C::N.  in: complex (idist complex, >= C::N+1), out: real (odist floats).
+template <class C>
+__global__ __launch_bounds__(C::BLOCK) void k_c2r(ofx_args a) {
+    __shared__ float2 lds[C::TPB * C::LDS_N];
+    const int tl = threadIdx.x / C::T;
+    const int tid = threadIdx.x - tl * C::T;
+    const long long t0 = (long long)blockIdx.x * C::TPB;
+    const long long t = t0 + tl;
+    const bool ok = t < a.batch;
+    const long long tt = ok ? t : 0;
+    const float2* __restrict__ tw = static_cast<const float2*>(a.tw);
+    float2* lt = lds + tl * C::LDS_N;
+    const float hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 0.5f : 1.f;
+    GLoadC2RPre gin{static_cast<const float2*>(a.in) + tt * a.idist, static_cast<const float2*>(a.tw_aux), C::N, hw};
+    if constexpr (C::STAGED) {
+        const int nvalid = (int)min((long long)C::TPB, a.batch - t0);
+        float2* dst = reinterpret_cast<float2*>(static_cast<float*>(a.out)) + t0 * C::N;
+        LdsWrite<C> gout{lt};
+        stockham_passes<C, false, true>(lt, tid, tw, gin, gout);
+        __syncthreads();
+        stage_out<C>(lds, dst, nvalid, -1.f, a.scale);
+    } else {
+        if constexpr (C::WAVE_SYNC) {
+            if (!ok) return;
+        }
+        GStoreC gout{reinterpret_cast<float2*>(static_cast<float*>(a.out) + tt * a.odist), -1.f, a.scale, ok};
+        stockham_passes<C, false, false>(lt, tid, tw, gin, gout);
+    }
+}
+
+// R2C, odd n = C::N.
+template <class C>
+__global__ __launch_bounds__(C::BLOCK) void k_r2c_odd(ofx_args a) {
+    __shared__ float2 lds[C::TPB * C::LDS_N];
+    const int tl = threadIdx.x / C::T;
+    const int tid = threadIdx.x - tl * C::T;
+    const long long t = (long long)blockIdx.x * C::TPB + tl;
+    const bool ok = t < a.batch;
+    const long long tt = ok ? t : 0;
+    if constexpr (C::WAVE_SYNC) {
+        if (!ok) return;
+    }
+    const float wgt = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 2.f : 1.f;
+    GLoadRealAsC gin{static_cast<const float*>(a.in) + tt * a.idist};
+    GStoreHalfSpectrum gout{static_cast<float2*>(a.out) + tt * a.odist, C::N / 2, a.scale, wgt, ok};
+    stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, static_cast<const float2*>(a.tw), gin, gout);
+}
+
+// C2R, odd n = C::N.
+template <class C>
+__global__ __launch_bounds__(C::BLOCK) void k_c2r_odd(ofx_args a) {
+    __shared__ float2 lds[C::TPB * C::LDS_N];
+    const int tl = threadIdx.x / C::T;
+    const int tid = threadIdx.x - tl * C::T;
+    const long long t = (long long)blockIdx.x * C::TPB + tl;
+    const bool ok = t < a.batch;
+    const long long tt = ok ? t : 0;
+    if constexpr (C::WAVE_SYNC) {
+        if (!ok) return;
+    }
+    const float hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 0.5f : 1.f;
+    GLoadHermExpand gin{static_cast<const float2*>(a.in) + tt * a.idist, C::N, hw};
+    GStoreReal gout{static_cast<float*>(a.out) + tt * a.odist, a.scale, ok};
+    stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, static_cast<const float2*>(a.tw), gin, gout);
+}
+
+// ---------------------------------------------------------------------------------------------
+// launchers + registry
+// ---------------------------------------------------------------------------------------------
+typedef hipError_t (*launch_fn)(const ofx_args*, hipStream_t);
+
+template <class C, int KIND>
+hipError_t launch(const ofx_args* a, hipStream_t s) {
+    if (a->batch <= 0) return hipSuccess;
+    const long long blocks = (a->batch + C::TPB - 1) / C::TPB;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+    if constexpr (C::STAGED) {
+        if (KIND == OFX_KIND_C2C && (a->idist != C::N || a->odist != C::N)) return hipErrorInvalidValue;
+        if (KIND == OFX_KIND_R2C && a->idist != 2 * C::N) return hipErrorInvalidValue;
+        if (KIND == OFX_KIND_C2R && a->odist != 2 * C::N) return hipErrorInvalidValue;
+    }
+    if constexpr (KIND == OFX_KIND_C2C) hipLaunchKernelGGL(k_c2c<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a);
+    if constexpr (KIND == OFX_KIND_R2C) hipLaunchKernelGGL(k_r2c<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a);
+    if constexpr (KIND == OFX_KIND_C2R) hipLaunchKernelGGL(k_c2r<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a);
+    if constexpr (KIND == OFX_KIND_R2C_ODD)
+        hipLaunchKernelGGL(k_r2c_odd<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a);
+    if constexpr (KIND == OFX_KIND_C2R_ODD)
+        hipLaunchKernelGGL(k_c2r_odd<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a);
+    return hipGetLastError();
+}
+
+struct Entry {
+    ofx_kernel_info info;
+    launch_fn fn;
+};
+
+template <class C, int KIND>
+Entry make_entry() {
+    Entry e{};
+    e.info.kind = KIND;
+    e.info.n = C::N;
+    e.info.threads_per_transform = C::T;
+    e.info.transforms_per_block = C::TPB;
+    e.info.block_threads = C::BLOCK;
+    e.info.npass = C::NP;
+    for (int p = 0; p < C::NP; ++p) e.info.radices[p] = C::radix(p);
+    e.info.staged = C::STAGED ? 1 : 0;
+    e.info.lds_bytes = (int)(C::TPB * C::LDS_N * sizeof(float2));
+    e.info.tw_len = C::tw_len();
+    e.fn = &launch<C, KIND>;
+    return e;
+}
+
+#define OFX_CFG(N, T, TPB, PADSH, STG, ...) StockhamCfg<N, T, TPB, PADSH, STG, Radices<__VA_ARGS__>>
+#define OFX_E(KIND, N, T, TPB, PADSH, STG, ...) make_entry<OFX_CFG(N, T, TPB, PADSH, STG, __VA_ARGS__), KIND>()
+
+#include "registry_1d.inc"
+
+static const int kNumEntries = (int)(sizeof(kRegistry) / sizeof(kRegistry[0]));
+
+// ---------------------------------------------------------------------------------------------
+// twiddle tables: FP64 sincospi on the GPU, rounded once to FP32
+// ---------------------------------------------------------------------------------------------
+__global__ void k_gen_pass_tw(float2* tw, int off, int ns, int r) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cnt = ns * (r - 1);
+    if (idx >= cnt) return;
+    const long long m = idx / (r - 1);
+    const long long k = idx % (r - 1) + 1;
+    const long long L = (long long)ns * r;
+    const long long e = (m * k) % L;
+    double s, c;
+    sincospi(2.0 * (double)e / (double)L, &s, &c);
+    tw[off + idx] = make_float2((float)c, (float)(-s));
+}
+
+__global__ void k_gen_roots(float2* tw, long long len, long long count) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    const long long e = idx % len;
+    double s, c;
+    sincospi(2.0 * (double)e / (double)len, &s, &c);
+    tw[idx] = make_float2((float)c, (float)(-s));
+}
+
+}  // namespace ofx
+
+using namespace ofx;
+
+extern "C" int ofx_kernel_count(void) { return kNumEntries; }
+
+extern "C" int ofx_kernel_info_get(int id, ofx_kernel_info* info) {
+    if (id < 0 || id >= kNumEntries || info == nullptr) return (int)hipErrorInvalidValue;
+    *info = kRegistry[id].info;
+    return 0;
+}
+
+extern "C" int ofx_launch(int id, const ofx_args* args, hipStream_t stream) {
+    if (id < 0 || id >= kNumEntries || args == nullptr) return (int)hipErrorInvalidValue;
+    return (int)kRegistry[id].fn(args, stream);
+}
+
+extern "C" int ofx_gen_pass_twiddles(void* tw, int npass, const int* radices, hipStream_t stream) {
+    if (npass < 0 || npass > OFX_MAX_PASSES) return (int)hipErrorInvalidValue;
+    int ns = 1, off = 0;
+    for (int p = 0; p < npass; ++p) {
+        const int r = radices[p];
+        if (p > 0) {
+            const int cnt = ns * (r - 1);
+            if (cnt > 0) {
+                hipLaunchKernelGGL(k_gen_pass_tw, dim3((cnt + 255) / 256), dim3(256), 0, stream,
+                                   static_cast<float2*>(tw), off, ns, r);
+                hipError_t e = hipGetLastError();
+                if (e != hipSuccess) return (int)e;
+            }
+            off += cnt;
+        }
+        ns *= r;
+    }
+    return 0;
+}
+
+extern "C" int ofx_gen_roots(void* tw, long long len, long long count, hipStream_t stream) {
+    if (count <= 0) return 0;
+    if (len <= 0) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gen_roots, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream,
+                       static_cast<float2*>(tw), len, count);
+    return (int)hipGetLastError();
+}

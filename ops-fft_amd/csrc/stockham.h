@@ -1,0 +1,158 @@
+// stockham.h - single-launch, LDS-resident Stockham autosort FFT for gfx950.
+//
+// One transform of length N is owned by T threads (T <= 64 keeps it inside one wave64, so passes
+// synchronise with wave-level fences instead of s_barrier); a workgroup holds TPB transforms.
+// Pass p of radix R with Ns = R_0*...*R_{p-1}:
+//     for j in [0, N/R):  v_k = src[j + k*N/R] * W_{Ns*R}^{k*(j mod Ns)}   (k = 0..R-1)
+//                         V   = DFT_R(v)
+//                         dst[(j div Ns)*Ns*R + (j mod Ns) + k*Ns] = V_k
+// which is the textbook Stockham autosort (the reference's mixed-radix stage,
+// src/fft1_d/arch35/mix/fft1_d_c2c_mix_kernel.h:1108-1176, uses the same index map).  Pass 0 reads
+// HBM directly (consecutive j -> consecutive addresses, so every load of a wave is one contiguous
+// 512 B run) and the last pass writes HBM directly (Ns = N/R, so the store index is j + k*N/R, again
+// contiguous); the passes in between exchange through LDS.  HBM is therefore touched exactly once
+// per element in each direction.  Optionally ("staged") the whole workgroup tile is moved between
+// HBM and LDS with 16-byte vector accesses first/last, which suits short transforms.
+//
+// LDS layout: transform t of the block at lds[t*LDS_N + pad(i)], pad(i) = i + (i >> PADSH).  The
+// pad breaks the power-of-two strides of the early passes (index j*R + k) across the 64 banks.
+//
+// Twiddle table (per plan, HBM, L1/L2 resident): pass p >= 1 stores W_{Ns*R}^{m*k} at
+// tw[tw_off(p) + m*(R-1) + (k-1)] for m in [0, Ns), k in [1, R).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "codelets.h"
+
+namespace ofx {
+
+template <int... Rs>
+struct Radices {
+    static constexpr int count = sizeof...(Rs);
+    static constexpr int v[count] = {Rs...};
+};
+
+template <int N_, int T_, int TPB_, int PADSH_, bool STAGED_, class RL>
+struct StockhamCfg {
+    static constexpr int N = N_;
+    static constexpr int T = T_;
+    static constexpr int TPB = TPB_;
+    static constexpr int PADSH = PADSH_;
+    static constexpr bool STAGED = STAGED_;
+    static constexpr int NP = RL::count;
+    static constexpr int BLOCK = T * TPB;
+    static constexpr bool WAVE_SYNC = (T <= 64) && (64 % T == 0);
+
+    static constexpr int radix(int p) { return RL::v[p]; }
+    static constexpr int ns(int p) {
+        int s = 1;
+        for (int q = 0; q < p; ++q) s *= RL::v[q];
+        return s;
+    }
+    static constexpr int tw_off(int p) {
+        int o = 0;
+        for (int q = 1; q < p; ++q) o += ns(q) * (RL::v[q] - 1);
+        return o;
+    }
+    static constexpr int tw_len() { return tw_off(NP); }
+    static constexpr int LDS_N = N + (N >> PADSH);
+    static constexpr bool USES_LDS = STAGED || NP > 1;
+    static constexpr int LDS_ELEMS = USES_LDS ? TPB * LDS_N : 1;
+    static constexpr int max_elems_per_thread() {
+        int m = 0;
+        for (int p = 0; p < NP; ++p) {
+            int r = RL::v[p];
+            int nb = (N / r + T - 1) / T;
+            if (nb * r > m) m = nb * r;
+        }
+        return m;
+    }
+    __device__ __forceinline__ static int pad(int i) { return i + (i >> PADSH); }
+    __device__ __forceinline__ static void sync() {
+        if constexpr (WAVE_SYNC) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else {
+            __syncthreads();
+        }
+    }
+    static_assert(N == ns(NP), "radix product must equal N");
+};
+
+// Reads element i of the current transform from LDS.
+template <class C>
+struct LdsRead {
+    const float2* base;
+    __device__ __forceinline__ float2 operator()(int i) const { return base[C::pad(i)]; }
+};
+template <class C>
+struct LdsWrite {
+    float2* base;
+    __device__ __forceinline__ void operator()(int i, float2 v) const { base[C::pad(i)] = v; }
+};
+
+// One Stockham pass.  IN_LDS && OUT_LDS means source and destination are the same LDS buffer, so
+// all reads must complete before any write.
+template <class C, int P, bool IN_LDS, bool OUT_LDS, class Ld, class St>
+__device__ __forceinline__ void stockham_pass(int tid, const float2* __restrict__ tw, const Ld& ld, const St& st) {
+    constexpr int R = C::radix(P);
+    constexpr int Ns = C::ns(P);
+    constexpr int NBF = C::N / R;
+    constexpr int NB = (NBF + C::T - 1) / C::T;
+    constexpr bool EXACT = (NBF % C::T) == 0;
+    float2 v[NB][R];
+    float2 w[NB][R > 1 ? R - 1 : 1];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = tid + b * C::T;
+        if (EXACT || j < NBF) {
+#pragma unroll
+            for (int k = 0; k < R; ++k) v[b][k] = ld(j + k * NBF);
+            if constexpr (P > 0) {
+                const float2* t = tw + C::tw_off(P) + (j % Ns) * (R - 1);
+#pragma unroll
+                for (int k = 0; k < R - 1; ++k) w[b][k] = t[k];
+            }
+        }
+    }
+    if constexpr (IN_LDS && OUT_LDS) C::sync();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int j = tid + b * C::T;
+        if (EXACT || j < NBF) {
+            if constexpr (P > 0) {
+#pragma unroll
+                for (int k = 1; k < R; ++k) v[b][k] = cmul(v[b][k], w[b][k - 1]);
+            }
+            Dft<R>::run(v[b]);
+            const int base = (j / Ns) * (Ns * R) + (j % Ns);
+#pragma unroll
+            for (int k = 0; k < R; ++k) st(base + k * Ns, v[b][k]);
+        }
+    }
+    if constexpr (OUT_LDS) C::sync();
+}
+
+// All passes.  The first pass reads `gin`, the last pass writes `gout`; SRC_LDS / DST_LDS state
+// whether those functors address this transform's LDS buffer (staged tiles, fused pre/post
+// processing) rather than HBM, which decides where the passes must synchronise.
+template <class C, bool SRC_LDS, bool DST_LDS, int P = 0, class GIn, class GOut>
+__device__ __forceinline__ void stockham_passes(float2* lds_t, int tid, const float2* __restrict__ tw, const GIn& gin,
+                                                const GOut& gout) {
+    constexpr bool FIRST = (P == 0);
+    constexpr bool LAST = (P == C::NP - 1);
+    LdsRead<C> lr{lds_t};
+    LdsWrite<C> lw{lds_t};
+    if constexpr (FIRST && LAST) {
+        stockham_pass<C, P, SRC_LDS, DST_LDS>(tid, tw, gin, gout);
+    } else if constexpr (FIRST) {
+        stockham_pass<C, P, SRC_LDS, true>(tid, tw, gin, lw);
+    } else if constexpr (LAST) {
+        stockham_pass<C, P, true, DST_LDS>(tid, tw, lr, gout);
+    } else {
+        stockham_pass<C, P, true, true>(tid, tw, lr, lw);
+    }
+    if constexpr (!LAST) stockham_passes<C, SRC_LDS, DST_LDS, P + 1>(lds_t, tid, tw, gin, gout);
+}
+
+}  // namespace ofx

@@ -1,0 +1,101 @@
+// executor.h - plan-owned execution state: the launch list chosen at plan time, the device
+// twiddle tables, workspace, host-staging buffers and captured graphs.
+//
+// The reference builds nothing at plan time and, on every exec, regenerates twiddles on the host,
+// allocates 5-11 device buffers, copies, launches and synchronises
+// (src/fft1_d/arch35/fft/fft1_d_c2c.cpp:188-251).  Here the launch list is decided once
+// (MakePlan), device tables are generated on the GPU once (first execution), and an execution on
+// device pointers is a pure asynchronous launch sequence on the plan's stream.
+#ifndef OFX_EXECUTOR_H
+#define OFX_EXECUTOR_H
+
+#include <cstddef>
+#include <string>
+#include <vector>
+#include "cann_ops_fft.h"
+#include "ofx_launch.h"
+
+namespace ofx {
+
+enum BufId : int { BUF_IN = 0, BUF_OUT = 1, BUF_WORK0 = 2, BUF_WORK1 = 3 };
+
+// Device lookup tables a plan needs; all live in one allocation.
+struct TableSpec {
+    enum Kind { PASS, ROOTS } kind = PASS;
+    int npass = 0;
+    int radices[OFX_MAX_PASSES] = {0};
+    long long len = 0, count = 0;  // ROOTS: W_len^k, k < count
+    size_t offset = 0;             // bytes from the start of the table allocation
+    size_t bytes = 0;
+};
+
+// One launch.
+struct Step {
+    enum Type { KERNEL, COPY } type = KERNEL;
+    int kernel = -1;               // registry id (KERNEL)
+    int kind = OFX_KIND_C2C;
+    int n = 0;                     // complex length the kernel computes (for autotuning)
+    ofx_args args{};               // in/out/tw/tw_aux and signs filled at execution
+    int src = BUF_IN, dst = BUF_OUT;
+    size_t src_off = 0, dst_off = 0;  // byte offsets into the buffers
+    int tw_table = -1, aux_table = -1;
+    bool conj_in = false;          // C2C: apply the direction's conjugation on load / store
+    bool conj_out = false;
+    size_t copy_bytes = 0;         // COPY
+    std::vector<int> candidates;   // registry ids that can replace `kernel` (autotuning)
+    std::vector<int> candidate_tables;  // pass-twiddle table of each candidate
+};
+
+struct PlanDesc {
+    int rank = 1;
+    int n0 = 0, n1 = 0;            // 1D: n0 = length; 2D: n0 = nx (slow), n1 = ny (fast)
+    long long batch = 1;
+    aclfftType type = ACLFFT_C2C;
+    bool vertical = false;         // 1D VERTICAL layout: n0 rows x `batch` columns
+};
+
+class Executor {
+public:
+    // Host-only: choose kernels and build the launch list.  No device call is made.
+    static aclfftResult build(const PlanDesc& d, Executor** out);
+    ~Executor();
+
+    // Execute on `in` -> `out` (device or host pointers).  direction: ACLFFT_FORWARD/BACKWARD for
+    // C2C; ignored for R2C/C2R.
+    aclfftResult exec(const void* in, void* out, int direction, hipStream_t stream);
+
+    size_t work_bytes() const { return work_bytes_; }
+    size_t in_bytes() const { return in_bytes_; }
+    size_t out_bytes() const { return out_bytes_; }
+    const std::vector<Step>& steps() const { return steps_; }
+    std::string describe() const;
+
+private:
+    Executor() = default;
+    aclfftResult prepare_device(hipStream_t s);
+    aclfftResult run(const void* din, void* dout, int direction, hipStream_t s);
+    aclfftResult autotune(hipStream_t s);
+    int add_table(const TableSpec& t);
+
+    PlanDesc desc_{};
+    std::vector<Step> steps_;
+    std::vector<TableSpec> tables_;
+    size_t table_bytes_ = 0;
+    size_t work_bytes_ = 0;
+    size_t in_bytes_ = 0, out_bytes_ = 0;
+    bool in_place_unsafe_ = false;  // in == out must go through the workspace
+
+    int device_ = -1;
+    void* d_tables_ = nullptr;
+    void* d_work_ = nullptr;
+    void* d_stage_in_ = nullptr;
+    void* d_stage_out_ = nullptr;
+    void* d_inplace_ = nullptr;
+    bool tuned_ = false;
+
+    friend struct Builder;
+};
+
+}  // namespace ofx
+
+#endif

@@ -1,0 +1,25 @@
+// tune_cache.h - persisted autotuning choices, keyed by (type, rank, lengths, batch, step).
+//
+// File: $ACLFFT_TUNE_CACHE, default $HOME/.cache/ops_fft_amd/tune.txt.  One line per step:
+//   <type> <rank> <n0> <n1> <batch> <step> <signature>
+// where the signature names the kernel configuration (kind, n, T, TPB, staged, radices), so a
+// cache stays valid across rebuilds that reorder the registry.  The cache is the only state this
+// library persists.
+#ifndef OFX_TUNE_CACHE_H
+#define OFX_TUNE_CACHE_H
+
+#include <string>
+#include <vector>
+#include "executor.h"
+
+namespace ofx {
+
+std::string kernel_signature(const ofx_kernel_info& k);
+// Replace each step's kernel by a cached choice when one exists among its candidates.
+void apply_tune_cache(const PlanDesc& d, std::vector<Step>& steps, const ofx_kernel_info* reg, int nreg);
+// Record the steps' current choices.
+void store_tune_cache(const PlanDesc& d, const std::vector<Step>& steps, const ofx_kernel_info* reg);
+
+}  // namespace ofx
+
+#endif
