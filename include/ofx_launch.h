@@ -79,6 +79,61 @@ typedef struct ofx_args {
     int flags;             /* OFX_FLAG_*                                                   */
 } ofx_args;
 
+/* Runtime-radix ("generic") launches: any length whose radix sequence is listed, any layout.
+ * Transform t is addressed as (t_hi, t_lo) = (t / cnt_lo, t % cnt_lo):
+ *   input  element i at in  + t_hi*idist + t_lo*idist_lo + i*istride
+ *   output element k at out + t_hi*odist + t_lo*odist_lo + k*ostride
+ * (units: complex64 or float as for ofx_args).  All data of a workgroup's transforms is held in
+ * LDS (two buffers), so tpb * n <= OFX_GENERIC_MAX_ELEMS. */
+#define OFX_GENERIC_MAX_ELEMS 9216
+
+enum ofx_in_mode {
+    OFX_IN_COMPLEX = 0,    /* complex64, conjugated when in_sign = -1                         */
+    OFX_IN_REAL = 1,       /* float, imaginary part 0 (odd-n R2C)                              */
+    OFX_IN_HERMITIAN = 2,  /* n/2+1 bins expanded by conjugate symmetry (odd-n C2R)            */
+    OFX_IN_C2R_PACK = 3    /* even C2R pre-pack of 2n-point spectra into n complex values      */
+};
+enum ofx_out_mode {
+    OFX_OUT_COMPLEX = 0,   /* complex64, conjugated when out_sign = -1                         */
+    OFX_OUT_HALF = 1,      /* bins 0..n/2 only (odd-n R2C)                                     */
+    OFX_OUT_REAL = 2,      /* real part (odd-n C2R)                                            */
+    OFX_OUT_R2C_UNPACK = 3 /* even R2C Hermitian unpack of n complex values into n+1 bins      */
+};
+
+typedef struct ofx_generic_desc {
+    int n;                          /* complex FFT length                                     */
+    int npass;
+    int radices[OFX_MAX_PASSES];
+    int tw_off[OFX_MAX_PASSES];     /* offset of each pass in the pass-twiddle table          */
+    int tpb;                        /* transforms per workgroup                               */
+    int in_mode, out_mode;
+    int in_tmajor, out_tmajor;      /* 1: consecutive lanes walk consecutive transforms (strided
+                                       layouts), 0: consecutive elements of one transform      */
+    int pad_;
+    long long cnt_lo;               /* transforms per t_hi group (>= 1)                       */
+    long long idist_lo, odist_lo;
+    long long ft_len;               /* four-step: output k of transform t is multiplied by
+                                       W_ft_len^((t_lo + ft_base) * k); 0 = none               */
+    long long ft_base;
+} ofx_generic_desc;
+
+/* Radix sequences the generic kernel can run: every radix must be one of these. */
+OFX_API int ofx_generic_radix_supported(int radix);
+
+OFX_API int ofx_launch_generic(const ofx_args* args, const ofx_generic_desc* desc, hipStream_t stream);
+
+/* FP64 four-step twiddle table for W_len^e (two levels, ofx_ft_table_count(len) double2 entries),
+ * generated on the GPU with FP64 sincospi. */
+OFX_API long long ofx_ft_table_count(long long len);
+OFX_API int ofx_gen_ft_table(void* table, long long len, hipStream_t stream);
+
+/* Large even-n real transforms (n = 2M): Hermitian unpack of M-point spectra Z [batch][M] into
+ * X [batch][M+1] (W = W_n^k, k <= M), and the C2R pre-pack X -> conj(Z') [batch][M]. */
+OFX_API int ofx_r2c_unpack(const void* Z, void* X, const void* W, long long M, long long batch, float scale,
+                           int flags, hipStream_t stream);
+OFX_API int ofx_c2r_pack(const void* X, void* Z, const void* W, long long M, long long batch, int flags,
+                         hipStream_t stream);
+
 /* R2C: multiply bins 1..n/2-1 by 2 (adjoint of C2R).  C2R: multiply input bins 1..n/2-1 by 1/2
  * (adjoint of R2C).  Used by the hand-written adjoint kernels. */
 #define OFX_FLAG_HERMITIAN_WEIGHT 1

@@ -1,0 +1,364 @@
+// kernels_generic.hip - runtime-radix Stockham kernel for any length and layout, plus the
+// elementwise kernels of large real transforms and the FP64 four-step twiddle tables.
+//
+// The templated kernels (kernels_1d.hip) cover the hot lengths with fully unrolled,
+// register-resident passes.  This kernel covers everything else: the radix sequence is a kernel
+// argument, each pass dispatches to the matching register codelet through a switch, and the data of
+// the workgroup's transforms ping-pongs between two LDS buffers.  The same kernel serves strided
+// layouts (columns of a 2D transform, the VERTICAL layout, both steps of the four-step
+// decomposition) through the (t_hi, t_lo) addressing of ofx_generic_desc, with the load/store
+// lane mapping chosen so that consecutive lanes touch consecutive addresses.
+//
+// Four-step twiddles W_N^e (N up to 2^27) are formed in FP64 from two FP64 tables,
+// W_N^e = W_N^(e_hi*B) * W_N^e_lo with B = 2048, then rounded once to FP32.
+#include <hip/hip_runtime.h>
+#include "ofx_launch.h"
+#include "codelets.h"
+
+namespace ofx {
+
+constexpr int kGenBlock = 256;
+constexpr int kFtLoBits = 11;  // B = 2048
+constexpr long long kFtLo = 1LL << kFtLoBits;
+
+struct GenParams {
+    ofx_args a;
+    ofx_generic_desc d;
+};
+
+// ---------------------------------------------------------------------------------------------
+// one pass of radix R over all transforms of the workgroup (LDS src -> LDS dst)
+// ---------------------------------------------------------------------------------------------
+template <int R>
+__device__ __forceinline__ void gen_pass(const float2* __restrict__ src, float2* __restrict__ dst, int n, int ns,
+                                         int tpb, const float2* __restrict__ tw) {
+    const int nbf = n / R;
+    const int total = tpb * nbf;
+    for (int f = threadIdx.x; f < total; f += blockDim.x) {
+        const int t = f / nbf;
+        const int j = f - t * nbf;
+        const float2* s = src + t * n;
+        float2* d = dst + t * n;
+        float2 v[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) v[k] = s[j + k * nbf];
+        const int m = j % ns;
+        if (ns > 1) {
+            const float2* w = tw + m * (R - 1);
+#pragma unroll
+            for (int k = 1; k < R; ++k) v[k] = cmul(v[k], w[k - 1]);
+        }
+        Dft<R>::run(v);
+        const int base = (j - m) * R + m;
+#pragma unroll
+        for (int k = 0; k < R; ++k) d[base + k * ns] = v[k];
+    }
+}
+
+__device__ __forceinline__ bool run_pass(int r, const float2* src, float2* dst, int n, int ns, int tpb,
+                                         const float2* tw) {
+    switch (r) {
+        case 2: gen_pass<2>(src, dst, n, ns, tpb, tw); return true;
+        case 3: gen_pass<3>(src, dst, n, ns, tpb, tw); return true;
+        case 4: gen_pass<4>(src, dst, n, ns, tpb, tw); return true;
+        case 5: gen_pass<5>(src, dst, n, ns, tpb, tw); return true;
+        case 6: gen_pass<6>(src, dst, n, ns, tpb, tw); return true;
+        case 7: gen_pass<7>(src, dst, n, ns, tpb, tw); return true;
+        case 8: gen_pass<8>(src, dst, n, ns, tpb, tw); return true;
+        case 9: gen_pass<9>(src, dst, n, ns, tpb, tw); return true;
+        case 10: gen_pass<10>(src, dst, n, ns, tpb, tw); return true;
+        case 11: gen_pass<11>(src, dst, n, ns, tpb, tw); return true;
+        case 12: gen_pass<12>(src, dst, n, ns, tpb, tw); return true;
+        case 13: gen_pass<13>(src, dst, n, ns, tpb, tw); return true;
+        case 15: gen_pass<15>(src, dst, n, ns, tpb, tw); return true;
+        case 16: gen_pass<16>(src, dst, n, ns, tpb, tw); return true;
+        case 17: gen_pass<17>(src, dst, n, ns, tpb, tw); return true;
+        case 19: gen_pass<19>(src, dst, n, ns, tpb, tw); return true;
+        case 23: gen_pass<23>(src, dst, n, ns, tpb, tw); return true;
+        case 25: gen_pass<25>(src, dst, n, ns, tpb, tw); return true;
+        case 29: gen_pass<29>(src, dst, n, ns, tpb, tw); return true;
+        case 31: gen_pass<31>(src, dst, n, ns, tpb, tw); return true;
+        case 37: gen_pass<37>(src, dst, n, ns, tpb, tw); return true;
+        case 41: gen_pass<41>(src, dst, n, ns, tpb, tw); return true;
+        case 43: gen_pass<43>(src, dst, n, ns, tpb, tw); return true;
+        case 47: gen_pass<47>(src, dst, n, ns, tpb, tw); return true;
+        default: return false;
+    }
+}
+
+// FP64 four-step twiddle W_len^e from the two-level table (lo: kFtLo entries, hi: the rest)
+__device__ __forceinline__ float2 ft_twiddle(const double2* __restrict__ t, long long e, long long len) {
+    e %= len;
+    const long long lo = e & (kFtLo - 1);
+    const long long hi = e >> kFtLoBits;
+    const double2 a = t[lo];
+    const double2 b = t[kFtLo + hi];
+    return make_float2((float)(a.x * b.x - a.y * b.y), (float)(a.x * b.y + a.y * b.x));
+}
+
+// ---------------------------------------------------------------------------------------------
+// the kernel
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
+    extern __shared__ float2 lds[];
+    const ofx_args& a = p.a;
+    const ofx_generic_desc& d = p.d;
+    const int n = d.n;
+    const int tpb = d.tpb;
+    float2* buf0 = lds;
+    float2* buf1 = lds + (size_t)tpb * n;
+    const long long t0 = (long long)blockIdx.x * tpb;
+    const int nvalid = (int)min((long long)tpb, a.batch - t0);
+    const float2* tw = static_cast<const float2*>(a.tw);
+
+    // ---- load into buf0 ----------------------------------------------------------------------
+    const int h = n / 2;  // used by the real-data modes
+    for (int f = threadIdx.x; f < nvalid * n; f += blockDim.x) {
+        int t, i;
+        if (d.in_tmajor) {
+            t = f % nvalid;
+            i = f / nvalid;
+        } else {
+            t = f / n;
+            i = f - t * n;
+        }
+        const long long tg = t0 + t;
+        const long long thi = tg / d.cnt_lo, tlo = tg - thi * d.cnt_lo;
+        const long long base = thi * a.idist + tlo * d.idist_lo;
+        float2 v;
+        if (d.in_mode == OFX_IN_COMPLEX) {
+            v = static_cast<const float2*>(a.in)[base + i * a.istride];
+            v.y *= a.in_sign;
+        } else if (d.in_mode == OFX_IN_REAL) {
+            v = make_float2(static_cast<const float*>(a.in)[base + i * a.istride], 0.f);
+        } else if (d.in_mode == OFX_IN_HERMITIAN) {
+            // bins 0..h of an n-point Hermitian spectrum; conjugated for the backward transform
+            const float2* X = static_cast<const float2*>(a.in) + base;
+            const float hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 0.5f : 1.f;
+            if (i == 0) {
+                v = make_float2(X[0].x, 0.f);
+            } else if (i <= h) {
+                v = cscale(X[(long long)i * a.istride], hw);
+            } else {
+                v = cconj(cscale(X[(long long)(n - i) * a.istride], hw));
+            }
+            v = cconj(v);
+        } else {  // OFX_IN_C2R_PACK: n = M, spectra of 2M points with M+1 bins
+            const float2* X = static_cast<const float2*>(a.in) + base;
+            const float2* W = static_cast<const float2*>(a.tw_aux);
+            const float hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 0.5f : 1.f;
+            float2 xa = X[(long long)i * a.istride];
+            float2 xb = X[(long long)(n - i) * a.istride];
+            if (i == 0) {
+                xa.y = 0.f;
+                xb.y = 0.f;
+            } else {
+                xa = cscale(xa, hw);
+                xb = cscale(xb, hw);
+            }
+            const float2 bc = cconj(xb);
+            const float2 s = cadd(xa, bc);
+            const float2 df = csub(xa, bc);
+            v = cconj(cadd(s, mul_posi(cmulc(df, W[i]))));
+        }
+        buf0[t * n + i] = v;
+    }
+    __syncthreads();
+
+    // ---- passes -------------------------------------------------------------------------------
+    float2* src = buf0;
+    float2* dst = buf1;
+    int ns = 1;
+    for (int p = 0; p < d.npass; ++p) {
+        const int r = d.radices[p];
+        run_pass(r, src, dst, n, ns, nvalid, tw + d.tw_off[p]);
+        __syncthreads();
+        ns *= r;
+        float2* tmp = src;
+        src = dst;
+        dst = tmp;
+    }
+
+    // ---- store from src ------------------------------------------------------------------------
+    const double2* ftab = static_cast<const double2*>(a.tw_aux);
+    const int nout = (d.out_mode == OFX_OUT_HALF) ? h + 1 : (d.out_mode == OFX_OUT_R2C_UNPACK ? n + 1 : n);
+    for (int f = threadIdx.x; f < nvalid * nout; f += blockDim.x) {
+        int t, k;
+        if (d.out_tmajor) {
+            t = f % nvalid;
+            k = f / nvalid;
+        } else {
+            t = f / nout;
+            k = f - t * nout;
+        }
+        const long long tg = t0 + t;
+        const long long thi = tg / d.cnt_lo, tlo = tg - thi * d.cnt_lo;
+        const long long base = thi * a.odist + tlo * d.odist_lo;
+        const float2* s = src + t * n;
+        if (d.out_mode == OFX_OUT_COMPLEX) {
+            float2 v = s[k];
+            v.y *= a.out_sign;
+            if (d.ft_len > 0) v = cmul(v, ft_twiddle(ftab, (tlo + d.ft_base) * (long long)k, d.ft_len));
+            static_cast<float2*>(a.out)[base + k * a.ostride] = make_float2(v.x * a.scale, v.y * a.scale);
+        } else if (d.out_mode == OFX_OUT_HALF) {
+            const float w = (k == 0) ? a.scale : a.scale * ((a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 2.f : 1.f);
+            float2 v = cscale(s[k], w);
+            if (k == 0) v.y = 0.f;
+            static_cast<float2*>(a.out)[base + k * a.ostride] = v;
+        } else if (d.out_mode == OFX_OUT_REAL) {
+            static_cast<float*>(a.out)[base + k * a.ostride] = s[k].x * a.scale;
+        } else {  // OFX_OUT_R2C_UNPACK: k = 0..n (n = M)
+            const float2* W = reinterpret_cast<const float2*>(a.tw_aux);
+            const int k1 = (k == n) ? 0 : k;
+            const int k2 = (k == 0) ? 0 : n - k;
+            const float2 zk = s[k1];
+            const float2 zc = cconj(s[k2]);
+            const float2 e = cscale(cadd(zk, zc), 0.5f);
+            const float2 o = cscale(csub(zk, zc), 0.5f);
+            float2 x = cadd(e, mul_negi(cmul(W[k], o)));
+            float sc = a.scale;
+            if (k == 0 || k == n) {
+                x.y = 0.f;
+            } else if (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) {
+                sc *= 2.f;
+            }
+            static_cast<float2*>(a.out)[base + k * a.ostride] = make_float2(x.x * sc, x.y * sc);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// FP64 four-step table: t[e] = W_len^e for e < B, t[B + h] = W_len^(h*B) for h < ceil(len/B)
+// ---------------------------------------------------------------------------------------------
+__global__ void k_gen_ft_table(double2* t, long long len, long long count) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    const long long e = (idx < kFtLo) ? idx : (idx - kFtLo) * kFtLo;
+    double s, c;
+    sincospi(2.0 * (double)(e % len) / (double)len, &s, &c);
+    t[idx] = make_double2(c, -s);
+}
+
+// ---------------------------------------------------------------------------------------------
+// large even-n real transforms: the M-point complex FFT runs as a four-step plan in a workspace
+// and these kernels do the Hermitian unpack / pre-pack as separate passes.
+// ---------------------------------------------------------------------------------------------
+// Z: [batch][M] complex (workspace) -> X: [batch][M+1] complex
+__global__ void k_r2c_unpack(const float2* __restrict__ Z, float2* __restrict__ X, const float2* __restrict__ W,
+                             long long M, long long batch, float scale, int flags) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = M + 1;
+    if (idx >= per * batch) return;
+    const long long t = idx / per, k = idx - t * per;
+    const float2* z = Z + t * M;
+    const long long k1 = (k == M) ? 0 : k;
+    const long long k2 = (k == 0) ? 0 : M - k;
+    const float2 zk = z[k1];
+    const float2 zc = cconj(z[k2]);
+    const float2 e = cscale(cadd(zk, zc), 0.5f);
+    const float2 o = cscale(csub(zk, zc), 0.5f);
+    float2 x = cadd(e, mul_negi(cmul(W[k], o)));
+    float sc = scale;
+    if (k == 0 || k == M) {
+        x.y = 0.f;
+    } else if (flags & OFX_FLAG_HERMITIAN_WEIGHT) {
+        sc *= 2.f;
+    }
+    X[t * per + k] = make_float2(x.x * sc, x.y * sc);
+}
+
+// X: [batch][M+1] complex -> Z': [batch][M] complex, conjugated (the complex FFT that follows runs
+// forward and conjugates its output)
+__global__ void k_c2r_pack(const float2* __restrict__ X, float2* __restrict__ Z, const float2* __restrict__ W,
+                           long long M, long long batch, int flags) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M * batch) return;
+    const long long t = idx / M, k = idx - t * M;
+    const float2* x = X + t * (M + 1);
+    const float hw = (flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 0.5f : 1.f;
+    float2 xa = x[k], xb = x[M - k];
+    if (k == 0) {
+        xa.y = 0.f;
+        xb.y = 0.f;
+    } else {
+        xa = cscale(xa, hw);
+        xb = cscale(xb, hw);
+    }
+    const float2 bc = cconj(xb);
+    Z[idx] = cconj(cadd(cadd(xa, bc), mul_posi(cmulc(csub(xa, bc), W[k]))));
+}
+
+}  // namespace ofx
+
+using namespace ofx;
+
+extern "C" int ofx_r2c_unpack(const void* Z, void* X, const void* W, long long M, long long batch, float scale,
+                              int flags, hipStream_t stream) {
+    const long long total = (M + 1) * batch;
+    if (total <= 0) return 0;
+    hipLaunchKernelGGL(k_r2c_unpack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
+                       static_cast<const float2*>(Z), static_cast<float2*>(X), static_cast<const float2*>(W), M,
+                       batch, scale, flags);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ofx_c2r_pack(const void* X, void* Z, const void* W, long long M, long long batch, int flags,
+                            hipStream_t stream) {
+    const long long total = M * batch;
+    if (total <= 0) return 0;
+    hipLaunchKernelGGL(k_c2r_pack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
+                       static_cast<const float2*>(X), static_cast<float2*>(Z), static_cast<const float2*>(W), M, batch,
+                       flags);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ofx_generic_radix_supported(int r) {
+    switch (r) {
+        case 2: case 3: case 4: case 5: case 6: case 7: case 8: case 9: case 10: case 11: case 12: case 13:
+        case 15: case 16: case 17: case 19: case 23: case 25: case 29: case 31: case 37: case 41: case 43:
+        case 47:
+            return 1;
+        default:
+            return 0;
+    }
+}
+
+extern "C" int ofx_launch_generic(const ofx_args* args, const ofx_generic_desc* desc, hipStream_t stream) {
+    if (args == nullptr || desc == nullptr) return (int)hipErrorInvalidValue;
+    if (args->batch <= 0) return 0;
+    const ofx_generic_desc& d = *desc;
+    if (d.n < 1 || d.tpb < 1 || (long long)d.tpb * d.n > OFX_GENERIC_MAX_ELEMS || d.npass < 0 ||
+        d.npass > OFX_MAX_PASSES || d.cnt_lo < 1)
+        return (int)hipErrorInvalidValue;
+    long long prod = 1;
+    for (int p = 0; p < d.npass; ++p) {
+        if (!ofx_generic_radix_supported(d.radices[p])) return (int)hipErrorInvalidValue;
+        prod *= d.radices[p];
+    }
+    if (prod != d.n) return (int)hipErrorInvalidValue;
+    if (d.ft_len > 0 && d.out_mode != OFX_OUT_COMPLEX) return (int)hipErrorInvalidValue;
+    const long long blocks = (args->batch + d.tpb - 1) / d.tpb;
+    if (blocks > 0x7fffffffLL) return (int)hipErrorInvalidConfiguration;
+    const size_t lds = 2 * sizeof(float2) * (size_t)d.tpb * d.n;
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_generic),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)(2 * sizeof(float2) * OFX_GENERIC_MAX_ELEMS));
+        if (e != hipSuccess) return (int)e;
+        attr_set = true;
+    }
+    GenParams p{*args, d};
+    hipLaunchKernelGGL(k_generic, dim3((unsigned)blocks), dim3(kGenBlock), lds, stream, p);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ofx_gen_ft_table(void* t, long long len, hipStream_t stream) {
+    if (len <= 0) return (int)hipErrorInvalidValue;
+    const long long count = kFtLo + (len + kFtLo - 1) / kFtLo;
+    hipLaunchKernelGGL(k_gen_ft_table, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream,
+                       static_cast<double2*>(t), len, count);
+    return (int)hipGetLastError();
+}
+
+extern "C" long long ofx_ft_table_count(long long len) { return kFtLo + (len + kFtLo - 1) / kFtLo; }

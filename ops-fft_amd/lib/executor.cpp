@@ -57,13 +57,64 @@ int env_int(const char* name, int dflt) {
 // -------------------------------------------------------------------------------------------
 // plan construction
 // -------------------------------------------------------------------------------------------
+
+// Radix sequence for the runtime-radix kernel: prime factors grouped into supported codelets
+// (2^k -> 16/8/4/2, 3^k -> 9/3), then the two smallest radices merged while their product is a
+// supported codelet of at most 16 (e.g. 3000 -> 8 x 15 x 25 is not reached; 3000 -> 8,3,5,5,5 ->
+// 8,15,5,5 -> 8,15,25).  Empty result: a prime factor above 47.
+std::vector<int> generic_radices(long long n) {
+    std::vector<int> f;
+    if (n <= 1) return f;
+    long long m = n;
+    int twos = 0;
+    while (m % 2 == 0) {
+        m /= 2;
+        ++twos;
+    }
+    while (twos >= 4) {
+        f.push_back(16);
+        twos -= 4;
+    }
+    if (twos > 0) f.push_back(1 << twos);
+    int threes = 0;
+    while (m % 3 == 0) {
+        m /= 3;
+        ++threes;
+    }
+    while (threes >= 2) {
+        f.push_back(9);
+        threes -= 2;
+    }
+    if (threes) f.push_back(3);
+    for (long long p = 5; p * p <= m || m > 1; p += 2) {
+        if (p * p > m) p = m;
+        while (m % p == 0) {
+            if (!ofx_generic_radix_supported((int)p)) return {};
+            f.push_back((int)p);
+            m /= p;
+        }
+        if (m == 1) break;
+    }
+    for (;;) {
+        std::sort(f.begin(), f.end());
+        if (f.size() < 2) break;
+        const int prod = f[0] * f[1];
+        if (prod > 25 || !ofx_generic_radix_supported(prod)) break;
+        f.erase(f.begin(), f.begin() + 2);
+        f.push_back(prod);
+    }
+    std::sort(f.begin(), f.end(), [](int a, int b) { return a > b; });
+    return f;
+}
+
 struct Builder {
     Executor* e;
 
     // All registry entries computing `kind` at complex length n, in preference order; staged
     // entries only when the transforms are contiguous.
-    std::vector<int> kernels_for(int kind, int n, bool contiguous) {
+    std::vector<int> kernels_for(int kind, long long n, bool contiguous) {
         std::vector<int> ids;
+        if (env_int("ACLFFT_DISABLE_TEMPLATED", 0) != 0) return ids;
         const auto& reg = registry();
         for (int i = 0; i < (int)reg.size(); ++i) {
             if (reg[i].kind == kind && reg[i].n == n && (contiguous || !reg[i].staged)) ids.push_back(i);
@@ -71,15 +122,16 @@ struct Builder {
         return ids;
     }
 
-    int pass_table(const ofx_kernel_info& k) {
-        if (k.tw_len <= 0) return -1;
+    int pass_table_for(int npass, const int* radices, long long len) {
+        if (len <= 0) return -1;
         TableSpec t;
         t.kind = TableSpec::PASS;
-        t.npass = k.npass;
-        for (int p = 0; p < k.npass; ++p) t.radices[p] = k.radices[p];
-        t.bytes = (size_t)k.tw_len * 8;
+        t.npass = npass;
+        for (int p = 0; p < npass; ++p) t.radices[p] = radices[p];
+        t.bytes = (size_t)len * 8;
         return e->add_table(t);
     }
+    int pass_table(const ofx_kernel_info& k) { return pass_table_for(k.npass, k.radices, k.tw_len); }
     int roots_table(long long len, long long count) {
         TableSpec t;
         t.kind = TableSpec::ROOTS;
@@ -88,12 +140,20 @@ struct Builder {
         t.bytes = (size_t)count * 8;
         return e->add_table(t);
     }
+    int fourstep_table(long long len) {
+        TableSpec t;
+        t.kind = TableSpec::FOURSTEP;
+        t.len = len;
+        t.count = ofx_ft_table_count(len);
+        t.bytes = (size_t)t.count * 16;
+        return e->add_table(t);
+    }
 
-    Step kernel_step(int kind, int n, const std::vector<int>& ids) {
+    Step kernel_step(int kind, long long n, const std::vector<int>& ids) {
         Step s;
         s.type = Step::KERNEL;
         s.kind = kind;
-        s.n = n;
+        s.n = (int)n;
         s.candidates = ids;
         for (int id : ids) s.candidate_tables.push_back(pass_table(registry()[id]));
         const int pick = std::min<int>(std::max(0, env_int("ACLFFT_KERNEL_VARIANT", 0)), (int)ids.size() - 1);
@@ -105,66 +165,275 @@ struct Builder {
         return s;
     }
 
-    // batch of contiguous C2C transforms of length n
-    aclfftResult c2c_rows(long long n, long long batch, int src, int dst) {
+    // A runtime-radix step of length n; false if n has an unsupported prime factor or is too long.
+    bool generic_step(long long n, Step* out) {
+        if (n > OFX_GENERIC_MAX_ELEMS) return false;
+        std::vector<int> r = generic_radices(n);
+        if ((r.empty() && n != 1) || (int)r.size() > OFX_MAX_PASSES) return false;
+        Step s;
+        s.type = Step::GENERIC;
+        s.n = (int)n;
+        ofx_generic_desc& g = s.g;
+        std::memset(&g, 0, sizeof g);
+        g.n = (int)n;
+        g.npass = (int)r.size();
+        for (int p = 0; p < g.npass; ++p) g.radices[p] = r[p];
+        // pass p >= 1 starts after the entries of passes 1..p-1 (pass 0 needs no twiddles)
+        long long acc = 0, ns2 = r.empty() ? 1 : r[0];
+        g.tw_off[0] = 0;
+        for (int p = 1; p < g.npass; ++p) {
+            g.tw_off[p] = (int)acc;
+            acc += ns2 * (r[p] - 1);
+            ns2 *= r[p];
+        }
+        s.tw_table = pass_table_for(g.npass, g.radices, acc);
+        g.tpb = (int)std::max<long long>(1, std::min<long long>(64, 2048 / n));
+        g.cnt_lo = 1;
+        s.args.in_sign = 1.f;
+        s.args.out_sign = 1.f;
+        s.args.scale = 1.f;
+        *out = s;
+        return true;
+    }
+
+    // Batch of contiguous C2C transforms of length n from buffer src to dst.  The direction's
+    // conjugation is applied on the first load (conj_in) and the last store (conj_out).
+    aclfftResult c2c_rows(long long n, long long batch, int src, int dst, bool conj_in, bool conj_out) {
         if (n == 1) {
             Step s;
             s.type = Step::COPY;
             s.src = src;
             s.dst = dst;
             s.copy_bytes = (size_t)batch * 8;
-            s.conj_in = s.conj_out = false;
+            e->steps_.push_back(s);
+            // a length-1 transform is the identity in both directions
+            return ACLFFT_SUCCESS;
+        }
+        auto ids = kernels_for(OFX_KIND_C2C, n, true);
+        if (!ids.empty()) {
+            Step s = kernel_step(OFX_KIND_C2C, n, ids);
+            s.src = src;
+            s.dst = dst;
+            s.args.batch = batch;
+            s.args.idist = n;
+            s.args.odist = n;
+            s.conj_in = conj_in;
+            s.conj_out = conj_out;
             e->steps_.push_back(s);
             return ACLFFT_SUCCESS;
         }
-        auto ids = kernels_for(OFX_KIND_C2C, (int)n, true);
-        if (ids.empty()) return ACLFFT_NOT_IMPLEMENTED;
-        Step s = kernel_step(OFX_KIND_C2C, (int)n, ids);
+        Step s;
+        if (generic_step(n, &s)) {
+            s.src = src;
+            s.dst = dst;
+            s.args.batch = batch;
+            s.args.idist = n;
+            s.args.odist = n;
+            s.args.istride = 1;
+            s.args.ostride = 1;
+            s.conj_in = conj_in;
+            s.conj_out = conj_out;
+            e->steps_.push_back(s);
+            return ACLFFT_SUCCESS;
+        }
+        return c2c_fourstep(n, batch, src, dst, conj_in, conj_out);
+    }
+
+    // Split n = n1 * n2 for the four-step: both factors runnable by the runtime-radix kernel, as
+    // close to sqrt(n) as possible with n1 (the strided column length) the smaller one.
+    bool split(long long n, long long* n1, long long* n2) {
+        long long best = 0;
+        for (long long a = 2; a * a <= n; ++a) {
+            if (n % a) continue;
+            const long long b = n / a;
+            if (a > OFX_GENERIC_MAX_ELEMS || b > OFX_GENERIC_MAX_ELEMS) continue;
+            if (generic_radices(a).empty() || generic_radices(b).empty()) continue;
+            best = a;
+        }
+        if (best == 0) return false;
+        *n1 = best;
+        *n2 = n / best;
+        return true;
+    }
+
+    // Four-step: x[n1*N2 + n2] -> columns (length N1, twiddle W_N^(n2 k1)) -> workspace
+    // tmp[k1*N2 + n2] -> rows (length N2) -> X[k1 + N1*k2].
+    aclfftResult c2c_fourstep(long long n, long long batch, int src, int dst, bool conj_in, bool conj_out) {
+        long long N1 = 0, N2 = 0;
+        if (!split(n, &N1, &N2)) return ACLFFT_NOT_IMPLEMENTED;
+        const int ft = fourstep_table(n);
+        Step a, b;
+        if (!generic_step(N1, &a) || !generic_step(N2, &b)) return ACLFFT_NOT_IMPLEMENTED;
+        // column step
+        a.g.tpb = (int)std::max<long long>(1, std::min<long long>(16, OFX_GENERIC_MAX_ELEMS / N1));
+        a.g.cnt_lo = N2;
+        a.g.idist_lo = 1;
+        a.g.odist_lo = 1;
+        a.g.in_tmajor = 1;
+        a.g.out_tmajor = 1;
+        a.g.ft_len = n;
+        a.aux_table = ft;
+        a.src = src;
+        a.dst = BUF_WORK0;
+        a.args.batch = batch * N2;
+        a.args.idist = n;
+        a.args.odist = n;
+        a.args.istride = N2;
+        a.args.ostride = N2;
+        a.conj_in = conj_in;
+        // row step
+        b.g.tpb = (int)std::max<long long>(1, std::min<long long>(16, OFX_GENERIC_MAX_ELEMS / N2));
+        b.g.cnt_lo = N1;
+        b.g.idist_lo = N2;
+        b.g.odist_lo = 1;
+        b.g.in_tmajor = 0;
+        b.g.out_tmajor = 1;
+        b.src = BUF_WORK0;
+        b.dst = dst;
+        b.args.batch = batch * N1;
+        b.args.idist = n;
+        b.args.odist = n;
+        b.args.istride = 1;
+        b.args.ostride = N1;
+        b.conj_out = conj_out;
+        e->steps_.push_back(a);
+        e->steps_.push_back(b);
+        e->work_bytes_ = std::max(e->work_bytes_, (size_t)(n * batch * 8));
+        return ACLFFT_SUCCESS;
+    }
+
+    // Columns of [hi][rows][cols] row-major data: `cols` transforms of length `rows` per image.
+    aclfftResult c2c_columns(long long rows, long long cols, long long images, int src, int dst, bool conj_in,
+                             bool conj_out) {
+        Step s;
+        if (!generic_step(rows, &s)) return ACLFFT_NOT_IMPLEMENTED;
+        s.g.tpb = (int)std::max<long long>(1, std::min<long long>(16, OFX_GENERIC_MAX_ELEMS / rows));
+        s.g.cnt_lo = cols;
+        s.g.idist_lo = 1;
+        s.g.odist_lo = 1;
+        s.g.in_tmajor = 1;
+        s.g.out_tmajor = 1;
         s.src = src;
         s.dst = dst;
-        s.args.batch = batch;
-        s.args.idist = n;
-        s.args.odist = n;
-        s.conj_in = s.conj_out = true;
+        s.args.batch = cols * images;
+        s.args.idist = rows * cols;
+        s.args.odist = rows * cols;
+        s.args.istride = cols;
+        s.args.ostride = cols;
+        s.conj_in = conj_in;
+        s.conj_out = conj_out;
         e->steps_.push_back(s);
         return ACLFFT_SUCCESS;
     }
 
     aclfftResult r2c(long long n, long long batch) {
         const long long h = n / 2 + 1;
-        if (n == 1) {  // X[0] = x[0]
-            return ACLFFT_NOT_IMPLEMENTED;
-        }
-        const bool even = (n % 2) == 0;
-        auto ids = kernels_for(even ? OFX_KIND_R2C : OFX_KIND_R2C_ODD, (int)(even ? n / 2 : n), true);
-        if (ids.empty()) return ACLFFT_NOT_IMPLEMENTED;
-        Step s = kernel_step(even ? OFX_KIND_R2C : OFX_KIND_R2C_ODD, (int)(even ? n / 2 : n), ids);
-        s.src = BUF_IN;
-        s.dst = BUF_OUT;
-        s.args.batch = batch;
-        s.args.idist = n;
-        s.args.odist = h;
-        if (even) s.aux_table = roots_table(n, n / 2 + 1);
-        e->steps_.push_back(s);
         e->in_place_unsafe_ = true;
+        const bool even = (n % 2) == 0;
+        if (n == 1) {
+            // X[0] = x[0]: the odd-n path with the length-1 generic step (no passes)
+        }
+        auto ids = kernels_for(even ? OFX_KIND_R2C : OFX_KIND_R2C_ODD, even ? n / 2 : n, true);
+        if (!ids.empty()) {
+            Step s = kernel_step(even ? OFX_KIND_R2C : OFX_KIND_R2C_ODD, even ? n / 2 : n, ids);
+            s.args.batch = batch;
+            s.args.idist = n;
+            s.args.odist = h;
+            if (even) s.aux_table = roots_table(n, n / 2 + 1);
+            e->steps_.push_back(s);
+            return ACLFFT_SUCCESS;
+        }
+        const long long m = even ? n / 2 : n;
+        Step s;
+        if (generic_step(m, &s)) {
+            s.args.batch = batch;
+            s.args.istride = 1;
+            s.args.ostride = 1;
+            if (even) {
+                s.g.in_mode = OFX_IN_COMPLEX;
+                s.g.out_mode = OFX_OUT_R2C_UNPACK;
+                s.args.idist = m;  // complex units over the real data
+                s.args.odist = h;
+                s.aux_table = roots_table(n, n / 2 + 1);
+            } else {
+                s.g.in_mode = OFX_IN_REAL;
+                s.g.out_mode = OFX_OUT_HALF;
+                s.args.idist = n;
+                s.args.odist = h;
+            }
+            e->steps_.push_back(s);
+            return ACLFFT_SUCCESS;
+        }
+        if (!even) return ACLFFT_NOT_IMPLEMENTED;
+        // large even n: M-point four-step into the workspace, then the unpack pass
+        const size_t before = e->steps_.size();
+        aclfftResult r = c2c_rows(m, batch, BUF_IN, BUF_WORK1, false, false);
+        if (r != ACLFFT_SUCCESS) return r;
+        (void)before;
+        Step u;
+        u.type = Step::R2C_UNPACK;
+        u.src = BUF_WORK1;
+        u.dst = BUF_OUT;
+        u.m = m;
+        u.args.batch = batch;
+        u.args.scale = 1.f;
+        u.aux_table = roots_table(n, n / 2 + 1);
+        e->steps_.push_back(u);
+        e->work1_bytes_ = std::max(e->work1_bytes_, (size_t)(m * batch * 8));
         return ACLFFT_SUCCESS;
     }
 
     aclfftResult c2r(long long n, long long batch) {
         const long long h = n / 2 + 1;
-        if (n == 1) return ACLFFT_NOT_IMPLEMENTED;
-        const bool even = (n % 2) == 0;
-        auto ids = kernels_for(even ? OFX_KIND_C2R : OFX_KIND_C2R_ODD, (int)(even ? n / 2 : n), true);
-        if (ids.empty()) return ACLFFT_NOT_IMPLEMENTED;
-        Step s = kernel_step(even ? OFX_KIND_C2R : OFX_KIND_C2R_ODD, (int)(even ? n / 2 : n), ids);
-        s.src = BUF_IN;
-        s.dst = BUF_OUT;
-        s.args.batch = batch;
-        s.args.idist = h;
-        s.args.odist = n;
-        if (even) s.aux_table = roots_table(n, n / 2 + 1);
-        e->steps_.push_back(s);
         e->in_place_unsafe_ = true;
+        const bool even = (n % 2) == 0;
+        auto ids = kernels_for(even ? OFX_KIND_C2R : OFX_KIND_C2R_ODD, even ? n / 2 : n, true);
+        if (!ids.empty()) {
+            Step s = kernel_step(even ? OFX_KIND_C2R : OFX_KIND_C2R_ODD, even ? n / 2 : n, ids);
+            s.args.batch = batch;
+            s.args.idist = h;
+            s.args.odist = n;
+            if (even) s.aux_table = roots_table(n, n / 2 + 1);
+            e->steps_.push_back(s);
+            return ACLFFT_SUCCESS;
+        }
+        const long long m = even ? n / 2 : n;
+        Step s;
+        if (generic_step(m, &s)) {
+            s.args.batch = batch;
+            s.args.idist = h;
+            s.args.istride = 1;
+            s.args.ostride = 1;
+            if (even) {
+                s.g.in_mode = OFX_IN_C2R_PACK;
+                s.g.out_mode = OFX_OUT_COMPLEX;
+                s.args.out_sign = -1.f;
+                s.args.odist = m;  // complex units over the real output
+                s.aux_table = roots_table(n, n / 2 + 1);
+            } else {
+                s.g.in_mode = OFX_IN_HERMITIAN;
+                s.g.out_mode = OFX_OUT_REAL;
+                s.args.odist = n;
+            }
+            e->steps_.push_back(s);
+            return ACLFFT_SUCCESS;
+        }
+        if (!even) return ACLFFT_NOT_IMPLEMENTED;
+        Step pk;
+        pk.type = Step::C2R_PACK;
+        pk.src = BUF_IN;
+        pk.dst = BUF_WORK1;
+        pk.m = m;
+        pk.args.batch = batch;
+        pk.aux_table = roots_table(n, n / 2 + 1);
+        e->steps_.push_back(pk);
+        e->work1_bytes_ = std::max(e->work1_bytes_, (size_t)(m * batch * 8));
+        // the packed data is already conjugated: forward transform, conjugated output
+        const size_t first = e->steps_.size();
+        aclfftResult r = c2c_rows(m, batch, BUF_WORK1, BUF_OUT, false, false);
+        if (r != ACLFFT_SUCCESS) return r;
+        e->steps_.back().args.out_sign = -1.f;
+        (void)first;
         return ACLFFT_SUCCESS;
     }
 };
@@ -189,7 +458,7 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
         switch (d.type) {
             case ACLFFT_C2C:
                 e->in_bytes_ = e->out_bytes_ = (size_t)n * d.batch * 8;
-                r = b.c2c_rows(n, d.batch, BUF_IN, BUF_OUT);
+                r = b.c2c_rows(n, d.batch, BUF_IN, BUF_OUT, true, true);
                 break;
             case ACLFFT_R2C:
                 e->in_bytes_ = (size_t)n * d.batch * 4;
@@ -203,6 +472,24 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
                 break;
             default:
                 r = ACLFFT_NOT_SUPPORTED;
+        }
+    } else if (d.rank == 1 && d.vertical && d.type == ACLFFT_C2C) {
+        // [n][cols] row-major, one transform per column (reference layout: VERTICAL, golden
+        // src/fft1_d/tests/fft1_d_data/gen_data.py:79-81)
+        const long long n = d.n0, cols = d.batch;
+        e->in_bytes_ = e->out_bytes_ = (size_t)n * cols * 8;
+        r = b.c2c_columns(n, cols, 1, BUF_IN, BUF_OUT, true, true);
+    } else if (d.rank == 2 && d.type == ACLFFT_C2C) {
+        const long long nx = d.n0, ny = d.n1;
+        e->in_bytes_ = e->out_bytes_ = (size_t)nx * ny * d.batch * 8;
+        // rows (length ny, contiguous) then columns (length nx, stride ny), columns in place
+        if (nx == 1) {
+            r = b.c2c_rows(ny, d.batch, BUF_IN, BUF_OUT, true, true);
+        } else if (ny == 1) {
+            r = b.c2c_rows(nx, d.batch, BUF_IN, BUF_OUT, true, true);
+        } else {
+            r = b.c2c_rows(ny, nx * d.batch, BUF_IN, BUF_OUT, true, false);
+            if (r == ACLFFT_SUCCESS) r = b.c2c_columns(nx, ny, d.batch, BUF_OUT, BUF_OUT, false, true);
         }
     }
     if (r != ACLFFT_SUCCESS) {
@@ -218,6 +505,7 @@ Executor::~Executor() {
     if (device_ >= 0 && hipGetDevice(&cur) == hipSuccess && cur != device_) (void)hipSetDevice(device_);
     if (d_tables_) (void)hipFree(d_tables_);
     if (d_work_) (void)hipFree(d_work_);
+    if (d_work1_) (void)hipFree(d_work1_);
     if (d_stage_in_) (void)hipFree(d_stage_in_);
     if (d_stage_out_) (void)hipFree(d_stage_out_);
     if (d_inplace_) (void)hipFree(d_inplace_);
@@ -232,6 +520,16 @@ std::string Executor::describe() const {
         if (i) os << " | ";
         if (s.type == Step::COPY) {
             os << "copy";
+            continue;
+        }
+        if (s.type == Step::R2C_UNPACK || s.type == Step::C2R_PACK) {
+            os << (s.type == Step::R2C_UNPACK ? "r2c_unpack" : "c2r_pack") << " M=" << s.m;
+            continue;
+        }
+        if (s.type == Step::GENERIC) {
+            os << "generic n=" << s.g.n << " tpb=" << s.g.tpb << " radices=";
+            for (int p = 0; p < s.g.npass; ++p) os << (p ? "x" : "") << s.g.radices[p];
+            if (s.g.ft_len) os << " fourstep-twiddle";
             continue;
         }
         const ofx_kernel_info& k = reg[s.kernel];
@@ -258,13 +556,19 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
         (void)hipGetLastError();
         return ACLFFT_ALLOC_FAILED;
     }
+    if (work1_bytes_ > 0 && hipMalloc(&d_work1_, work1_bytes_) != hipSuccess) {
+        (void)hipGetLastError();
+        return ACLFFT_ALLOC_FAILED;
+    }
     for (const TableSpec& t : tables_) {
         void* p = static_cast<char*>(d_tables_) + t.offset;
         int rc = 0;
         if (t.kind == TableSpec::PASS)
             rc = ofx_gen_pass_twiddles(p, t.npass, t.radices, s);
-        else
+        else if (t.kind == TableSpec::ROOTS)
             rc = ofx_gen_roots(p, t.len, t.count, s);
+        else
+            rc = ofx_gen_ft_table(p, t.len, s);
         if (rc != 0) {
             (void)hip_log((hipError_t)rc, "twiddle-table generation");
             return ACLFFT_SETUP_FAILED;
@@ -283,36 +587,63 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
 }
 
 aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream_t s) {
-    void* bufs[4] = {const_cast<void*>(din), dout, d_work_, d_work_};
+    void* bufs[4] = {const_cast<void*>(din), dout, d_work_, d_work1_};
     const float dsign = (direction == ACLFFT_BACKWARD) ? -1.f : 1.f;
+    auto table = [&](int idx) -> const void* {
+        return idx >= 0 ? static_cast<const void*>(static_cast<char*>(d_tables_) + tables_[idx].offset) : nullptr;
+    };
     for (const Step& st : steps_) {
         const char* src = static_cast<const char*>(bufs[st.src]) + st.src_off;
         char* dst = static_cast<char*>(bufs[st.dst]) + st.dst_off;
-        if (st.type == Step::COPY) {
-            if (src != dst && hipMemcpyAsync(dst, src, st.copy_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
-                return ACLFFT_EXEC_FAILED;
-            continue;
-        }
-        int kid = st.kernel;
-        int twt = st.tw_table;
-        if (registry()[kid].staged && ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15)) {
-            // staged tiles move 16-byte vectors: fall back to a direct-I/O candidate
-            for (size_t c = 0; c < st.candidates.size(); ++c)
-                if (!registry()[st.candidates[c]].staged) {
-                    kid = st.candidates[c];
-                    twt = st.candidate_tables[c];
-                    break;
+        hipError_t err = hipSuccess;
+        switch (st.type) {
+            case Step::COPY:
+                if (src != dst) err = hipMemcpyAsync(dst, src, st.copy_bytes, hipMemcpyDeviceToDevice, s);
+                break;
+            case Step::R2C_UNPACK:
+                err = (hipError_t)ofx_r2c_unpack(src, dst, table(st.aux_table), st.m, st.args.batch, st.args.scale,
+                                                 st.args.flags, s);
+                break;
+            case Step::C2R_PACK:
+                err = (hipError_t)ofx_c2r_pack(src, dst, table(st.aux_table), st.m, st.args.batch, st.args.flags, s);
+                break;
+            case Step::GENERIC: {
+                ofx_args a = st.args;
+                a.in = src;
+                a.out = dst;
+                a.tw = table(st.tw_table);
+                a.tw_aux = table(st.aux_table);
+                if (st.conj_in) a.in_sign *= dsign;
+                if (st.conj_out) a.out_sign *= dsign;
+                err = (hipError_t)ofx_launch_generic(&a, &st.g, s);
+                break;
+            }
+            case Step::KERNEL: {
+                int kid = st.kernel;
+                int twt = st.tw_table;
+                if (registry()[kid].staged &&
+                    ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15)) {
+                    // staged tiles move 16-byte vectors: fall back to a direct-I/O candidate
+                    for (size_t c = 0; c < st.candidates.size(); ++c)
+                        if (!registry()[st.candidates[c]].staged) {
+                            kid = st.candidates[c];
+                            twt = st.candidate_tables[c];
+                            break;
+                        }
+                    if (registry()[kid].staged) return ACLFFT_UNALIGNED_DATA;
                 }
-            if (registry()[kid].staged) return ACLFFT_UNALIGNED_DATA;
+                ofx_args a = st.args;
+                a.in = src;
+                a.out = dst;
+                a.tw = table(twt);
+                a.tw_aux = table(st.aux_table);
+                if (st.conj_in) a.in_sign *= dsign;
+                if (st.conj_out) a.out_sign *= dsign;
+                err = (hipError_t)ofx_launch(kid, &a, s);
+                break;
+            }
         }
-        ofx_args a = st.args;
-        a.in = src;
-        a.out = dst;
-        a.tw = twt >= 0 ? static_cast<char*>(d_tables_) + tables_[twt].offset : nullptr;
-        a.tw_aux = st.aux_table >= 0 ? static_cast<char*>(d_tables_) + tables_[st.aux_table].offset : nullptr;
-        if (st.conj_in) a.in_sign *= dsign;
-        if (st.conj_out) a.out_sign *= dsign;
-        if (hip_log((hipError_t)ofx_launch(kid, &a, s), "kernel launch") != hipSuccess) return ACLFFT_EXEC_FAILED;
+        if (hip_log(err, "kernel launch") != hipSuccess) return ACLFFT_EXEC_FAILED;
     }
     return ACLFFT_SUCCESS;
 }

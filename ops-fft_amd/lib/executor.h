@@ -21,17 +21,17 @@ enum BufId : int { BUF_IN = 0, BUF_OUT = 1, BUF_WORK0 = 2, BUF_WORK1 = 3 };
 
 // Device lookup tables a plan needs; all live in one allocation.
 struct TableSpec {
-    enum Kind { PASS, ROOTS } kind = PASS;
+    enum Kind { PASS, ROOTS, FOURSTEP } kind = PASS;
     int npass = 0;
     int radices[OFX_MAX_PASSES] = {0};
-    long long len = 0, count = 0;  // ROOTS: W_len^k, k < count
+    long long len = 0, count = 0;  // ROOTS: W_len^k, k < count; FOURSTEP: FP64 table of W_len
     size_t offset = 0;             // bytes from the start of the table allocation
     size_t bytes = 0;
 };
 
 // One launch.
 struct Step {
-    enum Type { KERNEL, COPY } type = KERNEL;
+    enum Type { KERNEL, GENERIC, COPY, R2C_UNPACK, C2R_PACK } type = KERNEL;
     int kernel = -1;               // registry id (KERNEL)
     int kind = OFX_KIND_C2C;
     int n = 0;                     // complex length the kernel computes (for autotuning)
@@ -42,6 +42,8 @@ struct Step {
     bool conj_in = false;          // C2C: apply the direction's conjugation on load / store
     bool conj_out = false;
     size_t copy_bytes = 0;         // COPY
+    ofx_generic_desc g{};          // GENERIC
+    long long m = 0;               // R2C_UNPACK / C2R_PACK: complex length M = n/2
     std::vector<int> candidates;   // registry ids that can replace `kernel` (autotuning)
     std::vector<int> candidate_tables;  // pass-twiddle table of each candidate
 };
@@ -64,7 +66,7 @@ public:
     // C2C; ignored for R2C/C2R.
     aclfftResult exec(const void* in, void* out, int direction, hipStream_t stream);
 
-    size_t work_bytes() const { return work_bytes_; }
+    size_t work_bytes() const { return work_bytes_ + work1_bytes_; }
     size_t in_bytes() const { return in_bytes_; }
     size_t out_bytes() const { return out_bytes_; }
     const std::vector<Step>& steps() const { return steps_; }
@@ -81,13 +83,15 @@ private:
     std::vector<Step> steps_;
     std::vector<TableSpec> tables_;
     size_t table_bytes_ = 0;
-    size_t work_bytes_ = 0;
+    size_t work_bytes_ = 0;       // four-step intermediate
+    size_t work1_bytes_ = 0;      // large real transforms: the packed complex data
     size_t in_bytes_ = 0, out_bytes_ = 0;
     bool in_place_unsafe_ = false;  // in == out must go through the workspace
 
     int device_ = -1;
     void* d_tables_ = nullptr;
     void* d_work_ = nullptr;
+    void* d_work1_ = nullptr;
     void* d_stage_in_ = nullptr;
     void* d_stage_out_ = nullptr;
     void* d_inplace_ = nullptr;

@@ -104,7 +104,11 @@ static void plan_make(or_plan* pl, long long n) {
                 long double c, sn;
                 cs2pi(p * j, len, &c, &sn);
                 pl->tw[s][2 * (p * prev + j)] = (double)c;
+#ifdef OR_PLANT_TWIDDLE_SIGN /* deliberately wrong build, used only to show the pins catch it */
+                pl->tw[s][2 * (p * prev + j) + 1] = (double)sn;
+#else
                 pl->tw[s][2 * (p * prev + j) + 1] = (double)-sn;
+#endif
             }
         for (long long q = 0; q < r; ++q)
             for (long long p = 0; p < r; ++p) {

@@ -21,26 +21,31 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+def load(path):
+    """ctypes handle of an oracle build at `path` with the argument types set."""
+    L = ctypes.CDLL(path)
+    d = ctypes.POINTER(ctypes.c_double)
+    ll = ctypes.c_longlong
+    for name, args in (
+        ("oracle_dft_naive", [d, d, ll, ll, ctypes.c_int]),
+        ("oracle_c2c", [d, d, ll, ll, ctypes.c_int]),
+        ("oracle_r2c", [d, d, ll, ll]),
+        ("oracle_c2r", [d, d, ll, ll]),
+        ("oracle_c2c_2d", [d, d, ll, ll, ll, ctypes.c_int]),
+        ("oracle_c2c_vertical", [d, d, ll, ll, ctypes.c_int]),
+    ):
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = None
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        d = ctypes.POINTER(ctypes.c_double)
-        ll = ctypes.c_longlong
-        for name, args in (
-            ("oracle_dft_naive", [d, d, ll, ll, ctypes.c_int]),
-            ("oracle_c2c", [d, d, ll, ll, ctypes.c_int]),
-            ("oracle_r2c", [d, d, ll, ll]),
-            ("oracle_c2r", [d, d, ll, ll]),
-            ("oracle_c2c_2d", [d, d, ll, ll, ll, ctypes.c_int]),
-            ("oracle_c2c_vertical", [d, d, ll, ll, ctypes.c_int]),
-        ):
-            f = getattr(L, name)
-            f.argtypes = args
-            f.restype = None
-        _lib = L
+        _lib = load(_LIB_PATH)
     return _lib
 
 

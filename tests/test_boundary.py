@@ -1,0 +1,227 @@
+"""CPU tests of the drop-in boundary: exported symbols, header ABI, and the reference's
+negative-path result codes.  No GPU work is issued: every call here is rejected (or accepted)
+before the library touches a device.
+
+Expected codes come from the reference's own negative tests:
+  src/fft1_d/tests/fft1_d_test_arch32.cpp:58-86,183-222 (plan gates, entry_reject_regression 6-7)
+  src/fft2_d/tests/fft2_d_test_arch32.cpp:52-126 (MakePlan2d type / size gates)
+  lib/fft_exec_api.cpp:32-116, lib/fft_plan_api.cpp:24-150 (exec/plan checks)
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import aclfft
+from aclfft import lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+
+H = ctypes.c_void_p
+
+
+def plan1d(nx, t, batch, dim=aclfft.HORIZONTAL):
+    h = H()
+    r = lib.aclfftPlan1d(ctypes.byref(h), nx, t, batch, dim)
+    return r, h
+
+
+def declared_functions():
+    names = []
+    for f in sorted(os.listdir(INC)):
+        if f.endswith(".h"):
+            txt = "\n".join(l for l in open(os.path.join(INC, f)).read().splitlines()
+                            if not l.lstrip().startswith("#"))
+            names += re.findall(r"(?:ACLFFT_API|OFX_API)\s+[\w\s\*]+?\**\s*(\w+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def test_exports_every_declared_symbol():
+    out = subprocess.check_output(["nm", "-D", "--defined-only", aclfft.LIB_PATH]).decode()
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    decl = declared_functions()
+    assert set(aclfft.ACLFFT_SYMBOLS) <= set(decl)
+    missing = [n for n in decl if n not in exported]
+    assert not missing, missing
+    # exactly the reference's 25 aclfft* names plus this library's ext/ofx additions
+    acl = {n for n in exported if n.startswith("aclfft") and not n.startswith("aclfftExt")}
+    assert acl == set(aclfft.ACLFFT_SYMBOLS)
+
+
+def test_header_compiles_as_c_and_links(tmp_path):
+    src = tmp_path / "t.c"
+    src.write_text(
+        '#include "cann_ops_fft.h"\n#include "ops_fft_ext.h"\n#include "ofx_launch.h"\n'
+        "int main(void) { aclfftHandle p = 0; hipStream_t s = 0; aclrtStream a = s;\n"
+        "  (void)a; if (aclfftPlan1d(&p, 64, ACLFFT_C2C, 1, ACLFFT_HORIZONTAL) != ACLFFT_SUCCESS) return 1;\n"
+        "  if (aclfftSetStream(p, s) != ACLFFT_SUCCESS) return 2;\n"
+        "  if (aclfftDestroy(p) != ACLFFT_SUCCESS) return 3;\n"
+        "  return sizeof(aclfftComplex) == 8 && ACLFFT_NOT_SUPPORTED == 16 && ACLFFT_C2C == 0x29 ? 0 : 4; }\n")
+    exe = tmp_path / "t"
+    libdir = os.path.dirname(aclfft.LIB_PATH)
+    subprocess.check_call(["gcc", "-std=c11", "-Wall", "-Werror", "-I", INC, str(src), "-o", str(exe),
+                           "-L", libdir, "-lcann_ops_fft", "-Wl,-rpath," + libdir])
+    assert subprocess.call([str(exe)]) == 0
+
+
+def test_error_strings_match_reference():
+    # lib/fft_utils_api.cpp:19-56
+    expect = {0: "Success", 1: "Invalid plan handle", 2: "Memory allocation failed", 3: "Invalid data type",
+              4: "Invalid parameter value", 5: "Internal error", 6: "Execution failed", 7: "Setup failed",
+              8: "Invalid size parameter", 9: "Data not aligned", 10: "Incomplete parameter list",
+              11: "Invalid device", 12: "Parse error", 13: "No workspace available",
+              14: "Feature not implemented", 16: "Feature not supported", 15: "Unknown error", 99: "Unknown error"}
+    for code, s in expect.items():
+        assert aclfft.error_string(code) == s
+
+
+def test_plan1d_argument_gates():
+    assert lib.aclfftPlan1d(None, 64, aclfft.C2C, 1, 0) == aclfft.INVALID_VALUE
+    assert plan1d(0, aclfft.C2C, 1)[0] == aclfft.INVALID_SIZE
+    assert plan1d(-4, aclfft.C2C, 1)[0] == aclfft.INVALID_SIZE
+    assert plan1d(64, aclfft.C2C, 0)[0] == aclfft.INVALID_SIZE
+    assert plan1d(64, aclfft.C2C, 1, 2)[0] == aclfft.INVALID_VALUE
+    assert plan1d(64, 0x99, 1)[0] == aclfft.INVALID_TYPE
+    # issue #171 in the reference: double precision refused at plan time
+    for t in (aclfft.Z2Z, aclfft.D2Z, aclfft.Z2D):
+        r, h = plan1d(256, t, 1)
+        assert r == aclfft.NOT_SUPPORTED and not h.value
+    # 1D size domain (issue #190): horizontal <= 2^27; vertical pow2 in [256, 2^18]
+    assert plan1d((1 << 27) + 1, aclfft.C2C, 1)[0] == aclfft.INVALID_SIZE
+    for n in (128, 300, 64, (1 << 18) * 2):
+        assert plan1d(n, aclfft.C2C, 1, aclfft.VERTICAL)[0] == aclfft.INVALID_SIZE
+    r, h = plan1d(1 << 27, aclfft.C2C, 1)
+    assert r == aclfft.SUCCESS
+    lib.aclfftDestroy(h)
+
+
+def test_plan1d_success_and_reuse():
+    r, h = plan1d(1024, aclfft.C2C, 64)
+    assert r == aclfft.SUCCESS and h.value
+    ws = ctypes.c_size_t(123)
+    # MakePlan on an initialised plan is INVALID_VALUE (lib/fft_error.h:124-128)
+    assert lib.aclfftMakePlan1d(h, 1024, aclfft.C2C, 64, 0, ctypes.byref(ws)) == aclfft.INVALID_VALUE
+    assert lib.aclfftMakePlan1d(None, 1024, aclfft.C2C, 64, 0, ctypes.byref(ws)) == aclfft.INVALID_VALUE
+    assert lib.aclfftSetStream(h, None) == aclfft.SUCCESS
+    assert lib.aclfftDestroy(h) == aclfft.SUCCESS
+    assert lib.aclfftDestroy(None) == aclfft.INVALID_VALUE
+    assert lib.aclfftSetStream(None, None) == aclfft.INVALID_VALUE
+
+
+def test_makeplan_reports_workspace():
+    h = H()
+    assert lib.aclfftCreate(ctypes.byref(h)) == aclfft.SUCCESS
+    ws = ctypes.c_size_t(7)
+    assert lib.aclfftMakePlan1d(h, 1024, aclfft.C2C, 8, 0, ctypes.byref(ws)) == aclfft.SUCCESS
+    assert ws.value == 0  # single-kernel plan: no workspace
+    lib.aclfftDestroy(h)
+    h = H()
+    lib.aclfftCreate(ctypes.byref(h))
+    assert lib.aclfftMakePlan1d(h, 1 << 20, aclfft.C2C, 2, 0, ctypes.byref(ws)) == aclfft.SUCCESS
+    assert ws.value == (1 << 20) * 2 * 8  # four-step intermediate
+    lib.aclfftDestroy(h)
+
+
+def test_plan2d_gates(monkeypatch):
+    h = H()
+    assert lib.aclfftPlan2d(None, 1, 64, 64, aclfft.C2C) == aclfft.INVALID_VALUE
+    assert lib.aclfftPlan2d(ctypes.byref(h), 0, 64, 64, aclfft.C2C) == aclfft.INVALID_SIZE
+    assert lib.aclfftPlan2d(ctypes.byref(h), 1, 0, 64, aclfft.C2C) == aclfft.INVALID_SIZE
+    assert lib.aclfftPlan2d(ctypes.byref(h), 1, 64, -1, aclfft.C2C) == aclfft.INVALID_SIZE
+    assert lib.aclfftPlan2d(ctypes.byref(h), 1, 64, 64, 0x77) == aclfft.INVALID_TYPE
+    for t in (aclfft.Z2Z, aclfft.D2Z, aclfft.Z2D, aclfft.R2C, aclfft.C2R):
+        assert lib.aclfftPlan2d(ctypes.byref(h), 1, 64, 64, t) == aclfft.NOT_SUPPORTED
+    # makeplan2d_type_reject (fft2_d_test_arch32.cpp:52-86)
+    for t in (aclfft.Z2Z, aclfft.D2Z, aclfft.Z2D, aclfft.R2C, aclfft.C2R):
+        p = H()
+        lib.aclfftCreate(ctypes.byref(p))
+        assert lib.aclfftMakePlan2d(p, 1, 64, 64, t, None) == aclfft.NOT_SUPPORTED
+        lib.aclfftDestroy(p)
+    p = H()
+    lib.aclfftCreate(ctypes.byref(p))
+    assert lib.aclfftMakePlan2d(p, 1, 64, 64, aclfft.C2C, None) == aclfft.SUCCESS
+    lib.aclfftDestroy(p)
+
+
+@pytest.mark.parametrize("strict", [True, False])
+def test_makeplan2d_size_domain(monkeypatch, strict):
+    """makeplan2d_size_reject (fft2_d_test_arch32.cpp:88-126).  With
+    ACLFFT_STRICT_REFERENCE_DOMAIN=1 the reference's {32,64,128}^2 gate is reproduced exactly; by
+    default this library is a superset and accepts those sizes."""
+    monkeypatch.setenv("ACLFFT_STRICT_REFERENCE_DOMAIN", "1" if strict else "0")
+    for nx, ny in ((100, 100), (32, 100), (100, 32), (256, 256), (64, 48), (16, 32)):
+        p = H()
+        lib.aclfftCreate(ctypes.byref(p))
+        r = lib.aclfftMakePlan2d(p, 1, nx, ny, aclfft.C2C, None)
+        assert r == (aclfft.NOT_SUPPORTED if strict else aclfft.SUCCESS), (nx, ny, r)
+        lib.aclfftDestroy(p)
+    p = H()
+    lib.aclfftCreate(ctypes.byref(p))
+    assert lib.aclfftMakePlan2d(p, 1, 32, 128, aclfft.C2C, None) == aclfft.SUCCESS
+    lib.aclfftDestroy(p)
+
+
+def test_plan3d_and_placeholders():
+    h = H()
+    assert lib.aclfftPlan3d(ctypes.byref(h), 8, 8, 8, aclfft.C2C) == aclfft.NOT_IMPLEMENTED
+    assert not h.value
+    assert lib.aclfftPlan3d(ctypes.byref(h), 0, 8, 8, aclfft.C2C) == aclfft.INVALID_SIZE
+    assert lib.aclfftPlan3d(ctypes.byref(h), 8, 8, 8, 0x11) == aclfft.INVALID_TYPE
+    ws = ctypes.c_size_t()
+    assert lib.aclfftGetSize1d(None, 8, aclfft.C2C, 1, ctypes.byref(ws)) == aclfft.NOT_IMPLEMENTED
+    assert lib.aclfftGetSize2d(None, 8, 8, aclfft.C2C, ctypes.byref(ws)) == aclfft.NOT_IMPLEMENTED
+    assert lib.aclfftGetSize3d(None, 8, 8, 8, aclfft.C2C, ctypes.byref(ws)) == aclfft.NOT_IMPLEMENTED
+    assert lib.aclfftGetSize(None, ctypes.byref(ws)) == aclfft.NOT_IMPLEMENTED
+    assert lib.aclfftSetAutoAllocation(None, 1) == aclfft.NOT_IMPLEMENTED
+    assert lib.aclfftSetWorkArea(None, None) == aclfft.NOT_IMPLEMENTED
+    assert lib.aclfftSetStride(None, 4) == aclfft.NOT_IMPLEMENTED
+    v = ctypes.c_int()
+    assert lib.aclfftGetVersion(ctypes.byref(v)) == aclfft.NOT_IMPLEMENTED
+    assert lib.aclfftGetProperty(0, ctypes.byref(v)) == aclfft.NOT_IMPLEMENTED
+    assert lib.aclfftExecZ2Z(None, None, None, -1) == aclfft.NOT_IMPLEMENTED
+    assert lib.aclfftExecD2Z(None, None, None) == aclfft.NOT_IMPLEMENTED
+    assert lib.aclfftExecZ2D(None, None, None) == aclfft.NOT_IMPLEMENTED
+
+
+def test_exec_gates_before_any_device_call():
+    buf = (ctypes.c_float * 64)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    assert lib.aclfftExecC2C(None, p, p, -1) == aclfft.INVALID_VALUE
+    h = H()
+    lib.aclfftCreate(ctypes.byref(h))
+    assert lib.aclfftExecC2C(h, p, p, -1) == aclfft.INVALID_PLAN  # created, not initialised
+    assert lib.aclfftExecR2C(h, p, p) == aclfft.INVALID_PLAN
+    lib.aclfftDestroy(h)
+    r, h = plan1d(16, aclfft.C2C, 2)
+    assert lib.aclfftExecC2C(h, None, p, -1) == aclfft.INVALID_VALUE
+    assert lib.aclfftExecC2C(h, p, None, -1) == aclfft.INVALID_VALUE
+    assert lib.aclfftExecC2C(h, p, p, 0) == aclfft.INVALID_VALUE
+    assert lib.aclfftExecC2C(h, p, p, 2) == aclfft.INVALID_VALUE
+    assert lib.aclfftExecR2C(h, p, p) == aclfft.INVALID_TYPE
+    assert lib.aclfftExecC2R(h, p, p) == aclfft.INVALID_TYPE
+    lib.aclfftDestroy(h)
+    r, h = plan1d(16, aclfft.R2C, 2)
+    assert lib.aclfftExecC2C(h, p, p, -1) == aclfft.INVALID_TYPE
+    assert lib.aclfftExecC2R(h, p, p) == aclfft.INVALID_TYPE
+    lib.aclfftDestroy(h)
+    # VERTICAL R2C/C2R: valid plan, NOT_IMPLEMENTED at exec (src/common/host/fft_exec_helper.h:42-46)
+    for t, fn in ((aclfft.R2C, lib.aclfftExecR2C), (aclfft.C2R, lib.aclfftExecC2R)):
+        r, h = plan1d(256, t, 4, aclfft.VERTICAL)
+        assert r == aclfft.SUCCESS
+        assert fn(h, p, p) == aclfft.NOT_IMPLEMENTED
+        lib.aclfftDestroy(h)
+
+
+def test_unsupported_factor_is_not_implemented_at_exec():
+    """A length with a prime factor beyond the codelet set plans successfully and reports
+    NOT_IMPLEMENTED at execution, as the reference does for factors outside RADIX_MIX
+    (src/fft1_d/arch35/fft1_d_exec_api.cpp:54-61)."""
+    buf = (ctypes.c_float * 64)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    r, h = plan1d(10007, aclfft.C2C, 1)
+    assert r == aclfft.SUCCESS
+    assert lib.aclfftExecC2C(h, p, p, -1) == aclfft.NOT_IMPLEMENTED
+    lib.aclfftDestroy(h)
