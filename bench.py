@@ -76,12 +76,145 @@ def cpu_golden(x_cpu, torch):
                 kind="torch.fft (reference golden path)", sample="full workload, median of 3, %.3f s" % t)
 
 
+def time_steps(torch, fn, steps, warmup, stream):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def extra_copy(torch, x, steps, stream):
+    """Device-to-device copy of the same 1 GiB: the achievable HBM bandwidth reference."""
+    y = torch.empty_like(x)
+    ms = time_steps(torch, lambda: y.copy_(x), steps, 3, stream)
+    b = 2.0 * x.numel() * x.element_size()
+    return {"ms": round(ms, 5), "GB/s": round(b / ms / 1e6, 1), "frac_of_peak": round(b / ms / 1e6 / HBM_PEAK_GBS, 4)}
+
+
+def extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y, steps, stream, e32):
+    out = {}
+    bytes_ = 16.0 * N * BATCH
+    for mode, name in ((aclfft.PRECISION_BF16, "bf16"), (aclfft.PRECISION_FP16X2, "fp16x2"),
+                       (aclfft.PRECISION_BF16X3, "bf16x3")):
+        p = aclfft.Plan.plan1d(N, aclfft.C2C, BATCH)
+        p.set_precision(mode)
+        p.set_stream(stream)
+        ms = time_steps(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
+        ref = oracle.c2c(x_cpu[:256].numpy().astype(np.complex128))
+        err = oracle.rel_l2(y[:256].cpu().numpy(), ref)
+        out[name] = {"ms": round(ms, 5), "GFLOP/s": round(5.0 * N * math.log2(N) * BATCH / ms / 1e6, 1),
+                     "hbm_frac": round(bytes_ / ms / 1e6 / HBM_PEAK_GBS, 4), "rel_l2_vs_fp64": err,
+                     "err_ratio_vs_fp32": round(err / e32, 3)}
+        p.destroy()
+    return out
+
+
+def extra_real(torch, aclfft, oracle, np, steps, stream):
+    out = {}
+    for n in (4096, 3000):
+        b = 8192
+        g = torch.Generator().manual_seed(1)
+        xr = torch.randn(b, n, generator=g).cuda()
+        X = torch.empty(b, n // 2 + 1, dtype=torch.complex64, device="cuda")
+        xb = torch.empty_like(xr)
+        pr = aclfft.Plan.plan1d(n, aclfft.R2C, b)
+        pc = aclfft.Plan.plan1d(n, aclfft.C2R, b)
+        pr.set_stream(stream)
+        pc.set_stream(stream)
+        ms_r = time_steps(torch, lambda: pr.exec_r2c(xr, X), steps, 3, stream)
+        ms_c = time_steps(torch, lambda: pc.exec_c2r(X, xb), steps, 3, stream)
+        byts = (4.0 * n + 8.0 * (n // 2 + 1)) * b
+        rt = float(((xb / n - xr).norm() / xr.norm()).item())
+        err = oracle.rel_l2(X[:64].cpu().numpy(), oracle.r2c(xr[:64].cpu().numpy().astype(np.float64)))
+        out["n%d_b%d" % (n, b)] = {
+            "r2c_ms": round(ms_r, 5), "c2r_ms": round(ms_c, 5),
+            "r2c_hbm_frac": round(byts / ms_r / 1e6 / HBM_PEAK_GBS, 4),
+            "c2r_hbm_frac": round(byts / ms_c / 1e6 / HBM_PEAK_GBS, 4),
+            "r2c_GFLOP/s_2.5nlogn": round(2.5 * n * math.log2(n) * b / ms_r / 1e6, 1),
+            "roundtrip_rel_l2": rt, "r2c_rel_l2_vs_fp64": err, "plan_r2c": pr.describe()}
+        pr.destroy()
+        pc.destroy()
+    return out
+
+
+def extra_2d(torch, aclfft, oracle, np, steps, stream):
+    B, NX, NY = 4, 2048, 2048
+    g = torch.Generator().manual_seed(2)
+    x = torch.randn(B, NX, NY, dtype=torch.complex64, generator=g).cuda()
+    y = torch.empty_like(x)
+    p = aclfft.Plan.plan2d(B, NX, NY, aclfft.C2C)
+    p.set_stream(stream)
+    ms = time_steps(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
+    err = oracle.rel_l2(y[:1].cpu().numpy(), np.fft.fft2(x[:1].cpu().numpy().astype(np.complex128)))
+    pts = B * NX * NY
+    out = {"ms": round(ms, 5), "GFLOP/s": round(5.0 * pts * math.log2(NX * NY) / ms / 1e6, 1),
+           "hbm_frac_compulsory": round(16.0 * pts / ms / 1e6 / HBM_PEAK_GBS, 4),
+           "hbm_frac_two_pass": round(32.0 * pts / ms / 1e6 / HBM_PEAK_GBS, 4),
+           "rel_l2_vs_fp64": err, "plan": p.describe()}
+    p.destroy()
+    return out
+
+
+def extra_batched(torch, aclfft, oracle, np, steps, stream):
+    """512 C2C transforms, lengths log-uniform over the set, batches U{1..64}; persistent kernel +
+    hipGraph in one plan vs one plan (and launch) per group."""
+    lengths = (60, 128, 243, 1000, 1024, 2048, 4096, 6561)
+    rng = np.random.default_rng(4)
+    logs = np.log(np.array(lengths, float))
+    u = rng.uniform(logs.min(), logs.max(), 512)
+    ln = [int(lengths[int(np.argmin(np.abs(logs - v)))]) for v in u]
+    bt = [int(b) for b in rng.integers(1, 65, 512)]
+    total = sum(a * b for a, b in zip(ln, bt))
+    x = (torch.rand(total, dtype=torch.complex64, device="cuda") * 2 - (1 + 1j)).to(torch.complex64)
+    y = torch.empty_like(x)
+    p = aclfft.Plan.plan_many_uneven(ln, bt)
+    p.set_stream(stream)
+    ms = time_steps(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
+    # correctness on a few groups
+    off = 0
+    worst = 0.0
+    ycpu = y.cpu().numpy()
+    xcpu = x.cpu().numpy()
+    for gi, (n, b) in enumerate(zip(ln, bt)):
+        if gi % 64 == 0:
+            seg = xcpu[off:off + n * b].reshape(b, n).astype(np.complex128)
+            worst = max(worst, oracle.rel_l2(ycpu[off:off + n * b].reshape(b, n), oracle.c2c(seg)))
+        off += n * b
+    plans = [aclfft.Plan.plan1d(n, aclfft.C2C, b) for n, b in zip(ln, bt)]
+    offs = np.cumsum([0] + [a * b for a, b in zip(ln, bt)])
+
+    def per_group():
+        for i, q in enumerate(plans):
+            q.exec_c2c(x[offs[i]:offs[i + 1]], y[offs[i]:offs[i + 1]], aclfft.FORWARD)
+
+    for q in plans:
+        q.set_stream(stream)
+    ms_naive = time_steps(torch, per_group, max(3, steps // 5), 2, stream)
+    flops = sum(5.0 * n * math.log2(n) * b for n, b in zip(ln, bt))
+    out = {"groups": 512, "points": int(total), "persistent_graph_ms": round(ms, 5),
+           "per_group_launches_ms": round(ms_naive, 5), "speedup": round(ms_naive / ms, 2),
+           "GFLOP/s": round(flops / ms / 1e6, 1), "hbm_frac": round(16.0 * total / ms / 1e6 / HBM_PEAK_GBS, 4),
+           "worst_rel_l2_vs_fp64": worst, "plan": p.describe()}
+    for q in plans:
+        q.destroy()
+    p.destroy()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-extras", action="store_true", help="only the headline C2C configuration")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the FFT path has no multi-GPU mode in this build "
@@ -148,6 +281,20 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
+    if not args.no_extras:
+        extras = {}
+        for name, fn in (("copy_1GiB", lambda: extra_copy(torch, x, 20, stream)),
+                         ("mfma_split_n1024_b65536", lambda: extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y,
+                                                                        args.steps, stream, err)),
+                         ("real_r2c_c2r", lambda: extra_real(torch, aclfft, oracle, np, args.steps, stream)),
+                         ("c2c_2d_2048x2048_b4", lambda: extra_2d(torch, aclfft, oracle, np, 10, stream)),
+                         ("uneven_batched_512", lambda: extra_batched(torch, aclfft, oracle, np, 20, stream))):
+            try:
+                log("bench: extra %s" % name)
+                extras[name] = fn()
+            except Exception as ex:  # an extra must not hide the headline line
+                extras[name] = {"error": "%s: %s" % (type(ex).__name__, ex)}
+        out["extras"] = extras
     if not args.no_cpu:
         sample = x_cpu[:512].numpy().astype(np.complex128)
         out["cpu_baseline"] = cpu_baseline(sample)

@@ -134,6 +134,53 @@ OFX_API int ofx_r2c_unpack(const void* Z, void* X, const void* W, long long M, l
 OFX_API int ofx_c2r_pack(const void* X, void* Z, const void* W, long long M, long long batch, int flags,
                          hipStream_t stream);
 
+/* MFMA split-precision C2C (dense 32-point DFT stages on v_mfma_f32_32x32x16, FP32 accumulate).
+ * Operand precision: */
+enum ofx_mfma_mode {
+    OFX_MFMA_BF16 = 1,    /* one BF16 part per operand (about 3 significant digits)          */
+    OFX_MFMA_FP16X2 = 2,  /* FP16 hi + lo parts, 3 products (about FP32 accuracy)            */
+    OFX_MFMA_BF16X3 = 3   /* BF16 hi + mid + lo parts, 6 products (about FP32 accuracy)      */
+};
+/* bytes of the constant-operand table of `mode` and its generation (FP64 sincospi on the GPU) */
+OFX_API long long ofx_mfma_const_bytes(int mode);
+OFX_API int ofx_gen_mfma_consts(int mode, void* table, hipStream_t stream);
+/* Batch of length-n C2C transforms (n = 1024), contiguous (idist/odist in complex64 units).
+ * args->tw = constant table of `mode`, args->tw_aux = W_1024^k (k < 1024, ofx_gen_roots).
+ * blocks: workgroups of the grid-stride launch (0 = default). */
+OFX_API int ofx_launch_mfma_c2c(int mode, int n, const ofx_args* args, int blocks, hipStream_t stream);
+
+/* Persistent batched C2C over uneven lengths: compiled length classes. */
+enum ofx_batch_class {
+    OFX_BCLASS_60 = 0, OFX_BCLASS_128, OFX_BCLASS_243, OFX_BCLASS_256, OFX_BCLASS_512, OFX_BCLASS_1000,
+    OFX_BCLASS_1024, OFX_BCLASS_2048, OFX_BCLASS_4096, OFX_BCLASS_6561, OFX_BCLASS_COUNT
+};
+typedef struct ofx_batch_class_info {
+    int n;                          /* transform length                                       */
+    int transforms_per_tile;        /* transforms one workgroup processes together            */
+    int npass;
+    int radices[OFX_MAX_PASSES];
+    int tw_len;                     /* entries of the class's pass-twiddle table              */
+} ofx_batch_class_info;
+typedef struct ofx_batch_tile {
+    int cls;                        /* enum ofx_batch_class                                   */
+    int count;                      /* transforms in this tile (1 .. transforms_per_tile)     */
+    long long offset;               /* complex64 offset of the tile's first transform         */
+} ofx_batch_tile;
+typedef struct ofx_batch_args {
+    const void* in;
+    void* out;
+    const void* tw;                 /* all classes' pass-twiddle tables                       */
+    int tw_off[OFX_BCLASS_COUNT];   /* complex64 offset of each class's table in `tw`         */
+    const ofx_batch_tile* tiles;    /* device array, processed in order of the work counter   */
+    int ntiles;
+    float sign;                     /* +1 forward, -1 backward (conjugation on load/store)    */
+    unsigned int* counter;          /* device work counter (reset by the launch)              */
+} ofx_batch_args;
+OFX_API int ofx_batch_class_count(void);
+OFX_API int ofx_batch_class_info_get(int cls, ofx_batch_class_info* info);
+/* Resets the counter (one-thread kernel) and launches `blocks` persistent workgroups. */
+OFX_API int ofx_launch_batch_persistent(const ofx_batch_args* args, int blocks, hipStream_t stream);
+
 /* R2C: multiply bins 1..n/2-1 by 2 (adjoint of C2R).  C2R: multiply input bins 1..n/2-1 by 1/2
  * (adjoint of R2C).  Used by the hand-written adjoint kernels. */
 #define OFX_FLAG_HERMITIAN_WEIGHT 1

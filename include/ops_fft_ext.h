@@ -15,6 +15,48 @@ extern "C" {
  * into buf (NUL-terminated, truncated to len).  For tests, tuning and logs. */
 ACLFFT_API aclfftResult aclfftExtDescribe(aclfftHandle plan, char* buf, size_t len);
 
+/* Arithmetic of the plan's transforms:
+ *   ACLFFT_EXT_PRECISION_FP32     (0) FP32 Stockham kernels (default; the reference's precision)
+ *   ACLFFT_EXT_PRECISION_BF16     (1) MFMA dense-DFT stages, one BF16 part per operand
+ *   ACLFFT_EXT_PRECISION_FP16X2   (2) MFMA, FP16 hi + lo split operands (about FP32 accuracy)
+ *   ACLFFT_EXT_PRECISION_BF16X3   (3) MFMA, BF16 hi + mid + lo split operands
+ * The MFMA modes exist for 1D C2C plans of length 1024; other plans return
+ * ACLFFT_NOT_SUPPORTED and keep their current precision.  Rebuilds the plan's launch list. */
+#define ACLFFT_EXT_PRECISION_FP32 0
+#define ACLFFT_EXT_PRECISION_BF16 1
+#define ACLFFT_EXT_PRECISION_FP16X2 2
+#define ACLFFT_EXT_PRECISION_BF16X3 3
+ACLFFT_API aclfftResult aclfftExtSetPrecision(aclfftHandle plan, int precision);
+
+/* One plan for many C2C transforms of uneven lengths: group g holds batches[g] transforms of
+ * length lengths[g], stored contiguously, groups one after the other (group g starts at the
+ * complex64 offset sum_{i<g} lengths[i]*batches[i]).  Execute with aclfftExecC2C on the whole
+ * buffer.  Lengths with a compiled class (60, 128, 243, 256, 512, 1000, 1024, 2048, 4096, 6561)
+ * run in one persistent-kernel launch that drains a work queue of tiles; other lengths run as
+ * extra launches.  Executions on device pointers replay a hipGraph captured per (in, out,
+ * direction).  Errors: plan/lengths/batches null -> INVALID_VALUE; count, a length or a batch out of
+ * range (length in [1, 2^27], batch >= 1) -> INVALID_SIZE; type other than C2C -> NOT_SUPPORTED.
+ * No counterpart in the reference (its plans hold one length). */
+ACLFFT_API aclfftResult aclfftExtPlanManyUneven(aclfftHandle* plan, int count, const int* lengths, const int* batches,
+                                               aclfftType type);
+
+/* Capture the plan's launch sequence in a hipGraph per (in, out, direction) and replay it on
+ * later executions with device pointers (on = 1), or launch kernel by kernel (on = 0). */
+ACLFFT_API aclfftResult aclfftExtSetGraphMode(aclfftHandle plan, int on);
+
+/* Multiply every output of the plan by `scale` (normalisation: 1/n, 1/sqrt(n), ...), fused into
+ * the last launch.  Default 1 (the reference's unnormalised convention).  Rebuilds the plan. */
+ACLFFT_API aclfftResult aclfftExtSetScale(aclfftHandle plan, float scale);
+
+/* Hand-written adjoints of the real transforms (for reverse-mode differentiation).  For an R2C
+ * plan of length n: grad_x[m] = Re sum_{k=0}^{n/2} g[k] e^{+2 pi i km/n}, computed by the C2R
+ * kernels with the bins strictly between DC and Nyquist weighted by 1/2.  For a C2R plan:
+ * grad_X = R2C(g) with the same bins weighted by 2.  The plan's scale applies.  Same pointer rules
+ * and errors as aclfftExecC2R / aclfftExecR2C; a plan of the other type -> INVALID_TYPE.  The
+ * adjoint of a C2C transform is the C2C transform in the opposite direction. */
+ACLFFT_API aclfftResult aclfftExtExecR2CAdjoint(aclfftHandle plan, aclfftComplex* grad_out, aclfftReal* grad_in);
+ACLFFT_API aclfftResult aclfftExtExecC2RAdjoint(aclfftHandle plan, aclfftReal* grad_out, aclfftComplex* grad_in);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,8 @@ library is required; there is no CPU fallback.
 from ._lib import (LIB_PATH, AclfftError, check, error_string, kernel_registry, lib, ACLFFT_SYMBOLS,
                    C2C, R2C, C2R, Z2Z, D2Z, Z2D, FORWARD, BACKWARD, HORIZONTAL, VERTICAL,
                    SUCCESS, INVALID_PLAN, ALLOC_FAILED, INVALID_TYPE, INVALID_VALUE, INTERNAL_ERROR,
-                   EXEC_FAILED, SETUP_FAILED, INVALID_SIZE, NOT_IMPLEMENTED, NOT_SUPPORTED, INVALID_DEVICE)
+                   EXEC_FAILED, SETUP_FAILED, INVALID_SIZE, NOT_IMPLEMENTED, NOT_SUPPORTED, INVALID_DEVICE,
+                   PRECISION_FP32, PRECISION_BF16, PRECISION_FP16X2, PRECISION_BF16X3)
 from .plan import Plan
 
 __all__ = ["Plan", "AclfftError", "check", "error_string", "kernel_registry", "lib", "LIB_PATH",

@@ -35,6 +35,12 @@ D2Z = 0x6A
 Z2D = 0x6C
 Z2Z = 0x69
 
+# aclfftExtSetPrecision modes (include/ops_fft_ext.h)
+PRECISION_FP32 = 0
+PRECISION_BF16 = 1
+PRECISION_FP16X2 = 2
+PRECISION_BF16X3 = 3
+
 FORWARD = -1
 BACKWARD = 1
 HORIZONTAL = 0
@@ -124,6 +130,20 @@ for _name, _args in _SIGS.items():
     _f = getattr(lib, _name)
     _f.argtypes = _args
     _f.restype = _i
+lib.aclfftExtDescribe.argtypes = [_h, ctypes.c_char_p, ctypes.c_size_t]
+lib.aclfftExtDescribe.restype = _i
+lib.aclfftExtPlanManyUneven.argtypes = [_hp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), _i]
+lib.aclfftExtPlanManyUneven.restype = _i
+lib.aclfftExtSetGraphMode.argtypes = [_h, _i]
+lib.aclfftExtSetGraphMode.restype = _i
+lib.aclfftExtSetScale.argtypes = [_h, ctypes.c_float]
+lib.aclfftExtSetScale.restype = _i
+lib.aclfftExtExecR2CAdjoint.argtypes = [_h, _vp, _vp]
+lib.aclfftExtExecR2CAdjoint.restype = _i
+lib.aclfftExtExecC2RAdjoint.argtypes = [_h, _vp, _vp]
+lib.aclfftExtExecC2RAdjoint.restype = _i
+lib.aclfftExtSetPrecision.argtypes = [_h, _i]
+lib.aclfftExtSetPrecision.restype = _i
 lib.aclfftGetErrorString.argtypes = [_i]
 lib.aclfftGetErrorString.restype = ctypes.c_char_p
 lib.ofx_kernel_count.argtypes = []

@@ -59,6 +59,21 @@ class Plan:
         return cls(h, type, (nx, ny), batch)
 
     @classmethod
+    def plan_many_uneven(cls, lengths, batches, type=C2C):
+        """Groups of batches[g] transforms of length lengths[g], stored one group after the other
+        (aclfftExtPlanManyUneven)."""
+        n = len(lengths)
+        if len(batches) != n:
+            raise ValueError("lengths and batches differ in size")
+        L = (ctypes.c_int * n)(*lengths)
+        B = (ctypes.c_int * n)(*batches)
+        h = ctypes.c_void_p()
+        check(lib.aclfftExtPlanManyUneven(ctypes.byref(h), n, L, B, type), "aclfftExtPlanManyUneven")
+        p = cls(h, type, tuple(lengths), sum(batches))
+        p.groups = list(zip(lengths, batches))
+        return p
+
+    @classmethod
     def make_plan1d(cls, nx, type=C2C, batch=1, dim_type=HORIZONTAL):
         h = ctypes.c_void_p()
         check(lib.aclfftCreate(ctypes.byref(h)), "aclfftCreate")
@@ -98,6 +113,32 @@ class Plan:
     @property
     def handle(self):
         return self._h
+
+    # ---- library extensions (include/ops_fft_ext.h) --------------------------------------------
+    def describe(self):
+        buf = ctypes.create_string_buffer(1024)
+        check(lib.aclfftExtDescribe(self._h, buf, len(buf)), "aclfftExtDescribe")
+        return buf.value.decode()
+
+    def set_scale(self, scale):
+        check(lib.aclfftExtSetScale(self._h, float(scale)), "aclfftExtSetScale")
+
+    def exec_r2c_adjoint(self, grad_out, grad_in):
+        self._bind_stream(grad_out)
+        check(lib.aclfftExtExecR2CAdjoint(self._h, _ptr(grad_out), _ptr(grad_in)), "aclfftExtExecR2CAdjoint")
+        return grad_in
+
+    def exec_c2r_adjoint(self, grad_out, grad_in):
+        self._bind_stream(grad_out)
+        check(lib.aclfftExtExecC2RAdjoint(self._h, _ptr(grad_out), _ptr(grad_in)), "aclfftExtExecC2RAdjoint")
+        return grad_in
+
+    def set_graph_mode(self, on):
+        check(lib.aclfftExtSetGraphMode(self._h, 1 if on else 0), "aclfftExtSetGraphMode")
+
+    def set_precision(self, mode):
+        """0 FP32 Stockham; 1 MFMA BF16; 2 MFMA FP16 hi/lo; 3 MFMA BF16 hi/mid/lo."""
+        check(lib.aclfftExtSetPrecision(self._h, mode), "aclfftExtSetPrecision")
 
     # ---- execution ----------------------------------------------------------------------------
     def exec_c2c(self, idata, odata, direction=FORWARD):

@@ -1,0 +1,156 @@
+// kernels_batched.hip - one persistent launch for a plan of many C2C transforms of uneven lengths.
+//
+// The plan (aclfftExtPlanManyUneven) holds groups g of b_g transforms of length n_g, stored one
+// group after the other.  At plan time the groups are cut into tiles (a tile = the transforms one
+// workgroup of the length's class processes together) and the tiles are sorted by decreasing cost
+// (longest-processing-time order).  The kernel is launched with about two workgroups per CU; every
+// workgroup repeatedly takes the next tile index from a device counter (atomicAdd), runs the
+// Stockham body of that tile's length class, and exits when the queue is drained.  Each length
+// class is a compiled StockhamCfg with 256 threads per workgroup, so one launch serves them all;
+// the LDS buffer is sized for the largest class.  The counter is reset by a one-thread kernel
+// launched just before (no memset node, so the pair can be captured in a hipGraph and replayed).
+#include <hip/hip_runtime.h>
+
+#include "ofx_launch.h"
+#include "stockham.h"
+
+namespace ofx {
+
+constexpr int kBatchBlock = 256;
+
+// length classes (all with T * TPB = 256)
+typedef StockhamCfg<60, 4, 64, 4, false, Radices<4, 3, 5>> BC60;
+typedef StockhamCfg<128, 8, 32, 4, false, Radices<8, 16>> BC128;
+typedef StockhamCfg<243, 16, 16, 4, false, Radices<27, 9>> BC243;
+typedef StockhamCfg<1000, 64, 4, 4, false, Radices<10, 10, 10>> BC1000;
+typedef StockhamCfg<1024, 64, 4, 4, false, Radices<16, 16, 4>> BC1024;
+typedef StockhamCfg<2048, 128, 2, 4, false, Radices<16, 16, 8>> BC2048;
+typedef StockhamCfg<4096, 256, 1, 4, false, Radices<16, 16, 16>> BC4096;
+typedef StockhamCfg<6561, 256, 1, 4, false, Radices<27, 27, 9>> BC6561;
+typedef StockhamCfg<512, 32, 8, 4, false, Radices<8, 8, 8>> BC512;
+typedef StockhamCfg<256, 16, 16, 4, false, Radices<16, 16>> BC256;
+
+template <class C>
+constexpr int lds_of() {
+    return C::TPB * C::LDS_N;
+}
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int kBatchLds =
+    cmax(cmax(cmax(lds_of<BC60>(), lds_of<BC128>()), cmax(lds_of<BC243>(), lds_of<BC1000>())),
+         cmax(cmax(cmax(lds_of<BC1024>(), lds_of<BC2048>()), cmax(lds_of<BC4096>(), lds_of<BC6561>())),
+              cmax(lds_of<BC512>(), lds_of<BC256>())));
+
+static_assert(BC60::BLOCK == kBatchBlock && BC128::BLOCK == kBatchBlock && BC243::BLOCK == kBatchBlock &&
+                  BC1000::BLOCK == kBatchBlock && BC1024::BLOCK == kBatchBlock && BC2048::BLOCK == kBatchBlock &&
+                  BC4096::BLOCK == kBatchBlock && BC6561::BLOCK == kBatchBlock && BC512::BLOCK == kBatchBlock &&
+                  BC256::BLOCK == kBatchBlock,
+              "every class runs with the same workgroup size");
+
+struct GLoadB {
+    const float2* __restrict__ p;
+    float sg;
+    __device__ __forceinline__ float2 operator()(int i) const {
+        float2 v = p[i];
+        return make_float2(v.x, v.y * sg);
+    }
+};
+struct GStoreB {
+    float2* __restrict__ p;
+    float sg;
+    bool ok;
+    __device__ __forceinline__ void operator()(int i, float2 v) const {
+        if (ok) p[i] = make_float2(v.x, v.y * sg);
+    }
+};
+
+// One tile of class C: transforms [0, tile.count) starting at element offset tile.offset.  Classes
+// whose transforms fit in a wave skip empty slots; the others synchronise the whole workgroup
+// inside the passes, so every thread runs them and only the stores of empty slots are dropped.
+template <class C>
+__device__ __attribute__((noinline)) void run_tile(float2* lds, const ofx_batch_tile& tile, const float2* __restrict__ in,
+                                         float2* __restrict__ out, const float2* __restrict__ tw, float sg) {
+    const int tl = threadIdx.x / C::T;
+    const int tid = threadIdx.x - tl * C::T;
+    const bool ok = tl < tile.count;
+    if constexpr (C::WAVE_SYNC) {
+        if (!ok) return;
+    }
+    const long long base = tile.offset + (long long)(ok ? tl : 0) * C::N;
+    GLoadB gin{in + base, sg};
+    GStoreB gout{out + base, sg, ok};
+    stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, tw, gin, gout);
+}
+
+__global__ __launch_bounds__(kBatchBlock) void k_batch_reset(unsigned int* counter) { *counter = 0u; }
+
+__global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args a) {
+    __shared__ float2 lds[kBatchLds];
+    __shared__ int s_tile;
+    const float2* __restrict__ in = static_cast<const float2*>(a.in);
+    float2* __restrict__ out = static_cast<float2*>(a.out);
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = (int)atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const int t = s_tile;
+        if (t >= a.ntiles) break;  // uniform across the workgroup: every wave leaves here
+        const ofx_batch_tile tile = a.tiles[t];
+        const float2* tw = static_cast<const float2*>(a.tw) + a.tw_off[tile.cls];
+        switch (tile.cls) {
+            case OFX_BCLASS_60: run_tile<BC60>(lds, tile, in, out, tw, a.sign); break;
+            case OFX_BCLASS_128: run_tile<BC128>(lds, tile, in, out, tw, a.sign); break;
+            case OFX_BCLASS_243: run_tile<BC243>(lds, tile, in, out, tw, a.sign); break;
+            case OFX_BCLASS_256: run_tile<BC256>(lds, tile, in, out, tw, a.sign); break;
+            case OFX_BCLASS_512: run_tile<BC512>(lds, tile, in, out, tw, a.sign); break;
+            case OFX_BCLASS_1000: run_tile<BC1000>(lds, tile, in, out, tw, a.sign); break;
+            case OFX_BCLASS_1024: run_tile<BC1024>(lds, tile, in, out, tw, a.sign); break;
+            case OFX_BCLASS_2048: run_tile<BC2048>(lds, tile, in, out, tw, a.sign); break;
+            case OFX_BCLASS_4096: run_tile<BC4096>(lds, tile, in, out, tw, a.sign); break;
+            case OFX_BCLASS_6561: run_tile<BC6561>(lds, tile, in, out, tw, a.sign); break;
+            default: break;
+        }
+        __syncthreads();  // LDS and s_tile are reused by the next tile
+    }
+}
+
+template <class C>
+void fill_info(ofx_batch_class_info* c) {
+    c->n = C::N;
+    c->transforms_per_tile = C::TPB;
+    c->npass = C::NP;
+    for (int p = 0; p < C::NP; ++p) c->radices[p] = C::radix(p);
+    c->tw_len = C::tw_len();
+}
+
+}  // namespace ofx
+
+using namespace ofx;
+
+extern "C" int ofx_batch_class_count(void) { return OFX_BCLASS_COUNT; }
+
+extern "C" int ofx_batch_class_info_get(int cls, ofx_batch_class_info* c) {
+    if (c == nullptr) return (int)hipErrorInvalidValue;
+    switch (cls) {
+        case OFX_BCLASS_60: fill_info<BC60>(c); break;
+        case OFX_BCLASS_128: fill_info<BC128>(c); break;
+        case OFX_BCLASS_243: fill_info<BC243>(c); break;
+        case OFX_BCLASS_256: fill_info<BC256>(c); break;
+        case OFX_BCLASS_512: fill_info<BC512>(c); break;
+        case OFX_BCLASS_1000: fill_info<BC1000>(c); break;
+        case OFX_BCLASS_1024: fill_info<BC1024>(c); break;
+        case OFX_BCLASS_2048: fill_info<BC2048>(c); break;
+        case OFX_BCLASS_4096: fill_info<BC4096>(c); break;
+        case OFX_BCLASS_6561: fill_info<BC6561>(c); break;
+        default: return (int)hipErrorInvalidValue;
+    }
+    return 0;
+}
+
+extern "C" int ofx_launch_batch_persistent(const ofx_batch_args* a, int blocks, hipStream_t stream) {
+    if (a == nullptr || a->counter == nullptr || a->tiles == nullptr || blocks <= 0) return (int)hipErrorInvalidValue;
+    if (a->ntiles <= 0) return 0;
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, stream, a->counter);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(k_batch_persistent, dim3(blocks), dim3(kBatchBlock), 0, stream, *a);
+    return (int)hipGetLastError();
+}

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstdint>
@@ -326,6 +327,100 @@ struct Builder {
         return ACLFFT_SUCCESS;
     }
 
+    // MFMA split-precision C2C (dense 32-point DFT stages); only n = 1024 has this path.
+    aclfftResult c2c_mfma(long long n, long long batch, int mode) {
+        if (n != 1024 || ofx_mfma_const_bytes(mode) <= 0) return ACLFFT_NOT_SUPPORTED;
+        Step s;
+        s.type = Step::MFMA;
+        s.n = (int)n;
+        s.mfma_mode = mode;
+        TableSpec t;
+        t.kind = TableSpec::MFMA_CONST;
+        t.npass = mode;
+        t.bytes = (size_t)ofx_mfma_const_bytes(mode);
+        s.tw_table = e->add_table(t);
+        s.aux_table = roots_table(n, n);
+        s.args.batch = batch;
+        s.args.idist = n;
+        s.args.odist = n;
+        s.args.in_sign = s.args.out_sign = s.args.scale = 1.f;
+        s.conj_in = s.conj_out = true;
+        e->steps_.push_back(s);
+        return ACLFFT_SUCCESS;
+    }
+
+    // Groups of (length, batch) stored one after the other.  Lengths with a compiled batch class
+    // become tiles of the persistent kernel (one launch); any other length runs as its own step.
+    aclfftResult uneven(const std::vector<int>& len, const std::vector<int>& bat) {
+        struct ClassInfo {
+            ofx_batch_class_info info;
+            bool used = false;
+        };
+        std::vector<ClassInfo> cls(ofx_batch_class_count());
+        for (int c = 0; c < (int)cls.size(); ++c) ofx_batch_class_info_get(c, &cls[c].info);
+        std::vector<ofx_batch_tile> tiles;
+        std::vector<double> cost;
+        long long off = 0;
+        size_t total = 0;
+        for (size_t g = 0; g < len.size(); ++g) {
+            const long long n = len[g], b = bat[g];
+            int c = -1;
+            for (int k = 0; k < (int)cls.size(); ++k)
+                if (cls[k].info.n == n) c = k;
+            if (c >= 0) {
+                cls[c].used = true;
+                const int tpt = cls[c].info.transforms_per_tile;
+                for (long long i = 0; i < b; i += tpt) {
+                    ofx_batch_tile t;
+                    t.cls = c;
+                    t.count = (int)std::min<long long>(tpt, b - i);
+                    t.offset = off + i * n;
+                    tiles.push_back(t);
+                    cost.push_back((double)t.count * n * std::log2((double)n + 1.0));
+                }
+            } else {
+                const size_t first = e->steps_.size();
+                aclfftResult r = c2c_rows(n, b, BUF_IN, BUF_OUT, true, true);
+                if (r != ACLFFT_SUCCESS) return r;
+                for (size_t k = first; k < e->steps_.size(); ++k) {
+                    if (e->steps_[k].src == BUF_IN) e->steps_[k].src_off = (size_t)off * 8;
+                    if (e->steps_[k].dst == BUF_OUT) e->steps_[k].dst_off = (size_t)off * 8;
+                    if (e->steps_[k].src == BUF_OUT) e->steps_[k].src_off = (size_t)off * 8;
+                }
+            }
+            off += n * b;
+            total += (size_t)(n * b);
+        }
+        e->in_bytes_ = e->out_bytes_ = total * 8;
+        if (tiles.empty()) return ACLFFT_SUCCESS;
+        // longest-processing-time order
+        std::vector<size_t> order(tiles.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+        Step s;
+        s.type = Step::PERSISTENT;
+        s.conj_in = s.conj_out = true;
+        for (int c = 0; c < (int)cls.size(); ++c) {
+            s.class_tables[c] = -1;
+            if (cls[c].used) s.class_tables[c] = pass_table_for(cls[c].info.npass, cls[c].info.radices, cls[c].info.tw_len);
+        }
+        TableSpec tt;
+        tt.kind = TableSpec::BLOB;
+        tt.bytes = tiles.size() * sizeof(ofx_batch_tile);
+        tt.blob.resize(tt.bytes);
+        for (size_t i = 0; i < order.size(); ++i)
+            std::memcpy(tt.blob.data() + i * sizeof(ofx_batch_tile), &tiles[order[i]], sizeof(ofx_batch_tile));
+        s.tiles_table = e->add_table(tt);
+        TableSpec ct;
+        ct.kind = TableSpec::BLOB;
+        ct.bytes = 4;
+        ct.blob.assign(4, 0);
+        s.counter_table = e->add_table(ct);
+        s.b.ntiles = (int)tiles.size();
+        e->steps_.insert(e->steps_.begin(), s);
+        return ACLFFT_SUCCESS;
+    }
+
     aclfftResult r2c(long long n, long long batch) {
         const long long h = n / 2 + 1;
         e->in_place_unsafe_ = true;
@@ -453,12 +548,19 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
     e->desc_ = d;
     Builder b{e};
     aclfftResult r = ACLFFT_NOT_IMPLEMENTED;
-    if (d.rank == 1 && !d.vertical) {
+    if (d.precision != 0 && !(d.rank == 1 && !d.vertical && d.type == ACLFFT_C2C)) {
+        delete e;
+        return ACLFFT_NOT_SUPPORTED;
+    }
+    if (d.uneven) {
+        r = (d.type == ACLFFT_C2C) ? b.uneven(d.ulen, d.ubatch) : ACLFFT_NOT_SUPPORTED;
+        if (r == ACLFFT_SUCCESS) e->use_graphs_ = true;
+    } else if (d.rank == 1 && !d.vertical) {
         const long long n = d.n0;
         switch (d.type) {
             case ACLFFT_C2C:
                 e->in_bytes_ = e->out_bytes_ = (size_t)n * d.batch * 8;
-                r = b.c2c_rows(n, d.batch, BUF_IN, BUF_OUT, true, true);
+                r = d.precision ? b.c2c_mfma(n, d.batch, d.precision) : b.c2c_rows(n, d.batch, BUF_IN, BUF_OUT, true, true);
                 break;
             case ACLFFT_R2C:
                 e->in_bytes_ = (size_t)n * d.batch * 4;
@@ -496,8 +598,30 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
         delete e;
         return r;
     }
+    if (d.hermitian_weight)
+        for (Step& st : e->steps_)
+            if (st.type != Step::COPY) st.args.flags |= OFX_FLAG_HERMITIAN_WEIGHT;
+    if (d.scale != 1.f) {
+        // fold the normalisation into the last launch that writes the output
+        for (auto it = e->steps_.rbegin(); it != e->steps_.rend(); ++it)
+            if (it->dst == BUF_OUT && it->type != Step::COPY && it->type != Step::PERSISTENT) {
+                it->args.scale *= d.scale;
+                break;
+            }
+    }
     *out = e;
     return ACLFFT_SUCCESS;
+}
+
+Executor* Executor::adjoint() {
+    if (adjoint_ == nullptr && (desc_.type == ACLFFT_R2C || desc_.type == ACLFFT_C2R)) {
+        PlanDesc d = desc_;
+        d.type = (desc_.type == ACLFFT_R2C) ? ACLFFT_C2R : ACLFFT_R2C;
+        d.hermitian_weight = true;
+        Executor* e = nullptr;
+        if (build(d, &e) == ACLFFT_SUCCESS) adjoint_ = e;
+    }
+    return adjoint_;
 }
 
 Executor::~Executor() {
@@ -509,6 +633,9 @@ Executor::~Executor() {
     if (d_stage_in_) (void)hipFree(d_stage_in_);
     if (d_stage_out_) (void)hipFree(d_stage_out_);
     if (d_inplace_) (void)hipFree(d_inplace_);
+    for (GraphEntry& g : graphs_) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(g.exec));
+    delete adjoint_;
+    if (capture_stream_) (void)hipStreamDestroy(capture_stream_);
     if (cur >= 0 && device_ >= 0 && cur != device_) (void)hipSetDevice(cur);
 }
 
@@ -524,6 +651,22 @@ std::string Executor::describe() const {
         }
         if (s.type == Step::R2C_UNPACK || s.type == Step::C2R_PACK) {
             os << (s.type == Step::R2C_UNPACK ? "r2c_unpack" : "c2r_pack") << " M=" << s.m;
+            continue;
+        }
+        if (s.type == Step::PERSISTENT) {
+            os << "persistent tiles=" << s.b.ntiles << " classes=";
+            bool first = true;
+            for (int c = 0; c < OFX_BCLASS_COUNT; ++c)
+                if (s.class_tables[c] >= 0) {
+                    ofx_batch_class_info ci;
+                    ofx_batch_class_info_get(c, &ci);
+                    os << (first ? "" : ",") << ci.n;
+                    first = false;
+                }
+            continue;
+        }
+        if (s.type == Step::MFMA) {
+            os << "mfma n=" << s.n << " mode=" << (s.mfma_mode == OFX_MFMA_BF16 ? "bf16" : s.mfma_mode == OFX_MFMA_FP16X2 ? "fp16x2" : "bf16x3");
             continue;
         }
         if (s.type == Step::GENERIC) {
@@ -567,6 +710,10 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
             rc = ofx_gen_pass_twiddles(p, t.npass, t.radices, s);
         else if (t.kind == TableSpec::ROOTS)
             rc = ofx_gen_roots(p, t.len, t.count, s);
+        else if (t.kind == TableSpec::MFMA_CONST)
+            rc = ofx_gen_mfma_consts(t.npass, p, s);
+        else if (t.kind == TableSpec::BLOB)
+            rc = (int)hipMemcpyAsync(p, t.blob.data(), t.bytes, hipMemcpyHostToDevice, s);
         else
             rc = ofx_gen_ft_table(p, t.len, s);
         if (rc != 0) {
@@ -575,6 +722,13 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
         }
     }
     if (hip_log(hipStreamSynchronize(s), "hipStreamSynchronize (plan setup)") != hipSuccess) return ACLFFT_SETUP_FAILED;
+    {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            num_cus_ = cus;
+        else
+            (void)hipGetLastError();
+    }
     device_ = dev;
     if (!tuned_ && env_int("ACLFFT_TUNE", 0) != 0) {
         aclfftResult r = autotune(s);
@@ -607,6 +761,31 @@ aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream
             case Step::C2R_PACK:
                 err = (hipError_t)ofx_c2r_pack(src, dst, table(st.aux_table), st.m, st.args.batch, st.args.flags, s);
                 break;
+            case Step::PERSISTENT: {
+                ofx_batch_args b = st.b;
+                b.in = src;
+                b.out = dst;
+                b.tw = d_tables_;
+                for (int c = 0; c < OFX_BCLASS_COUNT; ++c)
+                    b.tw_off[c] = st.class_tables[c] >= 0 ? (int)(tables_[st.class_tables[c]].offset / 8) : 0;
+                b.tiles = static_cast<const ofx_batch_tile*>(table(st.tiles_table));
+                b.counter = static_cast<unsigned int*>(const_cast<void*>(table(st.counter_table)));
+                b.sign = dsign;
+                const int per_cu = std::max(1, env_int("ACLFFT_PERSISTENT_BLOCKS_PER_CU", 2));
+                err = (hipError_t)ofx_launch_batch_persistent(&b, num_cus_ * per_cu, s);
+                break;
+            }
+            case Step::MFMA: {
+                ofx_args a = st.args;
+                a.in = src;
+                a.out = dst;
+                a.tw = table(st.tw_table);
+                a.tw_aux = table(st.aux_table);
+                if (st.conj_in) a.in_sign *= dsign;
+                if (st.conj_out) a.out_sign *= dsign;
+                err = (hipError_t)ofx_launch_mfma_c2c(st.mfma_mode, st.n, &a, env_int("ACLFFT_MFMA_BLOCKS", 0), s);
+                break;
+            }
             case Step::GENERIC: {
                 ofx_args a = st.args;
                 a.in = src;
@@ -664,6 +843,7 @@ aclfftResult Executor::exec(const void* in, void* out, int direction, hipStream_
                 return ACLFFT_EXEC_FAILED;
             return run(d_inplace_, out, direction, s);
         }
+        if (use_graphs_) return run_graph(in, out, direction, s);
         return run(in, out, direction, s);
     }
     // Host pointers: the reference's synchronous staging semantics
@@ -693,6 +873,45 @@ aclfftResult Executor::exec(const void* in, void* out, int direction, hipStream_
     if (!dout && hipMemcpyAsync(out, dst, out_bytes_, hipMemcpyDeviceToHost, s) != hipSuccess)
         return ACLFFT_EXEC_FAILED;
     if (hip_log(hipStreamSynchronize(s), "hipStreamSynchronize") != hipSuccess) return ACLFFT_EXEC_FAILED;
+    return ACLFFT_SUCCESS;
+}
+
+// -------------------------------------------------------------------------------------------
+// hipGraph per plan: the launch sequence for (in, out, direction) is captured once on a private
+// stream and replayed with hipGraphLaunch on the caller's stream.  The sequence contains kernel and
+// copy nodes only (the persistent kernel's work counter is reset by a kernel, not a memset).
+// -------------------------------------------------------------------------------------------
+aclfftResult Executor::run_graph(const void* din, void* dout, int direction, hipStream_t s) {
+    for (const GraphEntry& g : graphs_) {
+        if (g.in == din && g.out == dout && g.direction == direction) {
+            if (hip_log(hipGraphLaunch(static_cast<hipGraphExec_t>(g.exec), s), "hipGraphLaunch") != hipSuccess)
+                return ACLFFT_EXEC_FAILED;
+            return ACLFFT_SUCCESS;
+        }
+    }
+    if (capture_stream_ == nullptr &&
+        hip_log(hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking), "hipStreamCreate") != hipSuccess)
+        return ACLFFT_EXEC_FAILED;
+    if (hip_log(hipStreamBeginCapture(capture_stream_, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture") !=
+        hipSuccess)
+        return ACLFFT_EXEC_FAILED;
+    aclfftResult r = run(din, dout, direction, capture_stream_);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(capture_stream_, &graph);
+    if (r != ACLFFT_SUCCESS || hip_log(e, "hipStreamEndCapture") != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return r != ACLFFT_SUCCESS ? r : ACLFFT_EXEC_FAILED;
+    }
+    hipGraphExec_t ge = nullptr;
+    e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (hip_log(e, "hipGraphInstantiate") != hipSuccess) return ACLFFT_EXEC_FAILED;
+    if (graphs_.size() >= 8) {  // bounded cache: drop the oldest
+        (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(graphs_.front().exec));
+        graphs_.erase(graphs_.begin());
+    }
+    graphs_.push_back(GraphEntry{din, dout, direction, ge});
+    if (hip_log(hipGraphLaunch(ge, s), "hipGraphLaunch") != hipSuccess) return ACLFFT_EXEC_FAILED;
     return ACLFFT_SUCCESS;
 }
 

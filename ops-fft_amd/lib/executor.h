@@ -21,17 +21,18 @@ enum BufId : int { BUF_IN = 0, BUF_OUT = 1, BUF_WORK0 = 2, BUF_WORK1 = 3 };
 
 // Device lookup tables a plan needs; all live in one allocation.
 struct TableSpec {
-    enum Kind { PASS, ROOTS, FOURSTEP } kind = PASS;
-    int npass = 0;
+    enum Kind { PASS, ROOTS, FOURSTEP, MFMA_CONST, BLOB } kind = PASS;
+    int npass = 0;                 // PASS: radix sequence; MFMA_CONST: npass = mode
     int radices[OFX_MAX_PASSES] = {0};
     long long len = 0, count = 0;  // ROOTS: W_len^k, k < count; FOURSTEP: FP64 table of W_len
     size_t offset = 0;             // bytes from the start of the table allocation
     size_t bytes = 0;
+    std::vector<char> blob;        // BLOB: host bytes copied to the device at setup
 };
 
 // One launch.
 struct Step {
-    enum Type { KERNEL, GENERIC, COPY, R2C_UNPACK, C2R_PACK } type = KERNEL;
+    enum Type { KERNEL, GENERIC, COPY, R2C_UNPACK, C2R_PACK, MFMA, PERSISTENT } type = KERNEL;
     int kernel = -1;               // registry id (KERNEL)
     int kind = OFX_KIND_C2C;
     int n = 0;                     // complex length the kernel computes (for autotuning)
@@ -44,6 +45,10 @@ struct Step {
     size_t copy_bytes = 0;         // COPY
     ofx_generic_desc g{};          // GENERIC
     long long m = 0;               // R2C_UNPACK / C2R_PACK: complex length M = n/2
+    int mfma_mode = 0;             // MFMA: enum ofx_mfma_mode
+    ofx_batch_args b{};            // PERSISTENT: launch arguments (pointers filled at execution)
+    int class_tables[OFX_BCLASS_COUNT] = {0};  // PERSISTENT: pass table of each class (-1 none)
+    int tiles_table = -1, counter_table = -1;
     std::vector<int> candidates;   // registry ids that can replace `kernel` (autotuning)
     std::vector<int> candidate_tables;  // pass-twiddle table of each candidate
 };
@@ -54,6 +59,11 @@ struct PlanDesc {
     long long batch = 1;
     aclfftType type = ACLFFT_C2C;
     bool vertical = false;         // 1D VERTICAL layout: n0 rows x `batch` columns
+    int precision = 0;             // 0: FP32 Stockham; else enum ofx_mfma_mode (MFMA split path)
+    bool hermitian_weight = false; // adjoint real transforms: OFX_FLAG_HERMITIAN_WEIGHT on real steps
+    float scale = 1.f;             // multiplies the output (normalisation), fused in the last step
+    bool uneven = false;           // aclfftExtPlanManyUneven: groups of (length, batch)
+    std::vector<int> ulen, ubatch;
 };
 
 class Executor {
@@ -70,6 +80,7 @@ public:
     size_t in_bytes() const { return in_bytes_; }
     size_t out_bytes() const { return out_bytes_; }
     const std::vector<Step>& steps() const { return steps_; }
+    const PlanDesc& desc() const { return desc_; }
     std::string describe() const;
 
 private:
@@ -96,6 +107,27 @@ private:
     void* d_stage_out_ = nullptr;
     void* d_inplace_ = nullptr;
     bool tuned_ = false;
+    int num_cus_ = 256;
+
+    // captured launch sequences, keyed by (in, out, direction)
+    struct GraphEntry {
+        const void* in;
+        void* out;
+        int direction;
+        void* exec;  // hipGraphExec_t
+    };
+    bool use_graphs_ = false;
+    std::vector<GraphEntry> graphs_;
+    hipStream_t capture_stream_ = nullptr;
+    Executor* adjoint_ = nullptr;
+    aclfftResult run_graph(const void* din, void* dout, int direction, hipStream_t s);
+
+public:
+    void set_graphs(bool on) { use_graphs_ = on; }
+    // the adjoint of a real plan: the opposite real type with the Hermitian weights, built on first use
+    Executor* adjoint();
+
+    bool graphs() const { return use_graphs_; }
 
     friend struct Builder;
 };

@@ -27,8 +27,8 @@ struct aclfftHandle_t {
     // plan-owned device state, created by MakePlan (host-side choice) and filled lazily on the
     // first execution (device tables and buffers)
     ofx::Executor* exec = nullptr;
-    // uneven-length batched plans (aclfftPlanManyUneven): per-group lengths and batches
-    int32_t ngroups = 0;
+    // arithmetic selected with aclfftExtSetPrecision (0 = FP32 Stockham)
+    int32_t precision = 0;
 
     aclfftHandle_t() = default;
     ~aclfftHandle_t();

@@ -225,3 +225,19 @@ def test_unsupported_factor_is_not_implemented_at_exec():
     assert r == aclfft.SUCCESS
     assert lib.aclfftExecC2C(h, p, p, -1) == aclfft.NOT_IMPLEMENTED
     lib.aclfftDestroy(h)
+
+
+def test_uneven_plan_argument_gates():
+    import ctypes
+    h = ctypes.c_void_p()
+    L = (ctypes.c_int * 2)(64, 128)
+    B = (ctypes.c_int * 2)(1, 2)
+    lib = aclfft.lib
+    assert lib.aclfftExtPlanManyUneven(None, 2, L, B, aclfft.C2C) == aclfft.INVALID_VALUE
+    assert lib.aclfftExtPlanManyUneven(ctypes.byref(h), 0, L, B, aclfft.C2C) == aclfft.INVALID_SIZE
+    assert lib.aclfftExtPlanManyUneven(ctypes.byref(h), 2, L, None, aclfft.C2C) == aclfft.INVALID_VALUE
+    assert lib.aclfftExtPlanManyUneven(ctypes.byref(h), 2, L, B, aclfft.R2C) == aclfft.NOT_SUPPORTED
+    Bz = (ctypes.c_int * 2)(1, 0)
+    assert lib.aclfftExtPlanManyUneven(ctypes.byref(h), 2, L, Bz, aclfft.C2C) == aclfft.INVALID_SIZE
+    assert lib.aclfftExtPlanManyUneven(ctypes.byref(h), 2, L, B, aclfft.C2C) == aclfft.SUCCESS
+    assert lib.aclfftDestroy(h) == aclfft.SUCCESS
