@@ -215,6 +215,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-extras", action="store_true", help="only the headline C2C configuration")
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16x2", "bf16x3"],
+                    help="arithmetic of the timed loop (profiling runs; the headline value is fp32)")
+    ap.add_argument("--with-copy", action="store_true",
+                    help="also run one device copy of the 1 GiB input (counter calibration for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the FFT path has no multi-GPU mode in this build "
@@ -233,7 +237,15 @@ def main():
     y = torch.empty_like(x)
     stream = torch.cuda.current_stream(dev)
     plan = aclfft.Plan.plan1d(N, aclfft.C2C, BATCH)
+    if args.precision != "fp32":
+        plan.set_precision({"bf16": aclfft.PRECISION_BF16, "fp16x2": aclfft.PRECISION_FP16X2,
+                            "bf16x3": aclfft.PRECISION_BF16X3}[args.precision])
     plan.set_stream(stream)
+    if args.with_copy:
+        ycopy = torch.empty_like(x)
+        ycopy.copy_(x)
+        torch.cuda.synchronize()
+        del ycopy
     log("bench: plan ready, warmup %d" % args.warmup)
     for _ in range(args.warmup):
         plan.exec_c2c(x, y, aclfft.FORWARD)
@@ -276,7 +288,9 @@ def main():
         "dtype": "f32",
         "data": "synthetic (torch.randn complex64, seed 0), resident in HBM",
         "config": {"workload": "1D C2C forward FFT, complex64, N=1024, batch=65536, contiguous",
-                   "n": N, "batch": BATCH, "path": "FP32 Stockham (aclfftExecC2C, device pointers)"},
+                   "n": N, "batch": BATCH, "path": ("FP32 Stockham" if args.precision == "fp32" else
+                                                     "MFMA " + args.precision) + " (aclfftExecC2C, device pointers)",
+                   "kernel": plan.describe()},
         "rel_l2_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},

@@ -56,9 +56,11 @@ typedef struct ofx_kernel_info {
     int block_threads;              /* T * TPB                                              */
     int npass;                      /* number of Stockham passes                            */
     int radices[OFX_MAX_PASSES];    /* radix of each pass, first to last                    */
-    int staged;                     /* 1: HBM<->LDS tile copies with 16-byte accesses       */
+    int staged;                     /* 1: 16-byte HBM accesses (staged tiles or paired I/O):
+                                       needs 16-byte aligned rows                            */
     int lds_bytes;                  /* static LDS per workgroup                             */
     int tw_len;                     /* complex64 entries of the pass-twiddle table          */
+    int iov;                        /* 2: paired butterflies in the first/last pass         */
 } ofx_kernel_info;
 
 /* Arguments of every 1D launch.  Element distances are in complex64 units for complex data and

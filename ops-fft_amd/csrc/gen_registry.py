@@ -53,6 +53,18 @@ def tpb_for(n, t):
     return tpb
 
 
+# paired 16-byte I/O variants (IOV = 2): (kind, N, T, TPB, radices)
+PAIRED = [
+    ("C2C", 1024, 64, 4, (8, 16, 8)), ("C2C", 1024, 64, 8, (8, 16, 8)), ("C2C", 1024, 64, 2, (8, 16, 8)),
+    ("C2C", 1024, 128, 2, (8, 8, 16)),
+    ("C2C", 2048, 128, 2, (8, 16, 16)), ("C2C", 2048, 128, 2, (8, 4, 8, 8)),
+    ("C2C", 4096, 256, 1, (8, 8, 8, 8)),
+    ("C2C", 512, 32, 8, (8, 8, 8)),
+    ("R2C", 2048, 128, 2, (8, 16, 16)), ("C2R", 2048, 128, 2, (16, 16, 8)),
+    ("R2C", 4096, 256, 1, (8, 8, 8, 8)), ("C2R", 4096, 256, 1, (8, 8, 8, 8)),
+]
+
+
 def lines():
     out = []
     for n, plans in sorted(PLANS.items()):
@@ -87,8 +99,10 @@ def main():
             f.write("    OFX_E(%s, %d, %d, %d, %d, %s, %s),\n" % (
                 KINDS[kind], n, t, tpb, padsh, "true" if staged else "false",
                 ", ".join(str(r) for r in rad)))
+        for kind, n, t, tpb, rad in PAIRED:
+            f.write("    OFX_EV(%s, %d, %d, %d, 4, %s),\n" % (KINDS[kind], n, t, tpb, ", ".join(str(r) for r in rad)))
         f.write("};\n")
-    print("wrote %d registry entries" % len(rows))
+    print("wrote %d registry entries" % (len(rows) + len(PAIRED)))
 
 
 if __name__ == "__main__":

@@ -32,6 +32,11 @@ struct GLoadC {  // complex input, conjugated for the backward direction
         float2 v = p[i];
         return make_float2(v.x, v.y * sg);
     }
+    // elements i, i+1 (i even, 16-byte aligned rows)
+    __device__ __forceinline__ float4 load2(int i) const {
+        float4 v = *reinterpret_cast<const float4*>(p + i);
+        return make_float4(v.x, v.y * sg, v.z, v.w * sg);
+    }
 };
 struct GStoreC {
     float2* __restrict__ p;
@@ -39,6 +44,10 @@ struct GStoreC {
     bool ok;
     __device__ __forceinline__ void operator()(int i, float2 v) const {
         if (ok) p[i] = make_float2(v.x * scale, v.y * (sg * scale));
+    }
+    __device__ __forceinline__ void store2(int i, float2 a, float2 b) const {
+        const float si = sg * scale;
+        if (ok) *reinterpret_cast<float4*>(p + i) = make_float4(a.x * scale, a.y * si, b.x * scale, b.y * si);
     }
 };
 
@@ -67,6 +76,10 @@ struct GLoadC2RPre {
         float2 t = mul_posi(cmulc(d, w));
         float2 z = cadd(s, t);
         return cconj(z);
+    }
+    __device__ __forceinline__ float4 load2(int k) const {
+        const float2 a = (*this)(k), b = (*this)(k + 1);
+        return make_float4(a.x, a.y, b.x, b.y);
     }
 };
 
@@ -328,6 +341,9 @@ hipError_t launch(const ofx_args* a, hipStream_t s) {
     if (a->batch <= 0) return hipSuccess;
     const long long blocks = (a->batch + C::TPB - 1) / C::TPB;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
+    if constexpr (C::STAGED || C::IOV == 2) {
+        if (KIND == OFX_KIND_C2C && ((a->idist | a->odist) & 1)) return hipErrorInvalidValue;
+    }
     if constexpr (C::STAGED) {
         if (KIND == OFX_KIND_C2C && (a->idist != C::N || a->odist != C::N)) return hipErrorInvalidValue;
         if (KIND == OFX_KIND_R2C && a->idist != 2 * C::N) return hipErrorInvalidValue;
@@ -358,7 +374,8 @@ Entry make_entry() {
     e.info.block_threads = C::BLOCK;
     e.info.npass = C::NP;
     for (int p = 0; p < C::NP; ++p) e.info.radices[p] = C::radix(p);
-    e.info.staged = C::STAGED ? 1 : 0;
+    e.info.staged = (C::STAGED || C::IOV == 2) ? 1 : 0;
+    e.info.iov = C::IOV;
     e.info.lds_bytes = (int)(C::TPB * C::LDS_N * sizeof(float2));
     e.info.tw_len = C::tw_len();
     e.fn = &launch<C, KIND>;
@@ -367,6 +384,8 @@ Entry make_entry() {
 
 #define OFX_CFG(N, T, TPB, PADSH, STG, ...) StockhamCfg<N, T, TPB, PADSH, STG, Radices<__VA_ARGS__>>
 #define OFX_E(KIND, N, T, TPB, PADSH, STG, ...) make_entry<OFX_CFG(N, T, TPB, PADSH, STG, __VA_ARGS__), KIND>()
+#define OFX_CFGV(N, T, TPB, PADSH, ...) StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 2>
+#define OFX_EV(KIND, N, T, TPB, PADSH, ...) make_entry<OFX_CFGV(N, T, TPB, PADSH, __VA_ARGS__), KIND>()
 
 #include "registry_1d.inc"
 

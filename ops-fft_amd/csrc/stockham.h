@@ -31,13 +31,16 @@ struct Radices {
     static constexpr int v[count] = {Rs...};
 };
 
-template <int N_, int T_, int TPB_, int PADSH_, bool STAGED_, class RL>
+// IOV_ = 2: in the first and last pass a thread owns adjacent butterflies j, j+1, so its HBM loads
+// and stores move two complex64 values (16 bytes) per instruction.
+template <int N_, int T_, int TPB_, int PADSH_, bool STAGED_, class RL, int IOV_ = 1>
 struct StockhamCfg {
     static constexpr int N = N_;
     static constexpr int T = T_;
     static constexpr int TPB = TPB_;
     static constexpr int PADSH = PADSH_;
     static constexpr bool STAGED = STAGED_;
+    static constexpr int IOV = IOV_;
     static constexpr int NP = RL::count;
     static constexpr int BLOCK = T * TPB;
     static constexpr bool WAVE_SYNC = (T <= 64) && (64 % T == 0);
@@ -100,14 +103,33 @@ __device__ __forceinline__ void stockham_pass(int tid, const float2* __restrict_
     constexpr int NBF = C::N / R;
     constexpr int NB = (NBF + C::T - 1) / C::T;
     constexpr bool EXACT = (NBF % C::T) == 0;
+    // paired mapping (j = 2*tid + (b&1) + (b>>1)*2T) for 16-byte HBM access
+    constexpr bool VLOAD = C::IOV == 2 && P == 0 && !IN_LDS && EXACT && NB % 2 == 0;
+    constexpr bool VSTORE = C::IOV == 2 && P == C::NP - 1 && !OUT_LDS && EXACT && NB % 2 == 0 && Ns == NBF;
+    constexpr bool PAIRED = VLOAD || VSTORE;
+    auto jof = [tid](int b) { return PAIRED ? (2 * tid + (b & 1) + (b >> 1) * 2 * C::T) : (tid + b * C::T); };
     float2 v[NB][R];
     float2 w[NB][R > 1 ? R - 1 : 1];
+    if constexpr (VLOAD) {
+#pragma unroll
+        for (int b = 0; b < NB; b += 2) {
+            const int j = jof(b);
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const float4 q = ld.load2(j + k * NBF);
+                v[b][k] = make_float2(q.x, q.y);
+                v[b + 1][k] = make_float2(q.z, q.w);
+            }
+        }
+    }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const int j = tid + b * C::T;
+        const int j = jof(b);
         if (EXACT || j < NBF) {
+            if constexpr (!VLOAD) {
 #pragma unroll
-            for (int k = 0; k < R; ++k) v[b][k] = ld(j + k * NBF);
+                for (int k = 0; k < R; ++k) v[b][k] = ld(j + k * NBF);
+            }
             if constexpr (P > 0) {
                 const float2* t = tw + C::tw_off(P) + (j % Ns) * (R - 1);
 #pragma unroll
@@ -118,16 +140,27 @@ __device__ __forceinline__ void stockham_pass(int tid, const float2* __restrict_
     if constexpr (IN_LDS && OUT_LDS) C::sync();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        const int j = tid + b * C::T;
+        const int j = jof(b);
         if (EXACT || j < NBF) {
             if constexpr (P > 0) {
 #pragma unroll
                 for (int k = 1; k < R; ++k) v[b][k] = cmul(v[b][k], w[b][k - 1]);
             }
             Dft<R>::run(v[b]);
-            const int base = (j / Ns) * (Ns * R) + (j % Ns);
+            if constexpr (!VSTORE) {
+                const int base = (j / Ns) * (Ns * R) + (j % Ns);
 #pragma unroll
-            for (int k = 0; k < R; ++k) st(base + k * Ns, v[b][k]);
+                for (int k = 0; k < R; ++k) st(base + k * Ns, v[b][k]);
+            }
+        }
+    }
+    if constexpr (VSTORE) {
+        // last pass: Ns = N/R, so butterfly j stores to j + k*Ns and j, j+1 are adjacent
+#pragma unroll
+        for (int b = 0; b < NB; b += 2) {
+            const int j = jof(b);
+#pragma unroll
+            for (int k = 0; k < R; ++k) st.store2(j + k * Ns, v[b][k], v[b + 1][k]);
         }
     }
     if constexpr (OUT_LDS) C::sync();

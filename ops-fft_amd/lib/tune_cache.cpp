@@ -53,7 +53,7 @@ std::map<std::string, std::string> load() {
 std::string kernel_signature(const ofx_kernel_info& k) {
     std::ostringstream os;
     os << k.kind << ':' << k.n << ':' << k.threads_per_transform << ':' << k.transforms_per_block << ':' << k.staged
-       << ':';
+       << ':' << k.iov << ':';
     for (int p = 0; p < k.npass; ++p) os << (p ? "x" : "") << k.radices[p];
     return os.str();
 }

@@ -59,3 +59,56 @@ def test_c2c_in_place_device(cuda):
     torch.cuda.synchronize()
     ref = oracle.c2c(x.astype(np.complex128))
     assert oracle.rel_l2(xt.cpu().numpy(), ref) <= fp32_tol(n)
+
+
+def _variants():
+    """(kind, n, index among that (kind, n)'s candidates) for every compiled kernel."""
+    seen = {}
+    out = []
+    for k in aclfft.kernel_registry():
+        key = (k["kind"], k["n"])
+        idx = seen.get(key, 0)
+        seen[key] = idx + 1
+        out.append((k["kind"], k["n"], idx))
+    return out
+
+
+@pytest.mark.parametrize("kind,n,variant", _variants())
+def test_every_registry_kernel(cuda, monkeypatch, kind, n, variant):
+    """Each compiled configuration (radix plan, threads per transform, staged / paired I/O),
+    forced with ACLFFT_KERNEL_VARIANT, against the FP64 oracle; odd batch exercises partial
+    workgroups."""
+    import torch
+    monkeypatch.setenv("ACLFFT_KERNEL_VARIANT", str(variant))
+    batch = 21
+    if kind == 0:
+        x = crandn((batch, n), n + variant).astype(np.complex64)
+        p = aclfft.Plan.plan1d(n, aclfft.C2C, batch)
+        assert "id=" in p.describe()
+        xt = torch.from_numpy(x).cuda()
+        yt = torch.empty_like(xt)
+        p.exec_c2c(xt, yt, aclfft.BACKWARD)
+        torch.cuda.synchronize()
+        ref = oracle.c2c(x.astype(np.complex128), aclfft.BACKWARD)
+        err = oracle.rel_l2(yt.cpu().numpy(), ref)
+    elif kind == 1:
+        m = 2 * n
+        x = np.random.default_rng(variant).standard_normal((batch, m)).astype(np.float32)
+        p = aclfft.Plan.plan1d(m, aclfft.R2C, batch)
+        xt = torch.from_numpy(x).cuda()
+        yt = torch.empty((batch, m // 2 + 1), dtype=torch.complex64, device=cuda)
+        p.exec_r2c(xt, yt)
+        torch.cuda.synchronize()
+        err = oracle.rel_l2(yt.cpu().numpy(), oracle.r2c(x.astype(np.float64)))
+    else:
+        m = 2 * n
+        X = crandn((batch, m // 2 + 1), variant).astype(np.complex64)
+        p = aclfft.Plan.plan1d(m, aclfft.C2R, batch)
+        Xt = torch.from_numpy(X).cuda()
+        yt = torch.empty((batch, m), dtype=torch.float32, device=cuda)
+        p.exec_c2r(Xt, yt)
+        torch.cuda.synchronize()
+        err = oracle.rel_l2(yt.cpu().numpy(), oracle.c2r(X.astype(np.complex128), m))
+    print(p.describe(), "rel_l2 %.3e" % err)
+    p.destroy()
+    assert err <= fp32_tol(2 * n)
