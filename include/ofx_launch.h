@@ -79,6 +79,7 @@ typedef struct ofx_args {
     float out_sign;        /* -1 conjugates the output on store                            */
     float scale;           /* multiplies every output value                                */
     int flags;             /* OFX_FLAG_*                                                   */
+    long long cols;        /* COL kernels: columns per image (batch = images * cols)       */
 } ofx_args;
 
 /* Runtime-radix ("generic") launches: any length whose radix sequence is listed, any layout.
@@ -148,7 +149,9 @@ OFX_API long long ofx_mfma_const_bytes(int mode);
 OFX_API int ofx_gen_mfma_consts(int mode, void* table, hipStream_t stream);
 /* Batch of length-n C2C transforms (n = 1024), contiguous (idist/odist in complex64 units).
  * args->tw = constant table of `mode`, args->tw_aux = W_1024^k (k < 1024, ofx_gen_roots).
- * blocks: workgroups of the grid-stride launch (0 = default). */
+ * blocks: workgroups of the grid-stride launch (0 = default 1024); a negative value -b launches b
+ * workgroups of the software-pipelined variant (next transform's loads issued before the MFMA work
+ * of the current one). */
 OFX_API int ofx_launch_mfma_c2c(int mode, int n, const ofx_args* args, int blocks, hipStream_t stream);
 
 /* Persistent batched C2C over uneven lengths: compiled length classes. */
@@ -192,6 +195,13 @@ OFX_API int ofx_kernel_info_get(int id, ofx_kernel_info* info);
 
 /* Launch registry kernel `id` on `stream`.  Returns 0 or a hipError_t. */
 OFX_API int ofx_launch(int id, const ofx_args* args, hipStream_t stream);
+
+/* Column kernels (OFX_KIND_COL): `batch` = images * cols transforms of length n; column c of image
+ * m has element i at in[m*idist + c + i*istride] (out likewise with odist/ostride).
+ * transforms_per_block of the info = adjacent columns per workgroup. */
+OFX_API int ofx_col_kernel_count(void);
+OFX_API int ofx_col_kernel_info_get(int id, ofx_kernel_info* info);
+OFX_API int ofx_launch_col(int id, const ofx_args* args, hipStream_t stream);
 
 /* Fill `tw` (device, tw_len complex64) with the pass-twiddle table of a radix sequence:
  * FP64 sincospi on the GPU, rounded once to FP32. */

@@ -341,7 +341,8 @@ hipError_t launch(const ofx_args* a, hipStream_t s) {
     if (a->batch <= 0) return hipSuccess;
     const long long blocks = (a->batch + C::TPB - 1) / C::TPB;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
-    if constexpr (C::STAGED || C::IOV == 2) {
+    if constexpr (C::IOV == 2 || (C::STAGED && (C::N % 2) == 0)) {
+        // 16-byte accesses: every row must start 16-byte aligned
         if (KIND == OFX_KIND_C2C && ((a->idist | a->odist) & 1)) return hipErrorInvalidValue;
     }
     if constexpr (C::STAGED) {

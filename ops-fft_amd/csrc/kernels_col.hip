@@ -1,0 +1,122 @@
+// kernels_col.hip - column transforms of row-major data (2D second pass, the VERTICAL layout).
+//
+// Data: `images` matrices of rows x cols complex64, row-major; one transform per column, length
+// rows, element i of column c of image m at in[m*idist + c + i*istride] (istride = cols).  A
+// workgroup takes COLS adjacent columns of one image: lane l works on column l % COLS, so in every
+// load or store the COLS lanes of a group touch one contiguous COLS*8-byte run of a row.  The
+// Stockham passes are the same register codelets as the row kernels, exchanging through LDS with
+// the columns interleaved (stockham.h, COLS > 0).  Column groups are assigned XCD-aware: the 8 XCDs
+// each take a contiguous range of groups (blocks are dispatched round-robin over XCDs), so the
+// neighbouring groups that share HBM lines run on the same XCD at about the same time and meet in
+// its L2.
+#include <hip/hip_runtime.h>
+#include "ofx_launch.h"
+#include "stockham.h"
+
+namespace ofx {
+
+template <class C>
+struct ColLoad {
+    const float2* __restrict__ p;  // element 0 of this column
+    long long stride;
+    float sg;
+    __device__ __forceinline__ float2 operator()(int i) const {
+        float2 v = p[(long long)i * stride];
+        return make_float2(v.x, v.y * sg);
+    }
+};
+template <class C>
+struct ColStore {
+    float2* __restrict__ p;
+    long long stride;
+    float sg, scale;
+    bool ok;
+    __device__ __forceinline__ void operator()(int i, float2 v) const {
+        if (ok) p[(long long)i * stride] = make_float2(v.x * scale, v.y * (sg * scale));
+    }
+};
+
+template <class C>
+__global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, long long groups_per_image) {
+    __shared__ float2 lds[C::LDS_N];
+    const int c = threadIdx.x % C::COLS;
+    const int tid = threadIdx.x / C::COLS;
+    // XCD-aware group index (blocks are dispatched round-robin over the 8 XCDs)
+    const long long nblk = gridDim.x;
+    long long g = blockIdx.x;
+    if ((nblk & 7) == 0) g = (g & 7) * (nblk >> 3) + (g >> 3);
+    const long long img = g / groups_per_image;
+    const long long col = (g - img * groups_per_image) * C::COLS + c;
+    const bool ok = col < cols;
+    const long long cc = ok ? col : 0;
+    ColLoad<C> gin{static_cast<const float2*>(a.in) + img * a.idist + cc, a.istride, a.in_sign};
+    ColStore<C> gout{static_cast<float2*>(a.out) + img * a.odist + cc, a.ostride, a.out_sign, a.scale, ok};
+    stockham_passes<C, false, false>(lds + c, tid, static_cast<const float2*>(a.tw), gin, gout);
+}
+
+template <class C>
+hipError_t launch_col(const ofx_args* a, hipStream_t s) {
+    if (a->batch <= 0 || a->cols <= 0) return hipSuccess;
+    const long long images = a->batch / a->cols;
+    const long long gpi = (a->cols + C::COLS - 1) / C::COLS;
+    const long long blocks = images * gpi;
+    if (blocks > 0x7fffffffLL || images * a->cols != a->batch) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_col<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi);
+    return hipGetLastError();
+}
+
+struct ColEntry {
+    ofx_kernel_info info;
+    hipError_t (*fn)(const ofx_args*, hipStream_t);
+};
+
+template <class C>
+ColEntry col_entry() {
+    ColEntry e{};
+    e.info.kind = OFX_KIND_COL;
+    e.info.n = C::N;
+    e.info.threads_per_transform = C::T;
+    e.info.transforms_per_block = C::COLS;
+    e.info.block_threads = C::BLOCK;
+    e.info.npass = C::NP;
+    for (int p = 0; p < C::NP; ++p) e.info.radices[p] = C::radix(p);
+    e.info.lds_bytes = (int)(C::LDS_N * sizeof(float2));
+    e.info.tw_len = C::tw_len();
+    e.info.iov = 1;
+    e.fn = &launch_col<C>;
+    return e;
+}
+
+// (N, T, COLS, COLPAD, radices): T * COLS threads per workgroup
+#define OFX_COL(N, T, COLS, COLPAD, ...) \
+    col_entry<StockhamCfg<N, T, COLS, 4, false, Radices<__VA_ARGS__>, 1, COLS, COLPAD>>()
+
+static const ColEntry kCols[] = {
+    OFX_COL(64, 8, 16, 1, 8, 8),
+    OFX_COL(128, 8, 16, 1, 8, 16),
+    OFX_COL(256, 16, 16, 1, 16, 16),
+    OFX_COL(512, 32, 16, 1, 8, 8, 8),
+    OFX_COL(1024, 64, 16, 1, 16, 16, 4),
+    OFX_COL(1024, 64, 8, 1, 16, 16, 4),
+    OFX_COL(2048, 128, 8, 1, 16, 16, 8),
+    OFX_COL(2048, 128, 4, 1, 16, 16, 8),
+    OFX_COL(4096, 256, 4, 0, 16, 16, 16),
+};
+static const int kNumCols = (int)(sizeof(kCols) / sizeof(kCols[0]));
+
+}  // namespace ofx
+
+using namespace ofx;
+
+extern "C" int ofx_col_kernel_count(void) { return kNumCols; }
+
+extern "C" int ofx_col_kernel_info_get(int id, ofx_kernel_info* info) {
+    if (id < 0 || id >= kNumCols || info == nullptr) return (int)hipErrorInvalidValue;
+    *info = kCols[id].info;
+    return 0;
+}
+
+extern "C" int ofx_launch_col(int id, const ofx_args* args, hipStream_t stream) {
+    if (id < 0 || id >= kNumCols || args == nullptr) return (int)hipErrorInvalidValue;
+    return (int)kCols[id].fn(args, stream);
+}

@@ -115,8 +115,90 @@ __device__ __forceinline__ float wave_max(float v) {
 
 // Constant-operand table, generated at plan time in lane order:
 //   index ((((stage * 2 + tile) * 4 + kstep) * P + part) * 64 + lane) of 16-byte vectors
+// One transform: v holds its 16 lane values (loaded by the caller), the result goes to y.
 template <int MODE>
-__global__ __launch_bounds__(kMfmaBlock) void k_mfma_c2c_1024(ofx_args a) {
+__device__ __forceinline__ void mfma_transform(const float2 (&v)[16], float2* __restrict__ y,
+                                               const typename SplitTraits<MODE>::vec* cst, const float2* tw,
+                                               int lane, int r, int h, float in_sign, float out_sign, float scale) {
+    typedef typename SplitTraits<MODE>::vec vec;
+    constexpr int P = SplitTraits<MODE>::P;
+    float m = 0.f;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) m = fmaxf(m, fmaxf(fabsf(v[q].x), fabsf(v[q].y)));
+    m = wave_max(m);
+    const float s1 = (m > 0.f && m < 3.0e38f) ? ldexpf(1.f, 11 - ilogbf(m)) : 1.f;
+    const float si = s1 * in_sign;
+
+    // ---- stage 1 ------------------------------------------------------------------------------
+    floatx16 acc0 = {0}, acc1 = {0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        float av[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) av[j] = (s < 2) ? v[s * 8 + j].x * s1 : v[(s - 2) * 8 + j].y * si;
+        vec ap[P];
+        split8<MODE>(av, ap);
+        vec b0[P], b1[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            b0[p] = cst[(((0 * 2 + 0) * 4 + s) * P + p) * 64 + lane];
+            b1[p] = cst[(((0 * 2 + 1) * 4 + s) * P + p) * 64 + lane];
+        }
+        acc0 = mfma_split<MODE>(ap, b0, acc0);
+        acc1 = mfma_split<MODE>(ap, b1, acc1);
+    }
+
+    // ---- inter-stage twiddle W_1024^(n1 k2), k2 = r, and rescale by 2^-5 --------------------
+    float2 z[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int n1 = 8 * (q >> 2) + 4 * h + (q & 3);
+        const float2 w = tw[n1 * r];
+        const float2 d = make_float2(acc0[q] * 0.03125f, acc1[q] * 0.03125f);
+        z[q] = cmul(d, w);
+    }
+
+    // ---- stage 2 ------------------------------------------------------------------------------
+    floatx16 out0 = {0}, out1 = {0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        float bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bv[j] = (s < 2) ? z[s * 8 + j].x : z[(s - 2) * 8 + j].y;
+        vec bp[P];
+        split8<MODE>(bv, bp);
+        vec a0[P], a1[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            a0[p] = cst[(((1 * 2 + 0) * 4 + s) * P + p) * 64 + lane];
+            a1[p] = cst[(((1 * 2 + 1) * 4 + s) * P + p) * 64 + lane];
+        }
+        out0 = mfma_split<MODE>(a0, bp, out0);
+        out1 = mfma_split<MODE>(a1, bp, out1);
+    }
+
+    // ---- store X[32 k1 + k2] ------------------------------------------------------------------
+    const float inv = 32.f / s1 * scale;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int k1 = 8 * (q >> 2) + 4 * h + (q & 3);
+        y[32 * k1 + r] = make_float2(out0[q] * inv, out1[q] * (inv * out_sign));
+    }
+}
+
+__device__ __forceinline__ void mfma_load(float2 (&v)[16], const float2* __restrict__ x, int r, int h) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int n2 = 16 * (q >> 3) + 8 * h + (q & 7);
+        v[q] = x[r + 32 * n2];
+    }
+}
+
+// Constant-operand table, generated at plan time in lane order:
+//   index ((((stage * 2 + tile) * 4 + kstep) * P + part) * 64 + lane) of 16-byte vectors
+// PREFETCH: the next transform's loads are issued before the current transform's MFMA work.
+template <int MODE, bool PREFETCH>
+__global__ __launch_bounds__(kMfmaBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma_c2c_1024(ofx_args a) {
     typedef typename SplitTraits<MODE>::vec vec;
     constexpr int P = SplitTraits<MODE>::P;
     constexpr int NCONST = 2 * 2 * 4 * P * 64;
@@ -134,79 +216,29 @@ __global__ __launch_bounds__(kMfmaBlock) void k_mfma_c2c_1024(ofx_args a) {
     const int r = lane & 31;
     const int h = lane >> 5;
     const long long stride = (long long)gridDim.x * (kMfmaBlock / 64);
-    for (long long t = (long long)blockIdx.x * (kMfmaBlock / 64) + wave; t < a.batch; t += stride) {
-        // Re-read the constant operands from LDS in every iteration instead of letting the compiler
-        // keep all of them in VGPRs across the loop (that would leave one wave per SIMD).
-        __asm__ volatile("" ::: "memory");
-        const float2* __restrict__ x = static_cast<const float2*>(a.in) + t * a.idist;
-        float2 v[16];
+    const float2* __restrict__ in = static_cast<const float2*>(a.in);
+    float2* __restrict__ out = static_cast<float2*>(a.out);
+    long long t = (long long)blockIdx.x * (kMfmaBlock / 64) + wave;
+    if constexpr (PREFETCH) {
+        float2 cur[16];
+        if (t < a.batch) mfma_load(cur, in + t * a.idist, r, h);
+        for (; t < a.batch; t += stride) {
+            __asm__ volatile("" ::: "memory");  // keep the constant operands in LDS (see below)
+            float2 nxt[16];
+            const long long tn = t + stride;
+            if (tn < a.batch) mfma_load(nxt, in + tn * a.idist, r, h);
+            mfma_transform<MODE>(cur, out + t * a.odist, cst, tw, lane, r, h, a.in_sign, a.out_sign, a.scale);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int n2 = 16 * (q >> 3) + 8 * h + (q & 7);
-            v[q] = x[r + 32 * n2];
+            for (int q = 0; q < 16; ++q) cur[q] = nxt[q];
         }
-        float m = 0.f;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) m = fmaxf(m, fmaxf(fabsf(v[q].x), fabsf(v[q].y)));
-        m = wave_max(m);
-        const float s1 = (m > 0.f && m < 3.0e38f) ? ldexpf(1.f, 11 - ilogbf(m)) : 1.f;
-        const float si = s1 * a.in_sign;
-
-        // ---- stage 1 --------------------------------------------------------------------------
-        floatx16 acc0 = {0}, acc1 = {0};
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            float av[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) av[j] = (s < 2) ? v[s * 8 + j].x * s1 : v[(s - 2) * 8 + j].y * si;
-            vec ap[P];
-            split8<MODE>(av, ap);
-            vec b0[P], b1[P];
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                b0[p] = cst[(((0 * 2 + 0) * 4 + s) * P + p) * 64 + lane];
-                b1[p] = cst[(((0 * 2 + 1) * 4 + s) * P + p) * 64 + lane];
-            }
-            acc0 = mfma_split<MODE>(ap, b0, acc0);
-            acc1 = mfma_split<MODE>(ap, b1, acc1);
-        }
-
-        // ---- inter-stage twiddle W_1024^(n1 k2), k2 = r, and rescale by 2^-5 ----------------
-        float2 z[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int n1 = 8 * (q >> 2) + 4 * h + (q & 3);
-            const float2 w = tw[n1 * r];
-            const float2 d = make_float2(acc0[q] * 0.03125f, acc1[q] * 0.03125f);
-            z[q] = cmul(d, w);
-        }
-
-        // ---- stage 2 --------------------------------------------------------------------------
-        floatx16 out0 = {0}, out1 = {0};
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            float bv[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) bv[j] = (s < 2) ? z[s * 8 + j].x : z[(s - 2) * 8 + j].y;
-            vec bp[P];
-            split8<MODE>(bv, bp);
-            vec a0[P], a1[P];
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                a0[p] = cst[(((1 * 2 + 0) * 4 + s) * P + p) * 64 + lane];
-                a1[p] = cst[(((1 * 2 + 1) * 4 + s) * P + p) * 64 + lane];
-            }
-            out0 = mfma_split<MODE>(a0, bp, out0);
-            out1 = mfma_split<MODE>(a1, bp, out1);
-        }
-
-        // ---- store X[32 k1 + k2] ----------------------------------------------------------------
-        const float inv = 32.f / s1 * a.scale;
-        float2* __restrict__ y = static_cast<float2*>(a.out) + t * a.odist;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int k1 = 8 * (q >> 2) + 4 * h + (q & 3);
-            y[32 * k1 + r] = make_float2(out0[q] * inv, out1[q] * (inv * a.out_sign));
+    } else {
+        for (; t < a.batch; t += stride) {
+            // Re-read the constant operands from LDS in every iteration instead of letting the
+            // compiler keep all of them in VGPRs across the loop (that would leave one wave per SIMD).
+            __asm__ volatile("" ::: "memory");
+            float2 v[16];
+            mfma_load(v, in + t * a.idist, r, h);
+            mfma_transform<MODE>(v, out + t * a.odist, cst, tw, lane, r, h, a.in_sign, a.out_sign, a.scale);
         }
     }
 }
@@ -299,12 +331,22 @@ extern "C" int ofx_launch_mfma_c2c(int mode, int n, const ofx_args* a, int block
     if (a == nullptr || n != 1024) return (int)hipErrorInvalidValue;
     if (a->batch <= 0) return 0;
     const long long need = (a->batch + 3) / 4;
-    const unsigned g = (unsigned)std::min<long long>(need, blocks > 0 ? blocks : 1024);
+    const bool prefetch = blocks < 0;
+    const int nb = blocks < 0 ? -blocks : blocks;
+    const unsigned g = (unsigned)std::min<long long>(need, nb > 0 ? nb : 1024);
+#define OFX_MFMA_LAUNCH(M)                                                                                   \
+    do {                                                                                                     \
+        if (prefetch)                                                                                        \
+            hipLaunchKernelGGL((k_mfma_c2c_1024<M, true>), dim3(g), dim3(kMfmaBlock), 0, stream, *a);        \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_mfma_c2c_1024<M, false>), dim3(g), dim3(kMfmaBlock), 0, stream, *a);       \
+    } while (0)
     switch (mode) {
-        case OFX_MFMA_BF16: hipLaunchKernelGGL(k_mfma_c2c_1024<OFX_MFMA_BF16>, dim3(g), dim3(kMfmaBlock), 0, stream, *a); break;
-        case OFX_MFMA_FP16X2: hipLaunchKernelGGL(k_mfma_c2c_1024<OFX_MFMA_FP16X2>, dim3(g), dim3(kMfmaBlock), 0, stream, *a); break;
-        case OFX_MFMA_BF16X3: hipLaunchKernelGGL(k_mfma_c2c_1024<OFX_MFMA_BF16X3>, dim3(g), dim3(kMfmaBlock), 0, stream, *a); break;
+        case OFX_MFMA_BF16: OFX_MFMA_LAUNCH(OFX_MFMA_BF16); break;
+        case OFX_MFMA_FP16X2: OFX_MFMA_LAUNCH(OFX_MFMA_FP16X2); break;
+        case OFX_MFMA_BF16X3: OFX_MFMA_LAUNCH(OFX_MFMA_BF16X3); break;
         default: return (int)hipErrorInvalidValue;
     }
+#undef OFX_MFMA_LAUNCH
     return (int)hipGetLastError();
 }

@@ -33,7 +33,10 @@ struct Radices {
 
 // IOV_ = 2: in the first and last pass a thread owns adjacent butterflies j, j+1, so its HBM loads
 // and stores move two complex64 values (16 bytes) per instruction.
-template <int N_, int T_, int TPB_, int PADSH_, bool STAGED_, class RL, int IOV_ = 1>
+// COLS_ > 0: column kernels.  The workgroup holds COLS_ transforms (columns of a row-major matrix)
+// interleaved in LDS, element i of column c at lds[c + i*(COLS_ + COLPAD_)], and consecutive
+// lanes take consecutive columns, so each HBM row segment of COLS_ values is one contiguous access.
+template <int N_, int T_, int TPB_, int PADSH_, bool STAGED_, class RL, int IOV_ = 1, int COLS_ = 0, int COLPAD_ = 1>
 struct StockhamCfg {
     static constexpr int N = N_;
     static constexpr int T = T_;
@@ -42,8 +45,10 @@ struct StockhamCfg {
     static constexpr bool STAGED = STAGED_;
     static constexpr int IOV = IOV_;
     static constexpr int NP = RL::count;
+    static constexpr int COLS = COLS_;
+    static constexpr int COLSTRIDE = COLS_ + COLPAD_;
     static constexpr int BLOCK = T * TPB;
-    static constexpr bool WAVE_SYNC = (T <= 64) && (64 % T == 0);
+    static constexpr bool WAVE_SYNC = (COLS_ == 0) && (T <= 64) && (64 % T == 0);
 
     static constexpr int radix(int p) { return RL::v[p]; }
     static constexpr int ns(int p) {
@@ -57,7 +62,7 @@ struct StockhamCfg {
         return o;
     }
     static constexpr int tw_len() { return tw_off(NP); }
-    static constexpr int LDS_N = N + (N >> PADSH);
+    static constexpr int LDS_N = COLS_ > 0 ? N * COLSTRIDE : N + (N >> PADSH);
     static constexpr bool USES_LDS = STAGED || NP > 1;
     static constexpr int LDS_ELEMS = USES_LDS ? TPB * LDS_N : 1;
     static constexpr int max_elems_per_thread() {
@@ -69,7 +74,12 @@ struct StockhamCfg {
         }
         return m;
     }
-    __device__ __forceinline__ static int pad(int i) { return i + (i >> PADSH); }
+    __device__ __forceinline__ static int pad(int i) {
+        if constexpr (COLS_ > 0)
+            return i * COLSTRIDE;
+        else
+            return i + (i >> PADSH);
+    }
     __device__ __forceinline__ static void sync() {
         if constexpr (WAVE_SYNC) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

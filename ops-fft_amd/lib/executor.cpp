@@ -48,6 +48,15 @@ hipError_t hip_log(hipError_t e, const char* what) {
     return e;
 }
 
+const std::vector<ofx_kernel_info>& col_registry() {
+    static std::vector<ofx_kernel_info> reg = [] {
+        std::vector<ofx_kernel_info> r(ofx_col_kernel_count());
+        for (int i = 0; i < (int)r.size(); ++i) ofx_col_kernel_info_get(i, &r[i]);
+        return r;
+    }();
+    return reg;
+}
+
 int env_int(const char* name, int dflt) {
     const char* v = std::getenv(name);
     return (v && *v) ? std::atoi(v) : dflt;
@@ -306,6 +315,41 @@ struct Builder {
     // Columns of [hi][rows][cols] row-major data: `cols` transforms of length `rows` per image.
     aclfftResult c2c_columns(long long rows, long long cols, long long images, int src, int dst, bool conj_in,
                              bool conj_out) {
+        std::vector<int> ids;
+        if (env_int("ACLFFT_DISABLE_TEMPLATED", 0) == 0) {
+            const auto& creg = col_registry();
+            for (int i = 0; i < (int)creg.size(); ++i)
+                if (creg[i].n == rows) ids.push_back(i);
+            // widest column groups first only when a row has enough columns to fill them
+            std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) {
+                const bool fa = creg[a].transforms_per_block <= cols, fb = creg[b].transforms_per_block <= cols;
+                return fa > fb;
+            });
+        }
+        if (!ids.empty()) {
+            Step c;
+            c.type = Step::COLUMN;
+            c.kind = OFX_KIND_COL;
+            c.n = (int)rows;
+            c.candidates = ids;
+            for (int id : ids) c.candidate_tables.push_back(pass_table(col_registry()[id]));
+            const int pick = std::min<int>(std::max(0, env_int("ACLFFT_COLUMN_VARIANT", 0)), (int)ids.size() - 1);
+            c.kernel = ids[pick];
+            c.tw_table = c.candidate_tables[pick];
+            c.args.in_sign = c.args.out_sign = c.args.scale = 1.f;
+            c.args.batch = cols * images;
+            c.args.cols = cols;
+            c.args.idist = rows * cols;
+            c.args.odist = rows * cols;
+            c.args.istride = cols;
+            c.args.ostride = cols;
+            c.src = src;
+            c.dst = dst;
+            c.conj_in = conj_in;
+            c.conj_out = conj_out;
+            e->steps_.push_back(c);
+            return ACLFFT_SUCCESS;
+        }
         Step s;
         if (!generic_step(rows, &s)) return ACLFFT_NOT_IMPLEMENTED;
         s.g.tpb = (int)std::max<long long>(1, std::min<long long>(16, OFX_GENERIC_MAX_ELEMS / rows));
@@ -653,6 +697,13 @@ std::string Executor::describe() const {
             os << (s.type == Step::R2C_UNPACK ? "r2c_unpack" : "c2r_pack") << " M=" << s.m;
             continue;
         }
+        if (s.type == Step::COLUMN) {
+            const ofx_kernel_info& k = col_registry()[s.kernel];
+            os << "column n=" << k.n << " T=" << k.threads_per_transform << " cols=" << k.transforms_per_block
+               << " radices=";
+            for (int p = 0; p < k.npass; ++p) os << (p ? "x" : "") << k.radices[p];
+            continue;
+        }
         if (s.type == Step::PERSISTENT) {
             os << "persistent tiles=" << s.b.ntiles << " classes=";
             bool first = true;
@@ -761,6 +812,16 @@ aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream
             case Step::C2R_PACK:
                 err = (hipError_t)ofx_c2r_pack(src, dst, table(st.aux_table), st.m, st.args.batch, st.args.flags, s);
                 break;
+            case Step::COLUMN: {
+                ofx_args a = st.args;
+                a.in = src;
+                a.out = dst;
+                a.tw = table(st.tw_table);
+                if (st.conj_in) a.in_sign *= dsign;
+                if (st.conj_out) a.out_sign *= dsign;
+                err = (hipError_t)ofx_launch_col(st.kernel, &a, s);
+                break;
+            }
             case Step::PERSISTENT: {
                 ofx_batch_args b = st.b;
                 b.in = src;
@@ -921,7 +982,7 @@ aclfftResult Executor::run_graph(const void* din, void* dout, int direction, hip
 aclfftResult Executor::autotune(hipStream_t s) {
     bool any = false;
     for (const Step& st : steps_)
-        if (st.type == Step::KERNEL && st.candidates.size() > 1) any = true;
+        if ((st.type == Step::KERNEL || st.type == Step::COLUMN) && st.candidates.size() > 1) any = true;
     if (!any) return ACLFFT_SUCCESS;
     void* a = nullptr;
     void* b = nullptr;
@@ -936,7 +997,10 @@ aclfftResult Executor::autotune(hipStream_t s) {
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     for (Step& st : steps_) {
-        if (st.type != Step::KERNEL || st.candidates.size() < 2) continue;
+        if ((st.type != Step::KERNEL && st.type != Step::COLUMN) || st.candidates.size() < 2) continue;
+        auto launch_cand = [&](int id, const ofx_args* a) {
+            return st.type == Step::COLUMN ? ofx_launch_col(id, a, s) : ofx_launch(id, a, s);
+        };
         float best = 1e30f;
         size_t best_i = 0;
         for (size_t ci = 0; ci < st.candidates.size(); ++ci) {
@@ -946,9 +1010,9 @@ aclfftResult Executor::autotune(hipStream_t s) {
             const int tt = st.candidate_tables[ci];
             args.tw = tt >= 0 ? static_cast<char*>(d_tables_) + tables_[tt].offset : nullptr;
             args.tw_aux = st.aux_table >= 0 ? static_cast<char*>(d_tables_) + tables_[st.aux_table].offset : nullptr;
-            bool ok = ofx_launch(st.candidates[ci], &args, s) == 0;  // warm-up
+            bool ok = launch_cand(st.candidates[ci], &args) == 0;  // warm-up
             (void)hipEventRecord(e0, s);
-            for (int r = 0; ok && r < 5; ++r) ok = ofx_launch(st.candidates[ci], &args, s) == 0;
+            for (int r = 0; ok && r < 5; ++r) ok = launch_cand(st.candidates[ci], &args) == 0;
             (void)hipEventRecord(e1, s);
             (void)hipEventSynchronize(e1);
             float ms = 0.f;

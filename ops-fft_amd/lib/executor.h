@@ -32,7 +32,7 @@ struct TableSpec {
 
 // One launch.
 struct Step {
-    enum Type { KERNEL, GENERIC, COPY, R2C_UNPACK, C2R_PACK, MFMA, PERSISTENT } type = KERNEL;
+    enum Type { KERNEL, GENERIC, COPY, R2C_UNPACK, C2R_PACK, MFMA, PERSISTENT, COLUMN } type = KERNEL;
     int kernel = -1;               // registry id (KERNEL)
     int kind = OFX_KIND_C2C;
     int n = 0;                     // complex length the kernel computes (for autotuning)

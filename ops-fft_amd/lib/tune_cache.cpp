@@ -64,7 +64,7 @@ void apply_tune_cache(const PlanDesc& d, std::vector<Step>& steps, const ofx_ker
     if (m.empty()) return;
     for (size_t i = 0; i < steps.size(); ++i) {
         Step& st = steps[i];
-        if (st.type != Step::KERNEL || st.candidates.size() < 2) continue;
+        if (st.type != Step::KERNEL || st.candidates.size() < 2) continue;  // row kernels only
         auto it = m.find(key_of(d, i));
         if (it == m.end()) continue;
         for (size_t c = 0; c < st.candidates.size(); ++c) {

@@ -112,3 +112,45 @@ def test_every_registry_kernel(cuda, monkeypatch, kind, n, variant):
     print(p.describe(), "rel_l2 %.3e" % err)
     p.destroy()
     assert err <= fp32_tol(2 * n)
+
+
+def _col_variants():
+    import ctypes
+    lib = aclfft.lib
+    lib.ofx_col_kernel_count.restype = ctypes.c_int
+    out, seen = [], {}
+    for i in range(lib.ofx_col_kernel_count()):
+        info = aclfft._lib.KernelInfo()
+        lib.ofx_col_kernel_info_get(i, ctypes.byref(info))
+        idx = seen.get(info.n, 0)
+        seen[info.n] = idx + 1
+        out.append((info.n, idx))
+    return out
+
+
+@pytest.mark.parametrize("rows,variant", _col_variants())
+@pytest.mark.parametrize("cols", [40, 7])
+def test_every_column_kernel_2d_and_vertical(cuda, monkeypatch, rows, variant, cols):
+    """Column kernels (2D second pass and the VERTICAL layout) against the FP64 oracle, with a
+    column count that leaves a partial column group."""
+    import torch
+    monkeypatch.setenv("ACLFFT_COLUMN_VARIANT", str(variant))
+    x = crandn((3, rows, cols), rows + cols + variant).astype(np.complex64)
+    p = aclfft.Plan.plan2d(3, rows, cols, aclfft.C2C)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    p.exec_c2c(xt, yt, aclfft.BACKWARD)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(yt.cpu().numpy(), oracle.c2c_2d(x.astype(np.complex128), aclfft.BACKWARD))
+    print(p.describe(), "rel_l2 %.3e" % err)
+    assert err <= fp32_tol(rows * cols)
+    p.destroy()
+    if rows >= 256:  # VERTICAL plans exist for power-of-two rows in [256, 2^18]
+        v = crandn((rows, cols), variant).astype(np.complex64)
+        p = aclfft.Plan.plan1d(rows, aclfft.C2C, cols, aclfft.VERTICAL)
+        vt = torch.from_numpy(v).cuda()
+        p.exec_c2c(vt, vt, aclfft.FORWARD)
+        torch.cuda.synchronize()
+        err = oracle.rel_l2(vt.cpu().numpy(), oracle.c2c_vertical(v.astype(np.complex128)))
+        assert err <= fp32_tol(rows)
+        p.destroy()
