@@ -44,7 +44,9 @@ enum ofx_kind {
     OFX_KIND_C2R = 2,      /* even n: fused Hermitian pre-pack + n/2-point complex FFT     */
     OFX_KIND_COL = 3,      /* strided "column" transforms (2D second pass, four-step)      */
     OFX_KIND_R2C_ODD = 4,  /* odd n: full complex FFT of real input, first n/2+1 bins      */
-    OFX_KIND_C2R_ODD = 5   /* odd n: Hermitian expansion + full complex FFT, real part     */
+    OFX_KIND_C2R_ODD = 5,  /* odd n: Hermitian expansion + full complex FFT, real part     */
+    OFX_KIND_ROWT = 6      /* contiguous rows in, transposed out: transform t -> out +
+                              (t / cols)*odist + t % cols + k*ostride (four-step row step) */
 };
 
 /* Description of one compiled kernel configuration. */
@@ -80,6 +82,8 @@ typedef struct ofx_args {
     float scale;           /* multiplies every output value                                */
     int flags;             /* OFX_FLAG_*                                                   */
     long long cols;        /* COL kernels: columns per image (batch = images * cols)       */
+    long long ft_len;      /* COL kernels: > 0 multiplies output i of column c by the four-step
+                              twiddle W_ft_len^(c*i) (tw_aux = ofx_gen_ft_table of ft_len)   */
 } ofx_args;
 
 /* Runtime-radix ("generic") launches: any length whose radix sequence is listed, any layout.
@@ -149,7 +153,8 @@ OFX_API long long ofx_mfma_const_bytes(int mode);
 OFX_API int ofx_gen_mfma_consts(int mode, void* table, hipStream_t stream);
 /* Batch of length-n C2C transforms (n = 1024), contiguous (idist/odist in complex64 units).
  * args->tw = constant table of `mode`, args->tw_aux = W_1024^k (k < 1024, ofx_gen_roots).
- * blocks: workgroups of the grid-stride launch (0 = default 1024); a negative value -b launches b
+ * blocks: workgroups of the grid-stride launch (0 = default: 2048, or 512 for OFX_MFMA_BF16X3); a
+ * negative value -b launches b
  * workgroups of the software-pipelined variant (next transform's loads issued before the MFMA work
  * of the current one). */
 OFX_API int ofx_launch_mfma_c2c(int mode, int n, const ofx_args* args, int blocks, hipStream_t stream);

@@ -10,7 +10,7 @@ Run: python3 gen_registry.py  (rewrites registry_1d.inc next to this script)
 import os
 
 KINDS = {"C2C": "OFX_KIND_C2C", "R2C": "OFX_KIND_R2C", "C2R": "OFX_KIND_C2R",
-         "R2C_ODD": "OFX_KIND_R2C_ODD", "C2R_ODD": "OFX_KIND_C2R_ODD"}
+         "R2C_ODD": "OFX_KIND_R2C_ODD", "C2R_ODD": "OFX_KIND_C2R_ODD", "ROWT": "OFX_KIND_ROWT"}
 
 # Radix plans for the lengths that get a dedicated (fully unrolled) kernel.  Everything else runs
 # on the runtime-radix kernel (kernels_generic.hip).
@@ -82,6 +82,15 @@ def lines():
             tpb = tpb_for(n, t)
             for staged in ((False, True) if pi == 0 else (False,)):
                 out.append(("C2C", n, t, tpb, 4, staged, rad))
+    # four-step row step (rows in, transposed out): as many rows per workgroup as LDS allows,
+    # so that the transposed stores run over >= 32-byte segments
+    for n in (128, 256, 512, 1024, 2048, 4096):
+        rad = PLANS[n][0]
+        t = choose_t(n, rad)
+        tpb = 16
+        while tpb > 1 and (tpb * (n + n // 16) * 8 > 147456 or tpb * t > 1024):
+            tpb //= 2
+        out.append(("ROWT", n, t, tpb, 4, False, rad))
     # real transforms: even n = 2N uses the N-point complex kernel with fused pack/unpack
     for n in (1024, 2048, 1500, 4096, 512, 256, 128, 64, 32, 16, 8, 4, 2, 8192, 3000, 1000):
         rad = PLANS[n][0]

@@ -331,6 +331,37 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2r_odd(ofx_args a) {
     stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, static_cast<const float2*>(a.tw), gin, gout);
 }
 
+// Rows in, columns out: the row step of the four-step decomposition.  Transform t (length C::N,
+// contiguous at in + t*idist) is written to out + img*odist + r + k*ostride with img = t / cols,
+// r = t % cols.  The last pass stays in LDS and the workgroup stores its TPB transforms
+// interleaved, so consecutive lanes write consecutive addresses (TPB*8-byte runs).
+template <class C>
+__global__ __launch_bounds__(C::BLOCK) void k_rowt(ofx_args a) {
+    __shared__ float2 lds[C::TPB * C::LDS_N];
+    const int tl = threadIdx.x / C::T;
+    const int tid = threadIdx.x - tl * C::T;
+    const long long t0 = (long long)blockIdx.x * C::TPB;
+    const long long t = t0 + tl;
+    const long long tt = t < a.batch ? t : 0;
+    float2* lt = lds + tl * C::LDS_N;
+    GLoadC gin{static_cast<const float2*>(a.in) + tt * a.idist, a.in_sign};
+    LdsWrite<C> lw{lt};
+    stockham_passes<C, false, true>(lt, tid, static_cast<const float2*>(a.tw), gin, lw);
+    __syncthreads();
+    const int nvalid = (int)min((long long)C::TPB, a.batch - t0);
+    float2* __restrict__ out = static_cast<float2*>(a.out);
+    const float si = a.out_sign * a.scale;
+    for (int f = threadIdx.x; f < nvalid * C::N; f += C::BLOCK) {
+        const int r = f % nvalid;
+        const int k = f / nvalid;
+        const long long tg = t0 + r;
+        const long long img = tg / a.cols;
+        const long long k1 = tg - img * a.cols;
+        const float2 v = lds[r * C::LDS_N + C::pad(k)];
+        out[img * a.odist + k1 + (long long)k * a.ostride] = make_float2(v.x * a.scale, v.y * si);
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // launchers + registry
 // ---------------------------------------------------------------------------------------------
@@ -357,6 +388,10 @@ hipError_t launch(const ofx_args* a, hipStream_t s) {
         hipLaunchKernelGGL(k_r2c_odd<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a);
     if constexpr (KIND == OFX_KIND_C2R_ODD)
         hipLaunchKernelGGL(k_c2r_odd<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a);
+    if constexpr (KIND == OFX_KIND_ROWT) {
+        if (a->cols <= 0) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_rowt<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a);
+    }
     return hipGetLastError();
 }
 

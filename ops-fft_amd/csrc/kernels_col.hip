@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include "ofx_launch.h"
 #include "stockham.h"
+#include "twiddle64.h"
 
 namespace ofx {
 
@@ -25,19 +26,23 @@ struct ColLoad {
         return make_float2(v.x, v.y * sg);
     }
 };
-template <class C>
+template <class C, bool FT>
 struct ColStore {
     float2* __restrict__ p;
     long long stride;
     float sg, scale;
     bool ok;
+    const double2* __restrict__ ft;  // FT: four-step twiddle W_ftlen^(col * i) on output i
+    long long col, ftlen;
     __device__ __forceinline__ void operator()(int i, float2 v) const {
+        if constexpr (FT) v = cmul(v, ft_twiddle(ft, col * (long long)i, ftlen));
         if (ok) p[(long long)i * stride] = make_float2(v.x * scale, v.y * (sg * scale));
     }
 };
 
-template <class C>
-__global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, long long groups_per_image) {
+template <class C, bool FT>
+__global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, long long groups_per_image,
+                                                  long long ftlen) {
     __shared__ float2 lds[C::LDS_N];
     const int c = threadIdx.x % C::COLS;
     const int tid = threadIdx.x / C::COLS;
@@ -50,7 +55,8 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
     const bool ok = col < cols;
     const long long cc = ok ? col : 0;
     ColLoad<C> gin{static_cast<const float2*>(a.in) + img * a.idist + cc, a.istride, a.in_sign};
-    ColStore<C> gout{static_cast<float2*>(a.out) + img * a.odist + cc, a.ostride, a.out_sign, a.scale, ok};
+    ColStore<C, FT> gout{static_cast<float2*>(a.out) + img * a.odist + cc, a.ostride, a.out_sign, a.scale, ok,
+                         static_cast<const double2*>(a.tw_aux), cc, ftlen};
     stockham_passes<C, false, false>(lds + c, tid, static_cast<const float2*>(a.tw), gin, gout);
 }
 
@@ -61,7 +67,12 @@ hipError_t launch_col(const ofx_args* a, hipStream_t s) {
     const long long gpi = (a->cols + C::COLS - 1) / C::COLS;
     const long long blocks = images * gpi;
     if (blocks > 0x7fffffffLL || images * a->cols != a->batch) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_col<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi);
+    if (a->ft_len > 0) {
+        if (a->tw_aux == nullptr) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_col<C, true>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, a->ft_len);
+    } else {
+        hipLaunchKernelGGL((k_col<C, false>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, 0LL);
+    }
     return hipGetLastError();
 }
 

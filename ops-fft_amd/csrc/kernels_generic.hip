@@ -14,12 +14,11 @@
 #include <hip/hip_runtime.h>
 #include "ofx_launch.h"
 #include "codelets.h"
+#include "twiddle64.h"
 
 namespace ofx {
 
 constexpr int kGenBlock = 256;
-constexpr int kFtLoBits = 11;  // B = 2048
-constexpr long long kFtLo = 1LL << kFtLoBits;
 
 struct GenParams {
     ofx_args a;
@@ -86,15 +85,6 @@ __device__ __forceinline__ bool run_pass(int r, const float2* src, float2* dst, 
     }
 }
 
-// FP64 four-step twiddle W_len^e from the two-level table (lo: kFtLo entries, hi: the rest)
-__device__ __forceinline__ float2 ft_twiddle(const double2* __restrict__ t, long long e, long long len) {
-    e %= len;
-    const long long lo = e & (kFtLo - 1);
-    const long long hi = e >> kFtLoBits;
-    const double2 a = t[lo];
-    const double2 b = t[kFtLo + hi];
-    return make_float2((float)(a.x * b.x - a.y * b.y), (float)(a.x * b.y + a.y * b.x));
-}
 
 // ---------------------------------------------------------------------------------------------
 // the kernel

@@ -333,7 +333,10 @@ extern "C" int ofx_launch_mfma_c2c(int mode, int n, const ofx_args* a, int block
     const long long need = (a->batch + 3) / 4;
     const bool prefetch = blocks < 0;
     const int nb = blocks < 0 ? -blocks : blocks;
-    const unsigned g = (unsigned)std::min<long long>(need, nb > 0 ? nb : 1024);
+    // default grid (measured on MI355X): 2048 workgroups for the 1- and 2-part modes, 512 for the
+    // 3-part mode whose constant table takes 48 KiB of LDS per workgroup
+    const int dflt = (mode == OFX_MFMA_BF16X3) ? 512 : 2048;
+    const unsigned g = (unsigned)std::min<long long>(need, nb > 0 ? nb : dflt);
 #define OFX_MFMA_LAUNCH(M)                                                                                   \
     do {                                                                                                     \
         if (prefetch)                                                                                        \

@@ -266,9 +266,51 @@ struct Builder {
         return true;
     }
 
+    // A four-step split whose column length has a compiled column kernel and whose row length has a
+    // compiled transposed-store row kernel, as balanced as possible.
+    bool templated_split(long long n, long long* n1, long long* n2) {
+        if (env_int("ACLFFT_DISABLE_TEMPLATED", 0) != 0) return false;
+        double best = 1e30;
+        for (const ofx_kernel_info& c : col_registry()) {
+            if (n % c.n) continue;
+            const long long b = n / c.n;
+            if (kernels_for(OFX_KIND_ROWT, b, true).empty()) continue;
+            const double d = std::fabs(std::log((double)c.n) - std::log((double)b));
+            if (d < best) {
+                best = d;
+                *n1 = c.n;
+                *n2 = b;
+            }
+        }
+        return best < 1e30;
+    }
+
     // Four-step: x[n1*N2 + n2] -> columns (length N1, twiddle W_N^(n2 k1)) -> workspace
     // tmp[k1*N2 + n2] -> rows (length N2) -> X[k1 + N1*k2].
     aclfftResult c2c_fourstep(long long n, long long batch, int src, int dst, bool conj_in, bool conj_out) {
+        long long T1 = 0, T2 = 0;
+        if (templated_split(n, &T1, &T2)) {
+            // column step: N2 columns of length N1 per transform, four-step twiddle fused in the store
+            aclfftResult r = c2c_columns(T1, T2, batch, src, BUF_WORK0, conj_in, false);
+            if (r != ACLFFT_SUCCESS) return r;
+            Step& a = e->steps_.back();
+            a.args.ft_len = n;
+            a.aux_table = fourstep_table(n);
+            // row step: rows of length N2, transposed store X[k1 + N1*k2]
+            auto ids = kernels_for(OFX_KIND_ROWT, T2, true);
+            Step b = kernel_step(OFX_KIND_ROWT, T2, ids);
+            b.src = BUF_WORK0;
+            b.dst = dst;
+            b.args.batch = batch * T1;
+            b.args.idist = T2;
+            b.args.cols = T1;
+            b.args.odist = n;
+            b.args.ostride = T1;
+            b.conj_out = conj_out;
+            e->steps_.push_back(b);
+            e->work_bytes_ = std::max(e->work_bytes_, (size_t)(n * batch * 8));
+            return ACLFFT_SUCCESS;
+        }
         long long N1 = 0, N2 = 0;
         if (!split(n, &N1, &N2)) return ACLFFT_NOT_IMPLEMENTED;
         const int ft = fourstep_table(n);
@@ -702,6 +744,7 @@ std::string Executor::describe() const {
             os << "column n=" << k.n << " T=" << k.threads_per_transform << " cols=" << k.transforms_per_block
                << " radices=";
             for (int p = 0; p < k.npass; ++p) os << (p ? "x" : "") << k.radices[p];
+            if (s.args.ft_len) os << " fourstep-twiddle";
             continue;
         }
         if (s.type == Step::PERSISTENT) {
@@ -817,6 +860,7 @@ aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream
                 a.in = src;
                 a.out = dst;
                 a.tw = table(st.tw_table);
+                a.tw_aux = table(st.aux_table);
                 if (st.conj_in) a.in_sign *= dsign;
                 if (st.conj_out) a.out_sign *= dsign;
                 err = (hipError_t)ofx_launch_col(st.kernel, &a, s);
