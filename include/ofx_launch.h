@@ -46,7 +46,8 @@ enum ofx_kind {
     OFX_KIND_R2C_ODD = 4,  /* odd n: full complex FFT of real input, first n/2+1 bins      */
     OFX_KIND_C2R_ODD = 5,  /* odd n: Hermitian expansion + full complex FFT, real part     */
     OFX_KIND_ROWT = 6      /* contiguous rows in, transposed out: transform t -> out +
-                              (t / cols)*odist + t % cols + k*ostride (four-step row step) */
+                              (t / cols)*odist + rhi*odist_hi + rlo + k*ostride with
+                              t % cols = rhi*cols_lo + rlo (four-step row step)          */
 };
 
 /* Description of one compiled kernel configuration. */
@@ -84,6 +85,9 @@ typedef struct ofx_args {
     long long cols;        /* COL kernels: columns per image (batch = images * cols)       */
     long long ft_len;      /* COL kernels: > 0 multiplies output i of column c by the four-step
                               twiddle W_ft_len^(c*i) (tw_aux = ofx_gen_ft_table of ft_len)   */
+    long long cols_lo;     /* ROWT: splits the row index r = t % cols into r / cols_lo (stride
+                              odist_hi in `out`) and r % cols_lo (stride 1); 0 = no split    */
+    long long odist_hi;
 } ofx_args;
 
 /* Runtime-radix ("generic") launches: any length whose radix sequence is listed, any layout.

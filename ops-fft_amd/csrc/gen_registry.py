@@ -88,7 +88,7 @@ def lines():
         rad = PLANS[n][0]
         t = choose_t(n, rad)
         tpb = 16
-        while tpb > 1 and (tpb * (n + n // 16) * 8 > 147456 or tpb * t > 1024):
+        while tpb > 1 and (tpb * (n + n // 16 + 1) * 8 > 147456 or tpb * t > 1024):
             tpb //= 2
         out.append(("ROWT", n, t, tpb, 4, False, rad))
     # real transforms: even n = 2N uses the N-point complex kernel with fused pack/unpack

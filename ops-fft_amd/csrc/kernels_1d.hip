@@ -332,33 +332,46 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2r_odd(ofx_args a) {
 }
 
 // Rows in, columns out: the row step of the four-step decomposition.  Transform t (length C::N,
-// contiguous at in + t*idist) is written to out + img*odist + r + k*ostride with img = t / cols,
-// r = t % cols.  The last pass stays in LDS and the workgroup stores its TPB transforms
-// interleaved, so consecutive lanes write consecutive addresses (TPB*8-byte runs).
+// contiguous at in + t*idist) is written to out + img*odist + rhi*odist_hi + rlo + k*ostride with
+// img = t / cols and t % cols = rhi*cols_lo + rlo (cols_lo = 0: rhi = 0).  The last pass stays in LDS and the workgroup stores its TPB transforms
+// interleaved, so consecutive lanes write consecutive addresses (TPB*8-byte runs).  The LDS rows
+// are RS = LDS_N + 1 elements apart (an odd count of 8-byte elements), so the TPB lanes reading
+// element k of TPB different rows fall on distinct banks.
 template <class C>
 __global__ __launch_bounds__(C::BLOCK) void k_rowt(ofx_args a) {
-    __shared__ float2 lds[C::TPB * C::LDS_N];
+    constexpr int RS = C::LDS_N + 1;
+    constexpr int KSTEP = C::BLOCK / C::TPB;  // rows' elements stored per sweep of the workgroup
+    static_assert(C::BLOCK % C::TPB == 0, "transposed store assumes TPB divides the block");
+    __shared__ float2 lds[C::TPB * RS];
     const int tl = threadIdx.x / C::T;
     const int tid = threadIdx.x - tl * C::T;
     const long long t0 = (long long)blockIdx.x * C::TPB;
     const long long t = t0 + tl;
     const long long tt = t < a.batch ? t : 0;
-    float2* lt = lds + tl * C::LDS_N;
+    float2* lt = lds + tl * RS;
     GLoadC gin{static_cast<const float2*>(a.in) + tt * a.idist, a.in_sign};
     LdsWrite<C> lw{lt};
     stockham_passes<C, false, true>(lt, tid, static_cast<const float2*>(a.tw), gin, lw);
     __syncthreads();
-    const int nvalid = (int)min((long long)C::TPB, a.batch - t0);
-    float2* __restrict__ out = static_cast<float2*>(a.out);
+    // thread -> (row r, first element k0); a row past the batch end stores nothing
+    const int r = threadIdx.x % C::TPB;
+    const int k0 = threadIdx.x / C::TPB;
+    const long long tg = t0 + r;
+    if (tg >= a.batch) return;
+    const long long img = tg / a.cols;
+    long long r_in = tg - img * a.cols, r_off = r_in;
+    if (a.cols_lo > 0) {
+        const long long rhi = r_in / a.cols_lo;
+        r_off = rhi * a.odist_hi + (r_in - rhi * a.cols_lo);
+    }
+    float2* __restrict__ o = static_cast<float2*>(a.out) + img * a.odist + r_off + k0 * a.ostride;
+    const long long step = (long long)KSTEP * a.ostride;
+    const float2* lr = lds + r * RS;
     const float si = a.out_sign * a.scale;
-    for (int f = threadIdx.x; f < nvalid * C::N; f += C::BLOCK) {
-        const int r = f % nvalid;
-        const int k = f / nvalid;
-        const long long tg = t0 + r;
-        const long long img = tg / a.cols;
-        const long long k1 = tg - img * a.cols;
-        const float2 v = lds[r * C::LDS_N + C::pad(k)];
-        out[img * a.odist + k1 + (long long)k * a.ostride] = make_float2(v.x * a.scale, v.y * si);
+#pragma unroll 4
+    for (int k = k0; k < C::N; k += KSTEP, o += step) {
+        const float2 v = lr[C::pad(k)];
+        *o = make_float2(v.x * a.scale, v.y * si);
     }
 }
 
@@ -412,7 +425,7 @@ Entry make_entry() {
     for (int p = 0; p < C::NP; ++p) e.info.radices[p] = C::radix(p);
     e.info.staged = (C::STAGED || C::IOV == 2) ? 1 : 0;
     e.info.iov = C::IOV;
-    e.info.lds_bytes = (int)(C::TPB * C::LDS_N * sizeof(float2));
+    e.info.lds_bytes = (int)(C::TPB * (C::LDS_N + (KIND == OFX_KIND_ROWT ? 1 : 0)) * sizeof(float2));
     e.info.tw_len = C::tw_len();
     e.fn = &launch<C, KIND>;
     return e;

@@ -13,6 +13,7 @@
 #include "ofx_launch.h"
 #include "stockham.h"
 #include "twiddle64.h"
+#include "twiddle_pow2.h"
 
 namespace ofx {
 
@@ -26,23 +27,27 @@ struct ColLoad {
         return make_float2(v.x, v.y * sg);
     }
 };
-template <class C, bool FT>
+// FT: 0 = none; 1 = four-step twiddle W_ftlen^(col * i) on output i from the FP64 two-level table;
+// 2 = the same twiddle computed in registers (ftlen = 2^ftlog2).
+template <class C, int FT>
 struct ColStore {
     float2* __restrict__ p;
     long long stride;
     float sg, scale;
     bool ok;
-    const double2* __restrict__ ft;  // FT: four-step twiddle W_ftlen^(col * i) on output i
+    const double2* __restrict__ ft;
     long long col, ftlen;
+    int ftlog2;
     __device__ __forceinline__ void operator()(int i, float2 v) const {
-        if constexpr (FT) v = cmul(v, ft_twiddle(ft, col * (long long)i, ftlen));
+        if constexpr (FT == 1) v = cmul(v, ft_twiddle(ft, col * (long long)i, ftlen));
+        if constexpr (FT == 2) v = cmul(v, twiddle_pow2((unsigned long long)col * (unsigned)i, ftlog2));
         if (ok) p[(long long)i * stride] = make_float2(v.x * scale, v.y * (sg * scale));
     }
 };
 
-template <class C, bool FT>
+template <class C, int FT>
 __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, long long groups_per_image,
-                                                  long long ftlen) {
+                                                  long long ftlen, int ftlog2) {
     __shared__ float2 lds[C::LDS_N];
     const int c = threadIdx.x % C::COLS;
     const int tid = threadIdx.x / C::COLS;
@@ -56,7 +61,7 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
     const long long cc = ok ? col : 0;
     ColLoad<C> gin{static_cast<const float2*>(a.in) + img * a.idist + cc, a.istride, a.in_sign};
     ColStore<C, FT> gout{static_cast<float2*>(a.out) + img * a.odist + cc, a.ostride, a.out_sign, a.scale, ok,
-                         static_cast<const double2*>(a.tw_aux), cc, ftlen};
+                         static_cast<const double2*>(a.tw_aux), cc, ftlen, ftlog2};
     stockham_passes<C, false, false>(lds + c, tid, static_cast<const float2*>(a.tw), gin, gout);
 }
 
@@ -68,10 +73,17 @@ hipError_t launch_col(const ofx_args* a, hipStream_t s) {
     const long long blocks = images * gpi;
     if (blocks > 0x7fffffffLL || images * a->cols != a->batch) return hipErrorInvalidValue;
     if (a->ft_len > 0) {
-        if (a->tw_aux == nullptr) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_col<C, true>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, a->ft_len);
+        const long long f = a->ft_len;
+        int lg = 0;
+        while ((1LL << lg) < f) ++lg;
+        if ((1LL << lg) == f && lg >= 3 && lg <= 27) {
+            hipLaunchKernelGGL((k_col<C, 2>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, f, lg);
+        } else {
+            if (a->tw_aux == nullptr) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((k_col<C, 1>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, f, 0);
+        }
     } else {
-        hipLaunchKernelGGL((k_col<C, false>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, 0LL);
+        hipLaunchKernelGGL((k_col<C, 0>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, 0LL, 0);
     }
     return hipGetLastError();
 }

@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
         if (d.out_mode == OFX_OUT_COMPLEX) {
             float2 v = s[k];
             v.y *= a.out_sign;
-            if (d.ft_len > 0) v = cmul(v, ft_twiddle(ftab, (tlo + d.ft_base) * (long long)k, d.ft_len));
+            if (d.ft_len > 0) v = cmul(v, ft_twiddle(ftab, ((tlo + d.ft_base) * (long long)k) % d.ft_len, d.ft_len));
             static_cast<float2*>(a.out)[base + k * a.ostride] = make_float2(v.x * a.scale, v.y * a.scale);
         } else if (d.out_mode == OFX_OUT_HALF) {
             const float w = (k == 0) ? a.scale : a.scale * ((a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 2.f : 1.f);

@@ -11,8 +11,9 @@ namespace ofx {
 constexpr int kFtLoBits = 11;  // B = 2048
 constexpr long long kFtLo = 1LL << kFtLoBits;
 
+// e must lie in [0, len) (the four-step products column * row index do).
 __device__ __forceinline__ float2 ft_twiddle(const double2* __restrict__ t, long long e, long long len) {
-    e %= len;
+    (void)len;
     const long long lo = e & (kFtLo - 1);
     const long long hi = e >> kFtLoBits;
     const double2 a = t[lo];

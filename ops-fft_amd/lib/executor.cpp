@@ -285,11 +285,80 @@ struct Builder {
         return best < 1e30;
     }
 
+    // A three-factor split n = n1 * n2 * n3 with compiled column kernels for n1 and n2 and a
+    // transposed-store row kernel for n3, minimising the largest factor.
+    bool templated_split3(long long n, long long* n1, long long* n2, long long* n3) {
+        if (env_int("ACLFFT_DISABLE_TEMPLATED", 0) != 0) return false;
+        long long best = 0;
+        for (const ofx_kernel_info& a : col_registry()) {
+            if (n % a.n) continue;
+            for (const ofx_kernel_info& b : col_registry()) {
+                if ((n / a.n) % b.n) continue;
+                const long long c = n / a.n / b.n;
+                if (kernels_for(OFX_KIND_ROWT, c, true).empty()) continue;
+                const long long worst = std::max<long long>(std::max(a.n, b.n), c);
+                if (best == 0 || worst < best) {
+                    best = worst;
+                    *n1 = a.n;
+                    *n2 = b.n;
+                    *n3 = c;
+                }
+            }
+        }
+        return best > 0;
+    }
+
+    // Three-level four-step, n = N1*N2*N3, index n = n1*N2*N3 + n2*N3 + n3, k = k1 + N1*k2 + N1*N2*k3:
+    //   A: columns over n1 (stride N2*N3), twiddle W_n^(k1*(n2*N3 + n3)), src -> dst (in place safe)
+    //   B: per (b, k1), columns over n2 (stride N3), twiddle W_(N2*N3)^(k2*n3), stored to the
+    //      workspace as [k2][b][k1][n3] so that the rows of step C that land next to each other
+    //      in X are next to each other in the workspace
+    //   C: rows over n3, transposed store X[b*n + k1 + N1*k2 + N1*N2*k3].
+    aclfftResult c2c_fourstep3(long long n, long long batch, long long N1, long long N2, long long N3, int src,
+                               int dst, bool conj_in, bool conj_out) {
+        const int scratch = (dst == BUF_WORK0) ? BUF_WORK1 : BUF_WORK0;
+        aclfftResult r = c2c_columns(N1, N2 * N3, batch, src, dst, conj_in, false);
+        if (r != ACLFFT_SUCCESS) return r;
+        e->steps_.back().args.ft_len = n;
+        e->steps_.back().aux_table = fourstep_table(n);
+        r = c2c_columns(N2, N3, batch * N1, dst, scratch, false, false);
+        if (r != ACLFFT_SUCCESS) return r;
+        Step& b = e->steps_.back();
+        b.args.odist = N3;
+        b.args.ostride = batch * N1 * N3;
+        b.args.ft_len = N2 * N3;
+        b.aux_table = fourstep_table(N2 * N3);
+        Step c = kernel_step(OFX_KIND_ROWT, N3, kernels_for(OFX_KIND_ROWT, N3, true));
+        c.src = scratch;
+        c.dst = dst;
+        c.args.batch = batch * N1 * N2;
+        c.args.idist = N3;
+        c.args.cols = batch * N1;
+        c.args.cols_lo = N1;
+        c.args.odist = N1;
+        c.args.odist_hi = n;
+        c.args.ostride = N1 * N2;
+        c.conj_out = conj_out;
+        e->steps_.push_back(c);
+        size_t& wb = (scratch == BUF_WORK0) ? e->work_bytes_ : e->work1_bytes_;
+        wb = std::max(wb, (size_t)(n * batch * 8));
+        return ACLFFT_SUCCESS;
+    }
+
     // Four-step: x[n1*N2 + n2] -> columns (length N1, twiddle W_N^(n2 k1)) -> workspace
-    // tmp[k1*N2 + n2] -> rows (length N2) -> X[k1 + N1*k2].
+    // tmp[k1*N2 + n2] -> rows (length N2) -> X[k1 + N1*k2].  Two factors when both fit the compiled
+    // kernels' best configurations (<= 1024: 16 columns or rows per workgroup, 128-byte HBM runs),
+    // else three factors; ACLFFT_FOURSTEP_LEVELS=2|3 forces a form where it exists.
     aclfftResult c2c_fourstep(long long n, long long batch, int src, int dst, bool conj_in, bool conj_out) {
-        long long T1 = 0, T2 = 0;
-        if (templated_split(n, &T1, &T2)) {
+        long long T1 = 0, T2 = 0, U1 = 0, U2 = 0, U3 = 0;
+        const bool two = templated_split(n, &T1, &T2);
+        const bool three = templated_split3(n, &U1, &U2, &U3);
+        const int force = env_int("ACLFFT_FOURSTEP_LEVELS", 0);
+        bool use3 = three && (!two || std::max(T1, T2) > 1024);
+        if (force == 2 && two) use3 = false;
+        if (force == 3 && three) use3 = true;
+        if (use3) return c2c_fourstep3(n, batch, U1, U2, U3, src, dst, conj_in, conj_out);
+        if (two) {
             // column step: N2 columns of length N1 per transform, four-step twiddle fused in the store
             aclfftResult r = c2c_columns(T1, T2, batch, src, BUF_WORK0, conj_in, false);
             if (r != ACLFFT_SUCCESS) return r;

@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 LARGE = [
     (1 << 14, 3), (1 << 15, 3), (1 << 16, 2), (1 << 17, 2), (1 << 18, 2), (1 << 20, 2),
-    (1 << 22, 1), (1 << 24, 1), (3 << 14, 2), (1000000, 1), (46200, 3),
+    (1 << 21, 2), (1 << 22, 1), (1 << 24, 1), (3 << 14, 2), (1000000, 1), (46200, 3),
 ]
 
 
@@ -48,3 +48,59 @@ def test_large_c2c_in_place(cuda, n):
     p.destroy()
     err = oracle.rel_l2(xt.cpu().numpy(), oracle.c2c(x.astype(np.complex128)))
     assert err <= fp32_tol(n), err
+
+
+@pytest.mark.parametrize("n,batch,levels", [(1 << 19, 3, 3), (1 << 20, 2, 3), (1 << 22, 1, 2), (1 << 25, 1, 0)])
+def test_fourstep_forms(cuda, monkeypatch, n, batch, levels):
+    """The two- and three-factor four-step forms forced where both exist (ACLFFT_FOURSTEP_LEVELS),
+    with batch > 1 for the three-factor form's [k2][b][k1][n3] workspace layout."""
+    import torch
+    monkeypatch.setenv("ACLFFT_FOURSTEP_LEVELS", str(levels))
+    x = crandn((batch, n), seed=levels + n % 97).astype(np.complex64)
+    p = aclfft.Plan.plan1d(n, aclfft.C2C, batch)
+    desc = p.describe()
+    if levels:
+        assert desc.count("fourstep-twiddle") == levels - 1, desc
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    p.exec_c2c(xt, yt, aclfft.BACKWARD)
+    torch.cuda.synchronize()
+    p.destroy()
+    err = oracle.rel_l2(yt.cpu().numpy(), oracle.c2c(x.astype(np.complex128), aclfft.BACKWARD))
+    print(desc, "rel_l2 %.3e" % err)
+    assert err <= fp32_tol(n), err
+
+
+def test_max_length_properties(cuda):
+    """n = 2^27, the largest horizontal length a plan accepts.  The oracle takes minutes here, so
+    the check is size-independent: Parseval, the forward/backward round trip (backward is
+    unnormalised: x -> n x), and a few bins against their defining sums in FP64."""
+    import torch
+    n = 1 << 27
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.randn(n, dtype=torch.complex64, device="cuda", generator=g)
+    p = aclfft.Plan.plan1d(n, aclfft.C2C, 1)
+    X = torch.empty_like(x)
+    p.exec_c2c(x, X, aclfft.FORWARD)
+    torch.cuda.synchronize()
+    ex = float((x.abs().double() ** 2).sum())
+    eX = float((X.abs().double() ** 2).sum())
+    assert abs(eX / (n * ex) - 1.0) < 1e-5
+    y = torch.empty_like(x)
+    p.exec_c2c(X, y, aclfft.BACKWARD)
+    torch.cuda.synchronize()
+    rt = float(torch.linalg.vector_norm((y / n - x).to(torch.complex128)) / torch.linalg.vector_norm(x.to(torch.complex128)))
+    print("2^27 round trip rel_l2 %.3e" % rt)
+    assert rt <= 2 * fp32_tol(n), rt
+    # spot bins: X[k] = sum_j x[j] exp(-2 pi i j k / n), exact integer phase (j*k mod n)
+    xd = x.to(torch.complex128)
+    j = torch.arange(n, dtype=torch.int64, device="cuda")
+    for k in (0, 1, 12345, n // 2 + 7, n - 1):
+        ph = (j * k) % n
+        w = torch.polar(torch.ones(n, dtype=torch.float64, device="cuda"), ph.double() * (-2.0 * np.pi / n))
+        ref = complex((xd * w).sum())
+        got = complex(X[k].to(torch.complex128).item())
+        err = abs(got - ref) / np.sqrt(n)
+        print("bin %d err/sqrt(n) %.3e" % (k, err))
+        assert err < 1e-4, (k, got, ref)
+    p.destroy()
