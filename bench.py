@@ -76,10 +76,22 @@ def cpu_golden(x_cpu, torch):
                 kind="torch.fft (reference golden path)", sample="full workload, median of 3, %.3f s" % t)
 
 
-def time_steps(torch, fn, steps, warmup, stream):
+def warm(torch, fn, warmup, min_seconds=0.1):
+    """`warmup` untimed calls, continued until at least `min_seconds` of device time has passed:
+    the clocks settle over tens of milliseconds of sustained load, so a few calls of a
+    sub-millisecond workload are not enough."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < min_seconds:
+        for _ in range(10):
+            fn()
+        torch.cuda.synchronize()
+
+
+def time_steps(torch, fn, steps, warmup, stream):
+    warm(torch, fn, warmup)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
@@ -118,8 +130,7 @@ def extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y, steps, stream, e32):
 
 def extra_real(torch, aclfft, oracle, np, steps, stream):
     out = {}
-    for n in (4096, 3000):
-        b = 8192
+    for n, b in ((4096, 8192), (3000, 8192), (8192, 1024)):
         g = torch.Generator().manual_seed(1)
         xr = torch.randn(b, n, generator=g).cuda()
         X = torch.empty(b, n // 2 + 1, dtype=torch.complex64, device="cuda")
@@ -152,7 +163,7 @@ def extra_2d(torch, aclfft, oracle, np, steps, stream):
     p = aclfft.Plan.plan2d(B, NX, NY, aclfft.C2C)
     p.set_stream(stream)
     ms = time_steps(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
-    err = oracle.rel_l2(y[:1].cpu().numpy(), np.fft.fft2(x[:1].cpu().numpy().astype(np.complex128)))
+    err = oracle.rel_l2(y[:1].cpu().numpy(), oracle.c2c_2d(x[:1].cpu().numpy().astype(np.complex128)))
     pts = B * NX * NY
     out = {"ms": round(ms, 5), "GFLOP/s": round(5.0 * pts * math.log2(NX * NY) / ms / 1e6, 1),
            "hbm_frac_compulsory": round(16.0 * pts / ms / 1e6 / HBM_PEAK_GBS, 4),
@@ -163,8 +174,8 @@ def extra_2d(torch, aclfft, oracle, np, steps, stream):
 
 
 def extra_batched(torch, aclfft, oracle, np, steps, stream):
-    """512 C2C transforms, lengths log-uniform over the set, batches U{1..64}; persistent kernel +
-    hipGraph in one plan vs one plan (and launch) per group."""
+    """512 C2C transforms, lengths log-uniform over the set, batches U{1..64}; one plan (persistent
+    kernel, static schedule, replayed from a hipGraph) vs one plan (and launch) per group."""
     lengths = (60, 128, 243, 1000, 1024, 2048, 4096, 6561)
     rng = np.random.default_rng(4)
     logs = np.log(np.array(lengths, float))
@@ -212,7 +223,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-extras", action="store_true", help="only the headline C2C configuration")
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16x2", "bf16x3"],

@@ -45,9 +45,12 @@ enum ofx_kind {
     OFX_KIND_COL = 3,      /* strided "column" transforms (2D second pass, four-step)      */
     OFX_KIND_R2C_ODD = 4,  /* odd n: full complex FFT of real input, first n/2+1 bins      */
     OFX_KIND_C2R_ODD = 5,  /* odd n: Hermitian expansion + full complex FFT, real part     */
-    OFX_KIND_ROWT = 6      /* contiguous rows in, transposed out: transform t -> out +
+    OFX_KIND_ROWT = 6,     /* contiguous rows in, transposed out: transform t -> out +
                               (t / cols)*odist + rhi*odist_hi + rlo + k*ostride with
                               t % cols = rhi*cols_lo + rlo (four-step row step)          */
+    OFX_KIND_C2C_TILES = 7 /* C2C over a tile list (uneven batches): workgroup b runs
+                              tiles[b] = (offset, count <= TPB) contiguous transforms at
+                              element offset `offset` of in and out; batch = tile count  */
 };
 
 /* Description of one compiled kernel configuration. */
@@ -88,6 +91,7 @@ typedef struct ofx_args {
     long long cols_lo;     /* ROWT: splits the row index r = t % cols into r / cols_lo (stride
                               odist_hi in `out`) and r % cols_lo (stride 1); 0 = no split    */
     long long odist_hi;
+    const void* tiles;     /* C2C_TILES: ofx_batch_tile array of `batch` entries (device)    */
 } ofx_args;
 
 /* Runtime-radix ("generic") launches: any length whose radix sequence is listed, any layout.
@@ -188,11 +192,18 @@ typedef struct ofx_batch_args {
     const ofx_batch_tile* tiles;    /* device array, processed in order of the work counter   */
     int ntiles;
     float sign;                     /* +1 forward, -1 backward (conjugation on load/store)    */
-    unsigned int* counter;          /* device work counter (reset by the launch)              */
+    unsigned int* counter;          /* device work counter (reset by the launch; dynamic only)  */
+    int schedule;                   /* 0 dynamic: after its first tile (blockIdx), a workgroup
+                                       takes the next index from the counter, fetched one tile
+                                       ahead; 1 static: the k-th tile of workgroup b is entry
+                                       k*blocks + (k even ? b : blocks-1-b) of the list (the list
+                                       is sorted by cost, so the rounds alternate direction),
+                                       its descriptor loaded one tile ahead; no counter        */
 } ofx_batch_args;
 OFX_API int ofx_batch_class_count(void);
 OFX_API int ofx_batch_class_info_get(int cls, ofx_batch_class_info* info);
-/* Resets the counter (one-thread kernel) and launches `blocks` persistent workgroups. */
+/* Launches `blocks` persistent workgroups (dynamic schedule: after a one-thread kernel that
+ * resets the counter). */
 OFX_API int ofx_launch_batch_persistent(const ofx_batch_args* args, int blocks, hipStream_t stream);
 
 /* R2C: multiply bins 1..n/2-1 by 2 (adjoint of C2R).  C2R: multiply input bins 1..n/2-1 by 1/2

@@ -35,6 +35,19 @@ def _is_device(a):
     return a is not None and not isinstance(a, np.ndarray) and a.is_cuda
 
 
+def _nbytes(a):
+    if isinstance(a, np.ndarray):
+        return a.nbytes
+    return a.numel() * a.element_size()
+
+
+def _check_size(a, need, what):
+    """The C API takes raw pointers and trusts their extent; the Python layer knows the buffer
+    sizes and refuses one smaller than the plan reads or writes (a kernel would run past it)."""
+    if a is not None and _nbytes(a) < need:
+        raise ValueError("%s holds %d bytes, the plan needs %d" % (what, _nbytes(a), need))
+
+
 class Plan:
     def __init__(self, handle, kind, dims, batch, vertical=False):
         self._h = handle
@@ -124,11 +137,17 @@ class Plan:
         check(lib.aclfftExtSetScale(self._h, float(scale)), "aclfftExtSetScale")
 
     def exec_r2c_adjoint(self, grad_out, grad_in):
+        r, c = self._sizes()
+        _check_size(grad_out, 8 * c, "grad_out")
+        _check_size(grad_in, 4 * r, "grad_in")
         self._bind_stream(grad_out)
         check(lib.aclfftExtExecR2CAdjoint(self._h, _ptr(grad_out), _ptr(grad_in)), "aclfftExtExecR2CAdjoint")
         return grad_in
 
     def exec_c2r_adjoint(self, grad_out, grad_in):
+        r, c = self._sizes()
+        _check_size(grad_out, 4 * r, "grad_out")
+        _check_size(grad_in, 8 * c, "grad_in")
         self._bind_stream(grad_out)
         check(lib.aclfftExtExecC2RAdjoint(self._h, _ptr(grad_out), _ptr(grad_in)), "aclfftExtExecC2RAdjoint")
         return grad_in
@@ -141,17 +160,38 @@ class Plan:
         check(lib.aclfftExtSetPrecision(self._h, mode), "aclfftExtSetPrecision")
 
     # ---- execution ----------------------------------------------------------------------------
+    def _sizes(self):
+        """(real samples, complex bins) of the whole plan: the extents of the real and complex
+        sides of R2C/C2R, and of both sides of C2C (complex)."""
+        if getattr(self, "groups", None):
+            pts = sum(n * b for n, b in self.groups)
+            return pts, pts
+        full = int(np.prod(self.dims)) * self.batch
+        if self.type == C2C:
+            return full, full
+        nx = self.dims[-1]
+        return full, (nx // 2 + 1) * int(np.prod(self.dims[:-1])) * self.batch
+
     def exec_c2c(self, idata, odata, direction=FORWARD):
+        n, _ = self._sizes()
+        _check_size(idata, 8 * n, "idata")
+        _check_size(odata, 8 * n, "odata")
         self._bind_stream(idata)
         check(lib.aclfftExecC2C(self._h, _ptr(idata), _ptr(odata), direction), "aclfftExecC2C")
         return odata
 
     def exec_r2c(self, idata, odata):
+        r, c = self._sizes()
+        _check_size(idata, 4 * r, "idata")
+        _check_size(odata, 8 * c, "odata")
         self._bind_stream(idata)
         check(lib.aclfftExecR2C(self._h, _ptr(idata), _ptr(odata)), "aclfftExecR2C")
         return odata
 
     def exec_c2r(self, idata, odata):
+        r, c = self._sizes()
+        _check_size(idata, 8 * c, "idata")
+        _check_size(odata, 4 * r, "odata")
         self._bind_stream(idata)
         check(lib.aclfftExecC2R(self._h, _ptr(idata), _ptr(odata)), "aclfftExecC2R")
         return odata

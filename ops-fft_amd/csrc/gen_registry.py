@@ -10,7 +10,8 @@ Run: python3 gen_registry.py  (rewrites registry_1d.inc next to this script)
 import os
 
 KINDS = {"C2C": "OFX_KIND_C2C", "R2C": "OFX_KIND_R2C", "C2R": "OFX_KIND_C2R",
-         "R2C_ODD": "OFX_KIND_R2C_ODD", "C2R_ODD": "OFX_KIND_C2R_ODD", "ROWT": "OFX_KIND_ROWT"}
+         "R2C_ODD": "OFX_KIND_R2C_ODD", "C2R_ODD": "OFX_KIND_C2R_ODD", "ROWT": "OFX_KIND_ROWT",
+         "C2C_TILES": "OFX_KIND_C2C_TILES"}
 
 # Radix plans for the lengths that get a dedicated (fully unrolled) kernel.  Everything else runs
 # on the runtime-radix kernel (kernels_generic.hip).
@@ -65,10 +66,39 @@ PAIRED = [
 ]
 
 
+# further single-access (IOV = 1) configurations: (kind, N, T, TPB, radices)
+EXTRA = [
+    ("C2C", 2048, 64, 1, (16, 16, 8)), ("C2C", 2048, 64, 2, (16, 16, 8)), ("C2C", 2048, 256, 1, (8, 8, 8, 4)),
+    ("R2C", 2048, 64, 1, (16, 16, 8)), ("R2C", 2048, 64, 2, (16, 16, 8)), ("R2C", 2048, 128, 1, (16, 16, 8)),
+    ("C2R", 2048, 64, 1, (16, 16, 8)), ("C2R", 2048, 64, 2, (16, 16, 8)), ("C2R", 2048, 128, 1, (16, 16, 8)),
+    ("R2C", 1500, 64, 2, (12, 5, 25)), ("R2C", 1500, 64, 1, (12, 5, 25)),
+    ("C2R", 1500, 128, 2, (10, 10, 15)), ("C2R", 1500, 64, 2, (12, 5, 25)), ("C2R", 1500, 64, 1, (12, 5, 25)),
+    ("R2C", 4096, 128, 1, (16, 16, 16)), ("C2R", 4096, 128, 1, (16, 16, 16)),
+    ("C2R", 4096, 512, 1, (8, 8, 8, 8)),
+]
+
+
+# tile-list C2C kernels of the uneven-batch lengths (one launch per length): (N, T, TPB, radices)
+TILES = [
+    (60, 4, 64, (4, 3, 5)), (128, 8, 32, (8, 16)), (243, 16, 16, (27, 9)), (256, 16, 16, (16, 16)),
+    (512, 32, 8, (8, 8, 8)), (1000, 128, 2, (10, 10, 10)), (1024, 64, 2, (8, 16, 8)), (1024, 128, 2, (8, 8, 16)),
+    (2048, 256, 1, (8, 8, 8, 4)), (2048, 128, 2, (16, 16, 8)), (4096, 256, 1, (16, 16, 16)),
+    (4096, 512, 1, (8, 8, 8, 8)), (6561, 256, 1, (27, 27, 9)),
+]
+
+
+def padsh(n):
+    """LDS pad shift: odd lengths have no power-of-two strides, so they go unpadded."""
+    return 30 if n % 2 else 4
+
+
 # configurations that lead their length's candidate list (measured fastest on MI355X; the planner
 # takes the first candidate unless autotuning or a cached choice says otherwise)
 FIRST = [
     ("C2C", 1024, 64, 2, (8, 16, 8), 2),
+    ("R2C", 2048, 256, 1, (8, 8, 8, 4), 1), ("C2R", 2048, 256, 1, (8, 8, 8, 4), 1),
+    ("R2C", 1500, 128, 2, (10, 10, 15), 1),
+    ("R2C", 4096, 512, 1, (8, 8, 8, 8), 1),
 ]
 
 
@@ -81,7 +111,7 @@ def lines():
                 t = 32
             tpb = tpb_for(n, t)
             for staged in ((False, True) if pi == 0 else (False,)):
-                out.append(("C2C", n, t, tpb, 4, staged, rad))
+                out.append(("C2C", n, t, tpb, padsh(n), staged, rad))
     # four-step row step (rows in, transposed out): as many rows per workgroup as LDS allows,
     # so that the transposed stores run over >= 32-byte segments
     for n in (128, 256, 512, 1024, 2048, 4096):
@@ -111,16 +141,27 @@ def main():
         f.write("// generated by gen_registry.py - do not edit\n")
         f.write("static const Entry kRegistry[] = {\n")
         for kind, n, t, tpb, rad, iov in FIRST:
-            f.write("    OFX_EV(%s, %d, %d, %d, 4, %s),\n" % (KINDS[kind], n, t, tpb, ", ".join(str(r) for r in rad)))
-        for kind, n, t, tpb, padsh, staged, rad in rows:
+            assert t * tpb <= 1024
+            if iov == 2:
+                f.write("    OFX_EV(%s, %d, %d, %d, 4, %s),\n" % (KINDS[kind], n, t, tpb, ", ".join(str(r) for r in rad)))
+            else:
+                f.write("    OFX_E(%s, %d, %d, %d, 4, false, %s),\n" % (KINDS[kind], n, t, tpb, ", ".join(str(r) for r in rad)))
+        for kind, n, t, tpb, psh, staged, rad in rows:
             assert t * tpb <= 1024
             f.write("    OFX_E(%s, %d, %d, %d, %d, %s, %s),\n" % (
-                KINDS[kind], n, t, tpb, padsh, "true" if staged else "false",
+                KINDS[kind], n, t, tpb, psh, "true" if staged else "false",
                 ", ".join(str(r) for r in rad)))
+        for n, t, tpb, rad in TILES:
+            assert t * tpb <= 1024
+            f.write("    OFX_E(%s, %d, %d, %d, %d, false, %s),\n" % (KINDS["C2C_TILES"], n, t, tpb, padsh(n),
+                                                                 ", ".join(str(r) for r in rad)))
+        for kind, n, t, tpb, rad in EXTRA:
+            assert t * tpb <= 1024
+            f.write("    OFX_E(%s, %d, %d, %d, 4, false, %s),\n" % (KINDS[kind], n, t, tpb, ", ".join(str(r) for r in rad)))
         for kind, n, t, tpb, rad in PAIRED:
             f.write("    OFX_EV(%s, %d, %d, %d, 4, %s),\n" % (KINDS[kind], n, t, tpb, ", ".join(str(r) for r in rad)))
         f.write("};\n")
-    print("wrote %d registry entries" % (len(FIRST) + len(rows) + len(PAIRED)))
+    print("wrote %d registry entries" % (len(FIRST) + len(rows) + len(TILES) + len(EXTRA) + len(PAIRED)))
 
 
 if __name__ == "__main__":

@@ -375,6 +375,26 @@ __global__ __launch_bounds__(C::BLOCK) void k_rowt(ofx_args a) {
     }
 }
 
+// C2C over a tile list: workgroup b runs tiles[b], `count` (<= TPB) contiguous transforms starting
+// at complex offset `offset` of both in and out (the groups of an uneven batch, one launch per
+// length).  Offsets may be odd, so only single-element (8-byte) I/O configurations are built.
+template <class C>
+__global__ __launch_bounds__(C::BLOCK) void k_c2c_tiles(ofx_args a) {
+    static_assert(C::IOV == 1 && !C::STAGED, "tile offsets carry no 16-byte alignment");
+    __shared__ float2 lds[C::TPB * C::LDS_N];
+    const int tl = threadIdx.x / C::T;
+    const int tid = threadIdx.x - tl * C::T;
+    const ofx_batch_tile tile = static_cast<const ofx_batch_tile*>(a.tiles)[blockIdx.x];
+    const bool ok = tl < tile.count;
+    if constexpr (C::WAVE_SYNC) {
+        if (!ok) return;  // whole transforms per wave: no block-wide barrier follows
+    }
+    const long long base = tile.offset + (long long)(ok ? tl : 0) * C::N;
+    GLoadC gin{static_cast<const float2*>(a.in) + base, a.in_sign};
+    GStoreC gout{static_cast<float2*>(a.out) + base, a.out_sign, a.scale, ok};
+    stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, static_cast<const float2*>(a.tw), gin, gout);
+}
+
 // ---------------------------------------------------------------------------------------------
 // launchers + registry
 // ---------------------------------------------------------------------------------------------
@@ -383,7 +403,7 @@ typedef hipError_t (*launch_fn)(const ofx_args*, hipStream_t);
 template <class C, int KIND>
 hipError_t launch(const ofx_args* a, hipStream_t s) {
     if (a->batch <= 0) return hipSuccess;
-    const long long blocks = (a->batch + C::TPB - 1) / C::TPB;
+    const long long blocks = KIND == OFX_KIND_C2C_TILES ? a->batch : (a->batch + C::TPB - 1) / C::TPB;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
     if constexpr (C::IOV == 2 || (C::STAGED && (C::N % 2) == 0)) {
         // 16-byte accesses: every row must start 16-byte aligned
@@ -401,6 +421,10 @@ hipError_t launch(const ofx_args* a, hipStream_t s) {
         hipLaunchKernelGGL(k_r2c_odd<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a);
     if constexpr (KIND == OFX_KIND_C2R_ODD)
         hipLaunchKernelGGL(k_c2r_odd<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a);
+    if constexpr (KIND == OFX_KIND_C2C_TILES) {
+        if (a->tiles == nullptr) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_c2c_tiles<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a);
+    }
     if constexpr (KIND == OFX_KIND_ROWT) {
         if (a->cols <= 0) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_rowt<C>, dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a);

@@ -3,9 +3,10 @@
 // The plan (aclfftExtPlanManyUneven) holds groups g of b_g transforms of length n_g, stored one
 // group after the other.  At plan time the groups are cut into tiles (a tile = the transforms one
 // workgroup of the length's class processes together) and the tiles are sorted by decreasing cost
-// (longest-processing-time order).  The kernel is launched with about two workgroups per CU; every
-// workgroup repeatedly takes the next tile index from a device counter (atomicAdd), runs the
-// Stockham body of that tile's length class, and exits when the queue is drained.  Each length
+// (longest-processing-time order).  The kernel is launched with a few workgroups per CU; every
+// workgroup repeatedly takes its next tile (from a device counter, or from a static round-robin
+// over the sorted list), runs the Stockham body of that tile's length class, and exits when the
+// queue is drained.  Each length
 // class is a compiled StockhamCfg with 256 threads per workgroup, so one launch serves them all;
 // the LDS buffer is sized for the largest class.  The counter is reset by a one-thread kernel
 // launched just before (no memset node, so the pair can be captured in a hipGraph and replayed).
@@ -21,12 +22,12 @@ constexpr int kBatchBlock = 256;
 // length classes (all with T * TPB = 256)
 typedef StockhamCfg<60, 4, 64, 4, false, Radices<4, 3, 5>> BC60;
 typedef StockhamCfg<128, 8, 32, 4, false, Radices<8, 16>> BC128;
-typedef StockhamCfg<243, 16, 16, 4, false, Radices<27, 9>> BC243;
+typedef StockhamCfg<243, 16, 16, 30, false, Radices<27, 9>> BC243;  // odd: no LDS pad needed
 typedef StockhamCfg<1000, 64, 4, 4, false, Radices<10, 10, 10>> BC1000;
 typedef StockhamCfg<1024, 64, 4, 4, false, Radices<8, 8, 16>> BC1024;
 typedef StockhamCfg<2048, 128, 2, 4, false, Radices<16, 16, 8>> BC2048;
 typedef StockhamCfg<4096, 256, 1, 4, false, Radices<16, 16, 16>> BC4096;
-typedef StockhamCfg<6561, 256, 1, 4, false, Radices<27, 27, 9>> BC6561;
+typedef StockhamCfg<6561, 256, 1, 30, false, Radices<27, 27, 9>> BC6561;
 typedef StockhamCfg<512, 32, 8, 4, false, Radices<8, 8, 8>> BC512;
 typedef StockhamCfg<256, 16, 16, 4, false, Radices<16, 16>> BC256;
 
@@ -66,11 +67,21 @@ struct GStoreB {
 // One tile of class C: transforms [0, tile.count) starting at element offset tile.offset.  Classes
 // whose transforms fit in a wave skip empty slots; the others synchronise the whole workgroup
 // inside the passes, so every thread runs them and only the stores of empty slots are dropped.
+//
+// Register budget: the class bodies are inlined into the kernel's dispatch switch, so the kernel
+// needs the largest class's registers, not the sum.  Two things keep it that way: the lane index
+// is re-read through an opaque move at the top of each body (otherwise the compiler hoists every
+// class's per-lane address arithmetic above the switch, where it all stays live: ~200 VGPRs), and
+// the loop has a memory barrier per tile (twiddle loads are not hoisted out of the loop).  Calls
+// instead of inlining cost more: call arguments travel in VGPRs and the callee-saved register
+// ranges of the AMDGPU calling convention inflate each body's footprint.
 template <class C>
-__device__ __attribute__((noinline)) void run_tile(float2* lds, const ofx_batch_tile& tile, const float2* __restrict__ in,
+__device__ __forceinline__ void run_tile(float2* lds, const ofx_batch_tile& tile, const float2* __restrict__ in,
                                          float2* __restrict__ out, const float2* __restrict__ tw, float sg) {
-    const int tl = threadIdx.x / C::T;
-    const int tid = threadIdx.x - tl * C::T;
+    int tx;
+    __asm__ volatile("v_mov_b32 %0, %1" : "=v"(tx) : "v"((int)threadIdx.x));
+    const int tl = tx / C::T;
+    const int tid = tx - tl * C::T;
     const bool ok = tl < tile.count;
     if constexpr (C::WAVE_SYNC) {
         if (!ok) return;
@@ -83,32 +94,62 @@ __device__ __attribute__((noinline)) void run_tile(float2* lds, const ofx_batch_
 
 __global__ __launch_bounds__(kBatchBlock) void k_batch_reset(unsigned int* counter) { *counter = 0u; }
 
-__global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args a) {
-    __shared__ float2 lds[kBatchLds];
-    __shared__ int s_tile;
+__device__ __forceinline__ void run_class(float2* lds, const ofx_batch_tile& tile, const ofx_batch_args& a) {
     const float2* __restrict__ in = static_cast<const float2*>(a.in);
     float2* __restrict__ out = static_cast<float2*>(a.out);
-    for (;;) {
-        if (threadIdx.x == 0) s_tile = (int)atomicAdd(a.counter, 1u);
-        __syncthreads();
-        const int t = s_tile;
-        if (t >= a.ntiles) break;  // uniform across the workgroup: every wave leaves here
-        const ofx_batch_tile tile = a.tiles[t];
-        const float2* tw = static_cast<const float2*>(a.tw) + a.tw_off[tile.cls];
-        switch (tile.cls) {
-            case OFX_BCLASS_60: run_tile<BC60>(lds, tile, in, out, tw, a.sign); break;
-            case OFX_BCLASS_128: run_tile<BC128>(lds, tile, in, out, tw, a.sign); break;
-            case OFX_BCLASS_243: run_tile<BC243>(lds, tile, in, out, tw, a.sign); break;
-            case OFX_BCLASS_256: run_tile<BC256>(lds, tile, in, out, tw, a.sign); break;
-            case OFX_BCLASS_512: run_tile<BC512>(lds, tile, in, out, tw, a.sign); break;
-            case OFX_BCLASS_1000: run_tile<BC1000>(lds, tile, in, out, tw, a.sign); break;
-            case OFX_BCLASS_1024: run_tile<BC1024>(lds, tile, in, out, tw, a.sign); break;
-            case OFX_BCLASS_2048: run_tile<BC2048>(lds, tile, in, out, tw, a.sign); break;
-            case OFX_BCLASS_4096: run_tile<BC4096>(lds, tile, in, out, tw, a.sign); break;
-            case OFX_BCLASS_6561: run_tile<BC6561>(lds, tile, in, out, tw, a.sign); break;
-            default: break;
+    const float2* tw = static_cast<const float2*>(a.tw) + a.tw_off[tile.cls];
+    switch (tile.cls) {
+        case OFX_BCLASS_60: run_tile<BC60>(lds, tile, in, out, tw, a.sign); break;
+        case OFX_BCLASS_128: run_tile<BC128>(lds, tile, in, out, tw, a.sign); break;
+        case OFX_BCLASS_243: run_tile<BC243>(lds, tile, in, out, tw, a.sign); break;
+        case OFX_BCLASS_256: run_tile<BC256>(lds, tile, in, out, tw, a.sign); break;
+        case OFX_BCLASS_512: run_tile<BC512>(lds, tile, in, out, tw, a.sign); break;
+        case OFX_BCLASS_1000: run_tile<BC1000>(lds, tile, in, out, tw, a.sign); break;
+        case OFX_BCLASS_1024: run_tile<BC1024>(lds, tile, in, out, tw, a.sign); break;
+        case OFX_BCLASS_2048: run_tile<BC2048>(lds, tile, in, out, tw, a.sign); break;
+        case OFX_BCLASS_4096: run_tile<BC4096>(lds, tile, in, out, tw, a.sign); break;
+        case OFX_BCLASS_6561: run_tile<BC6561>(lds, tile, in, out, tw, a.sign); break;
+        default: break;
+    }
+}
+
+// Every loop below is uniform over the workgroup (tile indices come from blockIdx, the tile list
+// or LDS), so every wave leaves together once the index passes ntiles.
+template <int SCHED>
+__global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args a) {
+    __shared__ float2 lds[kBatchLds];
+    __shared__ int s_next[2];
+    const int G = (int)gridDim.x;
+    const int b = (int)blockIdx.x;
+    if constexpr (SCHED == 1) {
+        int t = b;
+        ofx_batch_tile cur{};
+        if (t < a.ntiles) cur = a.tiles[t];
+        for (int k = 1; t < a.ntiles; ++k) {
+            const int tn = k * G + ((k & 1) ? G - 1 - b : b);
+            ofx_batch_tile nxt{};
+            if (tn < a.ntiles) nxt = a.tiles[tn];  // in flight while this tile runs
+            __asm__ volatile("" ::: "memory");      // no load of the tile bodies moves out of the loop
+            run_class(lds, cur, a);
+            __syncthreads();  // LDS is reused by the next tile
+            cur = nxt;
+            t = tn;
         }
-        __syncthreads();  // LDS and s_tile are reused by the next tile
+    } else {
+        int t = b;
+        int parity = 0;
+        while (t < a.ntiles) {
+            unsigned int nx = 0;
+            if (threadIdx.x == 0) nx = atomicAdd(a.counter, 1u);  // the next index, used after this tile
+            const ofx_batch_tile cur = a.tiles[t];
+            __asm__ volatile("" ::: "memory");
+            run_class(lds, cur, a);
+            // two slots: a wave may still read this tile's slot while thread 0 writes the next one
+            if (threadIdx.x == 0) s_next[parity] = (int)nx + G;
+            __syncthreads();  // publishes s_next; LDS is reused by the next tile
+            t = s_next[parity];
+            parity ^= 1;
+        }
     }
 }
 
@@ -146,11 +187,17 @@ extern "C" int ofx_batch_class_info_get(int cls, ofx_batch_class_info* c) {
 }
 
 extern "C" int ofx_launch_batch_persistent(const ofx_batch_args* a, int blocks, hipStream_t stream) {
-    if (a == nullptr || a->counter == nullptr || a->tiles == nullptr || blocks <= 0) return (int)hipErrorInvalidValue;
+    if (a == nullptr || a->tiles == nullptr || blocks <= 0 || (a->schedule == 0 && a->counter == nullptr))
+        return (int)hipErrorInvalidValue;
     if (a->ntiles <= 0) return 0;
+    if (a->schedule == 1) {
+        hipLaunchKernelGGL(k_batch_persistent<1>, dim3(blocks), dim3(kBatchBlock), 0, stream, *a);
+        return (int)hipGetLastError();
+    }
+    if (a->schedule != 0) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, stream, a->counter);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL(k_batch_persistent, dim3(blocks), dim3(kBatchBlock), 0, stream, *a);
+    hipLaunchKernelGGL(k_batch_persistent<0>, dim3(blocks), dim3(kBatchBlock), 0, stream, *a);
     return (int)hipGetLastError();
 }

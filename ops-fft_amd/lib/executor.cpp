@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <sstream>
 
 #include "tune_cache.h"
@@ -504,9 +505,106 @@ struct Builder {
         return ACLFFT_SUCCESS;
     }
 
-    // Groups of (length, batch) stored one after the other.  Lengths with a compiled batch class
-    // become tiles of the persistent kernel (one launch); any other length runs as its own step.
+    // Uneven batch, one launch per length: every length with a compiled tile-list kernel gets one
+    // step over the tiles (<= TPB contiguous transforms of one group) of all its groups; any other
+    // length runs as ordinary steps per group.  Each length runs with its own kernel's register
+    // and LDS budget, which a single persistent kernel has to size for its largest class.
+    aclfftResult uneven_grouped(const std::vector<int>& len, const std::vector<int>& bat) {
+        std::vector<long long> offs(len.size());
+        long long off = 0;
+        for (size_t g = 0; g < len.size(); ++g) {
+            offs[g] = off;
+            off += (long long)len[g] * bat[g];
+        }
+        e->in_bytes_ = e->out_bytes_ = (size_t)off * 8;
+        // distinct lengths, largest total work first
+        std::map<int, double> work;
+        for (size_t g = 0; g < len.size(); ++g) work[len[g]] += (double)len[g] * bat[g];
+        std::vector<int> lens;
+        for (auto& kv : work) lens.push_back(kv.first);
+        std::stable_sort(lens.begin(), lens.end(), [&](int a, int b) { return work[a] > work[b]; });
+        std::vector<double> lane_load;
+        for (int n : lens) {
+            auto ids = kernels_for(OFX_KIND_C2C_TILES, n, true);
+            if (ids.empty()) {
+                for (size_t g = 0; g < len.size(); ++g) {
+                    if (len[g] != n) continue;
+                    const size_t first = e->steps_.size();
+                    aclfftResult r = c2c_rows(n, bat[g], BUF_IN, BUF_OUT, true, true);
+                    if (r != ACLFFT_SUCCESS) return r;
+                    for (size_t k = first; k < e->steps_.size(); ++k) {
+                        if (e->steps_[k].src == BUF_IN || e->steps_[k].src == BUF_OUT)
+                            e->steps_[k].src_off = (size_t)offs[g] * 8;
+                        if (e->steps_[k].dst == BUF_OUT) e->steps_[k].dst_off = (size_t)offs[g] * 8;
+                    }
+                }
+                continue;
+            }
+            Step s = kernel_step(OFX_KIND_C2C_TILES, n, ids);
+            // the tile list is cut for one TPB: only candidates with that TPB stay tunable
+            const int tpb = registry()[s.kernel].transforms_per_block;
+            std::vector<int> cand, tabs;
+            for (size_t c = 0; c < s.candidates.size(); ++c)
+                if (registry()[s.candidates[c]].transforms_per_block == tpb) {
+                    cand.push_back(s.candidates[c]);
+                    tabs.push_back(s.candidate_tables[c]);
+                }
+            s.candidates = cand;
+            s.candidate_tables = tabs;
+            std::vector<ofx_batch_tile> tiles;
+            for (size_t g = 0; g < len.size(); ++g) {
+                if (len[g] != n) continue;
+                for (long long i = 0; i < bat[g]; i += tpb) {
+                    ofx_batch_tile t;
+                    t.cls = 0;
+                    t.count = (int)std::min<long long>(tpb, bat[g] - i);
+                    t.offset = offs[g] + i * n;
+                    tiles.push_back(t);
+                }
+            }
+            TableSpec tt;
+            tt.kind = TableSpec::BLOB;
+            tt.bytes = tiles.size() * sizeof(ofx_batch_tile);
+            tt.blob.resize(tt.bytes);
+            std::memcpy(tt.blob.data(), tiles.data(), tt.bytes);
+            s.tiles_table = e->add_table(tt);
+            s.args.batch = (long long)tiles.size();
+            s.conj_in = s.conj_out = true;
+            // longest-processing-time assignment of lengths to the side-stream lanes
+            const int lanes = std::max(0, std::min(8, env_int("ACLFFT_UNEVEN_LANES", 4)));
+            if (lanes > 1) {
+                lane_load.resize(lanes, 0.0);
+                const size_t k = std::min_element(lane_load.begin(), lane_load.end()) - lane_load.begin();
+                lane_load[k] += work[n];
+                s.lane = (int)k + 1;
+            }
+            e->steps_.push_back(s);
+        }
+        // steps of lengths without a tile kernel (lane 0) first, so that the lanes fork once
+        std::stable_partition(e->steps_.begin(), e->steps_.end(), [](const Step& a) { return a.lane == 0; });
+        return ACLFFT_SUCCESS;
+    }
+
+    // Groups of (length, batch) stored one after the other.  ACLFFT_UNEVEN_MODE=1 selects the
+    // persistent kernel (below), 2 the per-length launches (uneven_grouped).  By default the
+    // per-length launches are used when every length carries at least 2^22 points (each launch
+    // then amortises its ramp-up and tail, and runs with its own kernel's register and LDS
+    // budget), the persistent kernel otherwise (many lengths with little work each).
     aclfftResult uneven(const std::vector<int>& len, const std::vector<int>& bat) {
+        int mode = env_int("ACLFFT_UNEVEN_MODE", 0);
+        if (mode != 1 && mode != 2) {
+            std::map<int, long long> pts;
+            for (size_t g = 0; g < len.size(); ++g) pts[len[g]] += (long long)len[g] * bat[g];
+            long long least = -1;
+            for (auto& kv : pts) least = least < 0 ? kv.second : std::min(least, kv.second);
+            mode = least >= (1LL << 22) ? 2 : 1;
+        }
+        return mode == 2 ? uneven_grouped(len, bat) : uneven_persistent(len, bat);
+    }
+
+    // Lengths with a compiled batch class become tiles of the persistent kernel (one launch); any
+    // other length runs as its own step.
+    aclfftResult uneven_persistent(const std::vector<int>& len, const std::vector<int>& bat) {
         struct ClassInfo {
             ofx_batch_class_info info;
             bool used = false;
@@ -709,7 +807,9 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
     }
     if (d.uneven) {
         r = (d.type == ACLFFT_C2C) ? b.uneven(d.ulen, d.ubatch) : ACLFFT_NOT_SUPPORTED;
-        if (r == ACLFFT_SUCCESS) e->use_graphs_ = true;
+        // replaying a graph costs a few microseconds per launch on the device: worth it only when
+        // it replaces several launches (e.g. the ordinary steps of lengths without a class)
+        if (r == ACLFFT_SUCCESS) e->use_graphs_ = e->steps_.size() > 3;
     } else if (d.rank == 1 && !d.vertical) {
         const long long n = d.n0;
         switch (d.type) {
@@ -791,6 +891,8 @@ Executor::~Executor() {
     for (GraphEntry& g : graphs_) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(g.exec));
     delete adjoint_;
     if (capture_stream_) (void)hipStreamDestroy(capture_stream_);
+    for (hipStream_t ls : lane_streams_) (void)hipStreamDestroy(ls);
+    for (hipEvent_t le : lane_events_) (void)hipEventDestroy(le);
     if (cur >= 0 && device_ >= 0 && cur != device_) (void)hipSetDevice(cur);
 }
 
@@ -800,6 +902,7 @@ std::string Executor::describe() const {
     for (size_t i = 0; i < steps_.size(); ++i) {
         const Step& s = steps_[i];
         if (i) os << " | ";
+        if (s.lane > 0) os << "[lane " << s.lane << "] ";
         if (s.type == Step::COPY) {
             os << "copy";
             continue;
@@ -885,6 +988,20 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
         }
     }
     if (hip_log(hipStreamSynchronize(s), "hipStreamSynchronize (plan setup)") != hipSuccess) return ACLFFT_SETUP_FAILED;
+    int lanes = 0;
+    for (const Step& st : steps_) lanes = std::max(lanes, st.lane);
+    for (int k = 0; k < lanes; ++k) {
+        hipStream_t ls = nullptr;
+        if (hip_log(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking), "hipStreamCreate (lane)") != hipSuccess)
+            return ACLFFT_SETUP_FAILED;
+        lane_streams_.push_back(ls);
+    }
+    for (int k = 0; lanes > 0 && k <= lanes; ++k) {
+        hipEvent_t ev = nullptr;
+        if (hip_log(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate (lane)") != hipSuccess)
+            return ACLFFT_SETUP_FAILED;
+        lane_events_.push_back(ev);
+    }
     {
         int cus = 0;
         if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
@@ -909,10 +1026,39 @@ aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream
     auto table = [&](int idx) -> const void* {
         return idx >= 0 ? static_cast<const void*>(static_cast<char*>(d_tables_) + tables_[idx].offset) : nullptr;
     };
+    // fork / join of the side-stream lanes: the side streams wait for everything issued so far on
+    // `s`, and `s` waits for every side stream before the next lane-0 step and at the end (under
+    // stream capture these become the parallel branches of the graph)
+    const hipStream_t s0 = s;
+    bool forked = false;
+    auto fork = [&]() -> bool {
+        if (hip_log(hipEventRecord(lane_events_[0], s0), "hipEventRecord (fork)") != hipSuccess) return false;
+        for (hipStream_t ls : lane_streams_)
+            if (hip_log(hipStreamWaitEvent(ls, lane_events_[0], 0), "hipStreamWaitEvent (fork)") != hipSuccess)
+                return false;
+        return true;
+    };
+    auto join = [&]() -> bool {
+        for (size_t k = 0; k < lane_streams_.size(); ++k) {
+            if (hip_log(hipEventRecord(lane_events_[k + 1], lane_streams_[k]), "hipEventRecord (join)") != hipSuccess ||
+                hip_log(hipStreamWaitEvent(s0, lane_events_[k + 1], 0), "hipStreamWaitEvent (join)") != hipSuccess)
+                return false;
+        }
+        return true;
+    };
     for (const Step& st : steps_) {
         const char* src = static_cast<const char*>(bufs[st.src]) + st.src_off;
         char* dst = static_cast<char*>(bufs[st.dst]) + st.dst_off;
         hipError_t err = hipSuccess;
+        if (st.lane > 0 && (size_t)st.lane <= lane_streams_.size()) {
+            if (!forked && !fork()) return ACLFFT_EXEC_FAILED;
+            forked = true;
+            s = lane_streams_[st.lane - 1];
+        } else {
+            if (forked && !join()) return ACLFFT_EXEC_FAILED;
+            forked = false;
+            s = s0;
+        }
         switch (st.type) {
             case Step::COPY:
                 if (src != dst) err = hipMemcpyAsync(dst, src, st.copy_bytes, hipMemcpyDeviceToDevice, s);
@@ -945,7 +1091,8 @@ aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream
                 b.tiles = static_cast<const ofx_batch_tile*>(table(st.tiles_table));
                 b.counter = static_cast<unsigned int*>(const_cast<void*>(table(st.counter_table)));
                 b.sign = dsign;
-                const int per_cu = std::max(1, env_int("ACLFFT_PERSISTENT_BLOCKS_PER_CU", 2));
+                b.schedule = env_int("ACLFFT_PERSISTENT_SCHEDULE", 1) == 0 ? 0 : 1;
+                const int per_cu = std::max(1, env_int("ACLFFT_PERSISTENT_BLOCKS_PER_CU", 3));
                 err = (hipError_t)ofx_launch_batch_persistent(&b, num_cus_ * per_cu, s);
                 break;
             }
@@ -990,14 +1137,19 @@ aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream
                 a.out = dst;
                 a.tw = table(twt);
                 a.tw_aux = table(st.aux_table);
+                if (st.tiles_table >= 0) a.tiles = table(st.tiles_table);
                 if (st.conj_in) a.in_sign *= dsign;
                 if (st.conj_out) a.out_sign *= dsign;
                 err = (hipError_t)ofx_launch(kid, &a, s);
                 break;
             }
         }
-        if (hip_log(err, "kernel launch") != hipSuccess) return ACLFFT_EXEC_FAILED;
+        if (hip_log(err, "kernel launch") != hipSuccess) {
+            if (forked) (void)join();
+            return ACLFFT_EXEC_FAILED;
+        }
     }
+    if (forked && !join()) return ACLFFT_EXEC_FAILED;
     return ACLFFT_SUCCESS;
 }
 
@@ -1123,6 +1275,7 @@ aclfftResult Executor::autotune(hipStream_t s) {
             const int tt = st.candidate_tables[ci];
             args.tw = tt >= 0 ? static_cast<char*>(d_tables_) + tables_[tt].offset : nullptr;
             args.tw_aux = st.aux_table >= 0 ? static_cast<char*>(d_tables_) + tables_[st.aux_table].offset : nullptr;
+            if (st.tiles_table >= 0) args.tiles = static_cast<char*>(d_tables_) + tables_[st.tiles_table].offset;
             bool ok = launch_cand(st.candidates[ci], &args) == 0;  // warm-up
             (void)hipEventRecord(e0, s);
             for (int r = 0; ok && r < 5; ++r) ok = launch_cand(st.candidates[ci], &args) == 0;

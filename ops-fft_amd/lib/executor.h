@@ -15,6 +15,8 @@
 #include "cann_ops_fft.h"
 #include "ofx_launch.h"
 
+struct ihipEvent_t;  // hipEvent_t, without pulling the HIP runtime headers into the plan layer
+
 namespace ofx {
 
 enum BufId : int { BUF_IN = 0, BUF_OUT = 1, BUF_WORK0 = 2, BUF_WORK1 = 3 };
@@ -49,6 +51,9 @@ struct Step {
     ofx_batch_args b{};            // PERSISTENT: launch arguments (pointers filled at execution)
     int class_tables[OFX_BCLASS_COUNT] = {0};  // PERSISTENT: pass table of each class (-1 none)
     int tiles_table = -1, counter_table = -1;
+    int lane = 0;                  // 0: the caller's stream; k > 0: forked onto the plan's k-th
+                                   // side stream (independent steps run concurrently; joined
+                                   // back before the next lane-0 step and at the end)
     std::vector<int> candidates;   // registry ids that can replace `kernel` (autotuning)
     std::vector<int> candidate_tables;  // pass-twiddle table of each candidate
 };
@@ -119,6 +124,8 @@ private:
     bool use_graphs_ = false;
     std::vector<GraphEntry> graphs_;
     hipStream_t capture_stream_ = nullptr;
+    std::vector<hipStream_t> lane_streams_;  // side streams of steps with lane > 0
+    std::vector<ihipEvent_t*> lane_events_;  // hipEvent_t: [0] = fork, [k] = join of lane k
     Executor* adjoint_ = nullptr;
     aclfftResult run_graph(const void* din, void* dout, int direction, hipStream_t s);
 

@@ -8,6 +8,8 @@ Expected codes come from the reference's own negative tests:
   lib/fft_exec_api.cpp:32-116, lib/fft_plan_api.cpp:24-150 (exec/plan checks)
 """
 import ctypes
+
+import numpy as np
 import os
 import re
 import subprocess
@@ -241,3 +243,35 @@ def test_uneven_plan_argument_gates():
     assert lib.aclfftExtPlanManyUneven(ctypes.byref(h), 2, L, Bz, aclfft.C2C) == aclfft.INVALID_SIZE
     assert lib.aclfftExtPlanManyUneven(ctypes.byref(h), 2, L, B, aclfft.C2C) == aclfft.SUCCESS
     assert lib.aclfftDestroy(h) == aclfft.SUCCESS
+
+
+def test_python_plan_refuses_short_buffers():
+    """The C entry points trust the caller's buffer extents; the Python layer checks them before
+    any device work (no GPU needed: the check fires before the library call)."""
+    c64 = np.complex64
+    p = aclfft.Plan.plan1d(1024, aclfft.C2C, 4)
+    with pytest.raises(ValueError):
+        p.exec_c2c(np.zeros(4 * 1024 - 1, c64), np.zeros(4 * 1024, c64))
+    with pytest.raises(ValueError):
+        p.exec_c2c(np.zeros(4 * 1024, c64), np.zeros(4 * 1024 - 1, c64))
+    p.destroy()
+    p = aclfft.Plan.plan1d(4096, aclfft.R2C, 3)
+    with pytest.raises(ValueError):
+        p.exec_r2c(np.zeros(3 * 4096 - 1, np.float32), np.zeros(3 * 2049, c64))
+    with pytest.raises(ValueError):
+        p.exec_r2c(np.zeros(3 * 4096, np.float32), np.zeros(3 * 2049 - 1, c64))
+    p.destroy()
+    p = aclfft.Plan.plan1d(3000, aclfft.C2R, 2)
+    with pytest.raises(ValueError):
+        p.exec_c2r(np.zeros(2 * 1501 - 1, c64), np.zeros(2 * 3000, np.float32))
+    with pytest.raises(ValueError):
+        p.exec_c2r_adjoint(np.zeros(2 * 3000 - 1, np.float32), np.zeros(2 * 1501, c64))
+    p.destroy()
+    p = aclfft.Plan.plan2d(2, 64, 32, aclfft.C2C)
+    with pytest.raises(ValueError):
+        p.exec_c2c(np.zeros(2 * 64 * 32 - 1, c64), np.zeros(2 * 64 * 32, c64))
+    p.destroy()
+    p = aclfft.Plan.plan_many_uneven([60, 1000], [3, 2])
+    with pytest.raises(ValueError):
+        p.exec_c2c(np.zeros(60 * 3 + 1000 * 2 - 1, c64), np.zeros(2180, c64))
+    p.destroy()

@@ -1,4 +1,5 @@
-"""GPU: uneven-length batched plans (aclfftExtPlanManyUneven) - persistent kernel + hipGraph."""
+"""GPU: uneven-length batched plans (aclfftExtPlanManyUneven) - per-length tile launches (default)
+and the persistent kernel (ACLFFT_UNEVEN_MODE=1), each replayed from a hipGraph."""
 import numpy as np
 import pytest
 
@@ -32,9 +33,16 @@ def check_groups(out, x, groups, direction):
         off += n * b
 
 
+# (ACLFFT_UNEVEN_MODE, ACLFFT_PERSISTENT_SCHEDULE): persistent dynamic, persistent static, per-length
+FORMS = [(1, 0), (1, 1), (2, 1)]
+
+
+@pytest.mark.parametrize("mode,sched", FORMS)
 @pytest.mark.parametrize("graph", [True, False])
-def test_uneven_plan_vs_oracle(cuda, graph):
+def test_uneven_plan_vs_oracle(cuda, monkeypatch, graph, mode, sched):
     import torch
+    monkeypatch.setenv("ACLFFT_UNEVEN_MODE", str(mode))
+    monkeypatch.setenv("ACLFFT_PERSISTENT_SCHEDULE", str(sched))
     ln, bt = uneven_workload(40, 7)
     groups = list(zip(ln, bt))
     total = sum(n * b for n, b in groups)
@@ -55,9 +63,12 @@ def test_uneven_plan_vs_oracle(cuda, graph):
     p.destroy()
 
 
-def test_uneven_plan_mixed_with_generic_lengths(cuda):
+@pytest.mark.parametrize("mode,sched", FORMS)
+def test_uneven_plan_mixed_with_generic_lengths(cuda, monkeypatch, mode, sched):
     """Lengths without a compiled class run as extra launches inside the same plan/graph."""
     import torch
+    monkeypatch.setenv("ACLFFT_UNEVEN_MODE", str(mode))
+    monkeypatch.setenv("ACLFFT_PERSISTENT_SCHEDULE", str(sched))
     groups = [(60, 3), (420, 5), (1024, 7), (46200, 1), (7, 9), (6561, 2)]
     ln = [g[0] for g in groups]
     bt = [g[1] for g in groups]
@@ -69,4 +80,60 @@ def test_uneven_plan_mixed_with_generic_lengths(cuda):
     p.exec_c2c(xt, xt, aclfft.FORWARD)  # in place
     torch.cuda.synchronize()
     check_groups(xt.cpu().numpy(), x, groups, aclfft.FORWARD)
+    p.destroy()
+
+
+def _tile_variants():
+    seen, out = {}, []
+    for k in aclfft.kernel_registry():
+        if k["kind"] != 7:
+            continue
+        idx = seen.get(k["n"], 0)
+        seen[k["n"]] = idx + 1
+        out.append((k["n"], idx))
+    return out
+
+
+@pytest.mark.parametrize("n,variant", _tile_variants())
+def test_every_tile_kernel(cuda, monkeypatch, n, variant):
+    """Each compiled tile-list kernel, forced with ACLFFT_KERNEL_VARIANT, on groups whose batches
+    leave partial tiles and odd element offsets."""
+    import torch
+    monkeypatch.setenv("ACLFFT_UNEVEN_MODE", "2")
+    monkeypatch.setenv("ACLFFT_KERNEL_VARIANT", str(variant))
+    groups = [(n, 3), (63, 1), (n, 5), (n, 1)]
+    ln = [g[0] for g in groups]
+    bt = [g[1] for g in groups]
+    total = sum(a * b for a, b in groups)
+    x = crandn(total, n + variant).astype(np.complex64)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    p = aclfft.Plan.plan_many_uneven(ln, bt)
+    d = p.describe()
+    assert "kind7 n=%d" % n in d, d
+    p.exec_c2c(xt, yt, aclfft.BACKWARD)
+    torch.cuda.synchronize()
+    print(d)
+    check_groups(yt.cpu().numpy(), x, groups, aclfft.BACKWARD)
+    p.destroy()
+
+
+@pytest.mark.parametrize("sched", [0, 1])
+@pytest.mark.parametrize("blocks_per_cu", [1, 3, 40])
+def test_persistent_grid_sizes(cuda, monkeypatch, sched, blocks_per_cu):
+    """More workgroups than tiles, and far fewer: every tile is taken exactly once either way."""
+    import torch
+    monkeypatch.setenv("ACLFFT_UNEVEN_MODE", "1")
+    monkeypatch.setenv("ACLFFT_PERSISTENT_SCHEDULE", str(sched))
+    monkeypatch.setenv("ACLFFT_PERSISTENT_BLOCKS_PER_CU", str(blocks_per_cu))
+    ln, bt = uneven_workload(24, 11)
+    groups = list(zip(ln, bt))
+    total = sum(n * b for n, b in groups)
+    x = crandn(total, 5).astype(np.complex64)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    p = aclfft.Plan.plan_many_uneven(ln, bt)
+    p.exec_c2c(xt, yt, aclfft.FORWARD)
+    torch.cuda.synchronize()
+    check_groups(yt.cpu().numpy(), x, groups, aclfft.FORWARD)
     p.destroy()
