@@ -116,14 +116,21 @@ ColEntry col_entry() {
 
 static const ColEntry kCols[] = {
     OFX_COL(64, 8, 16, 1, 8, 8),
+    OFX_COL(64, 8, 32, 1, 8, 8),
     OFX_COL(128, 8, 16, 1, 8, 16),
+    OFX_COL(128, 8, 32, 1, 8, 16),
     OFX_COL(256, 16, 16, 1, 16, 16),
+    OFX_COL(256, 16, 32, 1, 16, 16),
     OFX_COL(512, 32, 16, 1, 8, 8, 8),
+    OFX_COL(512, 32, 8, 1, 8, 8, 8),
     OFX_COL(1024, 64, 16, 1, 16, 16, 4),
     OFX_COL(1024, 64, 8, 1, 16, 16, 4),
+    OFX_COL(1024, 128, 8, 1, 8, 8, 16),
     OFX_COL(2048, 128, 8, 1, 16, 16, 8),
     OFX_COL(2048, 128, 4, 1, 16, 16, 8),
+    OFX_COL(2048, 256, 4, 1, 8, 8, 8, 4),
     OFX_COL(4096, 256, 4, 0, 16, 16, 16),
+    OFX_COL(4096, 256, 2, 1, 16, 16, 16),
 };
 static const int kNumCols = (int)(sizeof(kCols) / sizeof(kCols[0]));
 

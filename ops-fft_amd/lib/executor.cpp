@@ -360,11 +360,18 @@ struct Builder {
         if (force == 3 && three) use3 = true;
         if (use3) return c2c_fourstep3(n, batch, U1, U2, U3, src, dst, conj_in, conj_out);
         if (two) {
+            // The workspace rows are P = N2 + pad elements apart: with a power-of-two pitch every
+            // row of a column group starts on the same HBM channel set.  Measured on MI355X: a
+            // 16-element pad gains ~2% for N2 >= 512 and loses 1-2% for shorter rows
+            // (ACLFFT_FOURSTEP_PAD overrides it).
+            const long long P = T2 + std::max(0, env_int("ACLFFT_FOURSTEP_PAD", T2 >= 512 ? 16 : 0));
             // column step: N2 columns of length N1 per transform, four-step twiddle fused in the store
             aclfftResult r = c2c_columns(T1, T2, batch, src, BUF_WORK0, conj_in, false);
             if (r != ACLFFT_SUCCESS) return r;
             Step& a = e->steps_.back();
             a.args.ft_len = n;
+            a.args.odist = T1 * P;
+            a.args.ostride = P;
             a.aux_table = fourstep_table(n);
             // row step: rows of length N2, transposed store X[k1 + N1*k2]
             auto ids = kernels_for(OFX_KIND_ROWT, T2, true);
@@ -372,13 +379,13 @@ struct Builder {
             b.src = BUF_WORK0;
             b.dst = dst;
             b.args.batch = batch * T1;
-            b.args.idist = T2;
+            b.args.idist = P;
             b.args.cols = T1;
             b.args.odist = n;
             b.args.ostride = T1;
             b.conj_out = conj_out;
             e->steps_.push_back(b);
-            e->work_bytes_ = std::max(e->work_bytes_, (size_t)(n * batch * 8));
+            e->work_bytes_ = std::max(e->work_bytes_, (size_t)(batch * T1 * P * 8));
             return ACLFFT_SUCCESS;
         }
         long long N1 = 0, N2 = 0;

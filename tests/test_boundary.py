@@ -123,7 +123,8 @@ def test_makeplan_reports_workspace():
     h = H()
     lib.aclfftCreate(ctypes.byref(h))
     assert lib.aclfftMakePlan1d(h, 1 << 20, aclfft.C2C, 2, 0, ctypes.byref(ws)) == aclfft.SUCCESS
-    assert ws.value == (1 << 20) * 2 * 8  # four-step intermediate
+    # four-step intermediate: 1024 rows of 1024 + 16 (row pitch padded off a power of two) per transform
+    assert ws.value == 1024 * (1024 + 16) * 2 * 8
     lib.aclfftDestroy(h)
 
 
