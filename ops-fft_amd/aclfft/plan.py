@@ -127,6 +127,23 @@ class Plan:
     def handle(self):
         return self._h
 
+    # ---- workspace (reference: aclfftGetSize / aclfftSetAutoAllocation / aclfftSetWorkArea) ------
+    def get_size(self):
+        """Device workspace of one execution, in bytes (0 when the plan needs none)."""
+        ws = ctypes.c_size_t(0)
+        check(lib.aclfftGetSize(self._h, ctypes.byref(ws)), "aclfftGetSize")
+        return ws.value
+
+    def set_auto_allocation(self, on):
+        check(lib.aclfftSetAutoAllocation(self._h, 1 if on else 0), "aclfftSetAutoAllocation")
+
+    def set_work_area(self, area):
+        """area: a device tensor of at least get_size() bytes, kept alive by the caller while the
+        plan uses it."""
+        if _nbytes(area) < self.get_size():
+            raise ValueError("work area holds %d bytes, the plan needs %d" % (_nbytes(area), self.get_size()))
+        check(lib.aclfftSetWorkArea(self._h, ctypes.c_void_p(_ptr(area))), "aclfftSetWorkArea")
+
     # ---- library extensions (include/ops_fft_ext.h) --------------------------------------------
     def describe(self):
         buf = ctypes.create_string_buffer(1024)

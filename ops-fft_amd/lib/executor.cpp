@@ -881,7 +881,12 @@ Executor* Executor::adjoint() {
         d.type = (desc_.type == ACLFFT_R2C) ? ACLFFT_C2R : ACLFFT_R2C;
         d.hermitian_weight = true;
         Executor* e = nullptr;
-        if (build(d, &e) == ACLFFT_SUCCESS) adjoint_ = e;
+        if (build(d, &e) == ACLFFT_SUCCESS) {
+            adjoint_ = e;
+            // a caller-managed workspace is shared with the adjoint (same stream, never concurrent)
+            e->auto_alloc_ = auto_alloc_;
+            if (user_work_ != nullptr && e->work_bytes() <= work_bytes()) (void)e->set_work_area(user_work_);
+        }
     }
     return adjoint_;
 }
@@ -890,17 +895,51 @@ Executor::~Executor() {
     int cur = -1;
     if (device_ >= 0 && hipGetDevice(&cur) == hipSuccess && cur != device_) (void)hipSetDevice(device_);
     if (d_tables_) (void)hipFree(d_tables_);
-    if (d_work_) (void)hipFree(d_work_);
-    if (d_work1_) (void)hipFree(d_work1_);
+    if (owns_work_ && d_work_) (void)hipFree(d_work_);
+    if (owns_work_ && d_work1_) (void)hipFree(d_work1_);
     if (d_stage_in_) (void)hipFree(d_stage_in_);
     if (d_stage_out_) (void)hipFree(d_stage_out_);
-    if (d_inplace_) (void)hipFree(d_inplace_);
+    if (owns_work_ && d_inplace_) (void)hipFree(d_inplace_);
     for (GraphEntry& g : graphs_) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(g.exec));
     delete adjoint_;
     if (capture_stream_) (void)hipStreamDestroy(capture_stream_);
     for (hipStream_t ls : lane_streams_) (void)hipStreamDestroy(ls);
     for (hipEvent_t le : lane_events_) (void)hipEventDestroy(le);
     if (cur >= 0 && device_ >= 0 && cur != device_) (void)hipSetDevice(cur);
+}
+
+size_t Executor::work_bytes() const {
+    size_t b = align256(work_bytes_) + align256(work1_bytes_);
+    if (in_place_unsafe_) b += align256(in_bytes_);
+    return b;
+}
+
+void Executor::inherit_workspace(const Executor& old) {
+    auto_alloc_ = old.auto_alloc_;
+    if (old.user_work_ != nullptr && work_bytes() <= old.work_bytes()) (void)set_work_area(old.user_work_);
+}
+
+void Executor::drop_graphs() {
+    for (GraphEntry& g : graphs_) (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(g.exec));
+    graphs_.clear();
+}
+
+aclfftResult Executor::set_work_area(void* p) {
+    if (p == nullptr || !is_device_pointer(p)) return ACLFFT_INVALID_VALUE;
+    if (owns_work_) {
+        // plan-allocated buffers (if any) are released; hipFree waits for work using them
+        if (d_work_) (void)hipFree(d_work_);
+        if (d_work1_) (void)hipFree(d_work1_);
+        if (d_inplace_) (void)hipFree(d_inplace_);
+    }
+    char* base = static_cast<char*>(p);
+    user_work_ = base;
+    d_work_ = work_bytes_ > 0 ? base : nullptr;
+    d_work1_ = work1_bytes_ > 0 ? base + align256(work_bytes_) : nullptr;
+    d_inplace_ = in_place_unsafe_ ? base + align256(work_bytes_) + align256(work1_bytes_) : nullptr;
+    owns_work_ = false;
+    drop_graphs();  // captured launches hold the old addresses
+    return ACLFFT_SUCCESS;
 }
 
 std::string Executor::describe() const {
@@ -964,15 +1003,17 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
     int dev = 0;
     if (hip_log(hipGetDevice(&dev), "hipGetDevice") != hipSuccess) return ACLFFT_SETUP_FAILED;
     if (device_ >= 0) return dev == device_ ? ACLFFT_SUCCESS : ACLFFT_INVALID_DEVICE;
+    if (owns_work_ && !auto_alloc_ && (work_bytes_ > 0 || work1_bytes_ > 0))
+        return ACLFFT_NO_WORKSPACE;  // caller-managed workspace not set yet
     if (table_bytes_ > 0 && hipMalloc(&d_tables_, table_bytes_) != hipSuccess) {
         (void)hipGetLastError();
         return ACLFFT_ALLOC_FAILED;
     }
-    if (work_bytes_ > 0 && hipMalloc(&d_work_, work_bytes_) != hipSuccess) {
+    if (owns_work_ && work_bytes_ > 0 && hipMalloc(&d_work_, work_bytes_) != hipSuccess) {
         (void)hipGetLastError();
         return ACLFFT_ALLOC_FAILED;
     }
-    if (work1_bytes_ > 0 && hipMalloc(&d_work1_, work1_bytes_) != hipSuccess) {
+    if (owns_work_ && work1_bytes_ > 0 && hipMalloc(&d_work1_, work1_bytes_) != hipSuccess) {
         (void)hipGetLastError();
         return ACLFFT_ALLOC_FAILED;
     }
@@ -1168,6 +1209,7 @@ aclfftResult Executor::exec(const void* in, void* out, int direction, hipStream_
     if (din && dout) {
         if (in == out && in_place_unsafe_) {
             // the kernel's output layout overlaps other transforms' inputs: read from a copy
+            if (d_inplace_ == nullptr && !auto_alloc_) return ACLFFT_NO_WORKSPACE;
             if (d_inplace_ == nullptr && hipMalloc(&d_inplace_, in_bytes_) != hipSuccess) {
                 (void)hipGetLastError();
                 return ACLFFT_ALLOC_FAILED;

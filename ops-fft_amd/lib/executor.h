@@ -81,7 +81,16 @@ public:
     // C2C; ignored for R2C/C2R.
     aclfftResult exec(const void* in, void* out, int direction, hipStream_t stream);
 
-    size_t work_bytes() const { return work_bytes_ + work1_bytes_; }
+    // Device workspace of one execution: the four-step buffers, then (real plans) the copy an
+    // in-place execution on device memory reads from; 256-byte aligned parts.
+    size_t work_bytes() const;
+    // Caller-managed workspace (aclfftSetWorkArea / aclfftSetAutoAllocation).  With automatic
+    // allocation off and no work area, an execution that needs workspace returns NO_WORKSPACE.
+    aclfftResult set_work_area(void* p);
+    void set_auto_allocation(bool on) { auto_alloc_ = on; }
+    // a rebuilt plan (new precision or scale) keeps the caller's workspace settings; the work
+    // area carries over when it is large enough for the new launch list
+    void inherit_workspace(const Executor& old);
     size_t in_bytes() const { return in_bytes_; }
     size_t out_bytes() const { return out_bytes_; }
     const std::vector<Step>& steps() const { return steps_; }
@@ -112,6 +121,10 @@ private:
     void* d_stage_out_ = nullptr;
     void* d_inplace_ = nullptr;
     bool tuned_ = false;
+    bool auto_alloc_ = true;      // the plan allocates its workspace on first use
+    bool owns_work_ = true;       // d_work_ / d_work1_ / d_inplace_ were allocated here
+    char* user_work_ = nullptr;   // the caller's work area, when set
+    void drop_graphs();
     int num_cus_ = 256;
 
     // captured launch sequences, keyed by (in, out, direction)

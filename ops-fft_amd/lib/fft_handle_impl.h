@@ -29,6 +29,8 @@ struct aclfftHandle_t {
     ofx::Executor* exec = nullptr;
     // arithmetic selected with aclfftExtSetPrecision (0 = FP32 Stockham)
     int32_t precision = 0;
+    // aclfftSetAutoAllocation: 0 = the caller provides the workspace with aclfftSetWorkArea
+    int32_t auto_allocate = 1;
 
     aclfftHandle_t() = default;
     ~aclfftHandle_t();

@@ -42,6 +42,7 @@ aclfftResult attach_executor(aclfftHandle_t* p, const ofx::PlanDesc& d, size_t* 
         return ACLFFT_SUCCESS;
     }
     if (r != ACLFFT_SUCCESS) return r;
+    e->set_auto_allocation(p->auto_allocate != 0);
     p->exec = e;
     if (workSize != nullptr) *workSize = e->work_bytes();
     return ACLFFT_SUCCESS;

@@ -174,12 +174,14 @@ def test_plan3d_and_placeholders():
     assert lib.aclfftPlan3d(ctypes.byref(h), 0, 8, 8, aclfft.C2C) == aclfft.INVALID_SIZE
     assert lib.aclfftPlan3d(ctypes.byref(h), 8, 8, 8, 0x11) == aclfft.INVALID_TYPE
     ws = ctypes.c_size_t()
-    assert lib.aclfftGetSize1d(None, 8, aclfft.C2C, 1, ctypes.byref(ws)) == aclfft.NOT_IMPLEMENTED
-    assert lib.aclfftGetSize2d(None, 8, 8, aclfft.C2C, ctypes.byref(ws)) == aclfft.NOT_IMPLEMENTED
+    # the workspace API is implemented (a superset of the reference's placeholders); without a
+    # plan handle it reports INVALID_PLAN, and 3D stays unimplemented
+    assert lib.aclfftGetSize1d(None, 8, aclfft.C2C, 1, ctypes.byref(ws)) == aclfft.INVALID_PLAN
+    assert lib.aclfftGetSize2d(None, 8, 8, aclfft.C2C, ctypes.byref(ws)) == aclfft.INVALID_PLAN
     assert lib.aclfftGetSize3d(None, 8, 8, 8, aclfft.C2C, ctypes.byref(ws)) == aclfft.NOT_IMPLEMENTED
-    assert lib.aclfftGetSize(None, ctypes.byref(ws)) == aclfft.NOT_IMPLEMENTED
-    assert lib.aclfftSetAutoAllocation(None, 1) == aclfft.NOT_IMPLEMENTED
-    assert lib.aclfftSetWorkArea(None, None) == aclfft.NOT_IMPLEMENTED
+    assert lib.aclfftGetSize(None, ctypes.byref(ws)) == aclfft.INVALID_PLAN
+    assert lib.aclfftSetAutoAllocation(None, 1) == aclfft.INVALID_PLAN
+    assert lib.aclfftSetWorkArea(None, None) == aclfft.INVALID_PLAN
     assert lib.aclfftSetStride(None, 4) == aclfft.NOT_IMPLEMENTED
     v = ctypes.c_int()
     assert lib.aclfftGetVersion(ctypes.byref(v)) == aclfft.NOT_IMPLEMENTED
@@ -276,3 +278,44 @@ def test_python_plan_refuses_short_buffers():
     with pytest.raises(ValueError):
         p.exec_c2c(np.zeros(60 * 3 + 1000 * 2 - 1, c64), np.zeros(2180, c64))
     p.destroy()
+
+
+def test_workspace_api_host_side():
+    """aclfftGetSize* agree with MakePlan's workSize; argument checks follow MakePlan's codes;
+    a work area must be device memory (checked before it is used)."""
+    H = ctypes.c_void_p
+    h = H()
+    assert lib.aclfftCreate(ctypes.byref(h)) == aclfft.SUCCESS
+    ws = ctypes.c_size_t(123)
+    for n, t, b in ((1024, aclfft.C2C, 8), (1 << 20, aclfft.C2C, 2), (1 << 22, aclfft.C2C, 1),
+                    (3000, aclfft.R2C, 4), (1 << 21, aclfft.R2C, 1), (1 << 21, aclfft.C2R, 3)):
+        assert lib.aclfftGetSize1d(h, n, t, b, ctypes.byref(ws)) == aclfft.SUCCESS
+        est = ws.value
+        h2 = H()
+        assert lib.aclfftCreate(ctypes.byref(h2)) == aclfft.SUCCESS
+        assert lib.aclfftMakePlan1d(h2, n, t, b, 0, ctypes.byref(ws)) == aclfft.SUCCESS
+        assert ws.value == est, (n, t, b, ws.value, est)
+        got = ctypes.c_size_t(0)
+        assert lib.aclfftGetSize(h2, ctypes.byref(got)) == aclfft.SUCCESS and got.value == est
+        lib.aclfftDestroy(h2)
+    assert lib.aclfftGetSize1d(h, 1024, aclfft.C2C, 8, ctypes.byref(ws)) == aclfft.SUCCESS and ws.value == 0
+    # real plans also hold the input copy an in-place execution on device memory reads from
+    assert lib.aclfftGetSize1d(h, 4096, aclfft.R2C, 2, ctypes.byref(ws)) == aclfft.SUCCESS
+    assert ws.value >= 4096 * 2 * 4
+    # MakePlan's argument codes
+    assert lib.aclfftGetSize1d(h, 0, aclfft.C2C, 1, ctypes.byref(ws)) == aclfft.INVALID_SIZE
+    assert lib.aclfftGetSize1d(h, 64, 0x7, 1, ctypes.byref(ws)) == aclfft.INVALID_TYPE
+    assert lib.aclfftGetSize1d(h, 64, aclfft.C2C, 1, None) == aclfft.INVALID_VALUE
+    assert lib.aclfftGetSize2d(h, 2048, 2048, aclfft.C2C, ctypes.byref(ws)) == aclfft.SUCCESS
+    assert lib.aclfftGetSize2d(h, 64, 64, aclfft.R2C, ctypes.byref(ws)) == aclfft.NOT_SUPPORTED
+    assert lib.aclfftGetSize2d(h, 0, 64, aclfft.C2C, ctypes.byref(ws)) == aclfft.INVALID_SIZE
+    # GetSize / SetWorkArea need a made plan; SetAutoAllocation does not
+    assert lib.aclfftGetSize(h, ctypes.byref(ws)) == aclfft.INVALID_PLAN
+    assert lib.aclfftSetWorkArea(h, ctypes.c_void_p(4096)) == aclfft.INVALID_PLAN
+    assert lib.aclfftSetAutoAllocation(h, 0) == aclfft.SUCCESS
+    assert lib.aclfftMakePlan1d(h, 1 << 20, aclfft.C2C, 1, 0, ctypes.byref(ws)) == aclfft.SUCCESS
+    assert ws.value > 0
+    assert lib.aclfftSetWorkArea(h, None) == aclfft.INVALID_VALUE
+    host = (ctypes.c_char * ws.value)()
+    assert lib.aclfftSetWorkArea(h, ctypes.cast(host, ctypes.c_void_p)) == aclfft.INVALID_VALUE
+    lib.aclfftDestroy(h)
