@@ -155,6 +155,38 @@ def extra_real(torch, aclfft, oracle, np, steps, stream):
     return out
 
 
+def extra_small_latency(torch, aclfft, oracle, np, stream):
+    """The launch-bound configuration (C2C N=1024, batch 64, 512 KiB each way): time per call,
+    eager and replayed from a hipGraph, measured over many back-to-back calls (device time per
+    call with events, host time per call with the wall clock)."""
+    x = torch.randn(64, 1024, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    out = {}
+    for name, graph in (("eager", False), ("graph", True)):
+        p = aclfft.Plan.plan1d(1024, aclfft.C2C, 64)
+        p.set_stream(stream)
+        p.set_graph_mode(graph)
+        fn = lambda: p.exec_c2c(x, y, aclfft.FORWARD)  # noqa: E731
+        warm(torch, fn, 20)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        calls = 2000
+        e0.record(stream)
+        t0 = time.perf_counter()
+        for _ in range(calls):
+            fn()
+        t1 = time.perf_counter()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out[name] = {"device_us_per_call": round(e0.elapsed_time(e1) * 1e3 / calls, 2),
+                     "host_us_per_call": round((t1 - t0) * 1e6 / calls, 2)}
+        p.destroy()
+    err = oracle.rel_l2(y.cpu().numpy(), oracle.c2c(x.cpu().numpy().astype(np.complex128)))
+    out["rel_l2_vs_fp64"] = err
+    out["ideal_us_at_8TBs"] = round(2 * x.numel() * 8 / 8e12 * 1e6, 3)
+    return out
+
+
 def extra_2d(torch, aclfft, oracle, np, steps, stream):
     B, NX, NY = 4, 2048, 2048
     g = torch.Generator().manual_seed(2)
@@ -315,6 +347,7 @@ def main():
                          ("mfma_split_n1024_b65536", lambda: extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y,
                                                                         args.steps, stream, err)),
                          ("real_r2c_c2r", lambda: extra_real(torch, aclfft, oracle, np, args.steps, stream)),
+                         ("c2c_n1024_b64_latency", lambda: extra_small_latency(torch, aclfft, oracle, np, stream)),
                          ("c2c_2d_2048x2048_b4", lambda: extra_2d(torch, aclfft, oracle, np, 10, stream)),
                          ("uneven_batched_512", lambda: extra_batched(torch, aclfft, oracle, np, 20, stream))):
             try:
