@@ -48,9 +48,16 @@ enum ofx_kind {
     OFX_KIND_ROWT = 6,     /* contiguous rows in, transposed out: transform t -> out +
                               (t / cols)*odist + rhi*odist_hi + rlo + k*ostride with
                               t % cols = rhi*cols_lo + rlo (four-step row step)          */
-    OFX_KIND_C2C_TILES = 7 /* C2C over a tile list (uneven batches): workgroup b runs
+    OFX_KIND_C2C_TILES = 7,/* C2C over a tile list (uneven batches): workgroup b runs
                               tiles[b] = (offset, count <= TPB) contiguous transforms at
                               element offset `offset` of in and out; batch = tile count  */
+    /* complex128 / double counterparts of C2C, R2C, C2R, R2C_ODD, C2R_ODD: same layouts and
+       arguments, elements of 16 (complex) / 8 (real) bytes, tables from the *_z generators  */
+    OFX_KIND_Z2Z = 8,
+    OFX_KIND_D2Z = 9,
+    OFX_KIND_Z2D = 10,
+    OFX_KIND_D2Z_ODD = 11,
+    OFX_KIND_Z2D_ODD = 12
 };
 
 /* Description of one compiled kernel configuration. */
@@ -92,6 +99,7 @@ typedef struct ofx_args {
                               odist_hi in `out`) and r % cols_lo (stride 1); 0 = no split    */
     long long odist_hi;
     const void* tiles;     /* C2C_TILES: ofx_batch_tile array of `batch` entries (device)    */
+    double scale64;        /* complex128 kernels: output scale in double (0 = use `scale`)   */
 } ofx_args;
 
 /* Runtime-radix ("generic") launches: any length whose radix sequence is listed, any layout.
@@ -230,6 +238,9 @@ OFX_API int ofx_gen_pass_twiddles(void* tw, int npass, const int* radices, hipSt
 /* Fill `tw` (device, count complex64) with W_len^k = e^{-2 pi i k/len}, k = 0..count-1 (FP64
  * sincospi on the GPU, rounded to FP32). */
 OFX_API int ofx_gen_roots(void* tw, long long len, long long count, hipStream_t stream);
+/* complex128 versions of the two generators above (double2 entries, FP64 values not rounded). */
+OFX_API int ofx_gen_pass_twiddles_z(void* tw, int npass, const int* radices, hipStream_t stream);
+OFX_API int ofx_gen_roots_z(void* tw, long long len, long long count, hipStream_t stream);
 
 #ifdef __cplusplus
 }

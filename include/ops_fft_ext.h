@@ -53,6 +53,7 @@ ACLFFT_API aclfftResult aclfftExtSetScale(aclfftHandle plan, float scale);
  * kernels with the bins strictly between DC and Nyquist weighted by 1/2.  For a C2R plan:
  * grad_X = R2C(g) with the same bins weighted by 2.  The plan's scale applies.  Same pointer rules
  * and errors as aclfftExecC2R / aclfftExecR2C; a plan of the other type -> INVALID_TYPE.  The
+ * same entry points serve D2Z / Z2D plans (then the buffers hold double / complex128 values).  The
  * adjoint of a C2C transform is the C2C transform in the opposite direction. */
 ACLFFT_API aclfftResult aclfftExtExecR2CAdjoint(aclfftHandle plan, aclfftComplex* grad_out, aclfftReal* grad_in);
 ACLFFT_API aclfftResult aclfftExtExecC2RAdjoint(aclfftHandle plan, aclfftReal* grad_out, aclfftComplex* grad_in);

@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import lib, check, C2C, R2C, C2R, FORWARD, BACKWARD, HORIZONTAL, VERTICAL
+from ._lib import lib, check, C2C, R2C, C2R, Z2Z, D2Z, Z2D, FORWARD, BACKWARD, HORIZONTAL, VERTICAL
 
 
 def _ptr(a):
@@ -153,18 +153,24 @@ class Plan:
     def set_scale(self, scale):
         check(lib.aclfftExtSetScale(self._h, float(scale)), "aclfftExtSetScale")
 
+    def _elem(self):
+        """(real, complex) element bytes of the plan's precision."""
+        return (8, 16) if self.type in (Z2Z, D2Z, Z2D) else (4, 8)
+
     def exec_r2c_adjoint(self, grad_out, grad_in):
         r, c = self._sizes()
-        _check_size(grad_out, 8 * c, "grad_out")
-        _check_size(grad_in, 4 * r, "grad_in")
+        er, ec = self._elem()
+        _check_size(grad_out, ec * c, "grad_out")
+        _check_size(grad_in, er * r, "grad_in")
         self._bind_stream(grad_out)
         check(lib.aclfftExtExecR2CAdjoint(self._h, _ptr(grad_out), _ptr(grad_in)), "aclfftExtExecR2CAdjoint")
         return grad_in
 
     def exec_c2r_adjoint(self, grad_out, grad_in):
         r, c = self._sizes()
-        _check_size(grad_out, 4 * r, "grad_out")
-        _check_size(grad_in, 8 * c, "grad_in")
+        er, ec = self._elem()
+        _check_size(grad_out, er * r, "grad_out")
+        _check_size(grad_in, ec * c, "grad_in")
         self._bind_stream(grad_out)
         check(lib.aclfftExtExecC2RAdjoint(self._h, _ptr(grad_out), _ptr(grad_in)), "aclfftExtExecC2RAdjoint")
         return grad_in
@@ -184,7 +190,7 @@ class Plan:
             pts = sum(n * b for n, b in self.groups)
             return pts, pts
         full = int(np.prod(self.dims)) * self.batch
-        if self.type == C2C:
+        if self.type in (C2C, Z2Z):
             return full, full
         nx = self.dims[-1]
         return full, (nx // 2 + 1) * int(np.prod(self.dims[:-1])) * self.batch
@@ -211,4 +217,29 @@ class Plan:
         _check_size(odata, 4 * r, "odata")
         self._bind_stream(idata)
         check(lib.aclfftExecC2R(self._h, _ptr(idata), _ptr(odata)), "aclfftExecC2R")
+        return odata
+
+    # double precision (complex128 / float64 buffers)
+    def exec_z2z(self, idata, odata, direction=FORWARD):
+        n, _ = self._sizes()
+        _check_size(idata, 16 * n, "idata")
+        _check_size(odata, 16 * n, "odata")
+        self._bind_stream(idata)
+        check(lib.aclfftExecZ2Z(self._h, _ptr(idata), _ptr(odata), direction), "aclfftExecZ2Z")
+        return odata
+
+    def exec_d2z(self, idata, odata):
+        r, c = self._sizes()
+        _check_size(idata, 8 * r, "idata")
+        _check_size(odata, 16 * c, "odata")
+        self._bind_stream(idata)
+        check(lib.aclfftExecD2Z(self._h, _ptr(idata), _ptr(odata)), "aclfftExecD2Z")
+        return odata
+
+    def exec_z2d(self, idata, odata):
+        r, c = self._sizes()
+        _check_size(idata, 16 * c, "idata")
+        _check_size(odata, 8 * r, "odata")
+        self._bind_stream(idata)
+        check(lib.aclfftExecZ2D(self._h, _ptr(idata), _ptr(odata)), "aclfftExecZ2D")
         return odata

@@ -8,7 +8,8 @@
 // Radices: 2, 3, 4 explicit; odd R by the symmetric-pair formula (used for 5, 7, 9, 11, 13 ...);
 // composite R = R1*R2 by one in-register Cooley-Tukey split with constant twiddles.  All constants
 // are evaluated in double precision at compile time (constexpr series with exact quarter-turn
-// reduction) and rounded to float once.
+// reduction) and rounded once to the element type's scalar (float for complex64, double for
+// complex128; the series are summed to full double precision).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -65,91 +66,131 @@ constexpr cs_pair cos_sin_2pi(long long num, long long den) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// complex helpers (float2 = interleaved complex64)
+// complex helpers: float2 = interleaved complex64, double2 = interleaved complex128
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+template <class V>
+struct vscalar;
+template <>
+struct vscalar<float2> {
+    using type = float;
+};
+template <>
+struct vscalar<double2> {
+    using type = double;
+};
+template <class V>
+using scalar_t = typename vscalar<V>::type;
+
+__device__ __forceinline__ float vfma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double vfma(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float2 mkv(float x, float y) { return make_float2(x, y); }
+__device__ __forceinline__ double2 mkv(double x, double y) { return make_double2(x, y); }
+
+template <class V>
+__device__ __forceinline__ V cadd(V a, V b) {
+    return mkv(a.x + b.x, a.y + b.y);
+}
+template <class V>
+__device__ __forceinline__ V csub(V a, V b) {
+    return mkv(a.x - b.x, a.y - b.y);
+}
+template <class V>
+__device__ __forceinline__ V cmul(V a, V b) {
+    return mkv(vfma(a.x, b.x, -a.y * b.y), vfma(a.x, b.y, a.y * b.x));
 }
 // a * conj(b)
-__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
-    return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -a.x * b.y));
+template <class V>
+__device__ __forceinline__ V cmulc(V a, V b) {
+    return mkv(vfma(a.x, b.x, a.y * b.y), vfma(a.y, b.x, -a.x * b.y));
 }
-__device__ __forceinline__ float2 mul_negi(float2 a) { return make_float2(a.y, -a.x); }  // * (-i)
-__device__ __forceinline__ float2 mul_posi(float2 a) { return make_float2(-a.y, a.x); }  // * (+i)
-__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+template <class V>
+__device__ __forceinline__ V mul_negi(V a) {  // * (-i)
+    return mkv(a.y, -a.x);
+}
+template <class V>
+__device__ __forceinline__ V mul_posi(V a) {  // * (+i)
+    return mkv(-a.y, a.x);
+}
+template <class V>
+__device__ __forceinline__ V cconj(V a) {
+    return mkv(a.x, -a.y);
+}
+template <class V>
+__device__ __forceinline__ V cscale(V a, scalar_t<V> s) {
+    return mkv(a.x * s, a.y * s);
+}
 
 // multiply by the compile-time constant W_R^e = e^{-2 pi i e/R}
-template <int R, int E>
-__device__ __forceinline__ float2 twiddle_const(float2 a) {
+template <int R, int E, class V>
+__device__ __forceinline__ V twiddle_const(V a) {
+    using S = scalar_t<V>;
     constexpr int e = ((E % R) + R) % R;
     if constexpr (e == 0) {
         return a;
     } else if constexpr (4 * e == R) {
         return mul_negi(a);
     } else if constexpr (2 * e == R) {
-        return make_float2(-a.x, -a.y);
+        return mkv(-a.x, -a.y);
     } else if constexpr (4 * e == 3 * R) {
         return mul_posi(a);
     } else if constexpr (8 * e == R) {  // (1 - i)/sqrt2
-        constexpr float h = 0.70710678118654752440f;
-        return make_float2(h * (a.x + a.y), h * (a.y - a.x));
+        constexpr S h = S(0.70710678118654752440084436210484903928L);
+        return mkv(h * (a.x + a.y), h * (a.y - a.x));
     } else if constexpr (8 * e == 3 * R) {  // (-1 - i)/sqrt2
-        constexpr float h = 0.70710678118654752440f;
-        return make_float2(h * (a.y - a.x), -h * (a.x + a.y));
+        constexpr S h = S(0.70710678118654752440084436210484903928L);
+        return mkv(h * (a.y - a.x), -h * (a.x + a.y));
     } else {
         constexpr cs_pair w = cos_sin_2pi(e, R);
-        constexpr float wr = float(w.c);
-        constexpr float wi = float(-w.s);
-        return make_float2(fmaf(a.x, wr, -a.y * wi), fmaf(a.x, wi, a.y * wr));
+        constexpr S wr = S(w.c);
+        constexpr S wi = S(-w.s);
+        return mkv(vfma(a.x, wr, -a.y * wi), vfma(a.x, wi, a.y * wr));
     }
 }
 
 // ---------------------------------------------------------------------------------------------
-// codelets
+// codelets (V = float2 or double2; the constants are rounded once to V's scalar type)
 // ---------------------------------------------------------------------------------------------
-template <int R>
+template <int R, class V = float2>
 struct Dft;
 
-template <>
-struct Dft<1> {
-    __device__ __forceinline__ static void run(float2 (&)[1]) {}
+template <class V>
+struct Dft<1, V> {
+    __device__ __forceinline__ static void run(V (&)[1]) {}
 };
 
-template <>
-struct Dft<2> {
-    __device__ __forceinline__ static void run(float2 (&x)[2]) {
-        float2 a = x[0], b = x[1];
+template <class V>
+struct Dft<2, V> {
+    __device__ __forceinline__ static void run(V (&x)[2]) {
+        V a = x[0], b = x[1];
         x[0] = cadd(a, b);
         x[1] = csub(a, b);
     }
 };
 
-template <>
-struct Dft<3> {
-    __device__ __forceinline__ static void run(float2 (&x)[3]) {
-        constexpr float c1 = -0.5f;
-        constexpr float s1 = 0.86602540378443864676f;  // sin(2pi/3)
-        float2 s = cadd(x[1], x[2]);
-        float2 d = csub(x[1], x[2]);
-        float2 x0 = x[0];
+template <class V>
+struct Dft<3, V> {
+    __device__ __forceinline__ static void run(V (&x)[3]) {
+        using S = scalar_t<V>;
+        constexpr S c1 = S(-0.5);
+        constexpr S s1 = S(0.86602540378443864676372317075293618347L);  // sin(2pi/3)
+        V s = cadd(x[1], x[2]);
+        V d = csub(x[1], x[2]);
+        V x0 = x[0];
         x[0] = cadd(x0, s);
-        float2 t = make_float2(fmaf(c1, s.x, x0.x), fmaf(c1, s.y, x0.y));
+        V t = mkv(vfma(c1, s.x, x0.x), vfma(c1, s.y, x0.y));
         // -i*s1*d = (s1*d.y, -s1*d.x)
-        x[1] = make_float2(fmaf(s1, d.y, t.x), fmaf(-s1, d.x, t.y));
-        x[2] = make_float2(fmaf(-s1, d.y, t.x), fmaf(s1, d.x, t.y));
+        x[1] = mkv(vfma(s1, d.y, t.x), vfma(-s1, d.x, t.y));
+        x[2] = mkv(vfma(-s1, d.y, t.x), vfma(s1, d.x, t.y));
     }
 };
 
-template <>
-struct Dft<4> {
-    __device__ __forceinline__ static void run(float2 (&x)[4]) {
-        float2 t0 = cadd(x[0], x[2]);
-        float2 t1 = csub(x[0], x[2]);
-        float2 t2 = cadd(x[1], x[3]);
-        float2 t3 = mul_negi(csub(x[1], x[3]));
+template <class V>
+struct Dft<4, V> {
+    __device__ __forceinline__ static void run(V (&x)[4]) {
+        V t0 = cadd(x[0], x[2]);
+        V t1 = csub(x[0], x[2]);
+        V t2 = cadd(x[1], x[3]);
+        V t3 = mul_negi(csub(x[1], x[3]));
         x[0] = cadd(t0, t2);
         x[2] = csub(t0, t2);
         x[1] = cadd(t1, t3);
@@ -161,13 +202,14 @@ struct Dft<4> {
 //   X_k     = x0 + sum_m s_m cos(2 pi mk/R) - i sum_m d_m sin(2 pi mk/R)
 //   X_{R-k} = x0 + sum_m s_m cos(2 pi mk/R) + i sum_m d_m sin(2 pi mk/R)
 // with s_m = x_m + x_{R-m}, d_m = x_m - x_{R-m}, m, k = 1..(R-1)/2.
-template <int R>
+template <int R, class V>
 struct DftOdd {
+    using S = scalar_t<V>;
     static constexpr int H = (R - 1) / 2;
-    __device__ __forceinline__ static void run(float2 (&x)[R]) {
-        float2 s[H + 1], d[H + 1];
-        float2 x0 = x[0];
-        float2 sum = x0;
+    __device__ __forceinline__ static void run(V (&x)[R]) {
+        V s[H + 1], d[H + 1];
+        V x0 = x[0];
+        V sum = x0;
 #pragma unroll
         for (int m = 1; m <= H; ++m) {
             s[m] = cadd(x[m], x[R - m]);
@@ -178,60 +220,58 @@ struct DftOdd {
         body<1>(x, x0, s, d);
     }
     template <int K>
-    __device__ __forceinline__ static void body(float2 (&x)[R], float2 x0, const float2 (&s)[H + 1],
-                                                const float2 (&d)[H + 1]) {
+    __device__ __forceinline__ static void body(V (&x)[R], V x0, const V (&s)[H + 1], const V (&d)[H + 1]) {
         if constexpr (K <= H) {
-            float2 re = x0;
-            float2 im = make_float2(0.f, 0.f);
+            V re = x0;
+            V im = mkv(S(0), S(0));
             accum<K, 1>(re, im, s, d);
             // -i*im = (im.y, -im.x)
-            x[K] = make_float2(re.x + im.y, re.y - im.x);
-            x[R - K] = make_float2(re.x - im.y, re.y + im.x);
+            x[K] = mkv(re.x + im.y, re.y - im.x);
+            x[R - K] = mkv(re.x - im.y, re.y + im.x);
             body<K + 1>(x, x0, s, d);
         }
     }
     template <int K, int M>
-    __device__ __forceinline__ static void accum(float2& re, float2& im, const float2 (&s)[H + 1],
-                                                 const float2 (&d)[H + 1]) {
+    __device__ __forceinline__ static void accum(V& re, V& im, const V (&s)[H + 1], const V (&d)[H + 1]) {
         if constexpr (M <= H) {
             constexpr cs_pair w = cos_sin_2pi((long long)M * K, R);
-            constexpr float c = float(w.c);
-            constexpr float sn = float(w.s);
-            re.x = fmaf(c, s[M].x, re.x);
-            re.y = fmaf(c, s[M].y, re.y);
-            im.x = fmaf(sn, d[M].x, im.x);
-            im.y = fmaf(sn, d[M].y, im.y);
+            constexpr S c = S(w.c);
+            constexpr S sn = S(w.s);
+            re.x = vfma(c, s[M].x, re.x);
+            re.y = vfma(c, s[M].y, re.y);
+            im.x = vfma(sn, d[M].x, im.x);
+            im.y = vfma(sn, d[M].y, im.y);
             accum<K, M + 1>(re, im, s, d);
         }
     }
 };
 
 // Composite R = R1*R2: n = R2*n1 + n2, k = k1 + R1*k2.
-template <int R1, int R2>
+template <int R1, int R2, class V>
 struct DftSplit {
     static constexpr int R = R1 * R2;
-    __device__ __forceinline__ static void run(float2 (&x)[R]) {
-        float2 y[R2][R1];
+    __device__ __forceinline__ static void run(V (&x)[R]) {
+        V y[R2][R1];
 #pragma unroll
         for (int n2 = 0; n2 < R2; ++n2) {
 #pragma unroll
             for (int n1 = 0; n1 < R1; ++n1) y[n2][n1] = x[R2 * n1 + n2];
-            Dft<R1>::run(y[n2]);
+            Dft<R1, V>::run(y[n2]);
         }
         twid<1, 1>(y);
 #pragma unroll
         for (int k1 = 0; k1 < R1; ++k1) {
-            float2 z[R2];
+            V z[R2];
 #pragma unroll
             for (int n2 = 0; n2 < R2; ++n2) z[n2] = y[n2][k1];
-            Dft<R2>::run(z);
+            Dft<R2, V>::run(z);
 #pragma unroll
             for (int k2 = 0; k2 < R2; ++k2) x[k1 + R1 * k2] = z[k2];
         }
     }
     // y[n2][k1] *= W_R^{n2*k1}, unrolled at compile time so each factor is a constant
     template <int N2, int K1>
-    __device__ __forceinline__ static void twid(float2 (&y)[R2][R1]) {
+    __device__ __forceinline__ static void twid(V (&y)[R2][R1]) {
         if constexpr (N2 < R2) {
             if constexpr (K1 < R1) {
                 y[N2][K1] = twiddle_const<R, N2 * K1>(y[N2][K1]);
@@ -243,30 +283,38 @@ struct DftSplit {
     }
 };
 
-template <> struct Dft<5> : DftOdd<5> {};
-template <> struct Dft<7> : DftOdd<7> {};
-template <> struct Dft<11> : DftOdd<11> {};
-template <> struct Dft<13> : DftOdd<13> {};
-template <> struct Dft<17> : DftOdd<17> {};
-template <> struct Dft<19> : DftOdd<19> {};
-template <> struct Dft<23> : DftOdd<23> {};
-template <> struct Dft<29> : DftOdd<29> {};
-template <> struct Dft<31> : DftOdd<31> {};
-template <> struct Dft<37> : DftOdd<37> {};
-template <> struct Dft<41> : DftOdd<41> {};
-template <> struct Dft<43> : DftOdd<43> {};
-template <> struct Dft<47> : DftOdd<47> {};
-template <> struct Dft<6> : DftSplit<2, 3> {};
-template <> struct Dft<8> : DftSplit<2, 4> {};
-template <> struct Dft<9> : DftSplit<3, 3> {};
-template <> struct Dft<10> : DftSplit<2, 5> {};
-template <> struct Dft<12> : DftSplit<4, 3> {};
-template <> struct Dft<15> : DftSplit<3, 5> {};
-template <> struct Dft<16> : DftSplit<4, 4> {};
-template <> struct Dft<25> : DftSplit<5, 5> {};
-template <> struct Dft<27> : DftSplit<3, 9> {};
-template <> struct Dft<32> : DftSplit<4, 8> {};
-template <> struct Dft<49> : DftSplit<7, 7> {};
-template <> struct Dft<64> : DftSplit<8, 8> {};
+#define OFX_DFT_ODD(R) \
+    template <class V> \
+    struct Dft<R, V> : DftOdd<R, V> {};
+#define OFX_DFT_SPLIT(R, R1, R2) \
+    template <class V> \
+    struct Dft<R, V> : DftSplit<R1, R2, V> {};
+OFX_DFT_ODD(5)
+OFX_DFT_ODD(7)
+OFX_DFT_ODD(11)
+OFX_DFT_ODD(13)
+OFX_DFT_ODD(17)
+OFX_DFT_ODD(19)
+OFX_DFT_ODD(23)
+OFX_DFT_ODD(29)
+OFX_DFT_ODD(31)
+OFX_DFT_ODD(37)
+OFX_DFT_ODD(41)
+OFX_DFT_ODD(43)
+OFX_DFT_ODD(47)
+OFX_DFT_SPLIT(6, 2, 3)
+OFX_DFT_SPLIT(8, 2, 4)
+OFX_DFT_SPLIT(9, 3, 3)
+OFX_DFT_SPLIT(10, 2, 5)
+OFX_DFT_SPLIT(12, 4, 3)
+OFX_DFT_SPLIT(15, 3, 5)
+OFX_DFT_SPLIT(16, 4, 4)
+OFX_DFT_SPLIT(25, 5, 5)
+OFX_DFT_SPLIT(27, 3, 9)
+OFX_DFT_SPLIT(32, 4, 8)
+OFX_DFT_SPLIT(49, 7, 7)
+OFX_DFT_SPLIT(64, 8, 8)
+#undef OFX_DFT_ODD
+#undef OFX_DFT_SPLIT
 
 }  // namespace ofx

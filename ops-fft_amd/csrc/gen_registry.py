@@ -11,7 +11,8 @@ import os
 
 KINDS = {"C2C": "OFX_KIND_C2C", "R2C": "OFX_KIND_R2C", "C2R": "OFX_KIND_C2R",
          "R2C_ODD": "OFX_KIND_R2C_ODD", "C2R_ODD": "OFX_KIND_C2R_ODD", "ROWT": "OFX_KIND_ROWT",
-         "C2C_TILES": "OFX_KIND_C2C_TILES"}
+         "C2C_TILES": "OFX_KIND_C2C_TILES", "Z2Z": "OFX_KIND_Z2Z", "D2Z": "OFX_KIND_D2Z",
+         "Z2D": "OFX_KIND_Z2D", "D2Z_ODD": "OFX_KIND_D2Z_ODD", "Z2D_ODD": "OFX_KIND_Z2D_ODD"}
 
 # Radix plans for the lengths that get a dedicated (fully unrolled) kernel.  Everything else runs
 # on the runtime-radix kernel (kernels_generic.hip).
@@ -87,6 +88,37 @@ TILES = [
 ]
 
 
+def tpb_for_z(n, t):
+    """complex128: 16-byte elements, so half the transforms per ~64 KiB of LDS."""
+    tpb = max(1, 256 // t)
+    while tpb > 1 and tpb * (n + n // 16) * 16 > 65536:
+        tpb //= 2
+    return tpb
+
+
+def z_lines():
+    """complex128 kernels: Z2Z for every length with a radix plan, D2Z/Z2D for the even real
+    lengths up to 8192 (their half-length complex kernels) and for the odd lengths 243 and 6561."""
+    out = []
+    for n, plans in sorted(PLANS.items()):
+        if n > 6561:
+            continue
+        rad = plans[0]
+        t = choose_t(n, rad)
+        out.append(("Z2Z", n, t, tpb_for_z(n, t), padsh(n), rad))
+    for n in (2, 4, 8, 16, 32, 64, 128, 256, 512, 1000, 1024, 1500, 2048, 3000, 4096):
+        rad = PLANS[n][0]
+        t = choose_t(n, rad)
+        out.append(("D2Z", n, t, tpb_for_z(n, t), padsh(n), rad))
+        out.append(("Z2D", n, t, tpb_for_z(n, t), padsh(n), rad))
+    for n in (243, 6561):
+        rad = PLANS[n][0]
+        t = choose_t(n, rad)
+        out.append(("D2Z_ODD", n, t, tpb_for_z(n, t), padsh(n), rad))
+        out.append(("Z2D_ODD", n, t, tpb_for_z(n, t), padsh(n), rad))
+    return out
+
+
 def padsh(n):
     """LDS pad shift: odd lengths have no power-of-two strides, so they go unpadded."""
     return 30 if n % 2 else 4
@@ -160,8 +192,12 @@ def main():
             f.write("    OFX_E(%s, %d, %d, %d, 4, false, %s),\n" % (KINDS[kind], n, t, tpb, ", ".join(str(r) for r in rad)))
         for kind, n, t, tpb, rad in PAIRED:
             f.write("    OFX_EV(%s, %d, %d, %d, 4, %s),\n" % (KINDS[kind], n, t, tpb, ", ".join(str(r) for r in rad)))
+        for kind, n, t, tpb, psh, rad in z_lines():
+            assert t * tpb <= 1024
+            f.write("    OFX_EZ(%s, %d, %d, %d, %d, %s),\n" % (KINDS[kind], n, t, tpb, psh, ", ".join(str(r) for r in rad)))
         f.write("};\n")
-    print("wrote %d registry entries" % (len(FIRST) + len(rows) + len(TILES) + len(EXTRA) + len(PAIRED)))
+    print("wrote %d registry entries" % (len(FIRST) + len(rows) + len(TILES) + len(EXTRA) + len(PAIRED) +
+                                         len(z_lines())))
 
 
 if __name__ == "__main__":

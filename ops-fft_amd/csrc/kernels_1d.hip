@@ -23,96 +23,115 @@
 namespace ofx {
 
 // ---------------------------------------------------------------------------------------------
-// HBM functors
+// HBM functors (V = float2 for complex64 / float, double2 for complex128 / double)
 // ---------------------------------------------------------------------------------------------
-struct GLoadC {  // complex input, conjugated for the backward direction
-    const float2* __restrict__ p;
-    float sg;
-    __device__ __forceinline__ float2 operator()(int i) const {
-        float2 v = p[i];
-        return make_float2(v.x, v.y * sg);
+// The output scale of a launch in V's precision: complex128 kernels take ofx_args::scale64 when it
+// is set (a float scale would round factors such as 1/3 to single precision).
+template <class S>
+__device__ __forceinline__ S arg_scale(const ofx_args& a) {
+    if constexpr (sizeof(S) == 8) {
+        return a.scale64 != 0.0 ? a.scale64 : (double)a.scale;
+    } else {
+        return a.scale;
     }
-    // elements i, i+1 (i even, 16-byte aligned rows)
+}
+
+template <class V>
+struct GLoadC {  // complex input, conjugated for the backward direction
+    const V* __restrict__ p;
+    scalar_t<V> sg;
+    __device__ __forceinline__ V operator()(int i) const {
+        V v = p[i];
+        return mkv(v.x, v.y * sg);
+    }
+    // elements i, i+1 (complex64, i even, 16-byte aligned rows)
     __device__ __forceinline__ float4 load2(int i) const {
         float4 v = *reinterpret_cast<const float4*>(p + i);
         return make_float4(v.x, v.y * sg, v.z, v.w * sg);
     }
 };
+template <class V>
 struct GStoreC {
-    float2* __restrict__ p;
-    float sg, scale;
+    V* __restrict__ p;
+    scalar_t<V> sg, scale;
     bool ok;
-    __device__ __forceinline__ void operator()(int i, float2 v) const {
-        if (ok) p[i] = make_float2(v.x * scale, v.y * (sg * scale));
+    __device__ __forceinline__ void operator()(int i, V v) const {
+        if (ok) p[i] = mkv(v.x * scale, v.y * (sg * scale));
     }
-    __device__ __forceinline__ void store2(int i, float2 a, float2 b) const {
-        const float si = sg * scale;
+    __device__ __forceinline__ void store2(int i, V a, V b) const {
+        const scalar_t<V> si = sg * scale;
         if (ok) *reinterpret_cast<float4*>(p + i) = make_float4(a.x * scale, a.y * si, b.x * scale, b.y * si);
     }
 };
 
 // C2R (even n) pre-pack, computed while loading pass 0.  Output is conjugated (backward via the
 // forward kernel).
+template <class V>
 struct GLoadC2RPre {
-    const float2* __restrict__ X;   // M+1 bins of this transform
-    const float2* __restrict__ W;   // W_n^k, k = 0..M
+    using S = scalar_t<V>;
+    const V* __restrict__ X;   // M+1 bins of this transform
+    const V* __restrict__ W;   // W_n^k, k = 0..M
     int M;
-    float half_weight;              // 0.5 for the adjoint of R2C, 1 otherwise (bins 1..M-1)
-    __device__ __forceinline__ float2 operator()(int k) const {
-        float2 a = X[k];
-        float2 b = X[M - k];
+    S half_weight;             // 0.5 for the adjoint of R2C, 1 otherwise (bins 1..M-1)
+    __device__ __forceinline__ V operator()(int k) const {
+        V a = X[k];
+        V b = X[M - k];
         if (k == 0) {
-            a.y = 0.f;
-            b.y = 0.f;
+            a.y = S(0);
+            b.y = S(0);
         } else {
             a = cscale(a, half_weight);
             b = cscale(b, half_weight);
         }
-        float2 bc = cconj(b);
-        float2 s = cadd(a, bc);
-        float2 d = csub(a, bc);
-        float2 w = W[k];
+        V bc = cconj(b);
+        V s = cadd(a, bc);
+        V d = csub(a, bc);
+        V w = W[k];
         // i * conj(w) * d
-        float2 t = mul_posi(cmulc(d, w));
-        float2 z = cadd(s, t);
+        V t = mul_posi(cmulc(d, w));
+        V z = cadd(s, t);
         return cconj(z);
     }
     __device__ __forceinline__ float4 load2(int k) const {
-        const float2 a = (*this)(k), b = (*this)(k + 1);
+        const V a = (*this)(k), b = (*this)(k + 1);
         return make_float4(a.x, a.y, b.x, b.y);
     }
 };
 
 // odd n R2C: real input as complex with zero imaginary part; keep bins 0..n/2.
+template <class V>
 struct GLoadRealAsC {
-    const float* __restrict__ p;
-    __device__ __forceinline__ float2 operator()(int i) const { return make_float2(p[i], 0.f); }
+    const scalar_t<V>* __restrict__ p;
+    __device__ __forceinline__ V operator()(int i) const { return mkv(p[i], scalar_t<V>(0)); }
 };
+template <class V>
 struct GStoreHalfSpectrum {
-    float2* __restrict__ p;
+    using S = scalar_t<V>;
+    V* __restrict__ p;
     int half;      // n/2
-    float scale, weight;
+    S scale, weight;
     bool ok;
-    __device__ __forceinline__ void operator()(int i, float2 v) const {
+    __device__ __forceinline__ void operator()(int i, V v) const {
         if (ok && i <= half) {
             // odd n has no Nyquist bin: every bin but DC has a distinct conjugate partner
-            const float s = (i == 0) ? scale : scale * weight;
-            float2 r = make_float2(v.x * s, v.y * s);
-            if (i == 0) r.y = 0.f;
+            const S s = (i == 0) ? scale : scale * weight;
+            V r = mkv(v.x * s, v.y * s);
+            if (i == 0) r.y = S(0);
             p[i] = r;
         }
     }
 };
 // odd n C2R: Hermitian expansion on load (conjugated for the backward direction), real part out.
+template <class V>
 struct GLoadHermExpand {
-    const float2* __restrict__ X;
+    const V* __restrict__ X;
     int n;
-    float half_weight;
-    __device__ __forceinline__ float2 operator()(int i) const {
+    scalar_t<V> half_weight;
+    __device__ __forceinline__ V operator()(int i) const {
         const int h = n / 2;
-        float2 v;
+        V v;
         if (i == 0) {
-            v = make_float2(X[0].x, 0.f);
+            v = mkv(X[0].x, scalar_t<V>(0));
         } else if (i <= h) {
             v = cscale(X[i], half_weight);
         } else {
@@ -121,11 +140,12 @@ struct GLoadHermExpand {
         return cconj(v);
     }
 };
+template <class V>
 struct GStoreReal {
-    float* __restrict__ p;
-    float scale;
+    scalar_t<V>* __restrict__ p;
+    scalar_t<V> scale;
     bool ok;
-    __device__ __forceinline__ void operator()(int i, float2 v) const {
+    __device__ __forceinline__ void operator()(int i, V v) const {
         if (ok) p[i] = v.x * scale;
     }
 };
@@ -134,45 +154,48 @@ struct GStoreReal {
 // staged tile copies (HBM <-> LDS, 16-byte accesses): tile = `nvalid` contiguous transforms
 // ---------------------------------------------------------------------------------------------
 template <class C>
-__device__ __forceinline__ void stage_in(float2* lds, const float2* __restrict__ src, int nvalid, float sg) {
+__device__ __forceinline__ void stage_in(typename C::V* lds, const typename C::V* __restrict__ src, int nvalid,
+                                         scalar_t<typename C::V> sg) {
+    using V = typename C::V;
     const int total = nvalid * C::N;
-    if constexpr ((C::N & 1) == 0) {
+    if constexpr ((C::N & 1) == 0 && sizeof(V) == 8) {
         const float4* s4 = reinterpret_cast<const float4*>(src);
         for (int l = threadIdx.x; l < total / 2; l += C::BLOCK) {
             float4 v = s4[l];
             const int e = 2 * l;
             const int t = e / C::N, i = e - t * C::N;
-            float2* d = lds + t * C::LDS_N;
-            d[C::pad(i)] = make_float2(v.x, v.y * sg);
-            d[C::pad(i + 1)] = make_float2(v.z, v.w * sg);
+            V* d = lds + t * C::LDS_N;
+            d[C::pad(i)] = mkv(v.x, v.y * sg);
+            d[C::pad(i + 1)] = mkv(v.z, v.w * sg);
         }
     } else {
         for (int l = threadIdx.x; l < total; l += C::BLOCK) {
-            float2 v = src[l];
+            V v = src[l];
             const int t = l / C::N, i = l - t * C::N;
-            lds[t * C::LDS_N + C::pad(i)] = make_float2(v.x, v.y * sg);
+            lds[t * C::LDS_N + C::pad(i)] = mkv(v.x, v.y * sg);
         }
     }
 }
 template <class C>
-__device__ __forceinline__ void stage_out(const float2* lds, float2* __restrict__ dst, int nvalid, float sg,
-                                          float scale) {
+__device__ __forceinline__ void stage_out(const typename C::V* lds, typename C::V* __restrict__ dst, int nvalid,
+                                          scalar_t<typename C::V> sg, scalar_t<typename C::V> scale) {
+    using V = typename C::V;
     const int total = nvalid * C::N;
-    const float si = sg * scale;
-    if constexpr ((C::N & 1) == 0) {
+    const scalar_t<V> si = sg * scale;
+    if constexpr ((C::N & 1) == 0 && sizeof(V) == 8) {
         float4* d4 = reinterpret_cast<float4*>(dst);
         for (int l = threadIdx.x; l < total / 2; l += C::BLOCK) {
             const int e = 2 * l;
             const int t = e / C::N, i = e - t * C::N;
-            const float2* s = lds + t * C::LDS_N;
-            float2 a = s[C::pad(i)], b = s[C::pad(i + 1)];
+            const V* s = lds + t * C::LDS_N;
+            V a = s[C::pad(i)], b = s[C::pad(i + 1)];
             d4[l] = make_float4(a.x * scale, a.y * si, b.x * scale, b.y * si);
         }
     } else {
         for (int l = threadIdx.x; l < total; l += C::BLOCK) {
             const int t = l / C::N, i = l - t * C::N;
-            float2 a = lds[t * C::LDS_N + C::pad(i)];
-            dst[l] = make_float2(a.x * scale, a.y * si);
+            V a = lds[t * C::LDS_N + C::pad(i)];
+            dst[l] = mkv(a.x * scale, a.y * si);
         }
     }
 }
@@ -182,30 +205,33 @@ __device__ __forceinline__ void stage_out(const float2* lds, float2* __restrict_
 // ---------------------------------------------------------------------------------------------
 template <class C>
 __global__ __launch_bounds__(C::BLOCK) void k_c2c(ofx_args a) {
-    __shared__ float2 lds[C::TPB * C::LDS_N];
+    using V = typename C::V;
+    using S = scalar_t<V>;
+    const S ascale = arg_scale<S>(a);
+    __shared__ V lds[C::TPB * C::LDS_N];
     const int tl = threadIdx.x / C::T;
     const int tid = threadIdx.x - tl * C::T;
     const long long t0 = (long long)blockIdx.x * C::TPB;
     const long long t = t0 + tl;
-    const float2* __restrict__ tw = static_cast<const float2*>(a.tw);
-    float2* lt = lds + tl * C::LDS_N;
+    const V* __restrict__ tw = static_cast<const V*>(a.tw);
+    V* lt = lds + tl * C::LDS_N;
     if constexpr (C::STAGED) {
         // staged tiles require contiguous transforms (idist == odist == N): checked by the launcher
         const int nvalid = (int)min((long long)C::TPB, a.batch - t0);
-        const float2* src = static_cast<const float2*>(a.in) + t0 * C::N;
-        float2* dst = static_cast<float2*>(a.out) + t0 * C::N;
+        const V* src = static_cast<const V*>(a.in) + t0 * C::N;
+        V* dst = static_cast<V*>(a.out) + t0 * C::N;
         stage_in<C>(lds, src, nvalid, a.in_sign);
         __syncthreads();
         LdsRead<C> gin{lt};
         LdsWrite<C> gout{lt};
         stockham_passes<C, true, true>(lt, tid, tw, gin, gout);
         __syncthreads();
-        stage_out<C>(lds, dst, nvalid, a.out_sign, a.scale);
+        stage_out<C>(lds, dst, nvalid, a.out_sign, ascale);
     } else {
         const bool ok = t < a.batch;
         const long long tt = ok ? t : 0;
-        GLoadC gin{static_cast<const float2*>(a.in) + tt * a.idist, a.in_sign};
-        GStoreC gout{static_cast<float2*>(a.out) + tt * a.odist, a.out_sign, a.scale, ok};
+        GLoadC<V> gin{static_cast<const V*>(a.in) + tt * a.idist, a.in_sign};
+        GStoreC<V> gout{static_cast<V*>(a.out) + tt * a.odist, a.out_sign, ascale, ok};
         if constexpr (C::WAVE_SYNC) {
             if (!ok) return;  // whole transforms per wave: no block-wide barrier follows
         }
@@ -216,20 +242,23 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2c(ofx_args a) {
 // R2C, even n = 2*C::N.  in: real (idist floats), out: complex (odist complex, >= C::N+1).
 template <class C>
 __global__ __launch_bounds__(C::BLOCK) void k_r2c(ofx_args a) {
-    __shared__ float2 lds[C::TPB * C::LDS_N];
+    using V = typename C::V;
+    using S = scalar_t<V>;
+    const S ascale = arg_scale<S>(a);
+    __shared__ V lds[C::TPB * C::LDS_N];
     const int tl = threadIdx.x / C::T;
     const int tid = threadIdx.x - tl * C::T;
     const long long t0 = (long long)blockIdx.x * C::TPB;
     const long long t = t0 + tl;
     const bool ok = t < a.batch;
     const long long tt = ok ? t : 0;
-    const float2* __restrict__ tw = static_cast<const float2*>(a.tw);
-    float2* lt = lds + tl * C::LDS_N;
+    const V* __restrict__ tw = static_cast<const V*>(a.tw);
+    V* lt = lds + tl * C::LDS_N;
     LdsWrite<C> lw{lt};
     if constexpr (C::STAGED) {
         const int nvalid = (int)min((long long)C::TPB, a.batch - t0);
-        const float2* src = reinterpret_cast<const float2*>(static_cast<const float*>(a.in)) + t0 * C::N;
-        stage_in<C>(lds, src, nvalid, 1.f);
+        const V* src = reinterpret_cast<const V*>(static_cast<const S*>(a.in)) + t0 * C::N;
+        stage_in<C>(lds, src, nvalid, S(1));
         __syncthreads();
         LdsRead<C> gin{lt};
         stockham_passes<C, true, true>(lt, tid, tw, gin, lw);
@@ -237,60 +266,63 @@ __global__ __launch_bounds__(C::BLOCK) void k_r2c(ofx_args a) {
         if constexpr (C::WAVE_SYNC) {
             if (!ok) return;
         }
-        GLoadC gin{reinterpret_cast<const float2*>(static_cast<const float*>(a.in) + tt * a.idist), 1.f};
+        GLoadC<V> gin{reinterpret_cast<const V*>(static_cast<const S*>(a.in) + tt * a.idist), 1.f};
         stockham_passes<C, false, true>(lt, tid, tw, gin, lw);
     }
     if constexpr (!C::WAVE_SYNC || C::STAGED) __syncthreads();
     if (!ok) return;
     // fused Hermitian unpack
     constexpr int M = C::N;
-    const float2* __restrict__ W = static_cast<const float2*>(a.tw_aux);
-    float2* __restrict__ out = static_cast<float2*>(a.out) + tt * a.odist;
-    const float wgt = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 2.f : 1.f;
+    const V* __restrict__ W = static_cast<const V*>(a.tw_aux);
+    V* __restrict__ out = static_cast<V*>(a.out) + tt * a.odist;
+    const S wgt = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(2) : S(1);
     for (int k = tid; k <= M; k += C::T) {
         const int k1 = (k == M) ? 0 : k;
         const int k2 = (k == 0) ? 0 : M - k;
-        float2 zk = lt[C::pad(k1)];
-        float2 zc = cconj(lt[C::pad(k2)]);
-        float2 e = cscale(cadd(zk, zc), 0.5f);
-        float2 o = cscale(csub(zk, zc), 0.5f);  // (Z[k] - conj Z[M-k]) / 2
-        float2 x = cadd(e, mul_negi(cmul(W[k], o)));
-        float s = a.scale;
+        V zk = lt[C::pad(k1)];
+        V zc = cconj(lt[C::pad(k2)]);
+        V e = cscale(cadd(zk, zc), S(0.5));
+        V o = cscale(csub(zk, zc), S(0.5));  // (Z[k] - conj Z[M-k]) / 2
+        V x = cadd(e, mul_negi(cmul(W[k], o)));
+        S s = ascale;
         if (k == 0 || k == M) {
             x.y = 0.f;
         } else {
             s *= wgt;
         }
-        out[k] = make_float2(x.x * s, x.y * s);
+        out[k] = mkv(x.x * s, x.y * s);
     }
 }
 
 // C2R, even n = 2*C::N.  in: complex (idist complex, >= C::N+1), out: real (odist floats).
 template <class C>
 __global__ __launch_bounds__(C::BLOCK) void k_c2r(ofx_args a) {
-    __shared__ float2 lds[C::TPB * C::LDS_N];
+    using V = typename C::V;
+    using S = scalar_t<V>;
+    const S ascale = arg_scale<S>(a);
+    __shared__ V lds[C::TPB * C::LDS_N];
     const int tl = threadIdx.x / C::T;
     const int tid = threadIdx.x - tl * C::T;
     const long long t0 = (long long)blockIdx.x * C::TPB;
     const long long t = t0 + tl;
     const bool ok = t < a.batch;
     const long long tt = ok ? t : 0;
-    const float2* __restrict__ tw = static_cast<const float2*>(a.tw);
-    float2* lt = lds + tl * C::LDS_N;
-    const float hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 0.5f : 1.f;
-    GLoadC2RPre gin{static_cast<const float2*>(a.in) + tt * a.idist, static_cast<const float2*>(a.tw_aux), C::N, hw};
+    const V* __restrict__ tw = static_cast<const V*>(a.tw);
+    V* lt = lds + tl * C::LDS_N;
+    const S hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(0.5) : S(1);
+    GLoadC2RPre<V> gin{static_cast<const V*>(a.in) + tt * a.idist, static_cast<const V*>(a.tw_aux), C::N, hw};
     if constexpr (C::STAGED) {
         const int nvalid = (int)min((long long)C::TPB, a.batch - t0);
-        float2* dst = reinterpret_cast<float2*>(static_cast<float*>(a.out)) + t0 * C::N;
+        V* dst = reinterpret_cast<V*>(static_cast<S*>(a.out)) + t0 * C::N;
         LdsWrite<C> gout{lt};
         stockham_passes<C, false, true>(lt, tid, tw, gin, gout);
         __syncthreads();
-        stage_out<C>(lds, dst, nvalid, -1.f, a.scale);
+        stage_out<C>(lds, dst, nvalid, S(-1), ascale);
     } else {
         if constexpr (C::WAVE_SYNC) {
             if (!ok) return;
         }
-        GStoreC gout{reinterpret_cast<float2*>(static_cast<float*>(a.out) + tt * a.odist), -1.f, a.scale, ok};
+        GStoreC<V> gout{reinterpret_cast<V*>(static_cast<S*>(a.out) + tt * a.odist), S(-1), ascale, ok};
         stockham_passes<C, false, false>(lt, tid, tw, gin, gout);
     }
 }
@@ -298,7 +330,10 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2r(ofx_args a) {
 // R2C, odd n = C::N.
 template <class C>
 __global__ __launch_bounds__(C::BLOCK) void k_r2c_odd(ofx_args a) {
-    __shared__ float2 lds[C::TPB * C::LDS_N];
+    using V = typename C::V;
+    using S = scalar_t<V>;
+    const S ascale = arg_scale<S>(a);
+    __shared__ V lds[C::TPB * C::LDS_N];
     const int tl = threadIdx.x / C::T;
     const int tid = threadIdx.x - tl * C::T;
     const long long t = (long long)blockIdx.x * C::TPB + tl;
@@ -307,16 +342,19 @@ __global__ __launch_bounds__(C::BLOCK) void k_r2c_odd(ofx_args a) {
     if constexpr (C::WAVE_SYNC) {
         if (!ok) return;
     }
-    const float wgt = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 2.f : 1.f;
-    GLoadRealAsC gin{static_cast<const float*>(a.in) + tt * a.idist};
-    GStoreHalfSpectrum gout{static_cast<float2*>(a.out) + tt * a.odist, C::N / 2, a.scale, wgt, ok};
-    stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, static_cast<const float2*>(a.tw), gin, gout);
+    const S wgt = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(2) : S(1);
+    GLoadRealAsC<V> gin{static_cast<const S*>(a.in) + tt * a.idist};
+    GStoreHalfSpectrum<V> gout{static_cast<V*>(a.out) + tt * a.odist, C::N / 2, ascale, wgt, ok};
+    stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, static_cast<const V*>(a.tw), gin, gout);
 }
 
 // C2R, odd n = C::N.
 template <class C>
 __global__ __launch_bounds__(C::BLOCK) void k_c2r_odd(ofx_args a) {
-    __shared__ float2 lds[C::TPB * C::LDS_N];
+    using V = typename C::V;
+    using S = scalar_t<V>;
+    const S ascale = arg_scale<S>(a);
+    __shared__ V lds[C::TPB * C::LDS_N];
     const int tl = threadIdx.x / C::T;
     const int tid = threadIdx.x - tl * C::T;
     const long long t = (long long)blockIdx.x * C::TPB + tl;
@@ -325,10 +363,10 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2r_odd(ofx_args a) {
     if constexpr (C::WAVE_SYNC) {
         if (!ok) return;
     }
-    const float hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 0.5f : 1.f;
-    GLoadHermExpand gin{static_cast<const float2*>(a.in) + tt * a.idist, C::N, hw};
-    GStoreReal gout{static_cast<float*>(a.out) + tt * a.odist, a.scale, ok};
-    stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, static_cast<const float2*>(a.tw), gin, gout);
+    const S hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(0.5) : S(1);
+    GLoadHermExpand<V> gin{static_cast<const V*>(a.in) + tt * a.idist, C::N, hw};
+    GStoreReal<V> gout{static_cast<S*>(a.out) + tt * a.odist, ascale, ok};
+    stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, static_cast<const V*>(a.tw), gin, gout);
 }
 
 // Rows in, columns out: the row step of the four-step decomposition.  Transform t (length C::N,
@@ -339,19 +377,22 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2r_odd(ofx_args a) {
 // element k of TPB different rows fall on distinct banks.
 template <class C>
 __global__ __launch_bounds__(C::BLOCK) void k_rowt(ofx_args a) {
+    using V = typename C::V;
+    using S = scalar_t<V>;
+    const S ascale = arg_scale<S>(a);
     constexpr int RS = C::LDS_N + 1;
     constexpr int KSTEP = C::BLOCK / C::TPB;  // rows' elements stored per sweep of the workgroup
     static_assert(C::BLOCK % C::TPB == 0, "transposed store assumes TPB divides the block");
-    __shared__ float2 lds[C::TPB * RS];
+    __shared__ V lds[C::TPB * RS];
     const int tl = threadIdx.x / C::T;
     const int tid = threadIdx.x - tl * C::T;
     const long long t0 = (long long)blockIdx.x * C::TPB;
     const long long t = t0 + tl;
     const long long tt = t < a.batch ? t : 0;
-    float2* lt = lds + tl * RS;
-    GLoadC gin{static_cast<const float2*>(a.in) + tt * a.idist, a.in_sign};
+    V* lt = lds + tl * RS;
+    GLoadC<V> gin{static_cast<const V*>(a.in) + tt * a.idist, a.in_sign};
     LdsWrite<C> lw{lt};
-    stockham_passes<C, false, true>(lt, tid, static_cast<const float2*>(a.tw), gin, lw);
+    stockham_passes<C, false, true>(lt, tid, static_cast<const V*>(a.tw), gin, lw);
     __syncthreads();
     // thread -> (row r, first element k0); a row past the batch end stores nothing
     const int r = threadIdx.x % C::TPB;
@@ -364,14 +405,14 @@ __global__ __launch_bounds__(C::BLOCK) void k_rowt(ofx_args a) {
         const long long rhi = r_in / a.cols_lo;
         r_off = rhi * a.odist_hi + (r_in - rhi * a.cols_lo);
     }
-    float2* __restrict__ o = static_cast<float2*>(a.out) + img * a.odist + r_off + k0 * a.ostride;
+    V* __restrict__ o = static_cast<V*>(a.out) + img * a.odist + r_off + k0 * a.ostride;
     const long long step = (long long)KSTEP * a.ostride;
-    const float2* lr = lds + r * RS;
-    const float si = a.out_sign * a.scale;
+    const V* lr = lds + r * RS;
+    const S si = a.out_sign * ascale;
 #pragma unroll 4
     for (int k = k0; k < C::N; k += KSTEP, o += step) {
-        const float2 v = lr[C::pad(k)];
-        *o = make_float2(v.x * a.scale, v.y * si);
+        const V v = lr[C::pad(k)];
+        *o = mkv(v.x * ascale, v.y * si);
     }
 }
 
@@ -380,8 +421,11 @@ __global__ __launch_bounds__(C::BLOCK) void k_rowt(ofx_args a) {
 // length).  Offsets may be odd, so only single-element (8-byte) I/O configurations are built.
 template <class C>
 __global__ __launch_bounds__(C::BLOCK) void k_c2c_tiles(ofx_args a) {
+    using V = typename C::V;
+    using S = scalar_t<V>;
+    const S ascale = arg_scale<S>(a);
     static_assert(C::IOV == 1 && !C::STAGED, "tile offsets carry no 16-byte alignment");
-    __shared__ float2 lds[C::TPB * C::LDS_N];
+    __shared__ V lds[C::TPB * C::LDS_N];
     const int tl = threadIdx.x / C::T;
     const int tid = threadIdx.x - tl * C::T;
     const ofx_batch_tile tile = static_cast<const ofx_batch_tile*>(a.tiles)[blockIdx.x];
@@ -390,9 +434,9 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2c_tiles(ofx_args a) {
         if (!ok) return;  // whole transforms per wave: no block-wide barrier follows
     }
     const long long base = tile.offset + (long long)(ok ? tl : 0) * C::N;
-    GLoadC gin{static_cast<const float2*>(a.in) + base, a.in_sign};
-    GStoreC gout{static_cast<float2*>(a.out) + base, a.out_sign, a.scale, ok};
-    stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, static_cast<const float2*>(a.tw), gin, gout);
+    GLoadC<V> gin{static_cast<const V*>(a.in) + base, a.in_sign};
+    GStoreC<V> gout{static_cast<V*>(a.out) + base, a.out_sign, ascale, ok};
+    stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, static_cast<const V*>(a.tw), gin, gout);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -400,8 +444,22 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2c_tiles(ofx_args a) {
 // ---------------------------------------------------------------------------------------------
 typedef hipError_t (*launch_fn)(const ofx_args*, hipStream_t);
 
-template <class C, int KIND>
+// The kernel family of a kind: complex128 kinds run the same templates as their complex64
+// counterparts, with C::V = double2.
+constexpr int base_kind(int k) {
+    return k == OFX_KIND_Z2Z ? OFX_KIND_C2C
+           : k == OFX_KIND_D2Z ? OFX_KIND_R2C
+           : k == OFX_KIND_Z2D ? OFX_KIND_C2R
+           : k == OFX_KIND_D2Z_ODD ? OFX_KIND_R2C_ODD
+           : k == OFX_KIND_Z2D_ODD ? OFX_KIND_C2R_ODD
+                                   : k;
+}
+constexpr bool is_z_kind(int k) { return k >= OFX_KIND_Z2Z && k <= OFX_KIND_Z2D_ODD; }
+
+template <class C, int KIND_>
 hipError_t launch(const ofx_args* a, hipStream_t s) {
+    static_assert(is_z_kind(KIND_) == (sizeof(typename C::V) == 16), "kind and element type disagree");
+    constexpr int KIND = base_kind(KIND_);
     if (a->batch <= 0) return hipSuccess;
     const long long blocks = KIND == OFX_KIND_C2C_TILES ? a->batch : (a->batch + C::TPB - 1) / C::TPB;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidConfiguration;
@@ -449,7 +507,7 @@ Entry make_entry() {
     for (int p = 0; p < C::NP; ++p) e.info.radices[p] = C::radix(p);
     e.info.staged = (C::STAGED || C::IOV == 2) ? 1 : 0;
     e.info.iov = C::IOV;
-    e.info.lds_bytes = (int)(C::TPB * (C::LDS_N + (KIND == OFX_KIND_ROWT ? 1 : 0)) * sizeof(float2));
+    e.info.lds_bytes = (int)(C::TPB * (C::LDS_N + (KIND == OFX_KIND_ROWT ? 1 : 0)) * sizeof(typename C::V));
     e.info.tw_len = C::tw_len();
     e.fn = &launch<C, KIND>;
     return e;
@@ -459,6 +517,8 @@ Entry make_entry() {
 #define OFX_E(KIND, N, T, TPB, PADSH, STG, ...) make_entry<OFX_CFG(N, T, TPB, PADSH, STG, __VA_ARGS__), KIND>()
 #define OFX_CFGV(N, T, TPB, PADSH, ...) StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 2>
 #define OFX_EV(KIND, N, T, TPB, PADSH, ...) make_entry<OFX_CFGV(N, T, TPB, PADSH, __VA_ARGS__), KIND>()
+#define OFX_CFGZ(N, T, TPB, PADSH, ...) StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, double2>
+#define OFX_EZ(KIND, N, T, TPB, PADSH, ...) make_entry<OFX_CFGZ(N, T, TPB, PADSH, __VA_ARGS__), KIND>()
 
 #include "registry_1d.inc"
 
@@ -487,6 +547,29 @@ __global__ void k_gen_roots(float2* tw, long long len, long long count) {
     double s, c;
     sincospi(2.0 * (double)e / (double)len, &s, &c);
     tw[idx] = make_float2((float)c, (float)(-s));
+}
+
+// complex128 tables: the same FP64 values, not rounded
+__global__ void k_gen_pass_tw_z(double2* tw, int off, int ns, int r) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int cnt = ns * (r - 1);
+    if (idx >= cnt) return;
+    const long long m = idx / (r - 1);
+    const long long k = idx % (r - 1) + 1;
+    const long long L = (long long)ns * r;
+    const long long e = (m * k) % L;
+    double s, c;
+    sincospi(2.0 * (double)e / (double)L, &s, &c);
+    tw[off + idx] = make_double2(c, -s);
+}
+
+__global__ void k_gen_roots_z(double2* tw, long long len, long long count) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    const long long e = idx % len;
+    double s, c;
+    sincospi(2.0 * (double)e / (double)len, &s, &c);
+    tw[idx] = make_double2(c, -s);
 }
 
 }  // namespace ofx
@@ -531,5 +614,33 @@ extern "C" int ofx_gen_roots(void* tw, long long len, long long count, hipStream
     if (len <= 0) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(k_gen_roots, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream,
                        static_cast<float2*>(tw), len, count);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ofx_gen_pass_twiddles_z(void* tw, int npass, const int* radices, hipStream_t stream) {
+    if (npass < 0 || npass > OFX_MAX_PASSES) return (int)hipErrorInvalidValue;
+    int ns = 1, off = 0;
+    for (int p = 0; p < npass; ++p) {
+        const int r = radices[p];
+        if (p > 0) {
+            const int cnt = ns * (r - 1);
+            if (cnt > 0) {
+                hipLaunchKernelGGL(k_gen_pass_tw_z, dim3((cnt + 255) / 256), dim3(256), 0, stream,
+                                   static_cast<double2*>(tw), off, ns, r);
+                hipError_t e = hipGetLastError();
+                if (e != hipSuccess) return (int)e;
+            }
+            off += cnt;
+        }
+        ns *= r;
+    }
+    return 0;
+}
+
+extern "C" int ofx_gen_roots_z(void* tw, long long len, long long count, hipStream_t stream) {
+    if (count <= 0) return 0;
+    if (len <= 0) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gen_roots_z, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream,
+                       static_cast<double2*>(tw), len, count);
     return (int)hipGetLastError();
 }

@@ -36,8 +36,11 @@ struct Radices {
 // COLS_ > 0: column kernels.  The workgroup holds COLS_ transforms (columns of a row-major matrix)
 // interleaved in LDS, element i of column c at lds[c + i*(COLS_ + COLPAD_)], and consecutive
 // lanes take consecutive columns, so each HBM row segment of COLS_ values is one contiguous access.
-template <int N_, int T_, int TPB_, int PADSH_, bool STAGED_, class RL, int IOV_ = 1, int COLS_ = 0, int COLPAD_ = 1>
+// V_: the element type, float2 (complex64) or double2 (complex128).
+template <int N_, int T_, int TPB_, int PADSH_, bool STAGED_, class RL, int IOV_ = 1, int COLS_ = 0, int COLPAD_ = 1,
+          class V_ = float2>
 struct StockhamCfg {
+    using V = V_;
     static constexpr int N = N_;
     static constexpr int T = T_;
     static constexpr int TPB = TPB_;
@@ -90,24 +93,27 @@ struct StockhamCfg {
         }
     }
     static_assert(N == ns(NP), "radix product must equal N");
+    static_assert(IOV_ == 1 || sizeof(V_) == 8, "paired 16-byte I/O is for complex64 only");
 };
 
 // Reads element i of the current transform from LDS.
 template <class C>
 struct LdsRead {
-    const float2* base;
-    __device__ __forceinline__ float2 operator()(int i) const { return base[C::pad(i)]; }
+    const typename C::V* base;
+    __device__ __forceinline__ typename C::V operator()(int i) const { return base[C::pad(i)]; }
 };
 template <class C>
 struct LdsWrite {
-    float2* base;
-    __device__ __forceinline__ void operator()(int i, float2 v) const { base[C::pad(i)] = v; }
+    typename C::V* base;
+    __device__ __forceinline__ void operator()(int i, typename C::V v) const { base[C::pad(i)] = v; }
 };
 
 // One Stockham pass.  IN_LDS && OUT_LDS means source and destination are the same LDS buffer, so
 // all reads must complete before any write.
 template <class C, int P, bool IN_LDS, bool OUT_LDS, class Ld, class St>
-__device__ __forceinline__ void stockham_pass(int tid, const float2* __restrict__ tw, const Ld& ld, const St& st) {
+__device__ __forceinline__ void stockham_pass(int tid, const typename C::V* __restrict__ tw, const Ld& ld,
+                                              const St& st) {
+    using V = typename C::V;
     constexpr int R = C::radix(P);
     constexpr int Ns = C::ns(P);
     constexpr int NBF = C::N / R;
@@ -118,8 +124,8 @@ __device__ __forceinline__ void stockham_pass(int tid, const float2* __restrict_
     constexpr bool VSTORE = C::IOV == 2 && P == C::NP - 1 && !OUT_LDS && EXACT && NB % 2 == 0 && Ns == NBF;
     constexpr bool PAIRED = VLOAD || VSTORE;
     auto jof = [tid](int b) { return PAIRED ? (2 * tid + (b & 1) + (b >> 1) * 2 * C::T) : (tid + b * C::T); };
-    float2 v[NB][R];
-    float2 w[NB][R > 1 ? R - 1 : 1];
+    V v[NB][R];
+    V w[NB][R > 1 ? R - 1 : 1];
     if constexpr (VLOAD) {
 #pragma unroll
         for (int b = 0; b < NB; b += 2) {
@@ -141,7 +147,7 @@ __device__ __forceinline__ void stockham_pass(int tid, const float2* __restrict_
                 for (int k = 0; k < R; ++k) v[b][k] = ld(j + k * NBF);
             }
             if constexpr (P > 0) {
-                const float2* t = tw + C::tw_off(P) + (j % Ns) * (R - 1);
+                const V* t = tw + C::tw_off(P) + (j % Ns) * (R - 1);
 #pragma unroll
                 for (int k = 0; k < R - 1; ++k) w[b][k] = t[k];
             }
@@ -156,7 +162,7 @@ __device__ __forceinline__ void stockham_pass(int tid, const float2* __restrict_
 #pragma unroll
                 for (int k = 1; k < R; ++k) v[b][k] = cmul(v[b][k], w[b][k - 1]);
             }
-            Dft<R>::run(v[b]);
+            Dft<R, V>::run(v[b]);
             if constexpr (!VSTORE) {
                 const int base = (j / Ns) * (Ns * R) + (j % Ns);
 #pragma unroll
@@ -180,8 +186,8 @@ __device__ __forceinline__ void stockham_pass(int tid, const float2* __restrict_
 // whether those functors address this transform's LDS buffer (staged tiles, fused pre/post
 // processing) rather than HBM, which decides where the passes must synchronise.
 template <class C, bool SRC_LDS, bool DST_LDS, int P = 0, class GIn, class GOut>
-__device__ __forceinline__ void stockham_passes(float2* lds_t, int tid, const float2* __restrict__ tw, const GIn& gin,
-                                                const GOut& gout) {
+__device__ __forceinline__ void stockham_passes(typename C::V* lds_t, int tid, const typename C::V* __restrict__ tw,
+                                                const GIn& gin, const GOut& gout) {
     constexpr bool FIRST = (P == 0);
     constexpr bool LAST = (P == C::NP - 1);
     LdsRead<C> lr{lds_t};

@@ -142,7 +142,26 @@ struct Builder {
         t.bytes = (size_t)len * 8;
         return e->add_table(t);
     }
-    int pass_table(const ofx_kernel_info& k) { return pass_table_for(k.npass, k.radices, k.tw_len); }
+    int pass_table(const ofx_kernel_info& k) {
+        if (!is_z_kind(k.kind)) return pass_table_for(k.npass, k.radices, k.tw_len);
+        // complex128 kernel: the table in FP64 (16-byte entries)
+        if (k.tw_len <= 0) return -1;
+        TableSpec t;
+        t.kind = TableSpec::PASS_Z;
+        t.npass = k.npass;
+        for (int p = 0; p < k.npass; ++p) t.radices[p] = k.radices[p];
+        t.bytes = (size_t)k.tw_len * 16;
+        return e->add_table(t);
+    }
+    static bool is_z_kind(int k) { return k >= OFX_KIND_Z2Z && k <= OFX_KIND_Z2D_ODD; }
+    int roots_table_z(long long len, long long count) {
+        TableSpec t;
+        t.kind = TableSpec::ROOTS_Z;
+        t.len = len;
+        t.count = count;
+        t.bytes = (size_t)count * 16;
+        return e->add_table(t);
+    }
     int roots_table(long long len, long long count) {
         TableSpec t;
         t.kind = TableSpec::ROOTS;
@@ -681,6 +700,53 @@ struct Builder {
         return ACLFFT_SUCCESS;
     }
 
+    // complex128 / double transforms: compiled kernels only (lengths listed in the registry);
+    // other lengths plan successfully and report NOT_IMPLEMENTED at execution.
+    aclfftResult z2z_rows(long long n, long long batch) {
+        if (n == 1) {
+            Step s;
+            s.type = Step::COPY;
+            s.copy_bytes = (size_t)batch * 16;
+            e->steps_.push_back(s);
+            return ACLFFT_SUCCESS;
+        }
+        auto ids = kernels_for(OFX_KIND_Z2Z, n, true);
+        if (ids.empty()) return ACLFFT_NOT_IMPLEMENTED;
+        Step s = kernel_step(OFX_KIND_Z2Z, n, ids);
+        s.args.batch = batch;
+        s.args.idist = n;
+        s.args.odist = n;
+        s.conj_in = s.conj_out = true;
+        e->steps_.push_back(s);
+        return ACLFFT_SUCCESS;
+    }
+    aclfftResult d2z(long long n, long long batch) {
+        e->in_place_unsafe_ = true;
+        const bool even = (n % 2) == 0;
+        auto ids = kernels_for(even ? OFX_KIND_D2Z : OFX_KIND_D2Z_ODD, even ? n / 2 : n, true);
+        if (ids.empty()) return ACLFFT_NOT_IMPLEMENTED;
+        Step s = kernel_step(even ? OFX_KIND_D2Z : OFX_KIND_D2Z_ODD, even ? n / 2 : n, ids);
+        s.args.batch = batch;
+        s.args.idist = n;
+        s.args.odist = n / 2 + 1;
+        if (even) s.aux_table = roots_table_z(n, n / 2 + 1);
+        e->steps_.push_back(s);
+        return ACLFFT_SUCCESS;
+    }
+    aclfftResult z2d(long long n, long long batch) {
+        e->in_place_unsafe_ = true;
+        const bool even = (n % 2) == 0;
+        auto ids = kernels_for(even ? OFX_KIND_Z2D : OFX_KIND_Z2D_ODD, even ? n / 2 : n, true);
+        if (ids.empty()) return ACLFFT_NOT_IMPLEMENTED;
+        Step s = kernel_step(even ? OFX_KIND_Z2D : OFX_KIND_Z2D_ODD, even ? n / 2 : n, ids);
+        s.args.batch = batch;
+        s.args.idist = n / 2 + 1;
+        s.args.odist = n;
+        if (even) s.aux_table = roots_table_z(n, n / 2 + 1);
+        e->steps_.push_back(s);
+        return ACLFFT_SUCCESS;
+    }
+
     aclfftResult r2c(long long n, long long batch) {
         const long long h = n / 2 + 1;
         e->in_place_unsafe_ = true;
@@ -834,6 +900,20 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
                 e->out_bytes_ = (size_t)n * d.batch * 4;
                 r = b.c2r(n, d.batch);
                 break;
+            case ACLFFT_Z2Z:
+                e->in_bytes_ = e->out_bytes_ = (size_t)n * d.batch * 16;
+                r = b.z2z_rows(n, d.batch);
+                break;
+            case ACLFFT_D2Z:
+                e->in_bytes_ = (size_t)n * d.batch * 8;
+                e->out_bytes_ = (size_t)(n / 2 + 1) * d.batch * 16;
+                r = b.d2z(n, d.batch);
+                break;
+            case ACLFFT_Z2D:
+                e->in_bytes_ = (size_t)(n / 2 + 1) * d.batch * 16;
+                e->out_bytes_ = (size_t)n * d.batch * 8;
+                r = b.z2d(n, d.batch);
+                break;
             default:
                 r = ACLFFT_NOT_SUPPORTED;
         }
@@ -868,6 +948,7 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
         for (auto it = e->steps_.rbegin(); it != e->steps_.rend(); ++it)
             if (it->dst == BUF_OUT && it->type != Step::COPY && it->type != Step::PERSISTENT) {
                 it->args.scale *= d.scale;
+                it->args.scale64 = (double)it->args.scale;
                 break;
             }
     }
@@ -876,9 +957,10 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
 }
 
 Executor* Executor::adjoint() {
-    if (adjoint_ == nullptr && (desc_.type == ACLFFT_R2C || desc_.type == ACLFFT_C2R)) {
+    const aclfftType t = desc_.type;
+    if (adjoint_ == nullptr && (t == ACLFFT_R2C || t == ACLFFT_C2R || t == ACLFFT_D2Z || t == ACLFFT_Z2D)) {
         PlanDesc d = desc_;
-        d.type = (desc_.type == ACLFFT_R2C) ? ACLFFT_C2R : ACLFFT_R2C;
+        d.type = t == ACLFFT_R2C ? ACLFFT_C2R : t == ACLFFT_C2R ? ACLFFT_R2C : t == ACLFFT_D2Z ? ACLFFT_Z2D : ACLFFT_D2Z;
         d.hermitian_weight = true;
         Executor* e = nullptr;
         if (build(d, &e) == ACLFFT_SUCCESS) {
@@ -1026,6 +1108,10 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
             rc = ofx_gen_roots(p, t.len, t.count, s);
         else if (t.kind == TableSpec::MFMA_CONST)
             rc = ofx_gen_mfma_consts(t.npass, p, s);
+        else if (t.kind == TableSpec::PASS_Z)
+            rc = ofx_gen_pass_twiddles_z(p, t.npass, t.radices, s);
+        else if (t.kind == TableSpec::ROOTS_Z)
+            rc = ofx_gen_roots_z(p, t.len, t.count, s);
         else if (t.kind == TableSpec::BLOB)
             rc = (int)hipMemcpyAsync(p, t.blob.data(), t.bytes, hipMemcpyHostToDevice, s);
         else

@@ -23,7 +23,7 @@ enum BufId : int { BUF_IN = 0, BUF_OUT = 1, BUF_WORK0 = 2, BUF_WORK1 = 3 };
 
 // Device lookup tables a plan needs; all live in one allocation.
 struct TableSpec {
-    enum Kind { PASS, ROOTS, FOURSTEP, MFMA_CONST, BLOB } kind = PASS;
+    enum Kind { PASS, ROOTS, FOURSTEP, MFMA_CONST, BLOB, PASS_Z, ROOTS_Z } kind = PASS;  // *_Z: complex128
     int npass = 0;                 // PASS: radix sequence; MFMA_CONST: npass = mode
     int radices[OFX_MAX_PASSES] = {0};
     long long len = 0, count = 0;  // ROOTS: W_len^k, k < count; FOURSTEP: FP64 table of W_len

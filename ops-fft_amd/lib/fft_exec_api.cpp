@@ -1,4 +1,5 @@
-// fft_exec_api.cpp - aclfftExecC2C / aclfftExecR2C / aclfftExecC2R (+ double-precision stubs).
+// fft_exec_api.cpp - aclfftExecC2C / aclfftExecR2C / aclfftExecC2R and their double-precision
+// counterparts aclfftExecZ2Z / aclfftExecD2Z / aclfftExecZ2D.
 //
 // Checks and codes follow the reference lib/fft_exec_api.cpp:32-116 and the operator-level entry
 // checks (src/common/host/fft_exec_helper.h:37-61; src/fft1_d/arch35/fft1_d_exec_api.cpp:19-27):
@@ -59,27 +60,39 @@ ACLFFT_API aclfftResult aclfftExecC2R(aclfftHandle plan, aclfftComplex* idata, a
     return run(plan, idata, odata, ACLFFT_BACKWARD);
 }
 
+// Double precision.  The reference declares these (include/cann_ops_fft.h) and ships them as
+// NOT_IMPLEMENTED stubs (lib/fft_exec_api.cpp:87-116), refusing D2Z/Z2D/Z2Z plans at creation; here
+// double-precision 1D plans are accepted (ACLFFT_STRICT_REFERENCE_DOMAIN=1 restores the refusal)
+// and these entry points run them with the same checks as their single-precision counterparts.
 ACLFFT_API aclfftResult aclfftExecZ2Z(aclfftHandle plan, aclfftDoubleComplex* idata, aclfftDoubleComplex* odata,
                                       int direction) {
-    (void)plan;
-    (void)idata;
-    (void)odata;
-    (void)direction;
-    return ACLFFT_NOT_IMPLEMENTED;
+    aclfftResult r = common_checks(plan, idata, odata, ACLFFT_Z2Z);
+    if (r != ACLFFT_SUCCESS) return r;
+    if (direction != ACLFFT_FORWARD && direction != ACLFFT_BACKWARD) return ACLFFT_INVALID_VALUE;
+    if (plan->rank != 1) return ACLFFT_NOT_IMPLEMENTED;
+    return run(plan, idata, odata, direction);
 }
 
 ACLFFT_API aclfftResult aclfftExecD2Z(aclfftHandle plan, aclfftDoubleReal* idata, aclfftDoubleComplex* odata) {
-    (void)plan;
-    (void)idata;
-    (void)odata;
-    return ACLFFT_NOT_IMPLEMENTED;
+    aclfftResult r = common_checks(plan, idata, odata, ACLFFT_D2Z);
+    if (r != ACLFFT_SUCCESS) return r;
+    if (plan->rank != 1) return ACLFFT_NOT_IMPLEMENTED;
+    if (plan->stride[0] > 1) {
+        ofx::log_reject("D2Z", "VERTICAL layout is not supported");
+        return ACLFFT_NOT_IMPLEMENTED;
+    }
+    return run(plan, idata, odata, ACLFFT_FORWARD);
 }
 
 ACLFFT_API aclfftResult aclfftExecZ2D(aclfftHandle plan, aclfftDoubleComplex* idata, aclfftDoubleReal* odata) {
-    (void)plan;
-    (void)idata;
-    (void)odata;
-    return ACLFFT_NOT_IMPLEMENTED;
+    aclfftResult r = common_checks(plan, idata, odata, ACLFFT_Z2D);
+    if (r != ACLFFT_SUCCESS) return r;
+    if (plan->rank != 1) return ACLFFT_NOT_IMPLEMENTED;
+    if (plan->stride[0] > 1) {
+        ofx::log_reject("Z2D", "VERTICAL layout is not supported");
+        return ACLFFT_NOT_IMPLEMENTED;
+    }
+    return run(plan, idata, odata, ACLFFT_BACKWARD);
 }
 
 }  // extern "C"

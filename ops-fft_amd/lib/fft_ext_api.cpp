@@ -96,7 +96,9 @@ static aclfftResult exec_adjoint(aclfftHandle plan, const void* g, void* out, ac
     if (plan == nullptr) return ACLFFT_INVALID_VALUE;
     if (!plan->is_initialized) return ACLFFT_INVALID_PLAN;
     if (g == nullptr || out == nullptr) return ACLFFT_INVALID_VALUE;
-    if (plan->type != expect) return ACLFFT_INVALID_TYPE;
+    // the real-to-complex adjoint serves R2C and D2Z plans, the complex-to-real one C2R and Z2D
+    const aclfftType twin = expect == ACLFFT_R2C ? ACLFFT_D2Z : ACLFFT_Z2D;
+    if (plan->type != expect && plan->type != twin) return ACLFFT_INVALID_TYPE;
     if (plan->rank != 1 || plan->stride[0] > 1 || plan->exec == nullptr) return ACLFFT_NOT_IMPLEMENTED;
     ofx::Executor* a = plan->exec->adjoint();
     if (a == nullptr) return ACLFFT_NOT_IMPLEMENTED;

@@ -30,6 +30,15 @@ inline bool is_single_precision_type(aclfftType t) {
 
 inline void log_reject(const char* entry, const char* what) { std::fprintf(stderr, "[ops-fft] %s: %s\n", entry, what); }
 
+// When ACLFFT_STRICT_REFERENCE_DOMAIN=1 the plan entry points keep the reference's domain exactly:
+// 2D plans limited to {32,64,128}^2 (lib/fft_plan_validate.h:52-72) and double precision refused
+// (lib/fft_plan_api.cpp:58-62).  By default any 2D nx, ny in [1, 2^27] and 1D double precision
+// are accepted.
+inline bool strict_reference_domain() {
+    const char* v = std::getenv("ACLFFT_STRICT_REFERENCE_DOMAIN");
+    return v != nullptr && v[0] == '1';
+}
+
 // 1D length domain by layout (reference lib/fft_plan_validate.h:30-47).
 inline aclfftResult check_1d_domain(int dimType, int nx, const char* entry) {
     char msg[160];
@@ -56,20 +65,15 @@ inline aclfftResult check_1d_args(int nx, aclfftType type, int batch, int dimTyp
     if (batch <= 0) return ACLFFT_INVALID_SIZE;
     if (dimType != ACLFFT_HORIZONTAL && dimType != ACLFFT_VERTICAL) return ACLFFT_INVALID_VALUE;
     if (!is_known_type(type)) return ACLFFT_INVALID_TYPE;
-    if (!is_single_precision_type(type)) {
+    if (!is_single_precision_type(type) && strict_reference_domain()) {
+        // the reference refuses double precision at plan time (lib/fft_plan_api.cpp:58-62); by
+        // default this library accepts it (1D Z2Z / D2Z / Z2D)
         char msg[96];
         std::snprintf(msg, sizeof msg, "type=%d has no 1D execution implementation yet", (int)type);
         log_reject(entry, msg);
         return ACLFFT_NOT_SUPPORTED;
     }
     return check_1d_domain(dimType, nx, entry);
-}
-
-// When ACLFFT_STRICT_REFERENCE_DOMAIN=1, 2D plans are limited to the reference's {32,64,128}^2
-// domain (lib/fft_plan_validate.h:52-72).  By default any nx, ny in [1, 2^27] is accepted.
-inline bool strict_reference_domain() {
-    const char* v = std::getenv("ACLFFT_STRICT_REFERENCE_DOMAIN");
-    return v != nullptr && v[0] == '1';
 }
 
 inline aclfftResult check_2d_domain(int nx, int ny, const char* entry) {

@@ -87,10 +87,12 @@ def test_plan1d_argument_gates():
     assert plan1d(64, aclfft.C2C, 0)[0] == aclfft.INVALID_SIZE
     assert plan1d(64, aclfft.C2C, 1, 2)[0] == aclfft.INVALID_VALUE
     assert plan1d(64, 0x99, 1)[0] == aclfft.INVALID_TYPE
-    # issue #171 in the reference: double precision refused at plan time
+    # issue #171 in the reference: double precision refused at plan time.  This library accepts
+    # 1D double precision (a superset); the strict mode restores the reference's refusal.
     for t in (aclfft.Z2Z, aclfft.D2Z, aclfft.Z2D):
         r, h = plan1d(256, t, 1)
-        assert r == aclfft.NOT_SUPPORTED and not h.value
+        assert r == aclfft.SUCCESS and h.value
+        lib.aclfftDestroy(h)
     # 1D size domain (issue #190): horizontal <= 2^27; vertical pow2 in [256, 2^18]
     assert plan1d((1 << 27) + 1, aclfft.C2C, 1)[0] == aclfft.INVALID_SIZE
     for n in (128, 300, 64, (1 << 18) * 2):
@@ -186,9 +188,10 @@ def test_plan3d_and_placeholders():
     v = ctypes.c_int()
     assert lib.aclfftGetVersion(ctypes.byref(v)) == aclfft.NOT_IMPLEMENTED
     assert lib.aclfftGetProperty(0, ctypes.byref(v)) == aclfft.NOT_IMPLEMENTED
-    assert lib.aclfftExecZ2Z(None, None, None, -1) == aclfft.NOT_IMPLEMENTED
-    assert lib.aclfftExecD2Z(None, None, None) == aclfft.NOT_IMPLEMENTED
-    assert lib.aclfftExecZ2D(None, None, None) == aclfft.NOT_IMPLEMENTED
+    # double-precision execution is implemented: a null plan is INVALID_VALUE, as for C2C
+    assert lib.aclfftExecZ2Z(None, None, None, -1) == aclfft.INVALID_VALUE
+    assert lib.aclfftExecD2Z(None, None, None) == aclfft.INVALID_VALUE
+    assert lib.aclfftExecZ2D(None, None, None) == aclfft.INVALID_VALUE
 
 
 def test_exec_gates_before_any_device_call():
@@ -318,4 +321,32 @@ def test_workspace_api_host_side():
     assert lib.aclfftSetWorkArea(h, None) == aclfft.INVALID_VALUE
     host = (ctypes.c_char * ws.value)()
     assert lib.aclfftSetWorkArea(h, ctypes.cast(host, ctypes.c_void_p)) == aclfft.INVALID_VALUE
+    lib.aclfftDestroy(h)
+
+
+def test_double_precision_gates(monkeypatch):
+    """1D Z2Z / D2Z / Z2D plans: accepted by default, refused (reference issue #171) in the strict
+    mode; exec type and direction checks as for the single-precision entry points."""
+    H = ctypes.c_void_p
+    monkeypatch.setenv("ACLFFT_STRICT_REFERENCE_DOMAIN", "1")
+    for t in (aclfft.Z2Z, aclfft.D2Z, aclfft.Z2D):
+        h = H()
+        assert lib.aclfftPlan1d(ctypes.byref(h), 256, t, 1, 0) == aclfft.NOT_SUPPORTED and not h.value
+    monkeypatch.delenv("ACLFFT_STRICT_REFERENCE_DOMAIN")
+    buf = (ctypes.c_double * 1024)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    h = H()
+    assert lib.aclfftPlan1d(ctypes.byref(h), 256, aclfft.Z2Z, 2, 0) == aclfft.SUCCESS
+    assert lib.aclfftExecZ2Z(h, p, p, 0) == aclfft.INVALID_VALUE        # direction
+    assert lib.aclfftExecC2C(h, p, p, -1) == aclfft.INVALID_TYPE        # plan/exec type mismatch
+    assert lib.aclfftExecD2Z(h, p, p) == aclfft.INVALID_TYPE
+    assert lib.aclfftExecZ2Z(h, None, p, -1) == aclfft.INVALID_VALUE
+    lib.aclfftDestroy(h)
+    h = H()
+    assert lib.aclfftPlan1d(ctypes.byref(h), 512, aclfft.D2Z, 1, aclfft.VERTICAL) == aclfft.SUCCESS
+    assert lib.aclfftExecD2Z(h, p, p) == aclfft.NOT_IMPLEMENTED         # VERTICAL real layout
+    lib.aclfftDestroy(h)
+    h = H()
+    assert lib.aclfftPlan1d(ctypes.byref(h), 4096, aclfft.Z2D, 3, 0) == aclfft.SUCCESS
+    assert lib.aclfftExecZ2D(h, p, p) != aclfft.INVALID_TYPE
     lib.aclfftDestroy(h)
