@@ -155,6 +155,46 @@ def extra_real(torch, aclfft, oracle, np, steps, stream):
     return out
 
 
+def extra_double(torch, aclfft, oracle, np, steps, stream):
+    """Double precision (aclfftExecZ2Z / D2Z / Z2D): C2C n=1024 batch 32768 (512 MiB of complex128
+    each way, the headline's points at twice the bytes) and R2C/C2R n=4096 batch 4096."""
+    out = {}
+    n, b = 1024, 32768
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x = torch.randn(b, n, dtype=torch.complex128, device="cuda", generator=g)
+    y = torch.empty_like(x)
+    p = aclfft.Plan.plan1d(n, aclfft.Z2Z, b)
+    p.set_stream(stream)
+    ms = time_steps(torch, lambda: p.exec_z2z(x, y, aclfft.FORWARD), steps, 3, stream)
+    err = oracle.rel_l2(y[:64].cpu().numpy(), oracle.c2c(x[:64].cpu().numpy()))
+    out["z2z_n1024_b32768"] = {"ms": round(ms, 5), "hbm_frac": round(32.0 * n * b / ms / 1e6 / HBM_PEAK_GBS, 4),
+                               "GFLOP/s": round(5.0 * n * math.log2(n) * b / ms / 1e6, 1),
+                               "rel_l2_vs_fp64_oracle": err, "plan": p.describe()}
+    p.destroy()
+    del x, y
+    n, b = 4096, 4096
+    xr = torch.randn(b, n, dtype=torch.float64, device="cuda", generator=g)
+    X = torch.empty(b, n // 2 + 1, dtype=torch.complex128, device="cuda")
+    xb = torch.empty_like(xr)
+    pr = aclfft.Plan.plan1d(n, aclfft.D2Z, b)
+    pc = aclfft.Plan.plan1d(n, aclfft.Z2D, b)
+    pr.set_stream(stream)
+    pc.set_stream(stream)
+    ms_r = time_steps(torch, lambda: pr.exec_d2z(xr, X), steps, 3, stream)
+    ms_c = time_steps(torch, lambda: pc.exec_z2d(X, xb), steps, 3, stream)
+    byts = (8.0 * n + 16.0 * (n // 2 + 1)) * b
+    err = oracle.rel_l2(X[:32].cpu().numpy(), oracle.r2c(xr[:32].cpu().numpy()))
+    rt = float(((xb / n - xr).norm() / xr.norm()).item())
+    out["d2z_z2d_n4096_b4096"] = {"d2z_ms": round(ms_r, 5), "z2d_ms": round(ms_c, 5),
+                                  "d2z_hbm_frac": round(byts / ms_r / 1e6 / HBM_PEAK_GBS, 4),
+                                  "z2d_hbm_frac": round(byts / ms_c / 1e6 / HBM_PEAK_GBS, 4),
+                                  "d2z_rel_l2_vs_fp64_oracle": err, "roundtrip_rel_l2": rt,
+                                  "plan_d2z": pr.describe()}
+    pr.destroy()
+    pc.destroy()
+    return out
+
+
 def extra_small_latency(torch, aclfft, oracle, np, stream):
     """The launch-bound configuration (C2C N=1024, batch 64, 512 KiB each way): time per call,
     eager and replayed from a hipGraph, measured over many back-to-back calls (device time per
@@ -348,6 +388,7 @@ def main():
                                                                         args.steps, stream, err)),
                          ("real_r2c_c2r", lambda: extra_real(torch, aclfft, oracle, np, args.steps, stream)),
                          ("c2c_n1024_b64_latency", lambda: extra_small_latency(torch, aclfft, oracle, np, stream)),
+                         ("double_precision", lambda: extra_double(torch, aclfft, oracle, np, args.steps, stream)),
                          ("c2c_2d_2048x2048_b4", lambda: extra_2d(torch, aclfft, oracle, np, 10, stream)),
                          ("uneven_batched_512", lambda: extra_batched(torch, aclfft, oracle, np, 20, stream))):
             try:

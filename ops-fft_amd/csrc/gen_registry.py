@@ -96,10 +96,21 @@ def tpb_for_z(n, t):
     return tpb
 
 
+# complex128 configurations measured fastest on MI355X (about 8 elements per thread: complex128
+# doubles the registers per element); they lead their length's candidate list
+ZFIRST = [
+    ("Z2Z", 256, 32, 4, (4, 8, 8)), ("Z2Z", 512, 64, 2, (8, 8, 8)), ("Z2Z", 1024, 128, 2, (8, 8, 16)),
+    ("Z2Z", 2048, 256, 1, (8, 8, 8, 4)), ("Z2Z", 4096, 512, 1, (8, 8, 8, 8)),
+    ("D2Z", 1024, 128, 1, (8, 8, 16)), ("Z2D", 1024, 128, 1, (8, 8, 16)),
+    ("D2Z", 2048, 256, 1, (8, 8, 8, 4)), ("Z2D", 2048, 256, 1, (8, 8, 8, 4)),
+    ("D2Z", 4096, 512, 1, (8, 8, 8, 8)), ("Z2D", 4096, 512, 1, (8, 8, 8, 8)),
+]
+
+
 def z_lines():
     """complex128 kernels: Z2Z for every length with a radix plan, D2Z/Z2D for the even real
     lengths up to 8192 (their half-length complex kernels) and for the odd lengths 243 and 6561."""
-    out = []
+    out = [(kind, n, t, tpb, padsh(n), rad) for kind, n, t, tpb, rad in ZFIRST]
     for n, plans in sorted(PLANS.items()):
         if n > 6561:
             continue
@@ -116,6 +127,7 @@ def z_lines():
         t = choose_t(n, rad)
         out.append(("D2Z_ODD", n, t, tpb_for_z(n, t), padsh(n), rad))
         out.append(("Z2D_ODD", n, t, tpb_for_z(n, t), padsh(n), rad))
+    out.append(("Z2Z", 1024, 128, 1, padsh(1024), (8, 8, 16)))
     return out
 
 

@@ -107,3 +107,53 @@ def test_double_real_adjoints(cuda, n):
     rhs = np.sum(x * gx.cpu().numpy())
     assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + np.linalg.norm(x) * np.linalg.norm(g))
     p.destroy()
+
+
+def _z_variants():
+    seen, out = {}, []
+    for k in aclfft.kernel_registry():
+        if k["kind"] not in (8, 9, 10, 11, 12):
+            continue
+        key = (k["kind"], k["n"])
+        idx = seen.get(key, 0)
+        seen[key] = idx + 1
+        out.append((k["kind"], k["n"], idx))
+    return out
+
+
+@pytest.mark.parametrize("kind,n,variant", _z_variants())
+def test_every_double_kernel(cuda, monkeypatch, kind, n, variant):
+    """Each compiled complex128 configuration, forced with ACLFFT_KERNEL_VARIANT."""
+    import torch
+    monkeypatch.setenv("ACLFFT_KERNEL_VARIANT", str(variant))
+    batch = 11
+    if kind == 8:
+        x = crandn((batch, n), n + variant)
+        p = aclfft.Plan.plan1d(n, aclfft.Z2Z, batch)
+        y = torch.empty((batch, n), dtype=torch.complex128, device="cuda")
+        p.exec_z2z(torch.from_numpy(x).cuda(), y, aclfft.BACKWARD)
+        torch.cuda.synchronize()
+        err = oracle.rel_l2(y.cpu().numpy(), oracle.c2c(x, aclfft.BACKWARD))
+        m = n
+    elif kind in (9, 11):
+        m = 2 * n if kind == 9 else n
+        x = np.random.default_rng(variant).standard_normal((batch, m))
+        p = aclfft.Plan.plan1d(m, aclfft.D2Z, batch)
+        y = torch.empty((batch, m // 2 + 1), dtype=torch.complex128, device="cuda")
+        p.exec_d2z(torch.from_numpy(x).cuda(), y)
+        torch.cuda.synchronize()
+        err = oracle.rel_l2(y.cpu().numpy(), oracle.r2c(x))
+    else:
+        m = 2 * n if kind == 10 else n
+        X = crandn((batch, m // 2 + 1), variant)
+        X[:, 0] = X[:, 0].real
+        if m % 2 == 0:
+            X[:, -1] = X[:, -1].real
+        p = aclfft.Plan.plan1d(m, aclfft.Z2D, batch)
+        y = torch.empty((batch, m), dtype=torch.float64, device="cuda")
+        p.exec_z2d(torch.from_numpy(X).cuda(), y)
+        torch.cuda.synchronize()
+        err = oracle.rel_l2(y.cpu().numpy(), oracle.c2r(X, m))
+    print(p.describe(), "rel_l2 %.3e" % err)
+    p.destroy()
+    assert err <= fp64_tol(m), err
