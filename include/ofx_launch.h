@@ -128,7 +128,7 @@ typedef struct ofx_generic_desc {
     int npass;
     int radices[OFX_MAX_PASSES];
     int tw_off[OFX_MAX_PASSES];     /* offset of each pass in the pass-twiddle table          */
-    int tpb;                        /* transforms per workgroup                               */
+    int tpb;                        /* transforms per workgroup, 1 .. 64                       */
     int in_mode, out_mode;
     int in_tmajor, out_tmajor;      /* 1: consecutive lanes walk consecutive transforms (strided
                                        layouts), 0: consecutive elements of one transform      */

@@ -19,6 +19,7 @@
 namespace ofx {
 
 constexpr int kGenBlock = 256;
+constexpr int kGenMaxTpb = 64;  // transforms per workgroup (ofx_generic_desc::tpb)
 
 struct GenParams {
     ofx_args a;
@@ -28,19 +29,25 @@ struct GenParams {
 // ---------------------------------------------------------------------------------------------
 // one pass of radix R over all transforms of the workgroup (LDS src -> LDS dst)
 // ---------------------------------------------------------------------------------------------
+// LDS layout: transform t at buf + t*ln, element i at gpad(i) = i + i/16 (ln = gpad(n)).  The pad
+// breaks the power-of-two strides of the passes (pass 0 writes element j*R + k from lane j: without
+// it the lanes of a wave meet on a few banks).
+__device__ __forceinline__ int gpad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int gen_ln(int n) { return n + (n >> 4); }
+
 template <int R>
-__device__ __forceinline__ void gen_pass(const float2* __restrict__ src, float2* __restrict__ dst, int n, int ns,
-                                         int tpb, const float2* __restrict__ tw) {
+__device__ __forceinline__ void gen_pass(const float2* __restrict__ src, float2* __restrict__ dst, int n, int ln,
+                                         int ns, int tpb, const float2* __restrict__ tw) {
     const int nbf = n / R;
     const int total = tpb * nbf;
     for (int f = threadIdx.x; f < total; f += blockDim.x) {
         const int t = f / nbf;
         const int j = f - t * nbf;
-        const float2* s = src + t * n;
-        float2* d = dst + t * n;
+        const float2* s = src + t * ln;
+        float2* d = dst + t * ln;
         float2 v[R];
 #pragma unroll
-        for (int k = 0; k < R; ++k) v[k] = s[j + k * nbf];
+        for (int k = 0; k < R; ++k) v[k] = s[gpad(j + k * nbf)];
         const int m = j % ns;
         if (ns > 1) {
             const float2* w = tw + m * (R - 1);
@@ -50,37 +57,61 @@ __device__ __forceinline__ void gen_pass(const float2* __restrict__ src, float2*
         Dft<R>::run(v);
         const int base = (j - m) * R + m;
 #pragma unroll
-        for (int k = 0; k < R; ++k) d[base + k * ns] = v[k];
+        for (int k = 0; k < R; ++k) d[gpad(base + k * ns)] = v[k];
     }
 }
 
+// BIG = false: radices up to 16 only (the common smooth lengths; a much smaller register
+// footprint, since the kernel's VGPR count is set by its largest codelet); BIG = true adds the
+// prime radices 17..47.
+template <bool BIG>
 __device__ __forceinline__ bool run_pass(int r, const float2* src, float2* dst, int n, int ns, int tpb,
                                          const float2* tw) {
+    const int ln = gen_ln(n);
+    if constexpr (!BIG) {
+        switch (r) {
+            case 2: gen_pass<2>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 3: gen_pass<3>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 4: gen_pass<4>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 5: gen_pass<5>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 6: gen_pass<6>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 7: gen_pass<7>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 8: gen_pass<8>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 9: gen_pass<9>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 10: gen_pass<10>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 11: gen_pass<11>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 12: gen_pass<12>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 13: gen_pass<13>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 15: gen_pass<15>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 16: gen_pass<16>(src, dst, n, ln, ns, tpb, tw); return true;
+            default: return false;
+        }
+    }
     switch (r) {
-        case 2: gen_pass<2>(src, dst, n, ns, tpb, tw); return true;
-        case 3: gen_pass<3>(src, dst, n, ns, tpb, tw); return true;
-        case 4: gen_pass<4>(src, dst, n, ns, tpb, tw); return true;
-        case 5: gen_pass<5>(src, dst, n, ns, tpb, tw); return true;
-        case 6: gen_pass<6>(src, dst, n, ns, tpb, tw); return true;
-        case 7: gen_pass<7>(src, dst, n, ns, tpb, tw); return true;
-        case 8: gen_pass<8>(src, dst, n, ns, tpb, tw); return true;
-        case 9: gen_pass<9>(src, dst, n, ns, tpb, tw); return true;
-        case 10: gen_pass<10>(src, dst, n, ns, tpb, tw); return true;
-        case 11: gen_pass<11>(src, dst, n, ns, tpb, tw); return true;
-        case 12: gen_pass<12>(src, dst, n, ns, tpb, tw); return true;
-        case 13: gen_pass<13>(src, dst, n, ns, tpb, tw); return true;
-        case 15: gen_pass<15>(src, dst, n, ns, tpb, tw); return true;
-        case 16: gen_pass<16>(src, dst, n, ns, tpb, tw); return true;
-        case 17: gen_pass<17>(src, dst, n, ns, tpb, tw); return true;
-        case 19: gen_pass<19>(src, dst, n, ns, tpb, tw); return true;
-        case 23: gen_pass<23>(src, dst, n, ns, tpb, tw); return true;
-        case 25: gen_pass<25>(src, dst, n, ns, tpb, tw); return true;
-        case 29: gen_pass<29>(src, dst, n, ns, tpb, tw); return true;
-        case 31: gen_pass<31>(src, dst, n, ns, tpb, tw); return true;
-        case 37: gen_pass<37>(src, dst, n, ns, tpb, tw); return true;
-        case 41: gen_pass<41>(src, dst, n, ns, tpb, tw); return true;
-        case 43: gen_pass<43>(src, dst, n, ns, tpb, tw); return true;
-        case 47: gen_pass<47>(src, dst, n, ns, tpb, tw); return true;
+        case 2: gen_pass<2>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 3: gen_pass<3>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 4: gen_pass<4>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 5: gen_pass<5>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 6: gen_pass<6>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 7: gen_pass<7>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 8: gen_pass<8>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 9: gen_pass<9>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 10: gen_pass<10>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 11: gen_pass<11>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 12: gen_pass<12>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 13: gen_pass<13>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 15: gen_pass<15>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 16: gen_pass<16>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 17: gen_pass<17>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 19: gen_pass<19>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 23: gen_pass<23>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 25: gen_pass<25>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 29: gen_pass<29>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 31: gen_pass<31>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 37: gen_pass<37>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 41: gen_pass<41>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 43: gen_pass<43>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 47: gen_pass<47>(src, dst, n, ln, ns, tpb, tw); return true;
         default: return false;
     }
 }
@@ -89,17 +120,31 @@ __device__ __forceinline__ bool run_pass(int r, const float2* src, float2* dst, 
 // ---------------------------------------------------------------------------------------------
 // the kernel
 // ---------------------------------------------------------------------------------------------
+template <bool BIG>
 __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
     extern __shared__ float2 lds[];
     const ofx_args& a = p.a;
     const ofx_generic_desc& d = p.d;
     const int n = d.n;
     const int tpb = d.tpb;
+    const int ln = gen_ln(n);
     float2* buf0 = lds;
-    float2* buf1 = lds + (size_t)tpb * n;
+    float2* buf1 = lds + (size_t)tpb * ln;
     const long long t0 = (long long)blockIdx.x * tpb;
     const int nvalid = (int)min((long long)tpb, a.batch - t0);
     const float2* tw = static_cast<const float2*>(a.tw);
+
+    // per-transform addressing, once per workgroup: (t_hi, t_lo) needs a 64-bit division, which
+    // per element would cost more than the element's arithmetic
+    __shared__ long long s_ibase[kGenMaxTpb], s_obase[kGenMaxTpb], s_tlo[kGenMaxTpb];
+    if (threadIdx.x < nvalid) {
+        const long long tg = t0 + threadIdx.x;
+        const long long thi = tg / d.cnt_lo, tlo = tg - thi * d.cnt_lo;
+        s_ibase[threadIdx.x] = thi * a.idist + tlo * d.idist_lo;
+        s_obase[threadIdx.x] = thi * a.odist + tlo * d.odist_lo;
+        s_tlo[threadIdx.x] = tlo;
+    }
+    __syncthreads();
 
     // ---- load into buf0 ----------------------------------------------------------------------
     const int h = n / 2;  // used by the real-data modes
@@ -112,9 +157,7 @@ __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
             t = f / n;
             i = f - t * n;
         }
-        const long long tg = t0 + t;
-        const long long thi = tg / d.cnt_lo, tlo = tg - thi * d.cnt_lo;
-        const long long base = thi * a.idist + tlo * d.idist_lo;
+        const long long base = s_ibase[t];
         float2 v;
         if (d.in_mode == OFX_IN_COMPLEX) {
             v = static_cast<const float2*>(a.in)[base + i * a.istride];
@@ -151,7 +194,7 @@ __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
             const float2 df = csub(xa, bc);
             v = cconj(cadd(s, mul_posi(cmulc(df, W[i]))));
         }
-        buf0[t * n + i] = v;
+        buf0[t * ln + gpad(i)] = v;
     }
     __syncthreads();
 
@@ -161,7 +204,7 @@ __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
     int ns = 1;
     for (int p = 0; p < d.npass; ++p) {
         const int r = d.radices[p];
-        run_pass(r, src, dst, n, ns, nvalid, tw + d.tw_off[p]);
+        run_pass<BIG>(r, src, dst, n, ns, nvalid, tw + d.tw_off[p]);
         __syncthreads();
         ns *= r;
         float2* tmp = src;
@@ -181,28 +224,30 @@ __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
             t = f / nout;
             k = f - t * nout;
         }
-        const long long tg = t0 + t;
-        const long long thi = tg / d.cnt_lo, tlo = tg - thi * d.cnt_lo;
-        const long long base = thi * a.odist + tlo * d.odist_lo;
-        const float2* s = src + t * n;
+        const long long base = s_obase[t];
+        const float2* s = src + t * ln;
         if (d.out_mode == OFX_OUT_COMPLEX) {
-            float2 v = s[k];
+            float2 v = s[gpad(k)];
             v.y *= a.out_sign;
-            if (d.ft_len > 0) v = cmul(v, ft_twiddle(ftab, ((tlo + d.ft_base) * (long long)k) % d.ft_len, d.ft_len));
+            if (d.ft_len > 0) {
+                long long e = (s_tlo[t] + d.ft_base) * (long long)k;
+                if (e >= d.ft_len) e %= d.ft_len;  // never for the four-step's own steps
+                v = cmul(v, ft_twiddle(ftab, e, d.ft_len));
+            }
             static_cast<float2*>(a.out)[base + k * a.ostride] = make_float2(v.x * a.scale, v.y * a.scale);
         } else if (d.out_mode == OFX_OUT_HALF) {
             const float w = (k == 0) ? a.scale : a.scale * ((a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 2.f : 1.f);
-            float2 v = cscale(s[k], w);
+            float2 v = cscale(s[gpad(k)], w);
             if (k == 0) v.y = 0.f;
             static_cast<float2*>(a.out)[base + k * a.ostride] = v;
         } else if (d.out_mode == OFX_OUT_REAL) {
-            static_cast<float*>(a.out)[base + k * a.ostride] = s[k].x * a.scale;
+            static_cast<float*>(a.out)[base + k * a.ostride] = s[gpad(k)].x * a.scale;
         } else {  // OFX_OUT_R2C_UNPACK: k = 0..n (n = M)
             const float2* W = reinterpret_cast<const float2*>(a.tw_aux);
             const int k1 = (k == n) ? 0 : k;
             const int k2 = (k == 0) ? 0 : n - k;
-            const float2 zk = s[k1];
-            const float2 zc = cconj(s[k2]);
+            const float2 zk = s[gpad(k1)];
+            const float2 zc = cconj(s[gpad(k2)]);
             const float2 e = cscale(cadd(zk, zc), 0.5f);
             const float2 o = cscale(csub(zk, zc), 0.5f);
             float2 x = cadd(e, mul_negi(cmul(W[k], o)));
@@ -317,7 +362,7 @@ extern "C" int ofx_launch_generic(const ofx_args* args, const ofx_generic_desc* 
     if (args == nullptr || desc == nullptr) return (int)hipErrorInvalidValue;
     if (args->batch <= 0) return 0;
     const ofx_generic_desc& d = *desc;
-    if (d.n < 1 || d.tpb < 1 || (long long)d.tpb * d.n > OFX_GENERIC_MAX_ELEMS || d.npass < 0 ||
+    if (d.n < 1 || d.tpb < 1 || d.tpb > kGenMaxTpb || (long long)d.tpb * d.n > OFX_GENERIC_MAX_ELEMS || d.npass < 0 ||
         d.npass > OFX_MAX_PASSES || d.cnt_lo < 1)
         return (int)hipErrorInvalidValue;
     long long prod = 1;
@@ -329,17 +374,25 @@ extern "C" int ofx_launch_generic(const ofx_args* args, const ofx_generic_desc* 
     if (d.ft_len > 0 && d.out_mode != OFX_OUT_COMPLEX) return (int)hipErrorInvalidValue;
     const long long blocks = (args->batch + d.tpb - 1) / d.tpb;
     if (blocks > 0x7fffffffLL) return (int)hipErrorInvalidConfiguration;
-    const size_t lds = 2 * sizeof(float2) * (size_t)d.tpb * d.n;
+    const size_t lds = 2 * sizeof(float2) * (size_t)d.tpb * gen_ln(d.n);
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_generic),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)(2 * sizeof(float2) * OFX_GENERIC_MAX_ELEMS));
+        const int max_lds = (int)(2 * sizeof(float2) * gen_ln(OFX_GENERIC_MAX_ELEMS));
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_generic<false>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_generic<true>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
         if (e != hipSuccess) return (int)e;
         attr_set = true;
     }
+    bool big = false;
+    for (int q = 0; q < d.npass; ++q) big = big || d.radices[q] > 16;
     GenParams p{*args, d};
-    hipLaunchKernelGGL(k_generic, dim3((unsigned)blocks), dim3(kGenBlock), lds, stream, p);
+    if (big)
+        hipLaunchKernelGGL(k_generic<true>, dim3((unsigned)blocks), dim3(kGenBlock), lds, stream, p);
+    else
+        hipLaunchKernelGGL(k_generic<false>, dim3((unsigned)blocks), dim3(kGenBlock), lds, stream, p);
     return (int)hipGetLastError();
 }
 

@@ -217,7 +217,9 @@ struct Builder {
             ns2 *= r[p];
         }
         s.tw_table = pass_table_for(g.npass, g.radices, acc);
-        g.tpb = (int)std::max<long long>(1, std::min<long long>(64, 2048 / n));
+        // points per workgroup: enough bytes in flight per CU (ACLFFT_GENERIC_ELEMS overrides)
+        const long long per_block = std::max(64, std::min(OFX_GENERIC_MAX_ELEMS, env_int("ACLFFT_GENERIC_ELEMS", 2048)));
+        g.tpb = (int)std::max<long long>(1, std::min<long long>(64, per_block / n));
         g.cnt_lo = 1;
         s.args.in_sign = 1.f;
         s.args.out_sign = 1.f;
