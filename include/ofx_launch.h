@@ -74,6 +74,8 @@ typedef struct ofx_kernel_info {
     int lds_bytes;                  /* static LDS per workgroup                             */
     int tw_len;                     /* complex64 entries of the pass-twiddle table          */
     int iov;                        /* 2: paired butterflies in the first/last pass         */
+    int stream;                     /* 1: streaming (non-temporal) HBM input and output, for
+                                       working sets well beyond the last-level cache          */
 } ofx_kernel_info;
 
 /* Arguments of every 1D launch.  Element distances are in complex64 units for complex data and

@@ -75,7 +75,7 @@ class KernelInfo(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_int), ("threads_per_transform", ctypes.c_int),
                 ("transforms_per_block", ctypes.c_int), ("block_threads", ctypes.c_int), ("npass", ctypes.c_int),
                 ("radices", ctypes.c_int * 12), ("staged", ctypes.c_int), ("lds_bytes", ctypes.c_int),
-                ("tw_len", ctypes.c_int), ("iov", ctypes.c_int)]
+                ("tw_len", ctypes.c_int), ("iov", ctypes.c_int), ("stream", ctypes.c_int)]
 
 
 # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64.so.7, and its libraries
@@ -168,5 +168,5 @@ def kernel_registry():
         lib.ofx_kernel_info_get(i, ctypes.byref(k))
         out.append(dict(id=i, kind=k.kind, n=k.n, T=k.threads_per_transform, TPB=k.transforms_per_block,
                         block=k.block_threads, radices=list(k.radices[:k.npass]), staged=bool(k.staged),
-                        lds_bytes=k.lds_bytes, tw_len=k.tw_len, iov=k.iov))
+                        lds_bytes=k.lds_bytes, tw_len=k.tw_len, iov=k.iov, stream=bool(k.stream)))
     return out

@@ -131,6 +131,17 @@ def z_lines():
     return out
 
 
+# streaming variants (non-temporal HBM input/output) of the default C2C / Z2Z kernels, chosen by
+# the plan for working sets well beyond the last-level cache: (kind, N, T, TPB, radices, iov)
+STREAM = [
+    ("C2C", 128, 8, 32, (8, 16), 1), ("C2C", 256, 16, 16, (16, 16), 1), ("C2C", 512, 32, 8, (8, 8, 8), 1),
+    ("C2C", 1024, 64, 2, (8, 16, 8), 2), ("C2C", 2048, 128, 2, (16, 16, 8), 1),
+    ("C2C", 4096, 256, 1, (16, 16, 16), 1), ("C2C", 8192, 512, 1, (16, 16, 16, 2), 1),
+    ("Z2Z", 256, 32, 4, (4, 8, 8), 1), ("Z2Z", 512, 64, 2, (8, 8, 8), 1), ("Z2Z", 1024, 128, 2, (8, 8, 16), 1),
+    ("Z2Z", 2048, 256, 1, (8, 8, 8, 4), 1), ("Z2Z", 4096, 512, 1, (8, 8, 8, 8), 1),
+]
+
+
 def padsh(n):
     """LDS pad shift: odd lengths have no power-of-two strides, so they go unpadded."""
     return 30 if n % 2 else 4
@@ -207,9 +218,13 @@ def main():
         for kind, n, t, tpb, psh, rad in z_lines():
             assert t * tpb <= 1024
             f.write("    OFX_EZ(%s, %d, %d, %d, %d, %s),\n" % (KINDS[kind], n, t, tpb, psh, ", ".join(str(r) for r in rad)))
+        for kind, n, t, tpb, rad, iov in STREAM:
+            assert t * tpb <= 1024
+            mac = "OFX_EZS" if kind == "Z2Z" else ("OFX_EVS" if iov == 2 else "OFX_ES")
+            f.write("    %s(%s, %d, %d, %d, %d, %s),\n" % (mac, KINDS[kind], n, t, tpb, padsh(n), ", ".join(str(r) for r in rad)))
         f.write("};\n")
     print("wrote %d registry entries" % (len(FIRST) + len(rows) + len(TILES) + len(EXTRA) + len(PAIRED) +
-                                         len(z_lines())))
+                                         len(z_lines()) + len(STREAM)))
 
 
 if __name__ == "__main__":

@@ -36,31 +36,56 @@ __device__ __forceinline__ S arg_scale(const ofx_args& a) {
     }
 }
 
-template <class V>
+// NT: streaming (non-temporal) accesses.  Used only where every wave's accesses cover whole
+// 128-byte lines (rows of a multiple of 128 bytes, contiguous): there they skip the cache
+// retention of data touched once; on partial lines they turn into re-reads and masked writes.
+template <bool NT, class P>
+__device__ __forceinline__ auto ldx(const P* p) {
+    if constexpr (NT) {
+        return ld_nt(p);
+    } else {
+        return *p;
+    }
+}
+template <bool NT, class P>
+__device__ __forceinline__ void stx(P* p, P v) {
+    if constexpr (NT) {
+        st_nt(p, v);
+    } else {
+        *p = v;
+    }
+}
+// rows of N elements of type E (the real or complex row of a kernel) cover whole 128-byte lines
+template <class E, int N>
+constexpr bool rows_whole_lines() {
+    return (N * (int)sizeof(E)) % 128 == 0;
+}
+
+template <class V, bool NT = false>
 struct GLoadC {  // complex input, conjugated for the backward direction
     const V* __restrict__ p;
     scalar_t<V> sg;
     __device__ __forceinline__ V operator()(int i) const {
-        V v = p[i];
+        V v = ldx<NT>(p + i);
         return mkv(v.x, v.y * sg);
     }
     // elements i, i+1 (complex64, i even, 16-byte aligned rows)
     __device__ __forceinline__ float4 load2(int i) const {
-        float4 v = *reinterpret_cast<const float4*>(p + i);
+        float4 v = ldx<NT>(reinterpret_cast<const float4*>(p + i));
         return make_float4(v.x, v.y * sg, v.z, v.w * sg);
     }
 };
-template <class V>
+template <class V, bool NT = false>
 struct GStoreC {
     V* __restrict__ p;
     scalar_t<V> sg, scale;
     bool ok;
     __device__ __forceinline__ void operator()(int i, V v) const {
-        if (ok) p[i] = mkv(v.x * scale, v.y * (sg * scale));
+        if (ok) stx<NT>(p + i, mkv(v.x * scale, v.y * (sg * scale)));
     }
     __device__ __forceinline__ void store2(int i, V a, V b) const {
         const scalar_t<V> si = sg * scale;
-        if (ok) *reinterpret_cast<float4*>(p + i) = make_float4(a.x * scale, a.y * si, b.x * scale, b.y * si);
+        if (ok) stx<NT>(reinterpret_cast<float4*>(p + i), make_float4(a.x * scale, a.y * si, b.x * scale, b.y * si));
     }
 };
 
@@ -161,7 +186,7 @@ __device__ __forceinline__ void stage_in(typename C::V* lds, const typename C::V
     if constexpr ((C::N & 1) == 0 && sizeof(V) == 8) {
         const float4* s4 = reinterpret_cast<const float4*>(src);
         for (int l = threadIdx.x; l < total / 2; l += C::BLOCK) {
-            float4 v = s4[l];
+            float4 v = ldx<C::STREAM && rows_whole_lines<V, C::N>()>(s4 + l);
             const int e = 2 * l;
             const int t = e / C::N, i = e - t * C::N;
             V* d = lds + t * C::LDS_N;
@@ -170,7 +195,7 @@ __device__ __forceinline__ void stage_in(typename C::V* lds, const typename C::V
         }
     } else {
         for (int l = threadIdx.x; l < total; l += C::BLOCK) {
-            V v = src[l];
+            V v = ldx<C::STREAM && rows_whole_lines<V, C::N>()>(src + l);
             const int t = l / C::N, i = l - t * C::N;
             lds[t * C::LDS_N + C::pad(i)] = mkv(v.x, v.y * sg);
         }
@@ -189,13 +214,13 @@ __device__ __forceinline__ void stage_out(const typename C::V* lds, typename C::
             const int t = e / C::N, i = e - t * C::N;
             const V* s = lds + t * C::LDS_N;
             V a = s[C::pad(i)], b = s[C::pad(i + 1)];
-            d4[l] = make_float4(a.x * scale, a.y * si, b.x * scale, b.y * si);
+            stx<C::STREAM && rows_whole_lines<V, C::N>()>(d4 + l, make_float4(a.x * scale, a.y * si, b.x * scale, b.y * si));
         }
     } else {
         for (int l = threadIdx.x; l < total; l += C::BLOCK) {
             const int t = l / C::N, i = l - t * C::N;
             V a = lds[t * C::LDS_N + C::pad(i)];
-            dst[l] = mkv(a.x * scale, a.y * si);
+            stx<C::STREAM && rows_whole_lines<V, C::N>()>(dst + l, mkv(a.x * scale, a.y * si));
         }
     }
 }
@@ -230,8 +255,9 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2c(ofx_args a) {
     } else {
         const bool ok = t < a.batch;
         const long long tt = ok ? t : 0;
-        GLoadC<V> gin{static_cast<const V*>(a.in) + tt * a.idist, a.in_sign};
-        GStoreC<V> gout{static_cast<V*>(a.out) + tt * a.odist, a.out_sign, ascale, ok};
+        constexpr bool NT = C::STREAM && rows_whole_lines<V, C::N>();
+        GLoadC<V, NT> gin{static_cast<const V*>(a.in) + tt * a.idist, a.in_sign};
+        GStoreC<V, NT> gout{static_cast<V*>(a.out) + tt * a.odist, a.out_sign, ascale, ok};
         if constexpr (C::WAVE_SYNC) {
             if (!ok) return;  // whole transforms per wave: no block-wide barrier follows
         }
@@ -509,6 +535,7 @@ Entry make_entry() {
     e.info.iov = C::IOV;
     e.info.lds_bytes = (int)(C::TPB * (C::LDS_N + (KIND == OFX_KIND_ROWT ? 1 : 0)) * sizeof(typename C::V));
     e.info.tw_len = C::tw_len();
+    e.info.stream = C::STREAM ? 1 : 0;
     e.fn = &launch<C, KIND>;
     return e;
 }
@@ -519,6 +546,13 @@ Entry make_entry() {
 #define OFX_EV(KIND, N, T, TPB, PADSH, ...) make_entry<OFX_CFGV(N, T, TPB, PADSH, __VA_ARGS__), KIND>()
 #define OFX_CFGZ(N, T, TPB, PADSH, ...) StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, double2>
 #define OFX_EZ(KIND, N, T, TPB, PADSH, ...) make_entry<OFX_CFGZ(N, T, TPB, PADSH, __VA_ARGS__), KIND>()
+// streaming variants (STREAM_ = true) of a direct, a paired and a complex128 configuration
+#define OFX_ES(KIND, N, T, TPB, PADSH, ...) \
+    make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, float2, true>, KIND>()
+#define OFX_EVS(KIND, N, T, TPB, PADSH, ...) \
+    make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 2, 0, 1, float2, true>, KIND>()
+#define OFX_EZS(KIND, N, T, TPB, PADSH, ...) \
+    make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, double2, true>, KIND>()
 
 #include "registry_1d.inc"
 

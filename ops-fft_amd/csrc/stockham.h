@@ -37,10 +37,13 @@ struct Radices {
 // interleaved in LDS, element i of column c at lds[c + i*(COLS_ + COLPAD_)], and consecutive
 // lanes take consecutive columns, so each HBM row segment of COLS_ values is one contiguous access.
 // V_: the element type, float2 (complex64) or double2 (complex128).
+// STREAM_: the kernel's HBM input and output use streaming (non-temporal) accesses; for working
+// sets well beyond the last-level cache (chosen by the plan).
 template <int N_, int T_, int TPB_, int PADSH_, bool STAGED_, class RL, int IOV_ = 1, int COLS_ = 0, int COLPAD_ = 1,
-          class V_ = float2>
+          class V_ = float2, bool STREAM_ = false>
 struct StockhamCfg {
     using V = V_;
+    static constexpr bool STREAM = STREAM_;
     static constexpr int N = N_;
     static constexpr int T = T_;
     static constexpr int TPB = TPB_;

@@ -195,6 +195,22 @@ struct Builder {
         return s;
     }
 
+    // Streaming kernels (non-temporal HBM accesses) pay off when a launch's input plus output is
+    // well beyond the 256 MB last-level cache; below that, repeated or chained launches find part
+    // of their data in that cache, which streaming accesses forgo.  Unless a variant is forced
+    // (ACLFFT_KERNEL_VARIANT), the step takes the first candidate of the wanted kind.
+    static constexpr size_t kStreamBytes = size_t(384) << 20;
+    void pick_stream(Step& s, size_t io_bytes) {
+        if (std::getenv("ACLFFT_KERNEL_VARIANT") != nullptr) return;
+        const bool want = io_bytes >= kStreamBytes;
+        for (size_t c = 0; c < s.candidates.size(); ++c)
+            if ((registry()[s.candidates[c]].stream != 0) == want) {
+                s.kernel = s.candidates[c];
+                s.tw_table = s.candidate_tables[c];
+                return;
+            }
+    }
+
     // A runtime-radix step of length n; false if n has an unsupported prime factor or is too long.
     bool generic_step(long long n, Step* out) {
         if (n > OFX_GENERIC_MAX_ELEMS) return false;
@@ -244,6 +260,7 @@ struct Builder {
         auto ids = kernels_for(OFX_KIND_C2C, n, true);
         if (!ids.empty()) {
             Step s = kernel_step(OFX_KIND_C2C, n, ids);
+            pick_stream(s, (size_t)n * batch * 16);
             s.src = src;
             s.dst = dst;
             s.args.batch = batch;
@@ -715,6 +732,7 @@ struct Builder {
         auto ids = kernels_for(OFX_KIND_Z2Z, n, true);
         if (ids.empty()) return ACLFFT_NOT_IMPLEMENTED;
         Step s = kernel_step(OFX_KIND_Z2Z, n, ids);
+        pick_stream(s, (size_t)n * batch * 32);
         s.args.batch = batch;
         s.args.idist = n;
         s.args.odist = n;
@@ -1073,7 +1091,8 @@ std::string Executor::describe() const {
         }
         const ofx_kernel_info& k = reg[s.kernel];
         os << "kind" << k.kind << " n=" << k.n << " T=" << k.threads_per_transform << " TPB=" << k.transforms_per_block
-           << (k.iov == 2 ? " paired" : (k.staged ? " staged" : " direct")) << " radices=";
+           << (k.iov == 2 ? " paired" : (k.staged ? " staged" : " direct")) << (k.stream ? " stream" : "")
+           << " radices=";
         for (int p = 0; p < k.npass; ++p) os << (p ? "x" : "") << k.radices[p];
         os << " id=" << s.kernel;
     }
