@@ -57,7 +57,9 @@ enum ofx_kind {
     OFX_KIND_D2Z = 9,
     OFX_KIND_Z2D = 10,
     OFX_KIND_D2Z_ODD = 11,
-    OFX_KIND_Z2D_ODD = 12
+    OFX_KIND_Z2D_ODD = 12,
+    OFX_KIND_ROWT_Z = 13,  /* complex128 ROWT                                             */
+    OFX_KIND_COL_Z = 14    /* complex128 column kernels (ofx_col_kernel_* registry)          */
 };
 
 /* Description of one compiled kernel configuration. */

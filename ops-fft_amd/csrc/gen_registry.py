@@ -12,7 +12,8 @@ import os
 KINDS = {"C2C": "OFX_KIND_C2C", "R2C": "OFX_KIND_R2C", "C2R": "OFX_KIND_C2R",
          "R2C_ODD": "OFX_KIND_R2C_ODD", "C2R_ODD": "OFX_KIND_C2R_ODD", "ROWT": "OFX_KIND_ROWT",
          "C2C_TILES": "OFX_KIND_C2C_TILES", "Z2Z": "OFX_KIND_Z2Z", "D2Z": "OFX_KIND_D2Z",
-         "Z2D": "OFX_KIND_Z2D", "D2Z_ODD": "OFX_KIND_D2Z_ODD", "Z2D_ODD": "OFX_KIND_Z2D_ODD"}
+         "Z2D": "OFX_KIND_Z2D", "D2Z_ODD": "OFX_KIND_D2Z_ODD", "Z2D_ODD": "OFX_KIND_Z2D_ODD",
+         "ROWT_Z": "OFX_KIND_ROWT_Z"}
 
 # Radix plans for the lengths that get a dedicated (fully unrolled) kernel.  Everything else runs
 # on the runtime-radix kernel (kernels_generic.hip).
@@ -129,6 +130,13 @@ def z_lines():
         out.append(("D2Z_ODD", n, t, tpb_for_z(n, t), padsh(n), rad))
         out.append(("Z2D_ODD", n, t, tpb_for_z(n, t), padsh(n), rad))
     out.append(("Z2Z", 1024, 128, 1, padsh(1024), (8, 8, 16)))
+    # four-step row steps (transposed store), ~8 elements per thread, LDS <= 147456 bytes
+    for n, t, rad in ((128, 16, (8, 16)), (256, 32, (4, 8, 8)), (512, 64, (8, 8, 8)), (1024, 128, (8, 8, 16)),
+                      (2048, 256, (8, 8, 8, 4))):
+        tpb = 16
+        while tpb > 1 and (tpb * (n + n // 16 + 1) * 16 > 147456 or tpb * t > 1024):
+            tpb //= 2
+        out.append(("ROWT_Z", n, t, tpb, 4, rad))
     return out
 
 

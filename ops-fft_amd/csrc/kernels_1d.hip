@@ -478,9 +478,10 @@ constexpr int base_kind(int k) {
            : k == OFX_KIND_Z2D ? OFX_KIND_C2R
            : k == OFX_KIND_D2Z_ODD ? OFX_KIND_R2C_ODD
            : k == OFX_KIND_Z2D_ODD ? OFX_KIND_C2R_ODD
+           : k == OFX_KIND_ROWT_Z ? OFX_KIND_ROWT
                                    : k;
 }
-constexpr bool is_z_kind(int k) { return k >= OFX_KIND_Z2Z && k <= OFX_KIND_Z2D_ODD; }
+constexpr bool is_z_kind(int k) { return k >= OFX_KIND_Z2Z && k <= OFX_KIND_ROWT_Z; }
 
 template <class C, int KIND_>
 hipError_t launch(const ofx_args* a, hipStream_t s) {

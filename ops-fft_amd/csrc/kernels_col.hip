@@ -1,6 +1,7 @@
-// kernels_col.hip - column transforms of row-major data (2D second pass, the VERTICAL layout).
+// kernels_col.hip - column transforms of row-major data (2D second pass, the VERTICAL layout,
+// four-step column steps), complex64 and complex128.
 //
-// Data: `images` matrices of rows x cols complex64, row-major; one transform per column, length
+// Data: `images` matrices of rows x cols complex values, row-major; one transform per column, length
 // rows, element i of column c of image m at in[m*idist + c + i*istride] (istride = cols).  A
 // workgroup takes COLS adjacent columns of one image: lane l works on column l % COLS, so in every
 // load or store the COLS lanes of a group touch one contiguous COLS*8-byte run of a row.  The
@@ -19,36 +20,40 @@ namespace ofx {
 
 template <class C>
 struct ColLoad {
-    const float2* __restrict__ p;  // element 0 of this column
+    using V = typename C::V;
+    const V* __restrict__ p;  // element 0 of this column
     long long stride;
-    float sg;
-    __device__ __forceinline__ float2 operator()(int i) const {
-        float2 v = p[(long long)i * stride];
-        return make_float2(v.x, v.y * sg);
+    scalar_t<V> sg;
+    __device__ __forceinline__ V operator()(int i) const {
+        V v = p[(long long)i * stride];
+        return mkv(v.x, v.y * sg);
     }
 };
 // FT: 0 = none; 1 = four-step twiddle W_ftlen^(col * i) on output i from the FP64 two-level table;
-// 2 = the same twiddle computed in registers (ftlen = 2^ftlog2).
+// 2 = the same twiddle computed in registers (ftlen = 2^ftlog2).  In the element type's precision.
 template <class C, int FT>
 struct ColStore {
-    float2* __restrict__ p;
+    using V = typename C::V;
+    V* __restrict__ p;
     long long stride;
-    float sg, scale;
+    scalar_t<V> sg, scale;
     bool ok;
     const double2* __restrict__ ft;
     long long col, ftlen;
     int ftlog2;
-    __device__ __forceinline__ void operator()(int i, float2 v) const {
-        if constexpr (FT == 1) v = cmul(v, ft_twiddle(ft, col * (long long)i, ftlen));
-        if constexpr (FT == 2) v = cmul(v, twiddle_pow2((unsigned long long)col * (unsigned)i, ftlog2));
-        if (ok) p[(long long)i * stride] = make_float2(v.x * scale, v.y * (sg * scale));
+    __device__ __forceinline__ void operator()(int i, V v) const {
+        if constexpr (FT == 1) v = cmul(v, ft_twiddle_v((V*)nullptr, ft, col * (long long)i, ftlen));
+        if constexpr (FT == 2) v = cmul(v, twiddle_pow2_v((V*)nullptr, (unsigned long long)col * (unsigned)i, ftlog2));
+        if (ok) p[(long long)i * stride] = mkv(v.x * scale, v.y * (sg * scale));
     }
 };
 
 template <class C, int FT>
 __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, long long groups_per_image,
                                                   long long ftlen, int ftlog2) {
-    __shared__ float2 lds[C::LDS_N];
+    using V = typename C::V;
+    using S = scalar_t<V>;
+    __shared__ V lds[C::LDS_N];
     const int c = threadIdx.x % C::COLS;
     const int tid = threadIdx.x / C::COLS;
     // XCD-aware group index (blocks are dispatched round-robin over the 8 XCDs)
@@ -59,10 +64,11 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
     const long long col = (g - img * groups_per_image) * C::COLS + c;
     const bool ok = col < cols;
     const long long cc = ok ? col : 0;
-    ColLoad<C> gin{static_cast<const float2*>(a.in) + img * a.idist + cc, a.istride, a.in_sign};
-    ColStore<C, FT> gout{static_cast<float2*>(a.out) + img * a.odist + cc, a.ostride, a.out_sign, a.scale, ok,
+    const S scale = (sizeof(S) == 8 && a.scale64 != 0.0) ? (S)a.scale64 : (S)a.scale;
+    ColLoad<C> gin{static_cast<const V*>(a.in) + img * a.idist + cc, a.istride, a.in_sign};
+    ColStore<C, FT> gout{static_cast<V*>(a.out) + img * a.odist + cc, a.ostride, a.out_sign, scale, ok,
                          static_cast<const double2*>(a.tw_aux), cc, ftlen, ftlog2};
-    stockham_passes<C, false, false>(lds + c, tid, static_cast<const float2*>(a.tw), gin, gout);
+    stockham_passes<C, false, false>(lds + c, tid, static_cast<const V*>(a.tw), gin, gout);
 }
 
 template <class C>
@@ -96,14 +102,14 @@ struct ColEntry {
 template <class C>
 ColEntry col_entry() {
     ColEntry e{};
-    e.info.kind = OFX_KIND_COL;
+    e.info.kind = sizeof(typename C::V) == 16 ? OFX_KIND_COL_Z : OFX_KIND_COL;
     e.info.n = C::N;
     e.info.threads_per_transform = C::T;
     e.info.transforms_per_block = C::COLS;
     e.info.block_threads = C::BLOCK;
     e.info.npass = C::NP;
     for (int p = 0; p < C::NP; ++p) e.info.radices[p] = C::radix(p);
-    e.info.lds_bytes = (int)(C::LDS_N * sizeof(float2));
+    e.info.lds_bytes = (int)(C::LDS_N * sizeof(typename C::V));
     e.info.tw_len = C::tw_len();
     e.info.iov = 1;
     e.fn = &launch_col<C>;
@@ -113,6 +119,8 @@ ColEntry col_entry() {
 // (N, T, COLS, COLPAD, radices): T * COLS threads per workgroup
 #define OFX_COL(N, T, COLS, COLPAD, ...) \
     col_entry<StockhamCfg<N, T, COLS, 4, false, Radices<__VA_ARGS__>, 1, COLS, COLPAD>>()
+#define OFX_COLZ(N, T, COLS, COLPAD, ...) \
+    col_entry<StockhamCfg<N, T, COLS, 4, false, Radices<__VA_ARGS__>, 1, COLS, COLPAD, double2>>()
 
 static const ColEntry kCols[] = {
     OFX_COL(64, 8, 16, 1, 8, 8),
@@ -131,6 +139,14 @@ static const ColEntry kCols[] = {
     OFX_COL(2048, 256, 4, 1, 8, 8, 8, 4),
     OFX_COL(4096, 256, 4, 0, 16, 16, 16),
     OFX_COL(4096, 256, 2, 1, 16, 16, 16),
+    // complex128 (16-byte elements: half the columns per workgroup for the same LDS)
+    OFX_COLZ(64, 8, 16, 1, 8, 8),
+    OFX_COLZ(128, 16, 16, 1, 8, 16),
+    OFX_COLZ(256, 32, 16, 1, 4, 8, 8),
+    OFX_COLZ(512, 64, 8, 1, 8, 8, 8),
+    OFX_COLZ(1024, 128, 8, 1, 8, 8, 16),
+    OFX_COLZ(2048, 256, 4, 0, 8, 8, 8, 4),
+    OFX_COLZ(4096, 512, 2, 0, 8, 8, 8, 8),
 };
 static const int kNumCols = (int)(sizeof(kCols) / sizeof(kCols[0]));
 

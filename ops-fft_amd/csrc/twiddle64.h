@@ -21,4 +21,17 @@ __device__ __forceinline__ float2 ft_twiddle(const double2* __restrict__ t, long
     return make_float2((float)(a.x * b.x - a.y * b.y), (float)(a.x * b.y + a.y * b.x));
 }
 
+// the same twiddle kept in FP64 (complex128 kernels)
+__device__ __forceinline__ double2 ft_twiddle_z(const double2* __restrict__ t, long long e) {
+    const double2 a = t[e & (kFtLo - 1)];
+    const double2 b = t[kFtLo + (e >> kFtLoBits)];
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 ft_twiddle_v(float2*, const double2* __restrict__ t, long long e, long long len) {
+    return ft_twiddle(t, e, len);
+}
+__device__ __forceinline__ double2 ft_twiddle_v(double2*, const double2* __restrict__ t, long long e, long long) {
+    return ft_twiddle_z(t, e);
+}
+
 }  // namespace ofx

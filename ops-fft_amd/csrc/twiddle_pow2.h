@@ -31,4 +31,32 @@ __device__ __forceinline__ float2 twiddle_pow2(unsigned long long e, int log2n) 
     return make_float2(ct, -st);
 }
 
+// complex128: the same exact reduction, FP64 sincospi (x = 2s/N is exact in double for any
+// N <= 2^53), so the twiddle carries only sincospi's own rounding.
+__device__ __forceinline__ double2 twiddle_pow2_z(unsigned long long e, int log2n) {
+    const unsigned long long m = e & ((1ULL << log2n) - 1ULL);
+    const int sh = log2n - 2;
+    const unsigned long long q = (m + (1ULL << (sh - 1))) >> sh;
+    const long long s = (long long)m - (long long)(q << sh);
+    const double x = ldexp((double)s, 1 - log2n);
+    double sn, cs;
+    sincospi(x, &sn, &cs);
+    double ct, st;
+    switch ((int)(q & 3)) {
+        case 0: ct = cs; st = sn; break;
+        case 1: ct = -sn; st = cs; break;
+        case 2: ct = -cs; st = -sn; break;
+        default: ct = sn; st = -cs; break;
+    }
+    return make_double2(ct, -st);
+}
+
+// the four-step twiddle in the element type's precision
+__device__ __forceinline__ float2 twiddle_pow2_v(float2*, unsigned long long e, int log2n) {
+    return twiddle_pow2(e, log2n);
+}
+__device__ __forceinline__ double2 twiddle_pow2_v(double2*, unsigned long long e, int log2n) {
+    return twiddle_pow2_z(e, log2n);
+}
+
 }  // namespace ofx

@@ -153,7 +153,7 @@ struct Builder {
         t.bytes = (size_t)k.tw_len * 16;
         return e->add_table(t);
     }
-    static bool is_z_kind(int k) { return k >= OFX_KIND_Z2Z && k <= OFX_KIND_Z2D_ODD; }
+    static bool is_z_kind(int k) { return k >= OFX_KIND_Z2Z && k <= OFX_KIND_COL_Z; }
     int roots_table_z(long long len, long long count) {
         TableSpec t;
         t.kind = TableSpec::ROOTS_Z;
@@ -246,21 +246,22 @@ struct Builder {
 
     // Batch of contiguous C2C transforms of length n from buffer src to dst.  The direction's
     // conjugation is applied on the first load (conj_in) and the last store (conj_out).
-    aclfftResult c2c_rows(long long n, long long batch, int src, int dst, bool conj_in, bool conj_out) {
+    aclfftResult c2c_rows(long long n, long long batch, int src, int dst, bool conj_in, bool conj_out, bool z = false) {
         if (n == 1) {
             Step s;
             s.type = Step::COPY;
             s.src = src;
             s.dst = dst;
-            s.copy_bytes = (size_t)batch * 8;
+            s.copy_bytes = (size_t)batch * (z ? 16 : 8);
             e->steps_.push_back(s);
             // a length-1 transform is the identity in both directions
             return ACLFFT_SUCCESS;
         }
-        auto ids = kernels_for(OFX_KIND_C2C, n, true);
+        const int kind = z ? OFX_KIND_Z2Z : OFX_KIND_C2C;
+        auto ids = kernels_for(kind, n, true);
         if (!ids.empty()) {
-            Step s = kernel_step(OFX_KIND_C2C, n, ids);
-            pick_stream(s, (size_t)n * batch * 16);
+            Step s = kernel_step(kind, n, ids);
+            pick_stream(s, (size_t)n * batch * (z ? 32 : 16));
             s.src = src;
             s.dst = dst;
             s.args.batch = batch;
@@ -272,7 +273,7 @@ struct Builder {
             return ACLFFT_SUCCESS;
         }
         Step s;
-        if (generic_step(n, &s)) {
+        if (!z && generic_step(n, &s)) {
             s.src = src;
             s.dst = dst;
             s.args.batch = batch;
@@ -285,7 +286,7 @@ struct Builder {
             e->steps_.push_back(s);
             return ACLFFT_SUCCESS;
         }
-        return c2c_fourstep(n, batch, src, dst, conj_in, conj_out);
+        return c2c_fourstep(n, batch, src, dst, conj_in, conj_out, z);
     }
 
     // Split n = n1 * n2 for the four-step: both factors runnable by the runtime-radix kernel, as
@@ -307,13 +308,15 @@ struct Builder {
 
     // A four-step split whose column length has a compiled column kernel and whose row length has a
     // compiled transposed-store row kernel, as balanced as possible.
-    bool templated_split(long long n, long long* n1, long long* n2) {
+    static int col_kind(bool z) { return z ? OFX_KIND_COL_Z : OFX_KIND_COL; }
+    static int rowt_kind(bool z) { return z ? OFX_KIND_ROWT_Z : OFX_KIND_ROWT; }
+    bool templated_split(long long n, long long* n1, long long* n2, bool z = false) {
         if (env_int("ACLFFT_DISABLE_TEMPLATED", 0) != 0) return false;
         double best = 1e30;
         for (const ofx_kernel_info& c : col_registry()) {
-            if (n % c.n) continue;
+            if (c.kind != col_kind(z) || n % c.n) continue;
             const long long b = n / c.n;
-            if (kernels_for(OFX_KIND_ROWT, b, true).empty()) continue;
+            if (kernels_for(rowt_kind(z), b, true).empty()) continue;
             const double d = std::fabs(std::log((double)c.n) - std::log((double)b));
             if (d < best) {
                 best = d;
@@ -326,15 +329,15 @@ struct Builder {
 
     // A three-factor split n = n1 * n2 * n3 with compiled column kernels for n1 and n2 and a
     // transposed-store row kernel for n3, minimising the largest factor.
-    bool templated_split3(long long n, long long* n1, long long* n2, long long* n3) {
+    bool templated_split3(long long n, long long* n1, long long* n2, long long* n3, bool z = false) {
         if (env_int("ACLFFT_DISABLE_TEMPLATED", 0) != 0) return false;
         long long best = 0;
         for (const ofx_kernel_info& a : col_registry()) {
-            if (n % a.n) continue;
+            if (a.kind != col_kind(z) || n % a.n) continue;
             for (const ofx_kernel_info& b : col_registry()) {
-                if ((n / a.n) % b.n) continue;
+                if (b.kind != col_kind(z) || (n / a.n) % b.n) continue;
                 const long long c = n / a.n / b.n;
-                if (kernels_for(OFX_KIND_ROWT, c, true).empty()) continue;
+                if (kernels_for(rowt_kind(z), c, true).empty()) continue;
                 const long long worst = std::max<long long>(std::max(a.n, b.n), c);
                 if (best == 0 || worst < best) {
                     best = worst;
@@ -354,20 +357,21 @@ struct Builder {
     //      in X are next to each other in the workspace
     //   C: rows over n3, transposed store X[b*n + k1 + N1*k2 + N1*N2*k3].
     aclfftResult c2c_fourstep3(long long n, long long batch, long long N1, long long N2, long long N3, int src,
-                               int dst, bool conj_in, bool conj_out) {
+                               int dst, bool conj_in, bool conj_out, bool z = false) {
         const int scratch = (dst == BUF_WORK0) ? BUF_WORK1 : BUF_WORK0;
-        aclfftResult r = c2c_columns(N1, N2 * N3, batch, src, dst, conj_in, false);
+        const size_t elem = z ? 16 : 8;
+        aclfftResult r = c2c_columns(N1, N2 * N3, batch, src, dst, conj_in, false, z);
         if (r != ACLFFT_SUCCESS) return r;
         e->steps_.back().args.ft_len = n;
         e->steps_.back().aux_table = fourstep_table(n);
-        r = c2c_columns(N2, N3, batch * N1, dst, scratch, false, false);
+        r = c2c_columns(N2, N3, batch * N1, dst, scratch, false, false, z);
         if (r != ACLFFT_SUCCESS) return r;
         Step& b = e->steps_.back();
         b.args.odist = N3;
         b.args.ostride = batch * N1 * N3;
         b.args.ft_len = N2 * N3;
         b.aux_table = fourstep_table(N2 * N3);
-        Step c = kernel_step(OFX_KIND_ROWT, N3, kernels_for(OFX_KIND_ROWT, N3, true));
+        Step c = kernel_step(rowt_kind(z), N3, kernels_for(rowt_kind(z), N3, true));
         c.src = scratch;
         c.dst = dst;
         c.args.batch = batch * N1 * N2;
@@ -380,7 +384,7 @@ struct Builder {
         c.conj_out = conj_out;
         e->steps_.push_back(c);
         size_t& wb = (scratch == BUF_WORK0) ? e->work_bytes_ : e->work1_bytes_;
-        wb = std::max(wb, (size_t)(n * batch * 8));
+        wb = std::max(wb, (size_t)(n * batch) * elem);
         return ACLFFT_SUCCESS;
     }
 
@@ -388,15 +392,17 @@ struct Builder {
     // tmp[k1*N2 + n2] -> rows (length N2) -> X[k1 + N1*k2].  Two factors when both fit the compiled
     // kernels' best configurations (<= 1024: 16 columns or rows per workgroup, 128-byte HBM runs),
     // else three factors; ACLFFT_FOURSTEP_LEVELS=2|3 forces a form where it exists.
-    aclfftResult c2c_fourstep(long long n, long long batch, int src, int dst, bool conj_in, bool conj_out) {
+    aclfftResult c2c_fourstep(long long n, long long batch, int src, int dst, bool conj_in, bool conj_out,
+                              bool z = false) {
         long long T1 = 0, T2 = 0, U1 = 0, U2 = 0, U3 = 0;
-        const bool two = templated_split(n, &T1, &T2);
-        const bool three = templated_split3(n, &U1, &U2, &U3);
+        const size_t elem = z ? 16 : 8;
+        const bool two = templated_split(n, &T1, &T2, z);
+        const bool three = templated_split3(n, &U1, &U2, &U3, z);
         const int force = env_int("ACLFFT_FOURSTEP_LEVELS", 0);
         bool use3 = three && (!two || std::max(T1, T2) > 1024);
         if (force == 2 && two) use3 = false;
         if (force == 3 && three) use3 = true;
-        if (use3) return c2c_fourstep3(n, batch, U1, U2, U3, src, dst, conj_in, conj_out);
+        if (use3) return c2c_fourstep3(n, batch, U1, U2, U3, src, dst, conj_in, conj_out, z);
         if (two) {
             // The workspace rows are P = N2 + pad elements apart: with a power-of-two pitch every
             // row of a column group starts on the same HBM channel set.  Measured on MI355X: a
@@ -404,7 +410,7 @@ struct Builder {
             // (ACLFFT_FOURSTEP_PAD overrides it).
             const long long P = T2 + std::max(0, env_int("ACLFFT_FOURSTEP_PAD", T2 >= 512 ? 16 : 0));
             // column step: N2 columns of length N1 per transform, four-step twiddle fused in the store
-            aclfftResult r = c2c_columns(T1, T2, batch, src, BUF_WORK0, conj_in, false);
+            aclfftResult r = c2c_columns(T1, T2, batch, src, BUF_WORK0, conj_in, false, z);
             if (r != ACLFFT_SUCCESS) return r;
             Step& a = e->steps_.back();
             a.args.ft_len = n;
@@ -412,8 +418,8 @@ struct Builder {
             a.args.ostride = P;
             a.aux_table = fourstep_table(n);
             // row step: rows of length N2, transposed store X[k1 + N1*k2]
-            auto ids = kernels_for(OFX_KIND_ROWT, T2, true);
-            Step b = kernel_step(OFX_KIND_ROWT, T2, ids);
+            auto ids = kernels_for(rowt_kind(z), T2, true);
+            Step b = kernel_step(rowt_kind(z), T2, ids);
             b.src = BUF_WORK0;
             b.dst = dst;
             b.args.batch = batch * T1;
@@ -423,9 +429,10 @@ struct Builder {
             b.args.ostride = T1;
             b.conj_out = conj_out;
             e->steps_.push_back(b);
-            e->work_bytes_ = std::max(e->work_bytes_, (size_t)(batch * T1 * P * 8));
+            e->work_bytes_ = std::max(e->work_bytes_, (size_t)(batch * T1 * P) * elem);
             return ACLFFT_SUCCESS;
         }
+        if (z) return ACLFFT_NOT_IMPLEMENTED;  // complex128: compiled factors only
         long long N1 = 0, N2 = 0;
         if (!split(n, &N1, &N2)) return ACLFFT_NOT_IMPLEMENTED;
         const int ft = fourstep_table(n);
@@ -471,12 +478,12 @@ struct Builder {
 
     // Columns of [hi][rows][cols] row-major data: `cols` transforms of length `rows` per image.
     aclfftResult c2c_columns(long long rows, long long cols, long long images, int src, int dst, bool conj_in,
-                             bool conj_out) {
+                             bool conj_out, bool z = false) {
         std::vector<int> ids;
         if (env_int("ACLFFT_DISABLE_TEMPLATED", 0) == 0) {
             const auto& creg = col_registry();
             for (int i = 0; i < (int)creg.size(); ++i)
-                if (creg[i].n == rows) ids.push_back(i);
+                if (creg[i].n == rows && creg[i].kind == col_kind(z)) ids.push_back(i);
             // widest column groups first only when a row has enough columns to fill them
             std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) {
                 const bool fa = creg[a].transforms_per_block <= cols, fb = creg[b].transforms_per_block <= cols;
@@ -486,7 +493,7 @@ struct Builder {
         if (!ids.empty()) {
             Step c;
             c.type = Step::COLUMN;
-            c.kind = OFX_KIND_COL;
+            c.kind = col_kind(z);
             c.n = (int)rows;
             c.candidates = ids;
             for (int id : ids) c.candidate_tables.push_back(pass_table(col_registry()[id]));
@@ -508,7 +515,7 @@ struct Builder {
             return ACLFFT_SUCCESS;
         }
         Step s;
-        if (!generic_step(rows, &s)) return ACLFFT_NOT_IMPLEMENTED;
+        if (z || !generic_step(rows, &s)) return ACLFFT_NOT_IMPLEMENTED;
         s.g.tpb = (int)std::max<long long>(1, std::min<long long>(16, OFX_GENERIC_MAX_ELEMS / rows));
         s.g.cnt_lo = cols;
         s.g.idist_lo = 1;
@@ -719,26 +726,11 @@ struct Builder {
         return ACLFFT_SUCCESS;
     }
 
-    // complex128 / double transforms: compiled kernels only (lengths listed in the registry);
-    // other lengths plan successfully and report NOT_IMPLEMENTED at execution.
+    // complex128 rows: a compiled kernel where the registry has one, else the four-step over
+    // compiled complex128 column and row-step kernels; other lengths plan successfully and report
+    // NOT_IMPLEMENTED at execution.
     aclfftResult z2z_rows(long long n, long long batch) {
-        if (n == 1) {
-            Step s;
-            s.type = Step::COPY;
-            s.copy_bytes = (size_t)batch * 16;
-            e->steps_.push_back(s);
-            return ACLFFT_SUCCESS;
-        }
-        auto ids = kernels_for(OFX_KIND_Z2Z, n, true);
-        if (ids.empty()) return ACLFFT_NOT_IMPLEMENTED;
-        Step s = kernel_step(OFX_KIND_Z2Z, n, ids);
-        pick_stream(s, (size_t)n * batch * 32);
-        s.args.batch = batch;
-        s.args.idist = n;
-        s.args.odist = n;
-        s.conj_in = s.conj_out = true;
-        e->steps_.push_back(s);
-        return ACLFFT_SUCCESS;
+        return c2c_rows(n, batch, BUF_IN, BUF_OUT, true, true, true);
     }
     aclfftResult d2z(long long n, long long batch) {
         e->in_place_unsafe_ = true;
@@ -937,23 +929,25 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
             default:
                 r = ACLFFT_NOT_SUPPORTED;
         }
-    } else if (d.rank == 1 && d.vertical && d.type == ACLFFT_C2C) {
+    } else if (d.rank == 1 && d.vertical && (d.type == ACLFFT_C2C || d.type == ACLFFT_Z2Z)) {
         // [n][cols] row-major, one transform per column (reference layout: VERTICAL, golden
         // src/fft1_d/tests/fft1_d_data/gen_data.py:79-81)
+        const bool z = d.type == ACLFFT_Z2Z;
         const long long n = d.n0, cols = d.batch;
-        e->in_bytes_ = e->out_bytes_ = (size_t)n * cols * 8;
-        r = b.c2c_columns(n, cols, 1, BUF_IN, BUF_OUT, true, true);
-    } else if (d.rank == 2 && d.type == ACLFFT_C2C) {
+        e->in_bytes_ = e->out_bytes_ = (size_t)n * cols * (z ? 16 : 8);
+        r = b.c2c_columns(n, cols, 1, BUF_IN, BUF_OUT, true, true, z);
+    } else if (d.rank == 2 && (d.type == ACLFFT_C2C || d.type == ACLFFT_Z2Z)) {
+        const bool z = d.type == ACLFFT_Z2Z;
         const long long nx = d.n0, ny = d.n1;
-        e->in_bytes_ = e->out_bytes_ = (size_t)nx * ny * d.batch * 8;
+        e->in_bytes_ = e->out_bytes_ = (size_t)nx * ny * d.batch * (z ? 16 : 8);
         // rows (length ny, contiguous) then columns (length nx, stride ny), columns in place
         if (nx == 1) {
-            r = b.c2c_rows(ny, d.batch, BUF_IN, BUF_OUT, true, true);
+            r = b.c2c_rows(ny, d.batch, BUF_IN, BUF_OUT, true, true, z);
         } else if (ny == 1) {
-            r = b.c2c_rows(nx, d.batch, BUF_IN, BUF_OUT, true, true);
+            r = b.c2c_rows(nx, d.batch, BUF_IN, BUF_OUT, true, true, z);
         } else {
-            r = b.c2c_rows(ny, nx * d.batch, BUF_IN, BUF_OUT, true, false);
-            if (r == ACLFFT_SUCCESS) r = b.c2c_columns(nx, ny, d.batch, BUF_OUT, BUF_OUT, false, true);
+            r = b.c2c_rows(ny, nx * d.batch, BUF_IN, BUF_OUT, true, false, z);
+            if (r == ACLFFT_SUCCESS) r = b.c2c_columns(nx, ny, d.batch, BUF_OUT, BUF_OUT, false, true, z);
         }
     }
     if (r != ACLFFT_SUCCESS) {
@@ -1061,7 +1055,7 @@ std::string Executor::describe() const {
         }
         if (s.type == Step::COLUMN) {
             const ofx_kernel_info& k = col_registry()[s.kernel];
-            os << "column n=" << k.n << " T=" << k.threads_per_transform << " cols=" << k.transforms_per_block
+            os << (k.kind == OFX_KIND_COL_Z ? "column-z n=" : "column n=") << k.n << " T=" << k.threads_per_transform << " cols=" << k.transforms_per_block
                << " radices=";
             for (int p = 0; p < k.npass; ++p) os << (p ? "x" : "") << k.radices[p];
             if (s.args.ft_len) os << " fourstep-twiddle";

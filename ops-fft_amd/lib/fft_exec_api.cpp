@@ -62,14 +62,15 @@ ACLFFT_API aclfftResult aclfftExecC2R(aclfftHandle plan, aclfftComplex* idata, a
 
 // Double precision.  The reference declares these (include/cann_ops_fft.h) and ships them as
 // NOT_IMPLEMENTED stubs (lib/fft_exec_api.cpp:87-116), refusing D2Z/Z2D/Z2Z plans at creation; here
-// double-precision 1D plans are accepted (ACLFFT_STRICT_REFERENCE_DOMAIN=1 restores the refusal)
-// and these entry points run them with the same checks as their single-precision counterparts.
+// double-precision 1D plans (HORIZONTAL, and VERTICAL for Z2Z) and 2D Z2Z plans are accepted
+// (ACLFFT_STRICT_REFERENCE_DOMAIN=1 restores the refusal) and these entry points run them with the
+// same checks as their single-precision counterparts.
 ACLFFT_API aclfftResult aclfftExecZ2Z(aclfftHandle plan, aclfftDoubleComplex* idata, aclfftDoubleComplex* odata,
                                       int direction) {
     aclfftResult r = common_checks(plan, idata, odata, ACLFFT_Z2Z);
     if (r != ACLFFT_SUCCESS) return r;
     if (direction != ACLFFT_FORWARD && direction != ACLFFT_BACKWARD) return ACLFFT_INVALID_VALUE;
-    if (plan->rank != 1) return ACLFFT_NOT_IMPLEMENTED;
+    if (plan->rank != 1 && plan->rank != 2) return ACLFFT_NOT_IMPLEMENTED;
     return run(plan, idata, odata, direction);
 }
 

@@ -40,12 +40,7 @@ ACLFFT_API aclfftResult aclfftPlan2d(aclfftHandle* plan, int batch, int nx, int 
     if (plan == nullptr) return ACLFFT_INVALID_VALUE;
     if (nx <= 0 || ny <= 0 || batch <= 0) return ACLFFT_INVALID_SIZE;
     if (!ofx::is_known_type(type)) return ACLFFT_INVALID_TYPE;
-    if (type != ACLFFT_C2C) {
-        char msg[96];
-        std::snprintf(msg, sizeof msg, "type=%d has no 2D execution implementation yet", (int)type);
-        ofx::log_reject("aclfftPlan2d", msg);
-        return ACLFFT_NOT_SUPPORTED;
-    }
+    if (ofx::check_2d_type(type, "aclfftPlan2d") != ACLFFT_SUCCESS) return ACLFFT_NOT_SUPPORTED;
     aclfftResult r = aclfftCreate(plan);
     if (r != ACLFFT_SUCCESS) return r;
     r = aclfftMakePlan2d(*plan, batch, nx, ny, type, nullptr);

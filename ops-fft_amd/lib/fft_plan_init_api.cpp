@@ -85,12 +85,7 @@ ACLFFT_API aclfftResult aclfftMakePlan2d(aclfftHandle plan, int batch, int nx, i
     if (plan == nullptr || plan->is_initialized) return ACLFFT_INVALID_VALUE;
     if (nx <= 0 || ny <= 0 || batch <= 0) return ACLFFT_INVALID_SIZE;
     if (!ofx::is_known_type(type)) return ACLFFT_INVALID_TYPE;
-    if (type != ACLFFT_C2C) {
-        char msg[96];
-        std::snprintf(msg, sizeof msg, "type=%d has no 2D execution implementation yet", (int)type);
-        ofx::log_reject("aclfftMakePlan2d", msg);
-        return ACLFFT_NOT_SUPPORTED;
-    }
+    if (ofx::check_2d_type(type, "aclfftMakePlan2d") != ACLFFT_SUCCESS) return ACLFFT_NOT_SUPPORTED;
     aclfftResult r = ofx::check_2d_domain(nx, ny, "aclfftMakePlan2d");
     if (r != ACLFFT_SUCCESS) return r;
     plan->rank = 2;

@@ -85,7 +85,7 @@ ACLFFT_API aclfftResult aclfftGetSize2d(aclfftHandle plan, int nx, int ny, aclff
     if (workSize == nullptr) return ACLFFT_INVALID_VALUE;
     if (nx <= 0 || ny <= 0) return ACLFFT_INVALID_SIZE;
     if (!ofx::is_known_type(type)) return ACLFFT_INVALID_TYPE;
-    if (type != ACLFFT_C2C) return ACLFFT_NOT_SUPPORTED;
+    if (ofx::check_2d_type(type, "aclfftGetSize2d") != ACLFFT_SUCCESS) return ACLFFT_NOT_SUPPORTED;
     aclfftResult r = ofx::check_2d_domain(nx, ny, "aclfftGetSize2d");
     if (r != ACLFFT_SUCCESS) return r;
     ofx::PlanDesc d;

@@ -76,6 +76,16 @@ inline aclfftResult check_1d_args(int nx, aclfftType type, int batch, int dimTyp
     return check_1d_domain(dimType, nx, entry);
 }
 
+// 2D type gate: the reference implements 2D C2C only (lib/fft_plan_api.cpp:96-100,
+// lib/fft_plan_init_api.cpp:200-204); here 2D Z2Z runs too, except in strict mode.
+inline aclfftResult check_2d_type(aclfftType type, const char* entry) {
+    if (type == ACLFFT_C2C || (type == ACLFFT_Z2Z && !strict_reference_domain())) return ACLFFT_SUCCESS;
+    char msg[96];
+    std::snprintf(msg, sizeof msg, "type=%d has no 2D execution implementation yet", (int)type);
+    log_reject(entry, msg);
+    return ACLFFT_NOT_SUPPORTED;
+}
+
 inline aclfftResult check_2d_domain(int nx, int ny, const char* entry) {
     char msg[160];
     if (strict_reference_domain()) {

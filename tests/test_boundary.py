@@ -137,18 +137,29 @@ def test_plan2d_gates(monkeypatch):
     assert lib.aclfftPlan2d(ctypes.byref(h), 1, 0, 64, aclfft.C2C) == aclfft.INVALID_SIZE
     assert lib.aclfftPlan2d(ctypes.byref(h), 1, 64, -1, aclfft.C2C) == aclfft.INVALID_SIZE
     assert lib.aclfftPlan2d(ctypes.byref(h), 1, 64, 64, 0x77) == aclfft.INVALID_TYPE
-    for t in (aclfft.Z2Z, aclfft.D2Z, aclfft.Z2D, aclfft.R2C, aclfft.C2R):
-        assert lib.aclfftPlan2d(ctypes.byref(h), 1, 64, 64, t) == aclfft.NOT_SUPPORTED
-    # makeplan2d_type_reject (fft2_d_test_arch32.cpp:52-86)
-    for t in (aclfft.Z2Z, aclfft.D2Z, aclfft.Z2D, aclfft.R2C, aclfft.C2R):
+    # makeplan2d_type_reject (fft2_d_test_arch32.cpp:52-86): every type but C2C in the strict
+    # mode; by default 2D Z2Z is accepted as well
+    for strict in (True, False):
+        monkeypatch.setenv("ACLFFT_STRICT_REFERENCE_DOMAIN", "1" if strict else "0")
+        refused = (aclfft.D2Z, aclfft.Z2D, aclfft.R2C, aclfft.C2R) + ((aclfft.Z2Z,) if strict else ())
+        for t in refused:
+            assert lib.aclfftPlan2d(ctypes.byref(h), 1, 64, 64, t) == aclfft.NOT_SUPPORTED
+            p = H()
+            lib.aclfftCreate(ctypes.byref(p))
+            assert lib.aclfftMakePlan2d(p, 1, 64, 64, t, None) == aclfft.NOT_SUPPORTED
+            lib.aclfftDestroy(p)
+        ws = ctypes.c_size_t(0)
         p = H()
         lib.aclfftCreate(ctypes.byref(p))
-        assert lib.aclfftMakePlan2d(p, 1, 64, 64, t, None) == aclfft.NOT_SUPPORTED
+        assert lib.aclfftGetSize2d(p, 64, 64, aclfft.Z2Z, ctypes.byref(ws)) == (
+            aclfft.NOT_SUPPORTED if strict else aclfft.SUCCESS)
         lib.aclfftDestroy(p)
-    p = H()
-    lib.aclfftCreate(ctypes.byref(p))
-    assert lib.aclfftMakePlan2d(p, 1, 64, 64, aclfft.C2C, None) == aclfft.SUCCESS
-    lib.aclfftDestroy(p)
+        accepted = (aclfft.C2C,) + (() if strict else (aclfft.Z2Z,))
+        for t in accepted:
+            p = H()
+            lib.aclfftCreate(ctypes.byref(p))
+            assert lib.aclfftMakePlan2d(p, 1, 64, 64, t, None) == aclfft.SUCCESS
+            lib.aclfftDestroy(p)
 
 
 @pytest.mark.parametrize("strict", [True, False])

@@ -4,7 +4,7 @@ import pytest
 
 import aclfft
 from oracle import oracle
-from _util import crandn, fp32_tol
+from _util import col_variants, crandn, fp32_tol
 
 pytestmark = pytest.mark.gpu
 
@@ -117,21 +117,7 @@ def test_every_registry_kernel(cuda, monkeypatch, kind, n, variant):
     assert err <= fp32_tol(2 * n)
 
 
-def _col_variants():
-    import ctypes
-    lib = aclfft.lib
-    lib.ofx_col_kernel_count.restype = ctypes.c_int
-    out, seen = [], {}
-    for i in range(lib.ofx_col_kernel_count()):
-        info = aclfft._lib.KernelInfo()
-        lib.ofx_col_kernel_info_get(i, ctypes.byref(info))
-        idx = seen.get(info.n, 0)
-        seen[info.n] = idx + 1
-        out.append((info.n, idx))
-    return out
-
-
-@pytest.mark.parametrize("rows,variant", _col_variants())
+@pytest.mark.parametrize("rows,variant", col_variants(3))
 @pytest.mark.parametrize("cols", [40, 7])
 def test_every_column_kernel_2d_and_vertical(cuda, monkeypatch, rows, variant, cols):
     """Column kernels (2D second pass and the VERTICAL layout) against the FP64 oracle, with a

@@ -1,6 +1,7 @@
 """GPU parity of the double-precision transforms (aclfftExecZ2Z / D2Z / Z2D) against the FP64
 oracle.  Tolerance: rel-L2 <= 1e-14 * log2(n) (FP32 arithmetic would give ~1e-7), both
-directions, batches that leave partial workgroups, host and device pointers, in place, adjoints."""
+directions, batches that leave partial workgroups, host and device pointers, in place, adjoints,
+the complex128 four-step (two and three factors), 2D Z2Z and the VERTICAL Z2Z layout."""
 import math
 
 import numpy as np
@@ -8,7 +9,7 @@ import pytest
 
 import aclfft
 from oracle import oracle
-from _util import crandn
+from _util import col_variants, crandn
 
 pytestmark = pytest.mark.gpu
 
@@ -157,3 +158,98 @@ def test_every_double_kernel(cuda, monkeypatch, kind, n, variant):
     print(p.describe(), "rel_l2 %.3e" % err)
     p.destroy()
     assert err <= fp64_tol(m), err
+
+
+@pytest.mark.parametrize("n,batch,levels", [
+    (1 << 14, 3, 0), (1 << 16, 2, 0), (1 << 16, 2, 3), (1 << 20, 2, 2), (1 << 20, 1, 3),
+    (1 << 22, 1, 0)])
+def test_z2z_fourstep_vs_oracle(cuda, monkeypatch, n, batch, levels):
+    """Lengths beyond the single-workgroup kernels: the complex128 four-step over compiled column
+    and transposed-store row kernels, both factorisations (ACLFFT_FOURSTEP_LEVELS)."""
+    import torch
+    if levels:
+        monkeypatch.setenv("ACLFFT_FOURSTEP_LEVELS", str(levels))
+    x = crandn((batch, n), n + levels)
+    p = aclfft.Plan.plan1d(n, aclfft.Z2Z, batch)
+    if "not implemented" in p.describe():
+        p.destroy()
+        pytest.skip("no complex128 factorisation of %d" % n)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    for direction in (aclfft.FORWARD, aclfft.BACKWARD):
+        p.exec_z2z(xt, yt, direction)
+        torch.cuda.synchronize()
+        err = oracle.rel_l2(yt.cpu().numpy(), oracle.c2c(x, direction))
+        print(p.describe(), "dir", direction, "rel_l2 %.3e" % err)
+        assert err <= fp64_tol(n), err
+    p.destroy()
+
+
+def test_z2z_fourstep_in_place(cuda):
+    import torch
+    n, batch = 1 << 18, 2
+    x = crandn((batch, n), 9)
+    p = aclfft.Plan.plan1d(n, aclfft.Z2Z, batch)
+    xt = torch.from_numpy(x).cuda()
+    p.exec_z2z(xt, xt, aclfft.BACKWARD)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(xt.cpu().numpy(), oracle.c2c(x, aclfft.BACKWARD))
+    p.destroy()
+    assert err <= fp64_tol(n), err
+
+
+@pytest.mark.parametrize("batch,nx,ny", [(3, 64, 64), (2, 256, 512), (1, 2048, 2048), (2, 4096, 60),
+                                         (4, 1, 1024), (2, 512, 1), (1, 128, 1 << 16)])
+def test_z2z_2d_vs_oracle(cuda, batch, nx, ny):
+    """2D Z2Z: complex128 rows (length ny) then complex128 columns (length nx)."""
+    import torch
+    x = crandn((batch, nx, ny), nx * 7 + ny)
+    p = aclfft.Plan.plan2d(batch, nx, ny, aclfft.Z2Z)
+    if "not implemented" in p.describe():
+        p.destroy()
+        pytest.skip("no complex128 kernels for %dx%d" % (nx, ny))
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    for direction in (aclfft.FORWARD, aclfft.BACKWARD):
+        p.exec_z2z(xt, yt, direction)
+        torch.cuda.synchronize()
+        err = oracle.rel_l2(yt.cpu().numpy(), oracle.c2c_2d(x, direction))
+        print(p.describe(), "dir", direction, "rel_l2 %.3e" % err)
+        assert err <= fp64_tol(nx * ny), err
+    p.destroy()
+
+
+@pytest.mark.parametrize("n", [256, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("cols", [40, 7])
+def test_z2z_vertical_vs_oracle(cuda, n, cols):
+    """VERTICAL Z2Z ([n][cols] row-major, one transform per column), in place."""
+    import torch
+    v = crandn((n, cols), n + cols)
+    p = aclfft.Plan.plan1d(n, aclfft.Z2Z, cols, aclfft.VERTICAL)
+    vt = torch.from_numpy(v).cuda()
+    p.exec_z2z(vt, vt, aclfft.FORWARD)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(vt.cpu().numpy(), oracle.c2c_vertical(v))
+    print(p.describe(), "rel_l2 %.3e" % err)
+    p.destroy()
+    assert err <= fp64_tol(n), err
+
+
+@pytest.mark.parametrize("rows,variant", col_variants(14))
+@pytest.mark.parametrize("cols", [60, 243])
+def test_every_double_column_kernel(cuda, monkeypatch, rows, variant, cols):
+    """Each compiled complex128 column kernel as the second pass of a 2D Z2Z plan, with column
+    counts (row lengths that have complex128 row kernels) that leave a partial column group,
+    forced with ACLFFT_COLUMN_VARIANT."""
+    import torch
+    monkeypatch.setenv("ACLFFT_COLUMN_VARIANT", str(variant))
+    x = crandn((3, rows, cols), rows + cols + variant)
+    p = aclfft.Plan.plan2d(3, rows, cols, aclfft.Z2Z)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    p.exec_z2z(xt, yt, aclfft.BACKWARD)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(yt.cpu().numpy(), oracle.c2c_2d(x, aclfft.BACKWARD))
+    print(p.describe(), "rel_l2 %.3e" % err)
+    p.destroy()
+    assert err <= fp64_tol(rows * cols), err
