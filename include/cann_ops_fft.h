@@ -109,13 +109,15 @@ extern "C" {
  * ACLFFT_VERTICAL.  Replaces reference include/cann_ops_fft.h:152-156 (lib/fft_plan_api.cpp:47-85). */
 ACLFFT_API aclfftResult aclfftPlan1d(aclfftHandle* plan, int nx, aclfftType type, int batch, int dimType);
 
-/* Create + initialise a 2D C2C plan, row-major [batch][nx][ny] (ny fastest).  Argument order is
- * (plan, batch, nx, ny, type) as in the reference include/cann_ops_fft.h:174
+/* Create + initialise a 2D C2C (or Z2Z) plan, row-major [batch][nx][ny] (ny fastest).  Argument
+ * order is (plan, batch, nx, ny, type) as in the reference include/cann_ops_fft.h:174
  * (lib/fft_plan_api.cpp:90-122). */
 ACLFFT_API aclfftResult aclfftPlan2d(aclfftHandle* plan, int batch, int nx, int ny, aclfftType type);
 
-/* 3D placeholder: returns ACLFFT_NOT_IMPLEMENTED after argument checks, as the reference
- * (include/cann_ops_fft.h:192-193, lib/fft_plan_api.cpp:127-150). */
+/* Create + initialise a 3D C2C / Z2Z plan of one [nx][ny][nz] volume (nz fastest).  Replaces the
+ * reference's placeholder (include/cann_ops_fft.h:192-193, lib/fft_plan_api.cpp:127-150), whose
+ * argument checks come first here too; other types: ACLFFT_NOT_SUPPORTED.  With
+ * ACLFFT_STRICT_REFERENCE_DOMAIN=1 it returns ACLFFT_NOT_IMPLEMENTED like the reference. */
 ACLFFT_API aclfftResult aclfftPlan3d(aclfftHandle* plan, int nx, int ny, int nz, aclfftType type);
 
 /* Allocate an empty plan (reference include/cann_ops_fft.h:199, lib/fft_plan_api.cpp:24-36). */
@@ -133,12 +135,15 @@ ACLFFT_API aclfftResult aclfftMakePlan1d(aclfftHandle plan, int nx, aclfftType t
 ACLFFT_API aclfftResult aclfftMakePlan2d(aclfftHandle plan, int batch, int nx, int ny, aclfftType type,
                                          size_t* workSize);
 
-/* 3D placeholder (reference include/cann_ops_fft.h:259-260): ACLFFT_NOT_IMPLEMENTED. */
+/* Replaces the reference's placeholder (include/cann_ops_fft.h:259-260,
+ * lib/fft_plan_init_api.cpp:250-254): the 3D plan of aclfftPlan3d, NOT_IMPLEMENTED in strict mode. */
 ACLFFT_API aclfftResult aclfftMakePlan3d(aclfftHandle plan, int nx, int ny, int nz, aclfftType type,
                                          size_t* workSize);
 
-/* Workspace-query / stride / version placeholders: ACLFFT_NOT_IMPLEMENTED, as the reference
- * (include/cann_ops_fft.h:273-336,439,454,464; lib/fft_utils_api.cpp:66-109). */
+/* Workspace API (include/cann_ops_fft.h:273-336; placeholders in the reference,
+ * lib/fft_utils_api.cpp:66-94, implemented here): GetSize1d/2d/3d report the workspace a plan of
+ * those arguments would hold (host only), GetSize the built plan's; SetAutoAllocation(plan, 0) +
+ * SetWorkArea(plan, devptr) run the plan on the caller's device area. */
 ACLFFT_API aclfftResult aclfftGetSize1d(aclfftHandle plan, int nx, aclfftType type, int batch, size_t* workSize);
 ACLFFT_API aclfftResult aclfftGetSize2d(aclfftHandle plan, int nx, int ny, aclfftType type, size_t* workSize);
 ACLFFT_API aclfftResult aclfftGetSize3d(aclfftHandle plan, int nx, int ny, int nz, aclfftType type,

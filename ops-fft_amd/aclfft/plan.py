@@ -72,6 +72,13 @@ class Plan:
         return cls(h, type, (nx, ny), batch)
 
     @classmethod
+    def plan3d(cls, nx, ny, nz, type=C2C):
+        """One [nx][ny][nz] volume (nz fastest), C2C or Z2Z."""
+        h = ctypes.c_void_p()
+        check(lib.aclfftPlan3d(ctypes.byref(h), nx, ny, nz, type), "aclfftPlan3d")
+        return cls(h, type, (nx, ny, nz), 1)
+
+    @classmethod
     def plan_many_uneven(cls, lengths, batches, type=C2C):
         """Groups of batches[g] transforms of length lengths[g], stored one group after the other
         (aclfftExtPlanManyUneven)."""

@@ -123,6 +123,11 @@ ColEntry col_entry() {
     col_entry<StockhamCfg<N, T, COLS, 4, false, Radices<__VA_ARGS__>, 1, COLS, COLPAD, double2>>()
 
 static const ColEntry kCols[] = {
+    OFX_COL(2, 1, 64, 1, 2),
+    OFX_COL(4, 1, 64, 1, 4),
+    OFX_COL(8, 1, 64, 1, 8),
+    OFX_COL(16, 1, 64, 1, 16),
+    OFX_COL(32, 4, 32, 1, 4, 8),
     OFX_COL(64, 8, 16, 1, 8, 8),
     OFX_COL(64, 8, 32, 1, 8, 8),
     OFX_COL(128, 8, 16, 1, 8, 16),
@@ -140,6 +145,11 @@ static const ColEntry kCols[] = {
     OFX_COL(4096, 256, 4, 0, 16, 16, 16),
     OFX_COL(4096, 256, 2, 1, 16, 16, 16),
     // complex128 (16-byte elements: half the columns per workgroup for the same LDS)
+    OFX_COLZ(2, 1, 64, 1, 2),
+    OFX_COLZ(4, 1, 64, 1, 4),
+    OFX_COLZ(8, 1, 64, 1, 8),
+    OFX_COLZ(16, 1, 64, 1, 16),
+    OFX_COLZ(32, 4, 32, 1, 4, 8),
     OFX_COLZ(64, 8, 16, 1, 8, 8),
     OFX_COLZ(128, 16, 16, 1, 8, 16),
     OFX_COLZ(256, 32, 16, 1, 4, 8, 8),

@@ -936,18 +936,28 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
         const long long n = d.n0, cols = d.batch;
         e->in_bytes_ = e->out_bytes_ = (size_t)n * cols * (z ? 16 : 8);
         r = b.c2c_columns(n, cols, 1, BUF_IN, BUF_OUT, true, true, z);
-    } else if (d.rank == 2 && (d.type == ACLFFT_C2C || d.type == ACLFFT_Z2Z)) {
+    } else if ((d.rank == 2 || d.rank == 3) && (d.type == ACLFFT_C2C || d.type == ACLFFT_Z2Z)) {
         const bool z = d.type == ACLFFT_Z2Z;
-        const long long nx = d.n0, ny = d.n1;
-        e->in_bytes_ = e->out_bytes_ = (size_t)nx * ny * d.batch * (z ? 16 : 8);
-        // rows (length ny, contiguous) then columns (length nx, stride ny), columns in place
-        if (nx == 1) {
-            r = b.c2c_rows(ny, d.batch, BUF_IN, BUF_OUT, true, true, z);
-        } else if (ny == 1) {
-            r = b.c2c_rows(nx, d.batch, BUF_IN, BUF_OUT, true, true, z);
-        } else {
-            r = b.c2c_rows(ny, nx * d.batch, BUF_IN, BUF_OUT, true, false, z);
-            if (r == ACLFFT_SUCCESS) r = b.c2c_columns(nx, ny, d.batch, BUF_OUT, BUF_OUT, false, true, z);
+        // axes slowest first; length-1 axes are dropped (a 64x1x32 volume is a 64x32 image)
+        std::vector<long long> dims;
+        long long total = 1;
+        const int lens[3] = {d.n0, d.n1, d.n2};
+        for (int i = 0; i < d.rank; ++i) {
+            total *= lens[i];
+            if (lens[i] > 1) dims.push_back(lens[i]);
+        }
+        e->in_bytes_ = e->out_bytes_ = (size_t)total * d.batch * (z ? 16 : 8);
+        if (dims.empty()) dims.push_back(1);
+        // rows along the fastest axis (contiguous), then every slower axis as columns in place:
+        // axis i has length dims[i], stride inner = product of the faster axes, and
+        // outer = batch * product of the slower axes images
+        const int k = (int)dims.size();
+        long long inner = dims[k - 1];
+        r = b.c2c_rows(dims[k - 1], total / dims[k - 1] * d.batch, BUF_IN, BUF_OUT, true, k == 1, z);
+        for (int i = k - 2; i >= 0 && r == ACLFFT_SUCCESS; --i) {
+            const long long outer = total / (inner * dims[i]) * d.batch;
+            r = b.c2c_columns(dims[i], inner, outer, BUF_OUT, BUF_OUT, false, i == 0, z);
+            inner *= dims[i];
         }
     }
     if (r != ACLFFT_SUCCESS) {

@@ -60,7 +60,8 @@ struct Step {
 
 struct PlanDesc {
     int rank = 1;
-    int n0 = 0, n1 = 0;            // 1D: n0 = length; 2D: n0 = nx (slow), n1 = ny (fast)
+    int n0 = 0, n1 = 0, n2 = 0;    // 1D: n0 = length; 2D: n0 = nx (slow), n1 = ny (fast);
+                                   // 3D: n0 = nx (slowest), n1 = ny, n2 = nz (fastest)
     long long batch = 1;
     aclfftType type = ACLFFT_C2C;
     bool vertical = false;         // 1D VERTICAL layout: n0 rows x `batch` columns

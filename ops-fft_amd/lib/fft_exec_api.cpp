@@ -34,7 +34,7 @@ ACLFFT_API aclfftResult aclfftExecC2C(aclfftHandle plan, aclfftComplex* idata, a
     aclfftResult r = common_checks(plan, idata, odata, ACLFFT_C2C);
     if (r != ACLFFT_SUCCESS) return r;
     if (direction != ACLFFT_FORWARD && direction != ACLFFT_BACKWARD) return ACLFFT_INVALID_VALUE;
-    if (plan->rank != 1 && plan->rank != 2) return ACLFFT_NOT_IMPLEMENTED;
+    if (plan->rank < 1 || plan->rank > 3) return ACLFFT_NOT_IMPLEMENTED;
     return run(plan, idata, odata, direction);
 }
 
@@ -70,7 +70,7 @@ ACLFFT_API aclfftResult aclfftExecZ2Z(aclfftHandle plan, aclfftDoubleComplex* id
     aclfftResult r = common_checks(plan, idata, odata, ACLFFT_Z2Z);
     if (r != ACLFFT_SUCCESS) return r;
     if (direction != ACLFFT_FORWARD && direction != ACLFFT_BACKWARD) return ACLFFT_INVALID_VALUE;
-    if (plan->rank != 1 && plan->rank != 2) return ACLFFT_NOT_IMPLEMENTED;
+    if (plan->rank < 1 || plan->rank > 3) return ACLFFT_NOT_IMPLEMENTED;
     return run(plan, idata, odata, direction);
 }
 

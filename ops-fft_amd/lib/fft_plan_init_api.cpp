@@ -108,15 +108,36 @@ ACLFFT_API aclfftResult aclfftMakePlan2d(aclfftHandle plan, int batch, int nx, i
     return ACLFFT_SUCCESS;
 }
 
+// 3D C2C / Z2Z of one [nx][ny][nz] volume (nz fastest): rows along z, then columns along y and x.
+// The reference's is a stub returning NOT_IMPLEMENTED for any arguments
+// (lib/fft_plan_init_api.cpp:250-254); ACLFFT_STRICT_REFERENCE_DOMAIN=1 keeps that.
 ACLFFT_API aclfftResult aclfftMakePlan3d(aclfftHandle plan, int nx, int ny, int nz, aclfftType type,
                                          size_t* workSize) {
-    (void)plan;
-    (void)nx;
-    (void)ny;
-    (void)nz;
-    (void)type;
-    (void)workSize;
-    return ACLFFT_NOT_IMPLEMENTED;
+    if (ofx::strict_reference_domain()) return ACLFFT_NOT_IMPLEMENTED;
+    if (plan == nullptr || plan->is_initialized) return ACLFFT_INVALID_VALUE;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return ACLFFT_INVALID_SIZE;
+    if (!ofx::is_known_type(type)) return ACLFFT_INVALID_TYPE;
+    aclfftResult r = ofx::check_3d_type_domain(nx, ny, nz, type, "aclfftMakePlan3d");
+    if (r != ACLFFT_SUCCESS) return r;
+    plan->rank = 3;
+    plan->lengths[0] = nx;
+    plan->lengths[1] = ny;
+    plan->lengths[2] = nz;
+    plan->batch = 1;
+    plan->type = type;
+    plan->stride[0] = plan->stride[1] = plan->stride[2] = 1;
+    if (!io_sizes_fit(plan)) return ACLFFT_INVALID_SIZE;
+    ofx::PlanDesc d;
+    d.rank = 3;
+    d.n0 = nx;
+    d.n1 = ny;
+    d.n2 = nz;
+    d.batch = 1;
+    d.type = type;
+    r = attach_executor(plan, d, workSize);
+    if (r != ACLFFT_SUCCESS) return r;
+    plan->is_initialized = true;
+    return ACLFFT_SUCCESS;
 }
 
 }  // extern "C"

@@ -97,10 +97,25 @@ ACLFFT_API aclfftResult aclfftGetSize2d(aclfftHandle plan, int nx, int ny, aclff
     return size_of(d, workSize);
 }
 
+// The reference's is a stub (lib/fft_utils_api.cpp:76-79), kept in strict mode; otherwise the
+// checks of aclfftGetSize2d with the 3D gates of aclfftMakePlan3d.
 ACLFFT_API aclfftResult aclfftGetSize3d(aclfftHandle plan, int nx, int ny, int nz, aclfftType type,
                                         size_t* workSize) {
-    (void)plan; (void)nx; (void)ny; (void)nz; (void)type; (void)workSize;
-    return ACLFFT_NOT_IMPLEMENTED;
+    if (ofx::strict_reference_domain()) return ACLFFT_NOT_IMPLEMENTED;
+    if (plan == nullptr) return ACLFFT_INVALID_PLAN;
+    if (workSize == nullptr) return ACLFFT_INVALID_VALUE;
+    if (nx <= 0 || ny <= 0 || nz <= 0) return ACLFFT_INVALID_SIZE;
+    if (!ofx::is_known_type(type)) return ACLFFT_INVALID_TYPE;
+    aclfftResult r = ofx::check_3d_type_domain(nx, ny, nz, type, "aclfftGetSize3d");
+    if (r != ACLFFT_SUCCESS) return r;
+    ofx::PlanDesc d;
+    d.rank = 3;
+    d.n0 = nx;
+    d.n1 = ny;
+    d.n2 = nz;
+    d.batch = 1;
+    d.type = type;
+    return size_of(d, workSize);
 }
 
 ACLFFT_API aclfftResult aclfftGetSize(aclfftHandle plan, size_t* workSize) {

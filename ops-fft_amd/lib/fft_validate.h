@@ -106,6 +106,25 @@ inline aclfftResult check_2d_domain(int nx, int ny, const char* entry) {
     return ACLFFT_SUCCESS;
 }
 
+// 3D: the reference's aclfftMakePlan3d / aclfftGetSize3d are stubs returning NOT_IMPLEMENTED
+// (lib/fft_plan_init_api.cpp:250-254, lib/fft_utils_api.cpp:76-79; kept in strict mode).  Here 3D
+// C2C and Z2Z run, with the 2D gates per axis: NOT_SUPPORTED for another type or an axis beyond
+// 2^27.
+inline aclfftResult check_3d_type_domain(int nx, int ny, int nz, aclfftType type, const char* entry) {
+    char msg[160];
+    if (type != ACLFFT_C2C && type != ACLFFT_Z2Z) {
+        std::snprintf(msg, sizeof msg, "type=%d has no 3D execution implementation yet", (int)type);
+        log_reject(entry, msg);
+        return ACLFFT_NOT_SUPPORTED;
+    }
+    if (nx > kMaxHorizontalN || ny > kMaxHorizontalN || nz > kMaxHorizontalN) {
+        std::snprintf(msg, sizeof msg, "size=%dx%dx%d exceeds 2^27 in one dimension", nx, ny, nz);
+        log_reject(entry, msg);
+        return ACLFFT_NOT_SUPPORTED;
+    }
+    return ACLFFT_SUCCESS;
+}
+
 }  // namespace ofx
 
 #endif

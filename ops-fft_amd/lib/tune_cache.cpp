@@ -24,8 +24,8 @@ std::string cache_path() {
 
 std::string key_of(const PlanDesc& d, size_t step) {
     std::ostringstream os;
-    os << (int)d.type << ' ' << d.rank << ' ' << d.n0 << ' ' << d.n1 << ' ' << d.batch << ' ' << (d.vertical ? 1 : 0)
-       << ' ' << step;
+    os << (int)d.type << ' ' << d.rank << ' ' << d.n0 << ' ' << d.n1 << ' ' << d.n2 << ' ' << d.batch << ' '
+       << (d.vertical ? 1 : 0) << ' ' << step;
     return os.str();
 }
 

@@ -28,6 +28,11 @@
  *                      (golden torch.fft.fft2: src/fft2_d/tests/fft2_d_data/gen_data.py:59-61).
  *   oracle_c2c_vertical column transforms of an [n][batch] matrix (vertical layout; golden
  *                      src/fft1_d/tests/fft1_d_data/gen_data.py:79-81).
+ *   oracle_c2c_3d      3D transform of [batch][nx][ny][nz]: a 2D transform of every x-plane, then
+ *                      the transforms along x (the reference has no 3D execution, its
+ *                      aclfftMakePlan3d is a stub, lib/fft_plan_init_api.cpp:250-254; the
+ *                      convention is that of its 2D golden, torch.fft.fftn with an unnormalised
+ *                      inverse).
  *
  * Build: make -C oracle  (gcc, no dependencies) -> oracle/liboracle.so
  */
@@ -310,4 +315,17 @@ void oracle_c2c_vertical(const double* in, double* out, long long n, long long c
     }
     free(col);
     free(colo);
+}
+
+void oracle_c2c_3d(const double* in, double* out, long long nx, long long ny, long long nz, long long batch,
+                   int dir) {
+    const long long plane = ny * nz;
+    double* tmp = (double*)malloc(sizeof(double) * 2 * (size_t)(nx * plane));
+    oracle_c2c_2d(in, out, ny, nz, nx * batch, dir);
+    for (long long b = 0; b < batch; ++b) {
+        double* img = out + 2 * b * nx * plane;
+        oracle_c2c_vertical(img, tmp, nx, plane, dir);
+        memcpy(img, tmp, sizeof(double) * 2 * (size_t)(nx * plane));
+    }
+    free(tmp);
 }

@@ -33,6 +33,7 @@ def load(path):
         ("oracle_c2r", [d, d, ll, ll]),
         ("oracle_c2c_2d", [d, d, ll, ll, ll, ctypes.c_int]),
         ("oracle_c2c_vertical", [d, d, ll, ll, ctypes.c_int]),
+        ("oracle_c2c_3d", [d, d, ll, ll, ll, ll, ctypes.c_int]),
     ):
         f = getattr(L, name)
         f.argtypes = args
@@ -100,6 +101,15 @@ def c2c_2d(x, direction=FORWARD):
     nx, ny = x.shape[-2], x.shape[-1]
     out = np.empty_like(x)
     lib().oracle_c2c_2d(_p(x), _p(out), nx, ny, x.size // (nx * ny), direction)
+    return out
+
+
+def c2c_3d(x, direction=FORWARD):
+    """3D C2C over the last three axes ([..., nx, ny, nz], nz fastest)."""
+    x = _c128(x)
+    nx, ny, nz = x.shape[-3], x.shape[-2], x.shape[-1]
+    out = np.empty_like(x)
+    lib().oracle_c2c_3d(_p(x), _p(out), nx, ny, nz, x.size // (nx * ny * nz), direction)
     return out
 
 

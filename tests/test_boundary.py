@@ -180,18 +180,49 @@ def test_makeplan2d_size_domain(monkeypatch, strict):
     lib.aclfftDestroy(p)
 
 
-def test_plan3d_and_placeholders():
+def test_plan3d_gates(monkeypatch):
+    """The reference's 3D entry points are stubs (NOT_IMPLEMENTED for any arguments once Plan3d's
+    own checks pass, lib/fft_plan_api.cpp:127-149, lib/fft_plan_init_api.cpp:250-254,
+    lib/fft_utils_api.cpp:76-79): kept in the strict mode.  By default 3D C2C / Z2Z plan, and the
+    other types are NOT_SUPPORTED as in 2D."""
     h = H()
+    ws = ctypes.c_size_t()
+    monkeypatch.setenv("ACLFFT_STRICT_REFERENCE_DOMAIN", "1")
     assert lib.aclfftPlan3d(ctypes.byref(h), 8, 8, 8, aclfft.C2C) == aclfft.NOT_IMPLEMENTED
     assert not h.value
     assert lib.aclfftPlan3d(ctypes.byref(h), 0, 8, 8, aclfft.C2C) == aclfft.INVALID_SIZE
     assert lib.aclfftPlan3d(ctypes.byref(h), 8, 8, 8, 0x11) == aclfft.INVALID_TYPE
+    assert lib.aclfftMakePlan3d(None, 8, 8, 8, aclfft.C2C, None) == aclfft.NOT_IMPLEMENTED
+    assert lib.aclfftGetSize3d(None, 8, 8, 8, aclfft.C2C, ctypes.byref(ws)) == aclfft.NOT_IMPLEMENTED
+    monkeypatch.setenv("ACLFFT_STRICT_REFERENCE_DOMAIN", "0")
+    assert lib.aclfftPlan3d(None, 8, 8, 8, aclfft.C2C) == aclfft.INVALID_VALUE
+    assert lib.aclfftPlan3d(ctypes.byref(h), 8, 0, 8, aclfft.C2C) == aclfft.INVALID_SIZE
+    assert lib.aclfftPlan3d(ctypes.byref(h), 8, 8, 8, 0x11) == aclfft.INVALID_TYPE
+    for t in (aclfft.R2C, aclfft.C2R, aclfft.D2Z, aclfft.Z2D):
+        assert lib.aclfftPlan3d(ctypes.byref(h), 8, 8, 8, t) == aclfft.NOT_SUPPORTED and not h.value
+    assert lib.aclfftPlan3d(ctypes.byref(h), 8, 1 << 28, 8, aclfft.C2C) == aclfft.NOT_SUPPORTED
+    assert lib.aclfftMakePlan3d(None, 8, 8, 8, aclfft.C2C, None) == aclfft.INVALID_VALUE
+    assert lib.aclfftGetSize3d(None, 8, 8, 8, aclfft.C2C, ctypes.byref(ws)) == aclfft.INVALID_PLAN
+    for t in (aclfft.C2C, aclfft.Z2Z):
+        assert lib.aclfftPlan3d(ctypes.byref(h), 64, 32, 16, t) == aclfft.SUCCESS and h.value
+        assert lib.aclfftMakePlan3d(h, 64, 32, 16, t, None) == aclfft.INVALID_VALUE  # initialised
+        lib.aclfftDestroy(h)
+        p = H()
+        lib.aclfftCreate(ctypes.byref(p))
+        assert lib.aclfftGetSize3d(p, 64, 32, 16, t, ctypes.byref(ws)) == aclfft.SUCCESS
+        assert ws.value == 0  # compiled row and column kernels in place: no workspace
+        # a 2^16-point z axis runs as a four-step with a workspace the size of the data
+        assert lib.aclfftGetSize3d(p, 2, 2, 1 << 16, t, ctypes.byref(ws)) == aclfft.SUCCESS
+        assert ws.value >= 4 * (1 << 16) * (8 if t == aclfft.C2C else 16)
+        lib.aclfftDestroy(p)
+
+
+def test_placeholders():
     ws = ctypes.c_size_t()
     # the workspace API is implemented (a superset of the reference's placeholders); without a
-    # plan handle it reports INVALID_PLAN, and 3D stays unimplemented
+    # plan handle it reports INVALID_PLAN
     assert lib.aclfftGetSize1d(None, 8, aclfft.C2C, 1, ctypes.byref(ws)) == aclfft.INVALID_PLAN
     assert lib.aclfftGetSize2d(None, 8, 8, aclfft.C2C, ctypes.byref(ws)) == aclfft.INVALID_PLAN
-    assert lib.aclfftGetSize3d(None, 8, 8, 8, aclfft.C2C, ctypes.byref(ws)) == aclfft.NOT_IMPLEMENTED
     assert lib.aclfftGetSize(None, ctypes.byref(ws)) == aclfft.INVALID_PLAN
     assert lib.aclfftSetAutoAllocation(None, 1) == aclfft.INVALID_PLAN
     assert lib.aclfftSetWorkArea(None, None) == aclfft.INVALID_PLAN

@@ -126,6 +126,26 @@ def test_2d_separable_and_vertical():
     assert oracle.rel_l2(oracle.c2c_vertical(m), oracle.c2c(m.T).T) < 1e-14
 
 
+def test_3d_separable_delta_and_fftn():
+    """3D: separable inputs transform to the outer product of 1D transforms; a delta at (i, j, k)
+    to the closed-form phase ramp; random data matches numpy.fft.fftn / ifftn * N over the last
+    three axes (distinct nx, ny, nz, so a swapped axis fails)."""
+    nx, ny, nz = 6, 10, 12
+    a, b, c = crand(nx, 5), crand(ny, 6), crand(nz, 7)
+    vol = np.einsum("i,j,k->ijk", a, b, c)[None]
+    expect = np.einsum("i,j,k->ijk", oracle.c2c(a), oracle.c2c(b), oracle.c2c(c))[None]
+    assert oracle.rel_l2(oracle.c2c_3d(vol), expect) < 1e-13
+    d = np.zeros((1, nx, ny, nz), complex)
+    d[0, 2, 3, 5] = 1
+    u, v, w = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    ramp = np.exp(-2j * np.pi * (2 * u / nx + 3 * v / ny + 5 * w / nz))
+    assert np.max(np.abs(oracle.c2c_3d(d)[0] - ramp)) < 1e-13
+    x = crand((2, nx, ny, nz), 8)
+    assert oracle.rel_l2(oracle.c2c_3d(x), np.fft.fftn(x, axes=(1, 2, 3))) < 1e-13
+    assert oracle.rel_l2(oracle.c2c_3d(x, oracle.BACKWARD),
+                         np.fft.ifftn(x, axes=(1, 2, 3)) * nx * ny * nz) < 1e-13
+
+
 # ---- the reference's golden path (torch.fft), committed fixtures -------------------------------
 def _golden():
     return np.load(GOLDEN, allow_pickle=False)
