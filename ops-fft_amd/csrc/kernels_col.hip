@@ -53,7 +53,7 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
                                                   long long ftlen, int ftlog2) {
     using V = typename C::V;
     using S = scalar_t<V>;
-    __shared__ V lds[C::LDS_N];
+    __shared__ std::conditional_t<C::SPLITX, S, V> lds[C::LDS_N];
     const int c = threadIdx.x % C::COLS;
     const int tid = threadIdx.x / C::COLS;
     // XCD-aware group index (blocks are dispatched round-robin over the 8 XCDs)
@@ -68,7 +68,10 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
     ColLoad<C> gin{static_cast<const V*>(a.in) + img * a.idist + cc, a.istride, a.in_sign};
     ColStore<C, FT> gout{static_cast<V*>(a.out) + img * a.odist + cc, a.ostride, a.out_sign, scale, ok,
                          static_cast<const double2*>(a.tw_aux), cc, ftlen, ftlog2};
-    stockham_passes<C, false, false>(lds + c, tid, static_cast<const V*>(a.tw), gin, gout);
+    if constexpr (C::SPLITX)
+        split_fft<C>(lds + c, tid, static_cast<const V*>(a.tw), gin, gout);
+    else
+        stockham_passes<C, false, false>(lds + c, tid, static_cast<const V*>(a.tw), gin, gout);
 }
 
 template <class C>
@@ -109,7 +112,7 @@ ColEntry col_entry() {
     e.info.block_threads = C::BLOCK;
     e.info.npass = C::NP;
     for (int p = 0; p < C::NP; ++p) e.info.radices[p] = C::radix(p);
-    e.info.lds_bytes = (int)(C::LDS_N * sizeof(typename C::V));
+    e.info.lds_bytes = (int)(C::LDS_N * (C::SPLITX ? sizeof(scalar_t<typename C::V>) : sizeof(typename C::V)));
     e.info.tw_len = C::tw_len();
     e.info.iov = 1;
     e.fn = &launch_col<C>;
@@ -121,6 +124,11 @@ ColEntry col_entry() {
     col_entry<StockhamCfg<N, T, COLS, 4, false, Radices<__VA_ARGS__>, 1, COLS, COLPAD>>()
 #define OFX_COLZ(N, T, COLS, COLPAD, ...) \
     col_entry<StockhamCfg<N, T, COLS, 4, false, Radices<__VA_ARGS__>, 1, COLS, COLPAD, double2>>()
+// split (real / imaginary) LDS exchange: half the LDS per workgroup
+#define OFX_COLX(N, T, COLS, COLPAD, ...) \
+    col_entry<StockhamCfg<N, T, COLS, 4, false, Radices<__VA_ARGS__>, 1, COLS, COLPAD, float2, false, true>>()
+#define OFX_COLZX(N, T, COLS, COLPAD, ...) \
+    col_entry<StockhamCfg<N, T, COLS, 4, false, Radices<__VA_ARGS__>, 1, COLS, COLPAD, double2, false, true>>()
 
 static const ColEntry kCols[] = {
     OFX_COL(2, 1, 64, 1, 2),
@@ -139,11 +147,13 @@ static const ColEntry kCols[] = {
     OFX_COL(1024, 64, 16, 1, 16, 16, 4),
     OFX_COL(1024, 64, 8, 1, 16, 16, 4),
     OFX_COL(1024, 128, 8, 1, 8, 8, 16),
+    OFX_COLX(2048, 64, 8, 1, 16, 16, 8),  // two workgroups per CU: 58% vs 55% of HBM (2048 x 8192)
     OFX_COL(2048, 128, 8, 1, 16, 16, 8),
     OFX_COL(2048, 128, 4, 1, 16, 16, 8),
     OFX_COL(2048, 256, 4, 1, 8, 8, 8, 4),
     OFX_COL(4096, 256, 4, 0, 16, 16, 16),
     OFX_COL(4096, 256, 2, 1, 16, 16, 16),
+    OFX_COLX(1024, 32, 16, 1, 16, 8, 8),
     // complex128 (16-byte elements: half the columns per workgroup for the same LDS)
     OFX_COLZ(2, 1, 64, 1, 2),
     OFX_COLZ(4, 1, 64, 1, 4),

@@ -39,11 +39,14 @@ struct Radices {
 // V_: the element type, float2 (complex64) or double2 (complex128).
 // STREAM_: the kernel's HBM input and output use streaming (non-temporal) accesses; for working
 // sets well beyond the last-level cache (chosen by the plan).
+// SPLITX_ (column kernels): the LDS tile holds one scalar per element and each exchange between
+// passes moves the real parts, then the imaginary parts (split_fft below).
 template <int N_, int T_, int TPB_, int PADSH_, bool STAGED_, class RL, int IOV_ = 1, int COLS_ = 0, int COLPAD_ = 1,
-          class V_ = float2, bool STREAM_ = false>
+          class V_ = float2, bool STREAM_ = false, bool SPLITX_ = false>
 struct StockhamCfg {
     using V = V_;
     static constexpr bool STREAM = STREAM_;
+    static constexpr bool SPLITX = SPLITX_;
     static constexpr int N = N_;
     static constexpr int T = T_;
     static constexpr int TPB = TPB_;
@@ -96,6 +99,7 @@ struct StockhamCfg {
         }
     }
     static_assert(N == ns(NP), "radix product must equal N");
+    static_assert(!SPLITX_ || COLS_ > 0, "the split exchange is for column kernels");
     static_assert(IOV_ == 1 || sizeof(V_) == 8, "paired 16-byte I/O is for complex64 only");
 };
 
@@ -205,6 +209,104 @@ __device__ __forceinline__ void stockham_passes(typename C::V* lds_t, int tid, c
         stockham_pass<C, P, true, true>(tid, tw, lr, lw);
     }
     if constexpr (!LAST) stockham_passes<C, SRC_LDS, DST_LDS, P + 1>(lds_t, tid, tw, gin, gout);
+}
+
+// ---- split exchange -------------------------------------------------------------------------
+// The same passes with the exchange between pass P and P+1 done in two halves through a scalar
+// LDS tile: write the real parts of pass P's outputs, barrier, read the real parts of pass P+1's
+// inputs, barrier, and the same for the imaginary parts.  Four barriers per exchange instead of
+// two, and half the LDS: a 2048-point column group of 8 columns needs 72 KiB instead of 144 KiB,
+// so two workgroups share a CU and one loads from HBM while the other computes.
+template <class C, int P>
+struct PassDims {
+    static constexpr int R = C::radix(P);
+    static constexpr int Ns = C::ns(P);
+    static constexpr int NBF = C::N / R;
+    static constexpr int NB = (NBF + C::T - 1) / C::T;
+    static constexpr bool EXACT = (NBF % C::T) == 0;
+};
+
+// the registers of pass P: v[b][k] = input k of butterfly j = tid + b*T
+template <class C, int P>
+using PassRegs = typename C::V[PassDims<C, P>::NB][PassDims<C, P>::R];
+
+template <class C, int P, class GOut>
+__device__ __forceinline__ void split_passes(PassRegs<C, P>& v, int tid, scalar_t<typename C::V>* lds,
+                                             const typename C::V* __restrict__ tw, const GOut& gout) {
+    using V = typename C::V;
+    using D = PassDims<C, P>;
+#pragma unroll
+    for (int b = 0; b < D::NB; ++b) {
+        const int j = tid + b * C::T;
+        if (D::EXACT || j < D::NBF) {
+            if constexpr (P > 0) {
+                const V* t = tw + C::tw_off(P) + (j % D::Ns) * (D::R - 1);
+#pragma unroll
+                for (int k = 1; k < D::R; ++k) v[b][k] = cmul(v[b][k], t[k - 1]);
+            }
+            Dft<D::R, V>::run(v[b]);
+        }
+    }
+    if constexpr (P == C::NP - 1) {
+#pragma unroll
+        for (int b = 0; b < D::NB; ++b) {
+            const int j = tid + b * C::T;
+            if (D::EXACT || j < D::NBF) {
+                const int base = (j / D::Ns) * (D::Ns * D::R) + (j % D::Ns);
+#pragma unroll
+                for (int k = 0; k < D::R; ++k) gout(base + k * D::Ns, v[b][k]);
+            }
+        }
+    } else {
+        using E = PassDims<C, P + 1>;
+        PassRegs<C, P + 1> u;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+#pragma unroll
+            for (int b = 0; b < D::NB; ++b) {
+                const int j = tid + b * C::T;
+                if (D::EXACT || j < D::NBF) {
+                    const int base = (j / D::Ns) * (D::Ns * D::R) + (j % D::Ns);
+#pragma unroll
+                    for (int k = 0; k < D::R; ++k) lds[C::pad(base + k * D::Ns)] = half ? v[b][k].y : v[b][k].x;
+                }
+            }
+            C::sync();
+#pragma unroll
+            for (int b = 0; b < E::NB; ++b) {
+                const int j = tid + b * C::T;
+                if (E::EXACT || j < E::NBF) {
+#pragma unroll
+                    for (int k = 0; k < E::R; ++k) {
+                        const auto x = lds[C::pad(j + k * E::NBF)];
+                        if (half)
+                            u[b][k].y = x;
+                        else
+                            u[b][k].x = x;
+                    }
+                }
+            }
+            C::sync();
+        }
+        split_passes<C, P + 1>(u, tid, lds, tw, gout);
+    }
+}
+
+// All passes of one column with the split exchange; `lds` is this column's scalar tile.
+template <class C, class GIn, class GOut>
+__device__ __forceinline__ void split_fft(scalar_t<typename C::V>* lds, int tid, const typename C::V* __restrict__ tw,
+                                          const GIn& gin, const GOut& gout) {
+    using D = PassDims<C, 0>;
+    PassRegs<C, 0> v;
+#pragma unroll
+    for (int b = 0; b < D::NB; ++b) {
+        const int j = tid + b * C::T;
+        if (D::EXACT || j < D::NBF) {
+#pragma unroll
+            for (int k = 0; k < D::R; ++k) v[b][k] = gin(j + k * D::NBF);
+        }
+    }
+    split_passes<C, 0>(v, tid, lds, tw, gout);
 }
 
 }  // namespace ofx
