@@ -161,6 +161,11 @@ OFX_API int ofx_r2c_unpack(const void* Z, void* X, const void* W, long long M, l
 OFX_API int ofx_c2r_pack(const void* X, void* Z, const void* W, long long M, long long batch, int flags,
                          hipStream_t stream);
 
+/* Streaming device copy (16-byte non-temporal loads and stores), `bytes` a multiple of 16 and both
+ * pointers 16-byte aligned: the achievable-bandwidth yardstick the benchmarks measure beside the
+ * transforms.  No reference counterpart (a measurement aid). */
+OFX_API int ofx_stream_copy(const void* src, void* dst, long long bytes, hipStream_t stream);
+
 /* MFMA split-precision C2C (dense 32-point DFT stages on v_mfma_f32_32x32x16, FP32 accumulate).
  * Operand precision: */
 enum ofx_mfma_mode {

@@ -150,6 +150,8 @@ lib.ofx_kernel_count.argtypes = []
 lib.ofx_kernel_count.restype = _i
 lib.ofx_kernel_info_get.argtypes = [_i, ctypes.POINTER(KernelInfo)]
 lib.ofx_kernel_info_get.restype = _i
+lib.ofx_stream_copy.argtypes = [_vp, _vp, ctypes.c_longlong, _vp]
+lib.ofx_stream_copy.restype = _i
 
 
 def error_string(code):
@@ -170,3 +172,11 @@ def kernel_registry():
                         block=k.block_threads, radices=list(k.radices[:k.npass]), staged=bool(k.staged),
                         lds_bytes=k.lds_bytes, tw_len=k.tw_len, iov=k.iov, stream=bool(k.stream)))
     return out
+
+
+def stream_copy(src, dst, nbytes, stream=None):
+    """Streaming device copy of `nbytes` (a multiple of 16) between 16-byte aligned device
+    addresses on `stream` (a hipStream_t handle or None for the null stream)."""
+    code = lib.ofx_stream_copy(ctypes.c_void_p(src), ctypes.c_void_p(dst), nbytes, stream)
+    if code != 0:
+        raise RuntimeError("ofx_stream_copy failed: hip error %d" % code)

@@ -297,26 +297,29 @@ __global__ __launch_bounds__(C::BLOCK) void k_r2c(ofx_args a) {
     }
     if constexpr (!C::WAVE_SYNC || C::STAGED) __syncthreads();
     if (!ok) return;
-    // fused Hermitian unpack
+    // fused Hermitian unpack, bins k and M-k together: with E = (Z[k] + conj Z[M-k]) / 2,
+    // O = (Z[k] - conj Z[M-k]) / 2 and P = W^k O,
+    //     X[k] = E - iP,   X[M-k] = conj(E + iP)      (W^(M-k) = -conj(W^k), since W^M = -1)
+    // so each pair costs two LDS reads and one root of unity (k = 0 gives X[0] and X[M]).
     constexpr int M = C::N;
     const V* __restrict__ W = static_cast<const V*>(a.tw_aux);
     V* __restrict__ out = static_cast<V*>(a.out) + tt * a.odist;
-    const S wgt = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(2) : S(1);
-    for (int k = tid; k <= M; k += C::T) {
-        const int k1 = (k == M) ? 0 : k;
-        const int k2 = (k == 0) ? 0 : M - k;
-        V zk = lt[C::pad(k1)];
-        V zc = cconj(lt[C::pad(k2)]);
-        V e = cscale(cadd(zk, zc), S(0.5));
-        V o = cscale(csub(zk, zc), S(0.5));  // (Z[k] - conj Z[M-k]) / 2
-        V x = cadd(e, mul_negi(cmul(W[k], o)));
-        S s = ascale;
-        if (k == 0 || k == M) {
-            x.y = 0.f;
+    const S wgt = ((a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(2) : S(1)) * ascale;
+    for (int k = tid; k <= M / 2; k += C::T) {
+        const V zk = lt[C::pad(k)];
+        const V zc = cconj(lt[C::pad(k == 0 ? 0 : M - k)]);
+        const V e = cscale(cadd(zk, zc), S(0.5));
+        const V o = cscale(csub(zk, zc), S(0.5));
+        const V pk = cmul(W[k], o);
+        const V lo = cadd(e, mul_negi(pk));           // X[k]
+        const V hi = cconj(csub(e, mul_negi(pk)));     // X[M-k] = conj(E + iP)
+        if (k == 0) {
+            out[0] = mkv(lo.x * ascale, S(0));
+            out[M] = mkv(hi.x * ascale, S(0));
         } else {
-            s *= wgt;
+            out[k] = mkv(lo.x * wgt, lo.y * wgt);
+            if (2 * k != M) out[M - k] = mkv(hi.x * wgt, hi.y * wgt);
         }
-        out[k] = mkv(x.x * s, x.y * s);
     }
 }
 

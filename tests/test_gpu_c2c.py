@@ -143,3 +143,19 @@ def test_every_column_kernel_2d_and_vertical(cuda, monkeypatch, rows, variant, c
         err = oracle.rel_l2(vt.cpu().numpy(), oracle.c2c_vertical(v.astype(np.complex128)))
         assert err <= fp32_tol(rows)
         p.destroy()
+
+
+@pytest.mark.parametrize("nbytes", [16, 4096 + 48, (1 << 24) + 16 * 37])
+def test_stream_copy(cuda, nbytes):
+    """ofx_stream_copy (the benchmarks' bandwidth yardstick) copies exactly, including a last
+    partial workgroup, and leaves the bytes after the range untouched."""
+    import torch
+    from aclfft._lib import stream_copy
+    src = torch.randint(0, 255, (nbytes + 64,), dtype=torch.uint8, device="cuda")
+    dst = torch.zeros_like(src)
+    stream_copy(src.data_ptr(), dst.data_ptr(), nbytes)
+    torch.cuda.synchronize()
+    assert torch.equal(dst[:nbytes], src[:nbytes])
+    assert int(dst[nbytes:].abs().sum()) == 0
+    with pytest.raises(RuntimeError):
+        stream_copy(src.data_ptr(), dst.data_ptr(), 24)  # not a multiple of 16
