@@ -122,6 +122,33 @@ struct GLoadC2RPre {
         return make_float4(a.x, a.y, b.x, b.y);
     }
 };
+// The same pre-pack with the bins staged in LDS first (X[0..M-1] at lds[pad(i)], one coalesced
+// pass over HBM): the partner bin X[M-k] comes from LDS instead of a second, reversed global load.
+// X[M] (the partner of bin 0 only) is read from HBM.
+template <class C>
+struct LdsC2RPre {
+    using V = typename C::V;
+    using S = scalar_t<V>;
+    const V* lt;
+    const V* __restrict__ X;
+    const V* __restrict__ W;
+    S half_weight;
+    __device__ __forceinline__ V operator()(int k) const {
+        constexpr int M = C::N;
+        V a = lt[C::pad(k)];
+        V b = (k == 0) ? X[M] : lt[C::pad(M - k)];
+        if (k == 0) {
+            a.y = S(0);
+            b.y = S(0);
+        } else {
+            a = cscale(a, half_weight);
+            b = cscale(b, half_weight);
+        }
+        const V bc = cconj(b);
+        const V t = mul_posi(cmulc(csub(a, bc), W[k]));
+        return cconj(cadd(cadd(a, bc), t));
+    }
+};
 
 // odd n R2C: real input as complex with zero imaginary part; keep bins 0..n/2.
 template <class V>
@@ -352,7 +379,23 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2r(ofx_args a) {
             if (!ok) return;
         }
         GStoreC<V> gout{reinterpret_cast<V*>(static_cast<S*>(a.out) + tt * a.odist), S(-1), ascale, ok};
-        stockham_passes<C, false, false>(lt, tid, tw, gin, gout);
+        if constexpr (C::NP > 1) {
+            // bins into LDS (coalesced), then the pre-pack reads both partners there
+            const V* __restrict__ X = static_cast<const V*>(a.in) + tt * a.idist;
+            constexpr int IT = (C::N + C::T - 1) / C::T;
+            V xs[IT];  // every load in flight before the first LDS write
+#pragma unroll
+            for (int r = 0; r < IT; ++r)
+                if (C::N % C::T == 0 || tid + r * C::T < C::N) xs[r] = X[tid + r * C::T];
+#pragma unroll
+            for (int r = 0; r < IT; ++r)
+                if (C::N % C::T == 0 || tid + r * C::T < C::N) lt[C::pad(tid + r * C::T)] = xs[r];
+            C::sync();
+            LdsC2RPre<C> lin{lt, X, static_cast<const V*>(a.tw_aux), hw};
+            stockham_passes<C, true, false>(lt, tid, tw, lin, gout);
+        } else {
+            stockham_passes<C, false, false>(lt, tid, tw, gin, gout);
+        }
     }
 }
 
