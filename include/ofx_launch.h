@@ -222,6 +222,9 @@ OFX_API int ofx_batch_class_info_get(int cls, ofx_batch_class_info* info);
 /* Launches `blocks` persistent workgroups (dynamic schedule: after a one-thread kernel that
  * resets the counter). */
 OFX_API int ofx_launch_batch_persistent(const ofx_batch_args* args, int blocks, hipStream_t stream);
+/* Workgroups of the persistent kernel that fit on one CU at once (registers, LDS), for the current
+ * device; the grid for all of them resident is this times the CU count. */
+OFX_API int ofx_batch_blocks_per_cu(int schedule);
 
 /* R2C: multiply bins 1..n/2-1 by 2 (adjoint of C2R).  C2R: multiply input bins 1..n/2-1 by 1/2
  * (adjoint of R2C).  Used by the hand-written adjoint kernels. */

@@ -27,13 +27,15 @@ typedef StockhamCfg<1000, 64, 4, 4, false, Radices<10, 10, 10>> BC1000;
 typedef StockhamCfg<1024, 64, 4, 4, false, Radices<8, 8, 16>> BC1024;
 typedef StockhamCfg<2048, 128, 2, 4, false, Radices<16, 16, 8>> BC2048;
 typedef StockhamCfg<4096, 256, 1, 4, false, Radices<16, 16, 16>> BC4096;
-typedef StockhamCfg<6561, 256, 1, 30, false, Radices<27, 27, 9>> BC6561;
+// 6561: split exchange (26 KiB of LDS instead of 52), so the kernel's LDS budget is set by the
+// 35 KiB classes and four workgroups fit on a CU
+typedef StockhamCfg<6561, 256, 1, 30, false, Radices<27, 27, 9>, 1, 0, 1, float2, false, true> BC6561;
 typedef StockhamCfg<512, 32, 8, 4, false, Radices<8, 8, 8>> BC512;
 typedef StockhamCfg<256, 16, 16, 4, false, Radices<16, 16>> BC256;
 
 template <class C>
-constexpr int lds_of() {
-    return C::TPB * C::LDS_N;
+constexpr int lds_of() {  // in complex64 units
+    return C::SPLITX ? (C::TPB * C::LDS_N + 1) / 2 : C::TPB * C::LDS_N;
 }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr int kBatchLds =
@@ -89,7 +91,10 @@ __device__ __forceinline__ void run_tile(float2* lds, const ofx_batch_tile& tile
     const long long base = tile.offset + (long long)(ok ? tl : 0) * C::N;
     GLoadB gin{in + base, sg};
     GStoreB gout{out + base, sg, ok};
-    stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, tw, gin, gout);
+    if constexpr (C::SPLITX)
+        split_fft<C>(reinterpret_cast<float*>(lds) + tl * C::LDS_N, tid, tw, gin, gout);
+    else
+        stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, tw, gin, gout);
 }
 
 __global__ __launch_bounds__(kBatchBlock) void k_batch_reset(unsigned int* counter) { *counter = 0u; }
@@ -184,6 +189,25 @@ extern "C" int ofx_batch_class_info_get(int cls, ofx_batch_class_info* c) {
         default: return (int)hipErrorInvalidValue;
     }
     return 0;
+}
+
+extern "C" int ofx_batch_blocks_per_cu(int schedule) {
+    // cached per device and schedule (the query is a runtime call; plans ask on every execution)
+    static int cache[64][2] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = -1;
+    const int sc = schedule == 0 ? 0 : 1;
+    if (dev >= 0 && cache[dev][sc] > 0) return cache[dev][sc];
+    int n = 0;
+    const hipError_t e =
+        sc == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_persistent<0>, kBatchBlock, 0)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_persistent<1>, kBatchBlock, 0);
+    if (e != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        n = 1;
+    }
+    if (dev >= 0) cache[dev][sc] = n;
+    return n;
 }
 
 extern "C" int ofx_launch_batch_persistent(const ofx_batch_args* a, int blocks, hipStream_t stream) {

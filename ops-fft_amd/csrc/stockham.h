@@ -39,8 +39,8 @@ struct Radices {
 // V_: the element type, float2 (complex64) or double2 (complex128).
 // STREAM_: the kernel's HBM input and output use streaming (non-temporal) accesses; for working
 // sets well beyond the last-level cache (chosen by the plan).
-// SPLITX_ (column kernels): the LDS tile holds one scalar per element and each exchange between
-// passes moves the real parts, then the imaginary parts (split_fft below).
+// SPLITX_: the LDS tile holds one scalar per element and each exchange between passes moves the
+// real parts, then the imaginary parts (split_fft below).
 template <int N_, int T_, int TPB_, int PADSH_, bool STAGED_, class RL, int IOV_ = 1, int COLS_ = 0, int COLPAD_ = 1,
           class V_ = float2, bool STREAM_ = false, bool SPLITX_ = false>
 struct StockhamCfg {
@@ -99,7 +99,6 @@ struct StockhamCfg {
         }
     }
     static_assert(N == ns(NP), "radix product must equal N");
-    static_assert(!SPLITX_ || COLS_ > 0, "the split exchange is for column kernels");
     static_assert(IOV_ == 1 || sizeof(V_) == 8, "paired 16-byte I/O is for complex64 only");
 };
 
