@@ -2,16 +2,21 @@
 
     python3 ops-fft_amd/aclfft/profile_workloads.py [--reps 20]
 
-Workloads (each plan is called once to build its tables, then `reps` times):
+Workloads (each plan is called until 0.1 s of device time has passed, so that its tables exist and
+the clocks have settled, then `reps` more times: a trace's last `reps` dispatches of a kernel are
+the measured ones):
   R2C and C2R at n=4096 b=8192, n=3000 b=8192, n=8192 b=1024;
   2D C2C 2048x2048, batch 4;
   the 512-group uneven batch (lengths log-uniform over 60..6561, batches U{1..64}, seed 4);
-  1D C2C n=2^20, batch 64 (four-step).
+  1D C2C n=2^20, batch 64 (four-step);
+  3D C2C 256^3;
+  the streaming copy of 1 GiB (512 MiB read + 512 MiB written), the bandwidth yardstick.
 Every buffer is allocated at the size its plan reads or writes.
 """
 import argparse
 import os
 import sys
+import time
 
 sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
@@ -41,6 +46,11 @@ def main():
     def rep(fn):
         fn()
         torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < 0.1:
+            for _ in range(5):
+                fn()
+            torch.cuda.synchronize()
         for _ in range(args.reps):
             fn()
         torch.cuda.synchronize()
@@ -84,6 +94,21 @@ def main():
     rep(lambda: p.exec_c2c(x, y, aclfft.FORWARD))
     print("c2c n=2^20 b=64: %s" % p.describe(), flush=True)
     p.destroy()
+    del x, y
+    x = torch.randn(256, 256, 256, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    p = aclfft.Plan.plan3d(256, 256, 256, aclfft.C2C)
+    p.set_stream(s)
+    rep(lambda: p.exec_c2c(x, y, aclfft.FORWARD))
+    print("3d 256^3: %s" % p.describe(), flush=True)
+    p.destroy()
+    del x, y
+    from aclfft._lib import stream_copy
+    a = torch.empty(1 << 29, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    h = s.cuda_stream
+    rep(lambda: stream_copy(a.data_ptr(), b.data_ptr(), a.numel(), h))
+    print("stream copy 512 MiB", flush=True)
 
 
 if __name__ == "__main__":
