@@ -1173,7 +1173,8 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
         aclfftResult r = autotune(s);
         if (r != ACLFFT_SUCCESS) return r;
     } else if (!tuned_) {
-        apply_tune_cache(desc_, steps_, registry().data(), (int)registry().size());
+        apply_tune_cache(desc_, steps_, Registries{registry().data(), (int)registry().size(), col_registry().data(),
+                                                   (int)col_registry().size()});
     }
     tuned_ = true;
     return ACLFFT_SUCCESS;
@@ -1412,7 +1413,9 @@ aclfftResult Executor::autotune(hipStream_t s) {
     if (!any) return ACLFFT_SUCCESS;
     void* a = nullptr;
     void* b = nullptr;
-    const size_t bytes = std::max(in_bytes_, out_bytes_) + 256;
+    // candidates run on scratch copies of whichever buffers their step reads and writes (input,
+    // output or a four-step workspace with its padded pitch): size both for the largest
+    const size_t bytes = std::max(std::max(in_bytes_, out_bytes_), std::max(work_bytes_, work1_bytes_)) + 256;
     if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
         (void)hipGetLastError();
         if (a) (void)hipFree(a);
@@ -1456,7 +1459,8 @@ aclfftResult Executor::autotune(hipStream_t s) {
     (void)hipEventDestroy(e1);
     (void)hipFree(a);
     (void)hipFree(b);
-    store_tune_cache(desc_, steps_, registry().data());
+    store_tune_cache(desc_, steps_, Registries{registry().data(), (int)registry().size(), col_registry().data(),
+                                               (int)col_registry().size()});
     return ACLFFT_SUCCESS;
 }
 

@@ -34,16 +34,10 @@ std::map<std::string, std::string> load() {
     std::ifstream f(cache_path());
     std::string line;
     while (std::getline(f, line)) {
-        // key = first 7 fields, value = the rest
-        std::istringstream is(line);
-        std::string k[7], v;
-        bool ok = true;
-        for (auto& s : k)
-            if (!(is >> s)) ok = false;
-        if (!ok || !(is >> v)) continue;
-        std::string key = k[0];
-        for (int i = 1; i < 7; ++i) key += ' ' + k[i];
-        m[key] = v;
+        // "<key fields> <kernel signature>": the signature is the last field
+        const size_t sp = line.find_last_of(' ');
+        if (sp == std::string::npos || sp == 0 || sp + 1 >= line.size()) continue;
+        m[line.substr(0, sp)] = line.substr(sp + 1);
     }
     return m;
 }
@@ -58,19 +52,40 @@ std::string kernel_signature(const ofx_kernel_info& k) {
     return os.str();
 }
 
-void apply_tune_cache(const PlanDesc& d, std::vector<Step>& steps, const ofx_kernel_info* reg, int nreg) {
+namespace {
+
+// the registry entry a step's kernel index refers to, or null
+const ofx_kernel_info* entry_of(const Step& st, int id, const Registries& reg) {
+    if (st.type == Step::KERNEL) return (id >= 0 && id < reg.nrow) ? &reg.row[id] : nullptr;
+    if (st.type == Step::COLUMN) return (id >= 0 && id < reg.ncol) ? &reg.col[id] : nullptr;
+    return nullptr;
+}
+
+std::string step_signature(const Step& st, const ofx_kernel_info& k) {
+    std::string s = kernel_signature(k);
+    if (st.type == Step::COLUMN) s += ":c" + std::to_string(k.lds_bytes);
+    return s;
+}
+
+bool tunable(const Step& st) {
+    return (st.type == Step::KERNEL || st.type == Step::COLUMN) && st.candidates.size() >= 2;
+}
+
+}  // namespace
+
+void apply_tune_cache(const PlanDesc& d, std::vector<Step>& steps, const Registries& reg) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto m = load();
     if (m.empty()) return;
     for (size_t i = 0; i < steps.size(); ++i) {
         Step& st = steps[i];
-        if (st.type != Step::KERNEL || st.candidates.size() < 2) continue;  // row kernels only
+        if (!tunable(st)) continue;
         auto it = m.find(key_of(d, i));
         if (it == m.end()) continue;
         for (size_t c = 0; c < st.candidates.size(); ++c) {
-            const int id = st.candidates[c];
-            if (id >= 0 && id < nreg && kernel_signature(reg[id]) == it->second) {
-                st.kernel = id;
+            const ofx_kernel_info* k = entry_of(st, st.candidates[c], reg);
+            if (k != nullptr && step_signature(st, *k) == it->second) {
+                st.kernel = st.candidates[c];
                 st.tw_table = st.candidate_tables[c];
                 break;
             }
@@ -78,12 +93,13 @@ void apply_tune_cache(const PlanDesc& d, std::vector<Step>& steps, const ofx_ker
     }
 }
 
-void store_tune_cache(const PlanDesc& d, const std::vector<Step>& steps, const ofx_kernel_info* reg) {
+void store_tune_cache(const PlanDesc& d, const std::vector<Step>& steps, const Registries& reg) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto m = load();
     for (size_t i = 0; i < steps.size(); ++i) {
-        if (steps[i].type != Step::KERNEL || steps[i].candidates.size() < 2) continue;
-        m[key_of(d, i)] = kernel_signature(reg[steps[i].kernel]);
+        if (!tunable(steps[i])) continue;
+        const ofx_kernel_info* k = entry_of(steps[i], steps[i].kernel, reg);
+        if (k != nullptr) m[key_of(d, i)] = step_signature(steps[i], *k);
     }
     const std::string path = cache_path();
     // create the directory chain of the default location

@@ -1,10 +1,11 @@
 // tune_cache.h - persisted autotuning choices, keyed by (type, rank, lengths, batch, step).
 //
 // File: $ACLFFT_TUNE_CACHE, default $HOME/.cache/ops_fft_amd/tune.txt.  One line per step:
-//   <type> <rank> <n0> <n1> <batch> <step> <signature>
-// where the signature names the kernel configuration (kind, n, T, TPB, staged, radices), so a
-// cache stays valid across rebuilds that reorder the registry.  The cache is the only state this
-// library persists.
+//   <type> <rank> <n0> <n1> <n2> <batch> <vertical> <step> <signature>
+// where the signature names the kernel configuration (kind, n, T, TPB, staged, I/O width, radices;
+// for column kernels also the LDS footprint, which tells the split-exchange variants apart), so a
+// cache stays valid across rebuilds that reorder the registries.  Row-kernel and column-kernel steps
+// are recorded.  The cache is the only state this library persists.
 #ifndef OFX_TUNE_CACHE_H
 #define OFX_TUNE_CACHE_H
 
@@ -15,10 +16,17 @@
 namespace ofx {
 
 std::string kernel_signature(const ofx_kernel_info& k);
+// The registries a plan's steps index: row kernels (KERNEL steps) and column kernels (COLUMN steps).
+struct Registries {
+    const ofx_kernel_info* row;
+    int nrow;
+    const ofx_kernel_info* col;
+    int ncol;
+};
 // Replace each step's kernel by a cached choice when one exists among its candidates.
-void apply_tune_cache(const PlanDesc& d, std::vector<Step>& steps, const ofx_kernel_info* reg, int nreg);
+void apply_tune_cache(const PlanDesc& d, std::vector<Step>& steps, const Registries& reg);
 // Record the steps' current choices.
-void store_tune_cache(const PlanDesc& d, const std::vector<Step>& steps, const ofx_kernel_info* reg);
+void store_tune_cache(const PlanDesc& d, const std::vector<Step>& steps, const Registries& reg);
 
 }  // namespace ofx
 
