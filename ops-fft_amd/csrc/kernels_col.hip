@@ -29,8 +29,21 @@ struct ColLoad {
         return mkv(v.x, v.y * sg);
     }
 };
+// Factored register twiddles: when the last pass is exact (each of the T lanes of a column owns
+// butterflies j = lane + b*T, and N/R is a multiple of T), every output index a lane stores is
+// i = lane + T*m, m in [0, N/T), so W^(col*i) = W^(col*lane) * W^(col*T*m): one twiddle per lane
+// and a per-column table of N/T entries in LDS (filled by the column's lanes before the passes)
+// replace one twiddle evaluation per element.  Each factor carries sincospi's rounding, the product
+// about 2 ulp.
+template <class C>
+struct FtFactored {
+    static constexpr int E = C::N / C::T;
+    static constexpr bool OK = (C::N % C::T == 0) && ((C::N / C::radix(C::NP - 1)) % C::T == 0);
+};
+
 // FT: 0 = none; 1 = four-step twiddle W_ftlen^(col * i) on output i from the FP64 two-level table;
-// 2 = the same twiddle computed in registers (ftlen = 2^ftlog2).  In the element type's precision.
+// 2 = the same twiddle computed in registers (ftlen = 2^ftlog2); 3 = the factored register form
+// (FtFactored).  In the element type's precision.
 template <class C, int FT>
 struct ColStore {
     using V = typename C::V;
@@ -41,9 +54,13 @@ struct ColStore {
     const double2* __restrict__ ft;
     long long col, ftlen;
     int ftlog2;
+    const V* ftab;  // FT 3: this column's W^(col*T*m), m < N/T
+    V fbase;        // FT 3: W^(col*lane)
+    int lane;
     __device__ __forceinline__ void operator()(int i, V v) const {
         if constexpr (FT == 1) v = cmul(v, ft_twiddle_v((V*)nullptr, ft, col * (long long)i, ftlen));
         if constexpr (FT == 2) v = cmul(v, twiddle_pow2_v((V*)nullptr, (unsigned long long)col * (unsigned)i, ftlog2));
+        if constexpr (FT == 3) v = cmul(v, cmul(fbase, ftab[(i - lane) / C::T]));
         if (ok) p[(long long)i * stride] = mkv(v.x * scale, v.y * (sg * scale));
     }
 };
@@ -66,8 +83,18 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
     const long long cc = ok ? col : 0;
     const S scale = (sizeof(S) == 8 && a.scale64 != 0.0) ? (S)a.scale64 : (S)a.scale;
     ColLoad<C> gin{static_cast<const V*>(a.in) + img * a.idist + cc, a.istride, a.in_sign};
+    __shared__ V ftw[FT == 3 ? C::COLS * FtFactored<C>::E : 1];
+    V fbase = mkv(S(1), S(0));
+    if constexpr (FT == 3) {
+        constexpr int E = FtFactored<C>::E;
+        for (int m = tid; m < E; m += C::T)
+            ftw[c * E + m] = twiddle_pow2_v((V*)nullptr, (unsigned long long)cc * (unsigned)(C::T * m), ftlog2);
+        fbase = twiddle_pow2_v((V*)nullptr, (unsigned long long)cc * (unsigned)tid, ftlog2);
+        if constexpr (C::NP == 1) __syncthreads();  // otherwise the passes' barriers order the table
+    }
     ColStore<C, FT> gout{static_cast<V*>(a.out) + img * a.odist + cc, a.ostride, a.out_sign, scale, ok,
-                         static_cast<const double2*>(a.tw_aux), cc, ftlen, ftlog2};
+                         static_cast<const double2*>(a.tw_aux), cc, ftlen, ftlog2,
+                         ftw + (FT == 3 ? c * FtFactored<C>::E : 0), fbase, tid};
     if constexpr (C::SPLITX)
         split_fft<C>(lds + c, tid, static_cast<const V*>(a.tw), gin, gout);
     else
@@ -86,7 +113,8 @@ hipError_t launch_col(const ofx_args* a, hipStream_t s) {
         int lg = 0;
         while ((1LL << lg) < f) ++lg;
         if ((1LL << lg) == f && lg >= 3 && lg <= 27) {
-            hipLaunchKernelGGL((k_col<C, 2>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, f, lg);
+            constexpr int FT = FtFactored<C>::OK ? 3 : 2;
+            hipLaunchKernelGGL((k_col<C, FT>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, f, lg);
         } else {
             if (a->tw_aux == nullptr) return hipErrorInvalidValue;
             hipLaunchKernelGGL((k_col<C, 1>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, f, 0);
