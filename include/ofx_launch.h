@@ -160,6 +160,11 @@ OFX_API int ofx_r2c_unpack(const void* Z, void* X, const void* W, long long M, l
                            int flags, hipStream_t stream);
 OFX_API int ofx_c2r_pack(const void* X, void* Z, const void* W, long long M, long long batch, int flags,
                          hipStream_t stream);
+/* The same for complex128 data (D2Z / Z2D beyond the single-workgroup lengths; W in FP64). */
+OFX_API int ofx_r2c_unpack_z(const void* Z, void* X, const void* W, long long M, long long batch, double scale,
+                             int flags, hipStream_t stream);
+OFX_API int ofx_c2r_pack_z(const void* X, void* Z, const void* W, long long M, long long batch, int flags,
+                           hipStream_t stream);
 
 /* Streaming device copy (16-byte non-temporal loads and stores), `bytes` a multiple of 16 and both
  * pointers 16-byte aligned: the achievable-bandwidth yardstick the benchmarks measure beside the

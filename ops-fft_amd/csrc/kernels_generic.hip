@@ -279,47 +279,51 @@ __global__ void k_gen_ft_table(double2* t, long long len, long long count) {
 // and these kernels do the Hermitian unpack / pre-pack as separate passes.
 // ---------------------------------------------------------------------------------------------
 // Z: [batch][M] complex (workspace) -> X: [batch][M+1] complex
-__global__ void k_r2c_unpack(const float2* __restrict__ Z, float2* __restrict__ X, const float2* __restrict__ W,
-                             long long M, long long batch, float scale, int flags) {
+template <class V>
+__global__ void k_r2c_unpack(const V* __restrict__ Z, V* __restrict__ X, const V* __restrict__ W, long long M,
+                             long long batch, scalar_t<V> scale, int flags) {
+    using S = scalar_t<V>;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long per = M + 1;
     if (idx >= per * batch) return;
     const long long t = idx / per, k = idx - t * per;
-    const float2* z = Z + t * M;
+    const V* z = Z + t * M;
     const long long k1 = (k == M) ? 0 : k;
     const long long k2 = (k == 0) ? 0 : M - k;
-    const float2 zk = z[k1];
-    const float2 zc = cconj(z[k2]);
-    const float2 e = cscale(cadd(zk, zc), 0.5f);
-    const float2 o = cscale(csub(zk, zc), 0.5f);
-    float2 x = cadd(e, mul_negi(cmul(W[k], o)));
-    float sc = scale;
+    const V zk = z[k1];
+    const V zc = cconj(z[k2]);
+    const V e = cscale(cadd(zk, zc), S(0.5));
+    const V o = cscale(csub(zk, zc), S(0.5));
+    V x = cadd(e, mul_negi(cmul(W[k], o)));
+    S sc = scale;
     if (k == 0 || k == M) {
-        x.y = 0.f;
+        x.y = S(0);
     } else if (flags & OFX_FLAG_HERMITIAN_WEIGHT) {
-        sc *= 2.f;
+        sc *= S(2);
     }
-    X[t * per + k] = make_float2(x.x * sc, x.y * sc);
+    X[t * per + k] = mkv(x.x * sc, x.y * sc);
 }
 
 // X: [batch][M+1] complex -> Z': [batch][M] complex, conjugated (the complex FFT that follows runs
 // forward and conjugates its output)
-__global__ void k_c2r_pack(const float2* __restrict__ X, float2* __restrict__ Z, const float2* __restrict__ W,
-                           long long M, long long batch, int flags) {
+template <class V>
+__global__ void k_c2r_pack(const V* __restrict__ X, V* __restrict__ Z, const V* __restrict__ W, long long M,
+                           long long batch, int flags) {
+    using S = scalar_t<V>;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= M * batch) return;
     const long long t = idx / M, k = idx - t * M;
-    const float2* x = X + t * (M + 1);
-    const float hw = (flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 0.5f : 1.f;
-    float2 xa = x[k], xb = x[M - k];
+    const V* x = X + t * (M + 1);
+    const S hw = (flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(0.5) : S(1);
+    V xa = x[k], xb = x[M - k];
     if (k == 0) {
-        xa.y = 0.f;
-        xb.y = 0.f;
+        xa.y = S(0);
+        xb.y = S(0);
     } else {
         xa = cscale(xa, hw);
         xb = cscale(xb, hw);
     }
-    const float2 bc = cconj(xb);
+    const V bc = cconj(xb);
     Z[idx] = cconj(cadd(cadd(xa, bc), mul_posi(cmulc(csub(xa, bc), W[k]))));
 }
 
@@ -327,24 +331,48 @@ __global__ void k_c2r_pack(const float2* __restrict__ X, float2* __restrict__ Z,
 
 using namespace ofx;
 
-extern "C" int ofx_r2c_unpack(const void* Z, void* X, const void* W, long long M, long long batch, float scale,
-                              int flags, hipStream_t stream) {
+template <class V>
+static int launch_r2c_unpack(const void* Z, void* X, const void* W, long long M, long long batch, scalar_t<V> scale,
+                             int flags, hipStream_t stream) {
     const long long total = (M + 1) * batch;
     if (total <= 0) return 0;
-    hipLaunchKernelGGL(k_r2c_unpack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
-                       static_cast<const float2*>(Z), static_cast<float2*>(X), static_cast<const float2*>(W), M,
-                       batch, scale, flags);
+    const long long blocks = (total + 255) / 256;
+    if (blocks > 0x7fffffffLL) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_r2c_unpack<V>, dim3((unsigned)blocks), dim3(256), 0, stream, static_cast<const V*>(Z),
+                       static_cast<V*>(X), static_cast<const V*>(W), M, batch, scale, flags);
     return (int)hipGetLastError();
+}
+
+template <class V>
+static int launch_c2r_pack(const void* X, void* Z, const void* W, long long M, long long batch, int flags,
+                           hipStream_t stream) {
+    const long long total = M * batch;
+    if (total <= 0) return 0;
+    const long long blocks = (total + 255) / 256;
+    if (blocks > 0x7fffffffLL) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_c2r_pack<V>, dim3((unsigned)blocks), dim3(256), 0, stream, static_cast<const V*>(X),
+                       static_cast<V*>(Z), static_cast<const V*>(W), M, batch, flags);
+    return (int)hipGetLastError();
+}
+
+extern "C" int ofx_r2c_unpack(const void* Z, void* X, const void* W, long long M, long long batch, float scale,
+                              int flags, hipStream_t stream) {
+    return launch_r2c_unpack<float2>(Z, X, W, M, batch, scale, flags, stream);
 }
 
 extern "C" int ofx_c2r_pack(const void* X, void* Z, const void* W, long long M, long long batch, int flags,
                             hipStream_t stream) {
-    const long long total = M * batch;
-    if (total <= 0) return 0;
-    hipLaunchKernelGGL(k_c2r_pack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream,
-                       static_cast<const float2*>(X), static_cast<float2*>(Z), static_cast<const float2*>(W), M, batch,
-                       flags);
-    return (int)hipGetLastError();
+    return launch_c2r_pack<float2>(X, Z, W, M, batch, flags, stream);
+}
+
+extern "C" int ofx_r2c_unpack_z(const void* Z, void* X, const void* W, long long M, long long batch, double scale,
+                                int flags, hipStream_t stream) {
+    return launch_r2c_unpack<double2>(Z, X, W, M, batch, scale, flags, stream);
+}
+
+extern "C" int ofx_c2r_pack_z(const void* X, void* Z, const void* W, long long M, long long batch, int flags,
+                              hipStream_t stream) {
+    return launch_c2r_pack<double2>(X, Z, W, M, batch, flags, stream);
 }
 
 extern "C" int ofx_generic_radix_supported(int r) {

@@ -746,6 +746,26 @@ struct Builder {
         e->in_place_unsafe_ = true;
         const bool even = (n % 2) == 0;
         auto ids = kernels_for(even ? OFX_KIND_D2Z : OFX_KIND_D2Z_ODD, even ? n / 2 : n, true);
+        if (ids.empty() && even) {
+            // large even n: the complex128 four-step of the n/2 packed samples into the workspace,
+            // then the Hermitian unpack
+            const long long m = n / 2;
+            aclfftResult r = c2c_rows(m, batch, BUF_IN, BUF_WORK1, false, false, true);
+            if (r != ACLFFT_SUCCESS) return r;
+            Step u;
+            u.type = Step::R2C_UNPACK;
+            u.kind = OFX_KIND_D2Z;
+            u.src = BUF_WORK1;
+            u.dst = BUF_OUT;
+            u.m = m;
+            u.args.batch = batch;
+            u.args.scale = 1.f;
+            u.args.scale64 = 1.0;
+            u.aux_table = roots_table_z(n, n / 2 + 1);
+            e->steps_.push_back(u);
+            e->work1_bytes_ = std::max(e->work1_bytes_, (size_t)(m * batch) * 16);
+            return ACLFFT_SUCCESS;
+        }
         if (ids.empty()) return ACLFFT_NOT_IMPLEMENTED;
         Step s = kernel_step(even ? OFX_KIND_D2Z : OFX_KIND_D2Z_ODD, even ? n / 2 : n, ids);
         s.args.batch = batch;
@@ -759,6 +779,25 @@ struct Builder {
         e->in_place_unsafe_ = true;
         const bool even = (n % 2) == 0;
         auto ids = kernels_for(even ? OFX_KIND_Z2D : OFX_KIND_Z2D_ODD, even ? n / 2 : n, true);
+        if (ids.empty() && even) {
+            // large even n: Hermitian pre-pack into the workspace (conjugated), then the complex128
+            // four-step forward with a conjugated output
+            const long long m = n / 2;
+            Step pk;
+            pk.type = Step::C2R_PACK;
+            pk.kind = OFX_KIND_Z2D;
+            pk.src = BUF_IN;
+            pk.dst = BUF_WORK1;
+            pk.m = m;
+            pk.args.batch = batch;
+            pk.aux_table = roots_table_z(n, n / 2 + 1);
+            e->steps_.push_back(pk);
+            e->work1_bytes_ = std::max(e->work1_bytes_, (size_t)(m * batch) * 16);
+            aclfftResult r = c2c_rows(m, batch, BUF_WORK1, BUF_OUT, false, false, true);
+            if (r != ACLFFT_SUCCESS) return r;
+            e->steps_.back().args.out_sign = -1.f;
+            return ACLFFT_SUCCESS;
+        }
         if (ids.empty()) return ACLFFT_NOT_IMPLEMENTED;
         Step s = kernel_step(even ? OFX_KIND_Z2D : OFX_KIND_Z2D_ODD, even ? n / 2 : n, ids);
         s.args.batch = batch;
@@ -1234,11 +1273,17 @@ aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream
                 if (src != dst) err = hipMemcpyAsync(dst, src, st.copy_bytes, hipMemcpyDeviceToDevice, s);
                 break;
             case Step::R2C_UNPACK:
-                err = (hipError_t)ofx_r2c_unpack(src, dst, table(st.aux_table), st.m, st.args.batch, st.args.scale,
-                                                 st.args.flags, s);
+                err = st.kind == OFX_KIND_D2Z
+                          ? (hipError_t)ofx_r2c_unpack_z(src, dst, table(st.aux_table), st.m, st.args.batch,
+                                                         st.args.scale64 != 0.0 ? st.args.scale64 : st.args.scale,
+                                                         st.args.flags, s)
+                          : (hipError_t)ofx_r2c_unpack(src, dst, table(st.aux_table), st.m, st.args.batch,
+                                                       st.args.scale, st.args.flags, s);
                 break;
             case Step::C2R_PACK:
-                err = (hipError_t)ofx_c2r_pack(src, dst, table(st.aux_table), st.m, st.args.batch, st.args.flags, s);
+                err = st.kind == OFX_KIND_Z2D
+                          ? (hipError_t)ofx_c2r_pack_z(src, dst, table(st.aux_table), st.m, st.args.batch, st.args.flags, s)
+                          : (hipError_t)ofx_c2r_pack(src, dst, table(st.aux_table), st.m, st.args.batch, st.args.flags, s);
                 break;
             case Step::COLUMN: {
                 ofx_args a = st.args;

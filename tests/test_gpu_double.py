@@ -253,3 +253,38 @@ def test_every_double_column_kernel(cuda, monkeypatch, rows, variant, cols):
     print(p.describe(), "rel_l2 %.3e" % err)
     p.destroy()
     assert err <= fp64_tol(rows * cols), err
+
+
+@pytest.mark.parametrize("n,batch", [(16384, 3), (1 << 16, 2), (1 << 20, 1)])
+def test_d2z_z2d_fourstep_vs_oracle(cuda, n, batch):
+    """Even real lengths beyond the single-workgroup kernels: the complex128 four-step over the n/2
+    packed samples plus the Hermitian unpack (D2Z), the pre-pack plus the four-step (Z2D)."""
+    import torch
+    h = n // 2 + 1
+    x = np.random.default_rng(n + 1).standard_normal((batch, n))
+    p = aclfft.Plan.plan1d(n, aclfft.D2Z, batch)
+    X = torch.empty((batch, h), dtype=torch.complex128, device="cuda")
+    p.exec_d2z(torch.from_numpy(x).cuda(), X)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(X.cpu().numpy(), oracle.r2c(x))
+    print(p.describe(), "d2z rel_l2 %.3e" % err)
+    assert err <= fp64_tol(n), err
+    G = crandn((batch, h), n + 2)
+    G[:, 0] = G[:, 0].real
+    G[:, -1] = G[:, -1].real
+    q = aclfft.Plan.plan1d(n, aclfft.Z2D, batch)
+    y = torch.empty((batch, n), dtype=torch.float64, device="cuda")
+    q.exec_z2d(torch.from_numpy(G).cuda(), y)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(y.cpu().numpy(), oracle.c2r(G, n))
+    print(q.describe(), "z2d rel_l2 %.3e" % err)
+    assert err <= fp64_tol(n), err
+    # adjoint identity <D2Z x, g> = <x, D2Z^T g> on the four-step path
+    gx = torch.empty((batch, n), dtype=torch.float64, device="cuda")
+    p.exec_r2c_adjoint(torch.from_numpy(G).cuda(), gx)
+    torch.cuda.synchronize()
+    lhs = np.sum((X.cpu().numpy().conj() * G).real)
+    rhs = np.sum(x * gx.cpu().numpy())
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + np.linalg.norm(x) * np.linalg.norm(G))
+    p.destroy()
+    q.destroy()
