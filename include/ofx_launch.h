@@ -110,9 +110,11 @@ typedef struct ofx_args {
  * Transform t is addressed as (t_hi, t_lo) = (t / cnt_lo, t % cnt_lo):
  *   input  element i at in  + t_hi*idist + t_lo*idist_lo + i*istride
  *   output element k at out + t_hi*odist + t_lo*odist_lo + k*ostride
- * (units: complex64 or float as for ofx_args).  All data of a workgroup's transforms is held in
- * LDS (two buffers), so tpb * n <= OFX_GENERIC_MAX_ELEMS. */
+ * (units: complex64 or float as for ofx_args; complex128 or double when desc.z = 1).  All data of a
+ * workgroup's transforms is held in LDS (two buffers), so tpb * n <= OFX_GENERIC_MAX_ELEMS
+ * (OFX_GENERIC_MAX_ELEMS_Z for complex128). */
 #define OFX_GENERIC_MAX_ELEMS 9216
+#define OFX_GENERIC_MAX_ELEMS_Z 4608
 
 enum ofx_in_mode {
     OFX_IN_COMPLEX = 0,    /* complex64, conjugated when in_sign = -1                         */
@@ -136,7 +138,8 @@ typedef struct ofx_generic_desc {
     int in_mode, out_mode;
     int in_tmajor, out_tmajor;      /* 1: consecutive lanes walk consecutive transforms (strided
                                        layouts), 0: consecutive elements of one transform      */
-    int pad_;
+    int z;                          /* 1: complex128 data, FP64 tables (tpb * n <=
+                                       OFX_GENERIC_MAX_ELEMS_Z); 0: complex64                   */
     long long cnt_lo;               /* transforms per t_hi group (>= 1)                       */
     long long idist_lo, odist_lo;
     long long ft_len;               /* four-step: output k of transform t is multiplied by

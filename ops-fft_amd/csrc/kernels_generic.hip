@@ -35,26 +35,26 @@ struct GenParams {
 __device__ __forceinline__ int gpad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int gen_ln(int n) { return n + (n >> 4); }
 
-template <int R>
-__device__ __forceinline__ void gen_pass(const float2* __restrict__ src, float2* __restrict__ dst, int n, int ln,
-                                         int ns, int tpb, const float2* __restrict__ tw) {
+template <int R, class V>
+__device__ __forceinline__ void gen_pass(const V* __restrict__ src, V* __restrict__ dst, int n, int ln, int ns,
+                                         int tpb, const V* __restrict__ tw) {
     const int nbf = n / R;
     const int total = tpb * nbf;
     for (int f = threadIdx.x; f < total; f += blockDim.x) {
         const int t = f / nbf;
         const int j = f - t * nbf;
-        const float2* s = src + t * ln;
-        float2* d = dst + t * ln;
-        float2 v[R];
+        const V* s = src + t * ln;
+        V* d = dst + t * ln;
+        V v[R];
 #pragma unroll
         for (int k = 0; k < R; ++k) v[k] = s[gpad(j + k * nbf)];
         const int m = j % ns;
         if (ns > 1) {
-            const float2* w = tw + m * (R - 1);
+            const V* w = tw + m * (R - 1);
 #pragma unroll
             for (int k = 1; k < R; ++k) v[k] = cmul(v[k], w[k - 1]);
         }
-        Dft<R>::run(v);
+        Dft<R, V>::run(v);
         const int base = (j - m) * R + m;
 #pragma unroll
         for (int k = 0; k < R; ++k) d[gpad(base + k * ns)] = v[k];
@@ -64,54 +64,53 @@ __device__ __forceinline__ void gen_pass(const float2* __restrict__ src, float2*
 // BIG = false: radices up to 16 only (the common smooth lengths; a much smaller register
 // footprint, since the kernel's VGPR count is set by its largest codelet); BIG = true adds the
 // prime radices 17..47.
-template <bool BIG>
-__device__ __forceinline__ bool run_pass(int r, const float2* src, float2* dst, int n, int ns, int tpb,
-                                         const float2* tw) {
+template <bool BIG, class V>
+__device__ __forceinline__ bool run_pass(int r, const V* src, V* dst, int n, int ns, int tpb, const V* tw) {
     const int ln = gen_ln(n);
     if constexpr (!BIG) {
         switch (r) {
-            case 2: gen_pass<2>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 3: gen_pass<3>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 4: gen_pass<4>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 5: gen_pass<5>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 6: gen_pass<6>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 7: gen_pass<7>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 8: gen_pass<8>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 9: gen_pass<9>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 10: gen_pass<10>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 11: gen_pass<11>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 12: gen_pass<12>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 13: gen_pass<13>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 15: gen_pass<15>(src, dst, n, ln, ns, tpb, tw); return true;
-            case 16: gen_pass<16>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 2: gen_pass<2, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 3: gen_pass<3, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 4: gen_pass<4, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 5: gen_pass<5, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 6: gen_pass<6, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 7: gen_pass<7, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 8: gen_pass<8, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 9: gen_pass<9, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 10: gen_pass<10, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 11: gen_pass<11, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 12: gen_pass<12, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 13: gen_pass<13, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 15: gen_pass<15, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 16: gen_pass<16, V>(src, dst, n, ln, ns, tpb, tw); return true;
             default: return false;
         }
     }
     switch (r) {
-        case 2: gen_pass<2>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 3: gen_pass<3>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 4: gen_pass<4>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 5: gen_pass<5>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 6: gen_pass<6>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 7: gen_pass<7>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 8: gen_pass<8>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 9: gen_pass<9>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 10: gen_pass<10>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 11: gen_pass<11>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 12: gen_pass<12>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 13: gen_pass<13>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 15: gen_pass<15>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 16: gen_pass<16>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 17: gen_pass<17>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 19: gen_pass<19>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 23: gen_pass<23>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 25: gen_pass<25>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 29: gen_pass<29>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 31: gen_pass<31>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 37: gen_pass<37>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 41: gen_pass<41>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 43: gen_pass<43>(src, dst, n, ln, ns, tpb, tw); return true;
-        case 47: gen_pass<47>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 2: gen_pass<2, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 3: gen_pass<3, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 4: gen_pass<4, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 5: gen_pass<5, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 6: gen_pass<6, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 7: gen_pass<7, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 8: gen_pass<8, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 9: gen_pass<9, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 10: gen_pass<10, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 11: gen_pass<11, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 12: gen_pass<12, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 13: gen_pass<13, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 15: gen_pass<15, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 16: gen_pass<16, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 17: gen_pass<17, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 19: gen_pass<19, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 23: gen_pass<23, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 25: gen_pass<25, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 29: gen_pass<29, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 31: gen_pass<31, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 37: gen_pass<37, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 41: gen_pass<41, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 43: gen_pass<43, V>(src, dst, n, ln, ns, tpb, tw); return true;
+        case 47: gen_pass<47, V>(src, dst, n, ln, ns, tpb, tw); return true;
         default: return false;
     }
 }
@@ -120,19 +119,22 @@ __device__ __forceinline__ bool run_pass(int r, const float2* src, float2* dst, 
 // ---------------------------------------------------------------------------------------------
 // the kernel
 // ---------------------------------------------------------------------------------------------
-template <bool BIG>
+template <bool BIG, class V>
 __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
-    extern __shared__ float2 lds[];
+    using S = scalar_t<V>;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    V* lds = reinterpret_cast<V*>(lds_raw);
     const ofx_args& a = p.a;
     const ofx_generic_desc& d = p.d;
     const int n = d.n;
     const int tpb = d.tpb;
     const int ln = gen_ln(n);
-    float2* buf0 = lds;
-    float2* buf1 = lds + (size_t)tpb * ln;
+    V* buf0 = lds;
+    V* buf1 = lds + (size_t)tpb * ln;
     const long long t0 = (long long)blockIdx.x * tpb;
     const int nvalid = (int)min((long long)tpb, a.batch - t0);
-    const float2* tw = static_cast<const float2*>(a.tw);
+    const V* tw = static_cast<const V*>(a.tw);
+    const S scale = (sizeof(S) == 8 && a.scale64 != 0.0) ? S(a.scale64) : S(a.scale);
 
     // per-transform addressing, once per workgroup: (t_hi, t_lo) needs a 64-bit division, which
     // per element would cost more than the element's arithmetic
@@ -158,18 +160,18 @@ __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
             i = f - t * n;
         }
         const long long base = s_ibase[t];
-        float2 v;
+        V v;
         if (d.in_mode == OFX_IN_COMPLEX) {
-            v = static_cast<const float2*>(a.in)[base + i * a.istride];
-            v.y *= a.in_sign;
+            v = static_cast<const V*>(a.in)[base + i * a.istride];
+            v.y *= S(a.in_sign);
         } else if (d.in_mode == OFX_IN_REAL) {
-            v = make_float2(static_cast<const float*>(a.in)[base + i * a.istride], 0.f);
+            v = mkv(static_cast<const S*>(a.in)[base + i * a.istride], S(0));
         } else if (d.in_mode == OFX_IN_HERMITIAN) {
             // bins 0..h of an n-point Hermitian spectrum; conjugated for the backward transform
-            const float2* X = static_cast<const float2*>(a.in) + base;
-            const float hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 0.5f : 1.f;
+            const V* X = static_cast<const V*>(a.in) + base;
+            const S hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(0.5) : S(1);
             if (i == 0) {
-                v = make_float2(X[0].x, 0.f);
+                v = mkv(X[0].x, S(0));
             } else if (i <= h) {
                 v = cscale(X[(long long)i * a.istride], hw);
             } else {
@@ -177,21 +179,21 @@ __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
             }
             v = cconj(v);
         } else {  // OFX_IN_C2R_PACK: n = M, spectra of 2M points with M+1 bins
-            const float2* X = static_cast<const float2*>(a.in) + base;
-            const float2* W = static_cast<const float2*>(a.tw_aux);
-            const float hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 0.5f : 1.f;
-            float2 xa = X[(long long)i * a.istride];
-            float2 xb = X[(long long)(n - i) * a.istride];
+            const V* X = static_cast<const V*>(a.in) + base;
+            const V* W = static_cast<const V*>(a.tw_aux);
+            const S hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(0.5) : S(1);
+            V xa = X[(long long)i * a.istride];
+            V xb = X[(long long)(n - i) * a.istride];
             if (i == 0) {
-                xa.y = 0.f;
-                xb.y = 0.f;
+                xa.y = S(0);
+                xb.y = S(0);
             } else {
                 xa = cscale(xa, hw);
                 xb = cscale(xb, hw);
             }
-            const float2 bc = cconj(xb);
-            const float2 s = cadd(xa, bc);
-            const float2 df = csub(xa, bc);
+            const V bc = cconj(xb);
+            const V s = cadd(xa, bc);
+            const V df = csub(xa, bc);
             v = cconj(cadd(s, mul_posi(cmulc(df, W[i]))));
         }
         buf0[t * ln + gpad(i)] = v;
@@ -199,15 +201,15 @@ __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
     __syncthreads();
 
     // ---- passes -------------------------------------------------------------------------------
-    float2* src = buf0;
-    float2* dst = buf1;
+    V* src = buf0;
+    V* dst = buf1;
     int ns = 1;
     for (int p = 0; p < d.npass; ++p) {
         const int r = d.radices[p];
-        run_pass<BIG>(r, src, dst, n, ns, nvalid, tw + d.tw_off[p]);
+        run_pass<BIG, V>(r, src, dst, n, ns, nvalid, tw + d.tw_off[p]);
         __syncthreads();
         ns *= r;
-        float2* tmp = src;
+        V* tmp = src;
         src = dst;
         dst = tmp;
     }
@@ -225,39 +227,39 @@ __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
             k = f - t * nout;
         }
         const long long base = s_obase[t];
-        const float2* s = src + t * ln;
+        const V* s = src + t * ln;
         if (d.out_mode == OFX_OUT_COMPLEX) {
-            float2 v = s[gpad(k)];
-            v.y *= a.out_sign;
+            V v = s[gpad(k)];
+            v.y *= S(a.out_sign);
             if (d.ft_len > 0) {
                 long long e = (s_tlo[t] + d.ft_base) * (long long)k;
                 if (e >= d.ft_len) e %= d.ft_len;  // never for the four-step's own steps
-                v = cmul(v, ft_twiddle(ftab, e, d.ft_len));
+                v = cmul(v, ft_twiddle_v((V*)nullptr, ftab, e, d.ft_len));
             }
-            static_cast<float2*>(a.out)[base + k * a.ostride] = make_float2(v.x * a.scale, v.y * a.scale);
+            static_cast<V*>(a.out)[base + k * a.ostride] = mkv(v.x * scale, v.y * scale);
         } else if (d.out_mode == OFX_OUT_HALF) {
-            const float w = (k == 0) ? a.scale : a.scale * ((a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? 2.f : 1.f);
-            float2 v = cscale(s[gpad(k)], w);
-            if (k == 0) v.y = 0.f;
-            static_cast<float2*>(a.out)[base + k * a.ostride] = v;
+            const S w = (k == 0) ? scale : scale * ((a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(2) : S(1));
+            V v = cscale(s[gpad(k)], w);
+            if (k == 0) v.y = S(0);
+            static_cast<V*>(a.out)[base + k * a.ostride] = v;
         } else if (d.out_mode == OFX_OUT_REAL) {
-            static_cast<float*>(a.out)[base + k * a.ostride] = s[gpad(k)].x * a.scale;
+            static_cast<S*>(a.out)[base + k * a.ostride] = s[gpad(k)].x * scale;
         } else {  // OFX_OUT_R2C_UNPACK: k = 0..n (n = M)
-            const float2* W = reinterpret_cast<const float2*>(a.tw_aux);
+            const V* W = reinterpret_cast<const V*>(a.tw_aux);
             const int k1 = (k == n) ? 0 : k;
             const int k2 = (k == 0) ? 0 : n - k;
-            const float2 zk = s[gpad(k1)];
-            const float2 zc = cconj(s[gpad(k2)]);
-            const float2 e = cscale(cadd(zk, zc), 0.5f);
-            const float2 o = cscale(csub(zk, zc), 0.5f);
-            float2 x = cadd(e, mul_negi(cmul(W[k], o)));
-            float sc = a.scale;
+            const V zk = s[gpad(k1)];
+            const V zc = cconj(s[gpad(k2)]);
+            const V e = cscale(cadd(zk, zc), S(0.5));
+            const V o = cscale(csub(zk, zc), S(0.5));
+            V x = cadd(e, mul_negi(cmul(W[k], o)));
+            S sc = scale;
             if (k == 0 || k == n) {
-                x.y = 0.f;
+                x.y = S(0);
             } else if (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) {
-                sc *= 2.f;
+                sc *= S(2);
             }
-            static_cast<float2*>(a.out)[base + k * a.ostride] = make_float2(x.x * sc, x.y * sc);
+            static_cast<V*>(a.out)[base + k * a.ostride] = mkv(x.x * sc, x.y * sc);
         }
     }
 }
@@ -402,25 +404,38 @@ extern "C" int ofx_launch_generic(const ofx_args* args, const ofx_generic_desc* 
     if (d.ft_len > 0 && d.out_mode != OFX_OUT_COMPLEX) return (int)hipErrorInvalidValue;
     const long long blocks = (args->batch + d.tpb - 1) / d.tpb;
     if (blocks > 0x7fffffffLL) return (int)hipErrorInvalidConfiguration;
-    const size_t lds = 2 * sizeof(float2) * (size_t)d.tpb * gen_ln(d.n);
+    const bool z = d.z != 0;
+    if (z && (long long)d.tpb * d.n > OFX_GENERIC_MAX_ELEMS_Z) return (int)hipErrorInvalidValue;
+    const size_t esz = z ? sizeof(double2) : sizeof(float2);
+    const size_t lds = 2 * esz * (size_t)d.tpb * gen_ln(d.n);
     static bool attr_set = false;
     if (!attr_set) {
+        // both element types share one LDS ceiling: 9216 complex64 or 4608 complex128 points
         const int max_lds = (int)(2 * sizeof(float2) * gen_ln(OFX_GENERIC_MAX_ELEMS));
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_generic<false>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_generic<true>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-        if (e != hipSuccess) return (int)e;
+        const void* fns[4] = {reinterpret_cast<const void*>(&k_generic<false, float2>),
+                              reinterpret_cast<const void*>(&k_generic<true, float2>),
+                              reinterpret_cast<const void*>(&k_generic<false, double2>),
+                              reinterpret_cast<const void*>(&k_generic<true, double2>)};
+        for (const void* f : fns) {
+            const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+            if (e != hipSuccess) return (int)e;
+        }
         attr_set = true;
     }
     bool big = false;
     for (int q = 0; q < d.npass; ++q) big = big || d.radices[q] > 16;
     GenParams p{*args, d};
-    if (big)
-        hipLaunchKernelGGL(k_generic<true>, dim3((unsigned)blocks), dim3(kGenBlock), lds, stream, p);
-    else
-        hipLaunchKernelGGL(k_generic<false>, dim3((unsigned)blocks), dim3(kGenBlock), lds, stream, p);
+    if (z) {
+        if (big)
+            hipLaunchKernelGGL((k_generic<true, double2>), dim3((unsigned)blocks), dim3(kGenBlock), lds, stream, p);
+        else
+            hipLaunchKernelGGL((k_generic<false, double2>), dim3((unsigned)blocks), dim3(kGenBlock), lds, stream, p);
+    } else {
+        if (big)
+            hipLaunchKernelGGL((k_generic<true, float2>), dim3((unsigned)blocks), dim3(kGenBlock), lds, stream, p);
+        else
+            hipLaunchKernelGGL((k_generic<false, float2>), dim3((unsigned)blocks), dim3(kGenBlock), lds, stream, p);
+    }
     return (int)hipGetLastError();
 }
 

@@ -142,6 +142,15 @@ struct Builder {
         t.bytes = (size_t)len * 8;
         return e->add_table(t);
     }
+    int pass_table_z(int npass, const int* radices, long long len) {
+        if (len <= 0) return -1;
+        TableSpec t;
+        t.kind = TableSpec::PASS_Z;
+        t.npass = npass;
+        for (int p = 0; p < npass; ++p) t.radices[p] = radices[p];
+        t.bytes = (size_t)len * 16;
+        return e->add_table(t);
+    }
     int pass_table(const ofx_kernel_info& k) {
         if (!is_z_kind(k.kind)) return pass_table_for(k.npass, k.radices, k.tw_len);
         // complex128 kernel: the table in FP64 (16-byte entries)
@@ -212,8 +221,9 @@ struct Builder {
     }
 
     // A runtime-radix step of length n; false if n has an unsupported prime factor or is too long.
-    bool generic_step(long long n, Step* out) {
-        if (n > OFX_GENERIC_MAX_ELEMS) return false;
+    static long long generic_max(bool z) { return z ? OFX_GENERIC_MAX_ELEMS_Z : OFX_GENERIC_MAX_ELEMS; }
+    bool generic_step(long long n, Step* out, bool z = false) {
+        if (n > generic_max(z)) return false;
         std::vector<int> r = generic_radices(n);
         if ((r.empty() && n != 1) || (int)r.size() > OFX_MAX_PASSES) return false;
         Step s;
@@ -232,9 +242,11 @@ struct Builder {
             acc += ns2 * (r[p] - 1);
             ns2 *= r[p];
         }
-        s.tw_table = pass_table_for(g.npass, g.radices, acc);
+        g.z = z ? 1 : 0;
+        s.tw_table = z ? pass_table_z(g.npass, g.radices, acc) : pass_table_for(g.npass, g.radices, acc);
         // points per workgroup: enough bytes in flight per CU (ACLFFT_GENERIC_ELEMS overrides)
-        const long long per_block = std::max(64, std::min(OFX_GENERIC_MAX_ELEMS, env_int("ACLFFT_GENERIC_ELEMS", 2048)));
+        const long long per_block = std::max<long long>(
+            64, std::min<long long>(generic_max(z), env_int("ACLFFT_GENERIC_ELEMS", z ? 1024 : 2048)));
         g.tpb = (int)std::max<long long>(1, std::min<long long>(64, per_block / n));
         g.cnt_lo = 1;
         s.args.in_sign = 1.f;
@@ -273,7 +285,7 @@ struct Builder {
             return ACLFFT_SUCCESS;
         }
         Step s;
-        if (!z && generic_step(n, &s)) {
+        if (generic_step(n, &s, z)) {
             s.src = src;
             s.dst = dst;
             s.args.batch = batch;
@@ -291,12 +303,12 @@ struct Builder {
 
     // Split n = n1 * n2 for the four-step: both factors runnable by the runtime-radix kernel, as
     // close to sqrt(n) as possible with n1 (the strided column length) the smaller one.
-    bool split(long long n, long long* n1, long long* n2) {
+    bool split(long long n, long long* n1, long long* n2, bool z = false) {
         long long best = 0;
         for (long long a = 2; a * a <= n; ++a) {
             if (n % a) continue;
             const long long b = n / a;
-            if (a > OFX_GENERIC_MAX_ELEMS || b > OFX_GENERIC_MAX_ELEMS) continue;
+            if (a > generic_max(z) || b > generic_max(z)) continue;
             if (generic_radices(a).empty() || generic_radices(b).empty()) continue;
             best = a;
         }
@@ -432,14 +444,13 @@ struct Builder {
             e->work_bytes_ = std::max(e->work_bytes_, (size_t)(batch * T1 * P) * elem);
             return ACLFFT_SUCCESS;
         }
-        if (z) return ACLFFT_NOT_IMPLEMENTED;  // complex128: compiled factors only
         long long N1 = 0, N2 = 0;
-        if (!split(n, &N1, &N2)) return ACLFFT_NOT_IMPLEMENTED;
+        if (!split(n, &N1, &N2, z)) return ACLFFT_NOT_IMPLEMENTED;
         const int ft = fourstep_table(n);
         Step a, b;
-        if (!generic_step(N1, &a) || !generic_step(N2, &b)) return ACLFFT_NOT_IMPLEMENTED;
+        if (!generic_step(N1, &a, z) || !generic_step(N2, &b, z)) return ACLFFT_NOT_IMPLEMENTED;
         // column step
-        a.g.tpb = (int)std::max<long long>(1, std::min<long long>(16, OFX_GENERIC_MAX_ELEMS / N1));
+        a.g.tpb = (int)std::max<long long>(1, std::min<long long>(16, generic_max(z) / N1));
         a.g.cnt_lo = N2;
         a.g.idist_lo = 1;
         a.g.odist_lo = 1;
@@ -456,7 +467,7 @@ struct Builder {
         a.args.ostride = N2;
         a.conj_in = conj_in;
         // row step
-        b.g.tpb = (int)std::max<long long>(1, std::min<long long>(16, OFX_GENERIC_MAX_ELEMS / N2));
+        b.g.tpb = (int)std::max<long long>(1, std::min<long long>(16, generic_max(z) / N2));
         b.g.cnt_lo = N1;
         b.g.idist_lo = N2;
         b.g.odist_lo = 1;
@@ -472,7 +483,7 @@ struct Builder {
         b.conj_out = conj_out;
         e->steps_.push_back(a);
         e->steps_.push_back(b);
-        e->work_bytes_ = std::max(e->work_bytes_, (size_t)(n * batch * 8));
+        e->work_bytes_ = std::max(e->work_bytes_, (size_t)(n * batch) * elem);
         return ACLFFT_SUCCESS;
     }
 
@@ -525,8 +536,8 @@ struct Builder {
             return ACLFFT_SUCCESS;
         }
         Step s;
-        if (z || !generic_step(rows, &s)) return ACLFFT_NOT_IMPLEMENTED;
-        s.g.tpb = (int)std::max<long long>(1, std::min<long long>(16, OFX_GENERIC_MAX_ELEMS / rows));
+        if (!generic_step(rows, &s, z)) return ACLFFT_NOT_IMPLEMENTED;
+        s.g.tpb = (int)std::max<long long>(1, std::min<long long>(16, generic_max(z) / rows));
         s.g.cnt_lo = cols;
         s.g.idist_lo = 1;
         s.g.odist_lo = 1;
@@ -746,7 +757,7 @@ struct Builder {
         e->in_place_unsafe_ = true;
         const bool even = (n % 2) == 0;
         auto ids = kernels_for(even ? OFX_KIND_D2Z : OFX_KIND_D2Z_ODD, even ? n / 2 : n, true);
-        if (ids.empty() && even) {
+        if (ids.empty() && even && n / 2 > OFX_GENERIC_MAX_ELEMS_Z) {
             // large even n: the complex128 four-step of the n/2 packed samples into the workspace,
             // then the Hermitian unpack
             const long long m = n / 2;
@@ -766,7 +777,29 @@ struct Builder {
             e->work1_bytes_ = std::max(e->work1_bytes_, (size_t)(m * batch) * 16);
             return ACLFFT_SUCCESS;
         }
-        if (ids.empty()) return ACLFFT_NOT_IMPLEMENTED;
+        if (ids.empty()) {
+            // runtime-radix kernel: odd n as a full complex transform of the real samples (half
+            // spectrum stored), even n as the n/2-point transform of the packed samples with the
+            // Hermitian unpack fused into its store
+            Step g;
+            if (!generic_step(even ? n / 2 : n, &g, true)) return ACLFFT_NOT_IMPLEMENTED;
+            g.args.batch = batch;
+            g.args.istride = 1;
+            g.args.ostride = 1;
+            g.args.odist = n / 2 + 1;
+            if (even) {
+                g.g.in_mode = OFX_IN_COMPLEX;
+                g.g.out_mode = OFX_OUT_R2C_UNPACK;
+                g.args.idist = n / 2;
+                g.aux_table = roots_table_z(n, n / 2 + 1);
+            } else {
+                g.g.in_mode = OFX_IN_REAL;
+                g.g.out_mode = OFX_OUT_HALF;
+                g.args.idist = n;
+            }
+            e->steps_.push_back(g);
+            return ACLFFT_SUCCESS;
+        }
         Step s = kernel_step(even ? OFX_KIND_D2Z : OFX_KIND_D2Z_ODD, even ? n / 2 : n, ids);
         s.args.batch = batch;
         s.args.idist = n;
@@ -779,7 +812,7 @@ struct Builder {
         e->in_place_unsafe_ = true;
         const bool even = (n % 2) == 0;
         auto ids = kernels_for(even ? OFX_KIND_Z2D : OFX_KIND_Z2D_ODD, even ? n / 2 : n, true);
-        if (ids.empty() && even) {
+        if (ids.empty() && even && n / 2 > OFX_GENERIC_MAX_ELEMS_Z) {
             // large even n: Hermitian pre-pack into the workspace (conjugated), then the complex128
             // four-step forward with a conjugated output
             const long long m = n / 2;
@@ -798,7 +831,30 @@ struct Builder {
             e->steps_.back().args.out_sign = -1.f;
             return ACLFFT_SUCCESS;
         }
-        if (ids.empty()) return ACLFFT_NOT_IMPLEMENTED;
+        if (ids.empty()) {
+            // runtime-radix kernel: odd n from the Hermitian expansion of the half spectrum, even n
+            // as the pre-pack fused into the load of an n/2-point forward transform with a
+            // conjugated store
+            Step g;
+            if (!generic_step(even ? n / 2 : n, &g, true)) return ACLFFT_NOT_IMPLEMENTED;
+            g.args.batch = batch;
+            g.args.idist = n / 2 + 1;
+            g.args.istride = 1;
+            g.args.ostride = 1;
+            if (even) {
+                g.g.in_mode = OFX_IN_C2R_PACK;
+                g.g.out_mode = OFX_OUT_COMPLEX;
+                g.args.out_sign = -1.f;
+                g.args.odist = n / 2;
+                g.aux_table = roots_table_z(n, n / 2 + 1);
+            } else {
+                g.g.in_mode = OFX_IN_HERMITIAN;
+                g.g.out_mode = OFX_OUT_REAL;
+                g.args.odist = n;
+            }
+            e->steps_.push_back(g);
+            return ACLFFT_SUCCESS;
+        }
         Step s = kernel_step(even ? OFX_KIND_Z2D : OFX_KIND_Z2D_ODD, even ? n / 2 : n, ids);
         s.args.batch = batch;
         s.args.idist = n / 2 + 1;

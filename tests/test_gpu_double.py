@@ -288,3 +288,74 @@ def test_d2z_z2d_fourstep_vs_oracle(cuda, n, batch):
     assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + np.linalg.norm(x) * np.linalg.norm(G))
     p.destroy()
     q.destroy()
+
+
+@pytest.mark.parametrize("n,batch", [(1155, 5), (4608, 2), (2310, 3), (9000, 2), (12000, 2), (3 << 18, 1),
+                                     (1000000, 1)])
+def test_z2z_runtime_radix_vs_oracle(cuda, n, batch):
+    """Lengths without a compiled complex128 kernel: the runtime-radix kernel in FP64 (n <= 4608
+    in one step; beyond, the four-step over runtime-radix steps)."""
+    import torch
+    x = crandn((batch, n), n + 3)
+    p = aclfft.Plan.plan1d(n, aclfft.Z2Z, batch)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    for direction in (aclfft.FORWARD, aclfft.BACKWARD):
+        p.exec_z2z(xt, yt, direction)
+        torch.cuda.synchronize()
+        err = oracle.rel_l2(yt.cpu().numpy(), oracle.c2c(x, direction))
+        print(p.describe(), "dir", direction, "rel_l2 %.3e" % err)
+        assert err <= fp64_tol(n), err
+    p.destroy()
+
+
+@pytest.mark.parametrize("n", [1155, 2310, 4095, 10000, 30000])
+def test_d2z_z2d_runtime_radix_vs_oracle(cuda, n):
+    """D2Z / Z2D through the FP64 runtime-radix kernel: odd n as a full complex transform, even n
+    with the Hermitian unpack / pre-pack fused (four-step of the packed samples beyond 9216)."""
+    import torch
+    batch = 3
+    h = n // 2 + 1
+    x = np.random.default_rng(n).standard_normal((batch, n))
+    p = aclfft.Plan.plan1d(n, aclfft.D2Z, batch)
+    X = torch.empty((batch, h), dtype=torch.complex128, device="cuda")
+    p.exec_d2z(torch.from_numpy(x).cuda(), X)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(X.cpu().numpy(), oracle.r2c(x))
+    print(p.describe(), "d2z rel_l2 %.3e" % err)
+    assert err <= fp64_tol(n), err
+    G = crandn((batch, h), n + 5)
+    G[:, 0] = G[:, 0].real
+    if n % 2 == 0:
+        G[:, -1] = G[:, -1].real
+    q = aclfft.Plan.plan1d(n, aclfft.Z2D, batch)
+    y = torch.empty((batch, n), dtype=torch.float64, device="cuda")
+    q.exec_z2d(torch.from_numpy(G).cuda(), y)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(y.cpu().numpy(), oracle.c2r(G, n))
+    print(q.describe(), "z2d rel_l2 %.3e" % err)
+    p.destroy()
+    q.destroy()
+    assert err <= fp64_tol(n), err
+
+
+def test_z2z_2d_3d_runtime_radix(cuda):
+    import torch
+    x = crandn((2, 100, 60), 11)
+    p = aclfft.Plan.plan2d(2, 100, 60, aclfft.Z2Z)
+    y = torch.empty((2, 100, 60), dtype=torch.complex128, device="cuda")
+    p.exec_z2z(torch.from_numpy(x).cuda(), y, aclfft.BACKWARD)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(y.cpu().numpy(), oracle.c2c_2d(x, aclfft.BACKWARD))
+    print(p.describe(), "rel_l2 %.3e" % err)
+    p.destroy()
+    assert err <= fp64_tol(6000), err
+    v = crandn((30, 20, 10), 12)
+    p = aclfft.Plan.plan3d(30, 20, 10, aclfft.Z2Z)
+    y = torch.empty((30, 20, 10), dtype=torch.complex128, device="cuda")
+    p.exec_z2z(torch.from_numpy(v).cuda(), y, aclfft.FORWARD)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(y.cpu().numpy(), oracle.c2c_3d(v[None])[0])
+    print(p.describe(), "rel_l2 %.3e" % err)
+    p.destroy()
+    assert err <= fp64_tol(6000), err
