@@ -104,6 +104,9 @@ typedef struct ofx_args {
     long long odist_hi;
     const void* tiles;     /* C2C_TILES: ofx_batch_tile array of `batch` entries (device)    */
     double scale64;        /* complex128 kernels: output scale in double (0 = use `scale`)   */
+    long long ft_div;      /* COL kernels with ft_len: the twiddle uses column c / ft_div (the
+                              four-step of VERTICAL columns, where `cols` interleaves the
+                              transforms' columns); 0 or 1 = c                                 */
 } ofx_args;
 
 /* Runtime-radix ("generic") launches: any length whose radix sequence is listed, any layout.

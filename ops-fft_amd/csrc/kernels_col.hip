@@ -81,6 +81,7 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
     const long long col = (g - img * groups_per_image) * C::COLS + c;
     const bool ok = col < cols;
     const long long cc = ok ? col : 0;
+    const long long ccf = a.ft_div > 1 ? cc / a.ft_div : cc;  // the four-step twiddle's column index
     const S scale = (sizeof(S) == 8 && a.scale64 != 0.0) ? (S)a.scale64 : (S)a.scale;
     ColLoad<C> gin{static_cast<const V*>(a.in) + img * a.idist + cc, a.istride, a.in_sign};
     __shared__ V ftw[FT == 3 ? C::COLS * FtFactored<C>::E : 1];
@@ -88,12 +89,12 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
     if constexpr (FT == 3) {
         constexpr int E = FtFactored<C>::E;
         for (int m = tid; m < E; m += C::T)
-            ftw[c * E + m] = twiddle_pow2_v((V*)nullptr, (unsigned long long)cc * (unsigned)(C::T * m), ftlog2);
-        fbase = twiddle_pow2_v((V*)nullptr, (unsigned long long)cc * (unsigned)tid, ftlog2);
+            ftw[c * E + m] = twiddle_pow2_v((V*)nullptr, (unsigned long long)ccf * (unsigned)(C::T * m), ftlog2);
+        fbase = twiddle_pow2_v((V*)nullptr, (unsigned long long)ccf * (unsigned)tid, ftlog2);
         if constexpr (C::NP == 1) __syncthreads();  // otherwise the passes' barriers order the table
     }
     ColStore<C, FT> gout{static_cast<V*>(a.out) + img * a.odist + cc, a.ostride, a.out_sign, scale, ok,
-                         static_cast<const double2*>(a.tw_aux), cc, ftlen, ftlog2,
+                         static_cast<const double2*>(a.tw_aux), ccf, ftlen, ftlog2,
                          ftw + (FT == 3 ? c * FtFactored<C>::E : 0), fbase, tid};
     if constexpr (C::SPLITX)
         split_fft<C>(lds + c, tid, static_cast<const V*>(a.tw), gin, gout);

@@ -487,6 +487,45 @@ struct Builder {
         return ACLFFT_SUCCESS;
     }
 
+    // VERTICAL columns too long for one workgroup ([n][cols], n = N1 * N2, both factors with
+    // compiled column kernels): row r = n1*N2 + n2, output row k1 + N1*k2.
+    //   A: the N2*cols interleaved columns of length N1 (column n2*cols + c), twiddle
+    //      W_n^(k1*n2) (column index divided by cols), into the workspace at row k1*N2 + n2
+    //   B: per k1, the cols columns of length N2 of workspace rows k1*N2 .. k1*N2 + N2-1, stored
+    //      to row k1 + N1*k2 of the output.
+    aclfftResult vertical_fourstep(long long n, long long cols, bool z) {
+        if (env_int("ACLFFT_DISABLE_TEMPLATED", 0) != 0) return ACLFFT_NOT_IMPLEMENTED;
+        long long N1 = 0, N2 = 0;
+        double best = 1e30;
+        for (const ofx_kernel_info& a : col_registry()) {
+            if (a.kind != col_kind(z) || n % a.n) continue;
+            const long long m = n / a.n;
+            bool has = false;
+            for (const ofx_kernel_info& b : col_registry()) has = has || (b.kind == col_kind(z) && b.n == m);
+            if (!has) continue;
+            const double dist = std::fabs(std::log((double)a.n) - std::log((double)m));
+            if (dist < best) {
+                best = dist;
+                N1 = a.n;
+                N2 = m;
+            }
+        }
+        if (N1 == 0) return ACLFFT_NOT_IMPLEMENTED;
+        aclfftResult r = c2c_columns(N1, N2 * cols, 1, BUF_IN, BUF_WORK0, true, false, z);
+        if (r != ACLFFT_SUCCESS) return r;
+        Step& a = e->steps_.back();
+        a.args.ft_len = n;
+        a.args.ft_div = cols;
+        a.aux_table = fourstep_table(n);
+        r = c2c_columns(N2, cols, N1, BUF_WORK0, BUF_OUT, false, true, z);
+        if (r != ACLFFT_SUCCESS) return r;
+        Step& b = e->steps_.back();
+        b.args.odist = cols;
+        b.args.ostride = N1 * cols;
+        e->work_bytes_ = std::max(e->work_bytes_, (size_t)(n * cols) * (z ? 16 : 8));
+        return ACLFFT_SUCCESS;
+    }
+
     // Columns of [hi][rows][cols] row-major data: `cols` transforms of length `rows` per image.
     aclfftResult c2c_columns(long long rows, long long cols, long long images, int src, int dst, bool conj_in,
                              bool conj_out, bool z = false) {
@@ -1040,7 +1079,13 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
         const bool z = d.type == ACLFFT_Z2Z;
         const long long n = d.n0, cols = d.batch;
         e->in_bytes_ = e->out_bytes_ = (size_t)n * cols * (z ? 16 : 8);
-        r = b.c2c_columns(n, cols, 1, BUF_IN, BUF_OUT, true, true, z);
+        // one column kernel when one is compiled for n, else the column four-step over compiled
+        // factors, else the runtime-radix kernel
+        bool compiled = false;
+        for (const ofx_kernel_info& k : col_registry()) compiled = compiled || (k.kind == b.col_kind(z) && k.n == n);
+        r = ACLFFT_NOT_IMPLEMENTED;
+        if (!compiled) r = b.vertical_fourstep(n, cols, z);
+        if (r == ACLFFT_NOT_IMPLEMENTED) r = b.c2c_columns(n, cols, 1, BUF_IN, BUF_OUT, true, true, z);
     } else if ((d.rank == 2 || d.rank == 3) && (d.type == ACLFFT_C2C || d.type == ACLFFT_Z2Z)) {
         const bool z = d.type == ACLFFT_Z2Z;
         // axes slowest first; length-1 axes are dropped (a 64x1x32 volume is a 64x32 image)
