@@ -296,6 +296,33 @@ def extra_2d(torch, aclfft, oracle, np, steps, stream):
     return out
 
 
+def extra_loss_grad(torch, steps, stream):
+    """The differentiable configuration: loss = sum |W * FFT(x)|^2 (W ~ U[0,1), fixed) and its
+    gradient through torch.autograd over the library (forward transform, then the adjoint kernel),
+    for 2D C2C 2048x2048 batch 4 and 1D R2C n=8192 batch 1024.  One step = forward + backward."""
+    from aclfft import autograd as AG
+    out = {}
+    g = torch.Generator(device="cuda").manual_seed(7)
+    for name, shape, fn, wshape in (("fft2_2048x2048_b4", (4, 2048, 2048), AG.fft2, (4, 2048, 2048)),
+                                    ("rfft_8192_b1024", (1024, 8192), AG.rfft, (1024, 4097))):
+        if name.startswith("fft2"):
+            x = torch.randn(shape, dtype=torch.complex64, device="cuda", generator=g)
+        else:
+            x = torch.randn(shape, device="cuda", generator=g)
+        x.requires_grad_(True)
+        W = torch.rand(wshape, device="cuda", generator=g)
+
+        def step():
+            x.grad = None
+            loss = (W * fn(x)).abs().pow(2).sum()
+            loss.backward()
+        ms = time_steps(torch, step, steps, 3, stream)
+        fft_ms = time_steps(torch, lambda: fn(x.detach()), steps, 3, stream)
+        out[name] = {"fwd_bwd_ms": round(ms, 5), "transform_ms": round(fft_ms, 5),
+                     "note": "step = transform + loss (torch elementwise) + adjoint transform"}
+    return out
+
+
 def extra_batched(torch, aclfft, oracle, np, steps, stream):
     """512 C2C transforms, lengths log-uniform over the set, batches U{1..64}; one plan (persistent
     kernel, static schedule, replayed from a hipGraph) vs one plan (and launch) per group."""
@@ -441,7 +468,8 @@ def main():
                          ("c2c_n1024_b64_latency", lambda: extra_small_latency(torch, aclfft, oracle, np, stream)),
                          ("double_precision", lambda: extra_double(torch, aclfft, oracle, np, args.steps, stream)),
                          ("c2c_2d_2048x2048_b4", lambda: extra_2d(torch, aclfft, oracle, np, 10, stream)),
-                         ("uneven_batched_512", lambda: extra_batched(torch, aclfft, oracle, np, 20, stream))):
+                         ("uneven_batched_512", lambda: extra_batched(torch, aclfft, oracle, np, 20, stream)),
+                         ("loss_grad", lambda: extra_loss_grad(torch, 10, stream))):
             try:
                 log("bench: extra %s" % name)
                 extras[name] = fn()

@@ -12,6 +12,7 @@
 // Four-step twiddles W_N^e (N up to 2^27) are formed in FP64 from two FP64 tables,
 // W_N^e = W_N^(e_hi*B) * W_N^e_lo with B = 2048, then rounded once to FP32.
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include "ofx_launch.h"
 #include "codelets.h"
 #include "twiddle64.h"
@@ -408,8 +409,11 @@ extern "C" int ofx_launch_generic(const ofx_args* args, const ofx_generic_desc* 
     if (z && (long long)d.tpb * d.n > OFX_GENERIC_MAX_ELEMS_Z) return (int)hipErrorInvalidValue;
     const size_t esz = z ? sizeof(double2) : sizeof(float2);
     const size_t lds = 2 * esz * (size_t)d.tpb * gen_ln(d.n);
-    static bool attr_set = false;
-    if (!attr_set) {
+    // the dynamic-LDS ceiling is a per-device function attribute: set once on each device
+    static std::atomic<unsigned long long> attr_set{0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return (int)hipErrorInvalidDevice;
+    if ((attr_set.load() & (1ULL << dev)) == 0) {
         // both element types share one LDS ceiling: 9216 complex64 or 4608 complex128 points
         const int max_lds = (int)(2 * sizeof(float2) * gen_ln(OFX_GENERIC_MAX_ELEMS));
         const void* fns[4] = {reinterpret_cast<const void*>(&k_generic<false, float2>),
@@ -420,7 +424,7 @@ extern "C" int ofx_launch_generic(const ofx_args* args, const ofx_generic_desc* 
             const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
             if (e != hipSuccess) return (int)e;
         }
-        attr_set = true;
+        attr_set.fetch_or(1ULL << dev);
     }
     bool big = false;
     for (int q = 0; q < d.npass; ++q) big = big || d.radices[q] > 16;
