@@ -100,7 +100,9 @@ typedef struct ofx_args {
     long long ft_len;      /* COL kernels: > 0 multiplies output i of column c by the four-step
                               twiddle W_ft_len^(c*i) (tw_aux = ofx_gen_ft_table of ft_len)   */
     long long cols_lo;     /* ROWT: splits the row index r = t % cols into r / cols_lo (stride
-                              odist_hi in `out`) and r % cols_lo (stride 1); 0 = no split    */
+                              odist_hi in `out`) and r % cols_lo (stride 1); COL: splits the
+                              output image index m into m / cols_lo (stride odist_hi) and
+                              m % cols_lo (stride odist); 0 = no split                        */
     long long odist_hi;
     const void* tiles;     /* C2C_TILES: ofx_batch_tile array of `batch` entries (device)    */
     double scale64;        /* complex128 kernels: output scale in double (0 = use `scale`)   */

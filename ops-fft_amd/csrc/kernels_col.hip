@@ -93,7 +93,10 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
         fbase = twiddle_pow2_v((V*)nullptr, (unsigned long long)ccf * (unsigned)tid, ftlog2);
         if constexpr (C::NP == 1) __syncthreads();  // otherwise the passes' barriers order the table
     }
-    ColStore<C, FT> gout{static_cast<V*>(a.out) + img * a.odist + cc, a.ostride, a.out_sign, scale, ok,
+    // output image offset; cols_lo > 0 splits the image index into (img / cols_lo) * odist_hi +
+    // (img % cols_lo) * odist (the second step of a column four-step over several images)
+    const long long oimg = a.cols_lo > 0 ? (img / a.cols_lo) * a.odist_hi + (img % a.cols_lo) * a.odist : img * a.odist;
+    ColStore<C, FT> gout{static_cast<V*>(a.out) + oimg + cc, a.ostride, a.out_sign, scale, ok,
                          static_cast<const double2*>(a.tw_aux), ccf, ftlen, ftlog2,
                          ftw + (FT == 3 ? c * FtFactored<C>::E : 0), fbase, tid};
     if constexpr (C::SPLITX)

@@ -487,18 +487,20 @@ struct Builder {
         return ACLFFT_SUCCESS;
     }
 
-    // VERTICAL columns too long for one workgroup ([n][cols], n = N1 * N2, both factors with
+    // Columns too long for one workgroup ([images][n][cols], n = N1 * N2, both factors with
     // compiled column kernels): row r = n1*N2 + n2, output row k1 + N1*k2.
-    //   A: the N2*cols interleaved columns of length N1 (column n2*cols + c), twiddle
-    //      W_n^(k1*n2) (column index divided by cols), into the workspace at row k1*N2 + n2
-    //   B: per k1, the cols columns of length N2 of workspace rows k1*N2 .. k1*N2 + N2-1, stored
-    //      to row k1 + N1*k2 of the output.
-    aclfftResult vertical_fourstep(long long n, long long cols, bool z) {
+    //   A: per image, the N2*cols interleaved columns of length N1 (column n2*cols + c), twiddle
+    //      W_n^(k1*n2) (the column index divided by cols), into the workspace at row k1*N2 + n2
+    //   B: per (image, k1), the cols columns of length N2 of workspace rows k1*N2 .. k1*N2 + N2-1,
+    //      stored to row k1 + N1*k2 of that image of the output.
+    // Returns NOT_IMPLEMENTED, with no step added, when n has no such split.
+    aclfftResult column_fourstep(long long n, long long cols, long long images, int src, int dst, bool conj_in,
+                                 bool conj_out, bool z) {
         if (env_int("ACLFFT_DISABLE_TEMPLATED", 0) != 0) return ACLFFT_NOT_IMPLEMENTED;
         long long N1 = 0, N2 = 0;
         double best = 1e30;
         for (const ofx_kernel_info& a : col_registry()) {
-            if (a.kind != col_kind(z) || n % a.n) continue;
+            if (a.kind != col_kind(z) || n % a.n || a.n == n) continue;
             const long long m = n / a.n;
             bool has = false;
             for (const ofx_kernel_info& b : col_registry()) has = has || (b.kind == col_kind(z) && b.n == m);
@@ -511,18 +513,22 @@ struct Builder {
             }
         }
         if (N1 == 0) return ACLFFT_NOT_IMPLEMENTED;
-        aclfftResult r = c2c_columns(N1, N2 * cols, 1, BUF_IN, BUF_WORK0, true, false, z);
+        const int ws = (dst == BUF_WORK0 || src == BUF_WORK0) ? BUF_WORK1 : BUF_WORK0;
+        aclfftResult r = c2c_columns(N1, N2 * cols, images, src, ws, conj_in, false, z);
         if (r != ACLFFT_SUCCESS) return r;
         Step& a = e->steps_.back();
         a.args.ft_len = n;
         a.args.ft_div = cols;
         a.aux_table = fourstep_table(n);
-        r = c2c_columns(N2, cols, N1, BUF_WORK0, BUF_OUT, false, true, z);
+        r = c2c_columns(N2, cols, N1 * images, ws, dst, false, conj_out, z);
         if (r != ACLFFT_SUCCESS) return r;
         Step& b = e->steps_.back();
         b.args.odist = cols;
+        b.args.cols_lo = N1;
+        b.args.odist_hi = n * cols;
         b.args.ostride = N1 * cols;
-        e->work_bytes_ = std::max(e->work_bytes_, (size_t)(n * cols) * (z ? 16 : 8));
+        size_t& wb = (ws == BUF_WORK0) ? e->work_bytes_ : e->work1_bytes_;
+        wb = std::max(wb, (size_t)(n * cols * images) * (z ? 16 : 8));
         return ACLFFT_SUCCESS;
     }
 
@@ -574,6 +580,10 @@ struct Builder {
             e->steps_.push_back(c);
             return ACLFFT_SUCCESS;
         }
+        // no single column kernel: the column four-step over compiled factors, else the
+        // runtime-radix kernel
+        if (column_fourstep(rows, cols, images, src, dst, conj_in, conj_out, z) == ACLFFT_SUCCESS)
+            return ACLFFT_SUCCESS;
         Step s;
         if (!generic_step(rows, &s, z)) return ACLFFT_NOT_IMPLEMENTED;
         s.g.tpb = (int)std::max<long long>(1, std::min<long long>(16, generic_max(z) / rows));
@@ -1079,13 +1089,7 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
         const bool z = d.type == ACLFFT_Z2Z;
         const long long n = d.n0, cols = d.batch;
         e->in_bytes_ = e->out_bytes_ = (size_t)n * cols * (z ? 16 : 8);
-        // one column kernel when one is compiled for n, else the column four-step over compiled
-        // factors, else the runtime-radix kernel
-        bool compiled = false;
-        for (const ofx_kernel_info& k : col_registry()) compiled = compiled || (k.kind == b.col_kind(z) && k.n == n);
-        r = ACLFFT_NOT_IMPLEMENTED;
-        if (!compiled) r = b.vertical_fourstep(n, cols, z);
-        if (r == ACLFFT_NOT_IMPLEMENTED) r = b.c2c_columns(n, cols, 1, BUF_IN, BUF_OUT, true, true, z);
+        r = b.c2c_columns(n, cols, 1, BUF_IN, BUF_OUT, true, true, z);
     } else if ((d.rank == 2 || d.rank == 3) && (d.type == ACLFFT_C2C || d.type == ACLFFT_Z2Z)) {
         const bool z = d.type == ACLFFT_Z2Z;
         // axes slowest first; length-1 axes are dropped (a 64x1x32 volume is a 64x32 image)

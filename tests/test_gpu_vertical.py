@@ -1,4 +1,5 @@
-"""GPU parity of the VERTICAL layout over the reference's whole vertical domain (powers of two
+"""GPU parity of long column transforms: the VERTICAL layout over the reference's whole vertical
+domain, 2D plans with long columns and a 3D plan with a long slowest axis.  VERTICAL: (powers of two
 2^8 … 2^18, lib/fft_plan_validate.h:30-47): [n][cols] row-major, one transform per column, in
 both precisions, both directions, in place and out of place.  Columns up to 4096 points run in
 one column kernel; longer columns as the column four-step (two column steps over compiled
@@ -38,3 +39,40 @@ def test_vertical_domain_vs_oracle(cuda, typ, n, cols):
         print(p.describe(), "dir", direction, "rel_l2 %.3e" % err)
         assert err <= tol, err
     p.destroy()
+
+
+@pytest.mark.parametrize("typ", [aclfft.C2C, aclfft.Z2Z])
+@pytest.mark.parametrize("shape", [(2, 16384, 64), (2, 1 << 17, 8), (1, 8192, 1024), (3, 8192, 5)])
+def test_long_columns_2d_vs_oracle(cuda, typ, shape):
+    """2D plans whose column length has no single column kernel: the column four-step over
+    several images (the second step's output image index split into (image, k1))."""
+    import torch
+    z = typ == aclfft.Z2Z
+    b, nx, ny = shape
+    x = crandn(shape, nx + ny + b)
+    if not z:
+        x = x.astype(np.complex64)
+    p = aclfft.Plan.plan2d(b, nx, ny, typ)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    (p.exec_z2z if z else p.exec_c2c)(xt, yt, aclfft.FORWARD)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(yt.cpu().numpy(), oracle.c2c_2d(x.astype(np.complex128)))
+    print(p.describe(), "rel_l2 %.3e" % err)
+    p.destroy()
+    tol = 1e-14 * math.log2(nx * ny) if z else fp32_tol(nx * ny)
+    assert err <= tol, err
+
+
+def test_long_x_axis_3d_vs_oracle(cuda):
+    import torch
+    shape = (16384, 4, 8)
+    x = crandn(shape, 77).astype(np.complex64)
+    p = aclfft.Plan.plan3d(*shape, aclfft.C2C)
+    xt = torch.from_numpy(x).cuda()
+    p.exec_c2c(xt, xt, aclfft.BACKWARD)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(xt.cpu().numpy(), oracle.c2c_3d(x.astype(np.complex128)[None], aclfft.BACKWARD)[0])
+    print(p.describe(), "rel_l2 %.3e" % err)
+    p.destroy()
+    assert err <= fp32_tol(int(np.prod(shape))), err
