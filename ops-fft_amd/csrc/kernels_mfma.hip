@@ -116,7 +116,7 @@ __device__ __forceinline__ float wave_max(float v) {
 // Constant-operand table, generated at plan time in lane order:
 //   index ((((stage * 2 + tile) * 4 + kstep) * P + part) * 64 + lane) of 16-byte vectors
 // One transform: v holds its 16 lane values (loaded by the caller), the result goes to y.
-template <int MODE>
+template <int MODE, bool NT>
 __device__ __forceinline__ void mfma_transform(const float2 (&v)[16], float2* __restrict__ y,
                                                const typename SplitTraits<MODE>::vec* cst, const float2* tw,
                                                int lane, int r, int h, float in_sign, float out_sign, float scale) {
@@ -182,22 +182,28 @@ __device__ __forceinline__ void mfma_transform(const float2 (&v)[16], float2* __
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int k1 = 8 * (q >> 2) + 4 * h + (q & 3);
-        y[32 * k1 + r] = make_float2(out0[q] * inv, out1[q] * (inv * out_sign));
+        const float2 o = make_float2(out0[q] * inv, out1[q] * (inv * out_sign));
+        if constexpr (NT)
+            st_nt(y + 32 * k1 + r, o);
+        else
+            y[32 * k1 + r] = o;
     }
 }
 
+// NT: streaming (non-temporal) HBM access, for working sets well beyond the last-level cache
+template <bool NT>
 __device__ __forceinline__ void mfma_load(float2 (&v)[16], const float2* __restrict__ x, int r, int h) {
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
         const int n2 = 16 * (q >> 3) + 8 * h + (q & 7);
-        v[q] = x[r + 32 * n2];
+        v[q] = NT ? ld_nt(x + r + 32 * n2) : x[r + 32 * n2];
     }
 }
 
 // Constant-operand table, generated at plan time in lane order:
 //   index ((((stage * 2 + tile) * 4 + kstep) * P + part) * 64 + lane) of 16-byte vectors
 // PREFETCH: the next transform's loads are issued before the current transform's MFMA work.
-template <int MODE, bool PREFETCH>
+template <int MODE, bool PREFETCH, bool NT>
 __global__ __launch_bounds__(kMfmaBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_mfma_c2c_1024(ofx_args a) {
     typedef typename SplitTraits<MODE>::vec vec;
     constexpr int P = SplitTraits<MODE>::P;
@@ -221,13 +227,13 @@ __global__ __launch_bounds__(kMfmaBlock) __attribute__((amdgpu_waves_per_eu(2, 2
     long long t = (long long)blockIdx.x * (kMfmaBlock / 64) + wave;
     if constexpr (PREFETCH) {
         float2 cur[16];
-        if (t < a.batch) mfma_load(cur, in + t * a.idist, r, h);
+        if (t < a.batch) mfma_load<NT>(cur, in + t * a.idist, r, h);
         for (; t < a.batch; t += stride) {
             __asm__ volatile("" ::: "memory");  // keep the constant operands in LDS (see below)
             float2 nxt[16];
             const long long tn = t + stride;
-            if (tn < a.batch) mfma_load(nxt, in + tn * a.idist, r, h);
-            mfma_transform<MODE>(cur, out + t * a.odist, cst, tw, lane, r, h, a.in_sign, a.out_sign, a.scale);
+            if (tn < a.batch) mfma_load<NT>(nxt, in + tn * a.idist, r, h);
+            mfma_transform<MODE, NT>(cur, out + t * a.odist, cst, tw, lane, r, h, a.in_sign, a.out_sign, a.scale);
 #pragma unroll
             for (int q = 0; q < 16; ++q) cur[q] = nxt[q];
         }
@@ -237,8 +243,8 @@ __global__ __launch_bounds__(kMfmaBlock) __attribute__((amdgpu_waves_per_eu(2, 2
             // compiler keep all of them in VGPRs across the loop (that would leave one wave per SIMD).
             __asm__ volatile("" ::: "memory");
             float2 v[16];
-            mfma_load(v, in + t * a.idist, r, h);
-            mfma_transform<MODE>(v, out + t * a.odist, cst, tw, lane, r, h, a.in_sign, a.out_sign, a.scale);
+            mfma_load<NT>(v, in + t * a.idist, r, h);
+            mfma_transform<MODE, NT>(v, out + t * a.odist, cst, tw, lane, r, h, a.in_sign, a.out_sign, a.scale);
         }
     }
 }
@@ -337,12 +343,19 @@ extern "C" int ofx_launch_mfma_c2c(int mode, int n, const ofx_args* a, int block
     // 3-part mode whose constant table takes 48 KiB of LDS per workgroup
     const int dflt = (mode == OFX_MFMA_BF16X3) ? 512 : 2048;
     const unsigned g = (unsigned)std::min<long long>(need, nb > 0 ? nb : dflt);
+    // streaming HBM access when input and output together exceed the 256 MB last-level cache by
+    // half again (the threshold of the FP32 kernels' streaming variants)
+    const bool nt = a->batch * 1024LL * 16 >= (384LL << 20);
 #define OFX_MFMA_LAUNCH(M)                                                                                   \
     do {                                                                                                     \
-        if (prefetch)                                                                                        \
-            hipLaunchKernelGGL((k_mfma_c2c_1024<M, true>), dim3(g), dim3(kMfmaBlock), 0, stream, *a);        \
+        if (prefetch && nt)                                                                                  \
+            hipLaunchKernelGGL((k_mfma_c2c_1024<M, true, true>), dim3(g), dim3(kMfmaBlock), 0, stream, *a);  \
+        else if (prefetch)                                                                                   \
+            hipLaunchKernelGGL((k_mfma_c2c_1024<M, true, false>), dim3(g), dim3(kMfmaBlock), 0, stream, *a); \
+        else if (nt)                                                                                         \
+            hipLaunchKernelGGL((k_mfma_c2c_1024<M, false, true>), dim3(g), dim3(kMfmaBlock), 0, stream, *a); \
         else                                                                                                 \
-            hipLaunchKernelGGL((k_mfma_c2c_1024<M, false>), dim3(g), dim3(kMfmaBlock), 0, stream, *a);       \
+            hipLaunchKernelGGL((k_mfma_c2c_1024<M, false, false>), dim3(g), dim3(kMfmaBlock), 0, stream, *a); \
     } while (0)
     switch (mode) {
         case OFX_MFMA_BF16: OFX_MFMA_LAUNCH(OFX_MFMA_BF16); break;
