@@ -3,12 +3,12 @@
 // The plan (aclfftExtPlanManyUneven) holds groups g of b_g transforms of length n_g, stored one
 // group after the other.  At plan time the groups are cut into tiles (a tile = the transforms one
 // workgroup of the length's class processes together) and the tiles are sorted by decreasing cost
-// (longest-processing-time order).  The kernel is launched with a few workgroups per CU; every
-// workgroup repeatedly takes its next tile (from a device counter, or from a static round-robin
-// over the sorted list), runs the Stockham body of that tile's length class, and exits when the
-// queue is drained.  Each length
-// class is a compiled StockhamCfg with 256 threads per workgroup, so one launch serves them all;
-// the LDS buffer is sized for the largest class.  The counter is reset by a one-thread kernel
+// (longest-processing-time order).  The kernel is launched with as many workgroups as fit on the
+// device at once (ofx_batch_blocks_per_cu per CU); every workgroup repeatedly takes its next tile
+// (from a device counter, or from a static round-robin over the sorted list), runs the Stockham
+// body of that tile's length class, and exits when the queue is drained.  Each length class is a
+// compiled StockhamCfg with 256 threads per workgroup, so one launch serves them all; the LDS
+// buffer is sized for the largest class.  The counter is reset by a one-thread kernel
 // launched just before (no memset node, so the pair can be captured in a hipGraph and replayed).
 #include <hip/hip_runtime.h>
 

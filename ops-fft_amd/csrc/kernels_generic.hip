@@ -10,7 +10,9 @@
 // lane mapping chosen so that consecutive lanes touch consecutive addresses.
 //
 // Four-step twiddles W_N^e (N up to 2^27) are formed in FP64 from two FP64 tables,
-// W_N^e = W_N^(e_hi*B) * W_N^e_lo with B = 2048, then rounded once to FP32.
+// W_N^e = W_N^(e_hi*B) * W_N^e_lo with B = 2048, then rounded once to FP32 (kept in FP64 by the
+// complex128 instantiation, which serves double-precision plans with FP64 pass tables and half
+// the points per workgroup, since its LDS buffers hold 16-byte values).
 #include <hip/hip_runtime.h>
 #include <atomic>
 #include "ofx_launch.h"
