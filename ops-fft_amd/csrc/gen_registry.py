@@ -144,7 +144,9 @@ def z_lines():
 # the plan for working sets well beyond the last-level cache: (kind, N, T, TPB, radices, iov)
 STREAM = [
     ("C2C", 128, 8, 32, (8, 16), 1), ("C2C", 256, 16, 16, (16, 16), 1), ("C2C", 512, 32, 8, (8, 8, 8), 1),
-    ("C2C", 1024, 64, 2, (8, 16, 8), 2), ("C2C", 2048, 128, 2, (16, 16, 8), 1),
+    # 1024: radix 8 x 8 x 16 first (measured 171.8 vs 173.0 us for 8 x 16 x 8 at batch 65536)
+    ("C2C", 1024, 64, 2, (8, 8, 16), 2), ("C2C", 1024, 64, 2, (8, 16, 8), 2),
+    ("C2C", 2048, 128, 2, (16, 16, 8), 1),
     ("C2C", 4096, 256, 1, (16, 16, 16), 1), ("C2C", 8192, 512, 1, (16, 16, 16, 2), 1),
     ("Z2Z", 256, 32, 4, (4, 8, 8), 1), ("Z2Z", 512, 64, 2, (8, 8, 8), 1), ("Z2Z", 1024, 128, 2, (8, 8, 16), 1),
     ("Z2Z", 2048, 256, 1, (8, 8, 8, 4), 1), ("Z2Z", 4096, 512, 1, (8, 8, 8, 8), 1),
