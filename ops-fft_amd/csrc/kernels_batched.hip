@@ -23,7 +23,7 @@ constexpr int kBatchBlock = 256;
 typedef StockhamCfg<60, 4, 64, 4, false, Radices<4, 3, 5>> BC60;
 typedef StockhamCfg<128, 8, 32, 4, false, Radices<8, 16>> BC128;
 typedef StockhamCfg<243, 16, 16, 30, false, Radices<27, 9>> BC243;  // odd: no LDS pad needed
-typedef StockhamCfg<1000, 64, 4, 4, false, Radices<10, 10, 10>> BC1000;
+typedef StockhamCfg<1000, 128, 2, 4, false, Radices<10, 10, 10>> BC1000;
 typedef StockhamCfg<1024, 64, 4, 4, false, Radices<8, 8, 16>> BC1024;
 typedef StockhamCfg<2048, 128, 2, 4, false, Radices<16, 16, 8>> BC2048;
 typedef StockhamCfg<4096, 256, 1, 4, false, Radices<16, 16, 16>> BC4096;
