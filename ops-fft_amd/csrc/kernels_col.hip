@@ -199,6 +199,8 @@ static const ColEntry kCols[] = {
     OFX_COLZ(1024, 128, 8, 1, 8, 8, 16),
     OFX_COLZ(2048, 256, 4, 0, 8, 8, 8, 4),
     OFX_COLZ(4096, 512, 2, 0, 8, 8, 8, 8),
+    OFX_COLZX(1024, 64, 8, 1, 16, 16, 4),
+    OFX_COLZX(2048, 128, 4, 1, 16, 16, 8),
 };
 static const int kNumCols = (int)(sizeof(kCols) / sizeof(kCols[0]));
 
