@@ -382,8 +382,8 @@ def main():
                     help="also run one device copy of the 1 GiB input (counter calibration for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: the FFT path has no multi-GPU mode in this build "
-              "(batch sharding across GPUs needs no communication; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the FFT path has no multi-GPU mode (a batch would be "
+              "sharded across GPUs as contiguous ranges of transforms, with no communication)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -1,6 +1,6 @@
 """GPU: the MFMA split-precision C2C path (aclfftExtSetPrecision) against the FP64 oracle.
 
-Bars (BASELINE precision config): the FP16 hi/lo and BF16 hi/mid/lo paths must be no worse than
+Bars (the precision workload, C2C n=1024 with the 60 dB case): the FP16 hi/lo and BF16 hi/mid/lo paths must be no worse than
 twice the FP32 Stockham path's error against FP64 and within rel-L2 1e-6; single-BF16 is reported
 (about 3 significant digits) with a loose bound.
 """
