@@ -192,7 +192,14 @@ class Plan:
     # ---- execution ----------------------------------------------------------------------------
     def _sizes(self):
         """(real samples, complex bins) of the whole plan: the extents of the real and complex
-        sides of R2C/C2R, and of both sides of C2C (complex)."""
+        sides of R2C/C2R, and of both sides of C2C (complex).  Computed once per plan (the
+        executions check every buffer against it)."""
+        cached = self.__dict__.get("_sizes_cache")
+        if cached is None:
+            cached = self.__dict__["_sizes_cache"] = self._compute_sizes()
+        return cached
+
+    def _compute_sizes(self):
         if getattr(self, "groups", None):
             pts = sum(n * b for n, b in self.groups)
             return pts, pts
