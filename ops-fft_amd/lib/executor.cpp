@@ -401,9 +401,9 @@ struct Builder {
     }
 
     // Four-step: x[n1*N2 + n2] -> columns (length N1, twiddle W_N^(n2 k1)) -> workspace
-    // tmp[k1*N2 + n2] -> rows (length N2) -> X[k1 + N1*k2].  Two factors when both fit the compiled
-    // kernels' best configurations (<= 1024: 16 columns or rows per workgroup, 128-byte HBM runs),
-    // else three factors; ACLFFT_FOURSTEP_LEVELS=2|3 forces a form where it exists.
+    // tmp[k1*N2 + n2] -> rows (length N2) -> X[k1 + N1*k2].  Two factors when both are at most
+    // 2048 (whole columns and rows in one workgroup with 64-128-byte HBM runs), else three
+    // factors; ACLFFT_FOURSTEP_LEVELS=2|3 forces a form where it exists.
     aclfftResult c2c_fourstep(long long n, long long batch, int src, int dst, bool conj_in, bool conj_out,
                               bool z = false) {
         long long T1 = 0, T2 = 0, U1 = 0, U2 = 0, U3 = 0;
@@ -411,7 +411,9 @@ struct Builder {
         const bool two = templated_split(n, &T1, &T2, z);
         const bool three = templated_split3(n, &U1, &U2, &U3, z);
         const int force = env_int("ACLFFT_FOURSTEP_LEVELS", 0);
-        bool use3 = three && (!two || std::max(T1, T2) > 1024);
+        // two factors up to 2048 x 2048 (measured on MI355X, 2^22 x 16: 572 us as 2048 x 2048 vs
+        // 730 us as 64 x 256 x 256); three beyond (2^24 x 4: 664 us vs 931 us as 4096 x 4096)
+        bool use3 = three && (!two || std::max(T1, T2) > 2048);
         if (force == 2 && two) use3 = false;
         if (force == 3 && three) use3 = true;
         if (use3) return c2c_fourstep3(n, batch, U1, U2, U3, src, dst, conj_in, conj_out, z);
