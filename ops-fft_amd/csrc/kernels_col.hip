@@ -179,7 +179,11 @@ static const ColEntry kCols[] = {
     OFX_COL(1024, 64, 16, 1, 16, 16, 4),
     OFX_COL(1024, 64, 8, 1, 16, 16, 4),
     OFX_COL(1024, 128, 8, 1, 8, 8, 16),
-    OFX_COLX(2048, 64, 8, 1, 16, 16, 8),  // two workgroups per CU: 58% vs 55% of HBM (2048 x 8192)
+    // split exchange, two workgroups per CU (58% vs 55% of HBM for 2048 x 8192 VERTICAL); radix
+    // 8 x 16 x 16 first (2D 2048 x 2048 x 4: 93.1-94.2 us vs 95.6 us for 16 x 16 x 8)
+    OFX_COLX(2048, 64, 8, 1, 8, 16, 16),
+    OFX_COLX(2048, 64, 8, 1, 32, 8, 8),
+    OFX_COLX(2048, 64, 8, 1, 16, 16, 8),
     OFX_COL(2048, 128, 8, 1, 16, 16, 8),
     OFX_COL(2048, 128, 4, 1, 16, 16, 8),
     OFX_COL(2048, 256, 4, 1, 8, 8, 8, 4),
