@@ -104,3 +104,43 @@ def test_max_length_properties(cuda):
         print("bin %d err/sqrt(n) %.3e" % (k, err))
         assert err < 1e-4, (k, got, ref)
     p.destroy()
+
+
+def _rowt_variants():
+    seen, out = {}, []
+    for k in aclfft.kernel_registry():
+        if k["kind"] not in (6, 13):  # transposed-store row kernels, complex64 and complex128
+            continue
+        key = (k["kind"], k["n"])
+        out.append((k["kind"], k["n"], seen.get(key, 0)))
+        seen[key] = seen.get(key, 0) + 1
+    return out
+
+
+@pytest.mark.parametrize("kind,n2,variant", _rowt_variants())
+def test_every_rowt_kernel(cuda, monkeypatch, kind, n2, variant):
+    """Each compiled transposed-store row kernel as the row step of a two-factor four-step
+    (ACLFFT_FOURSTEP_LEVELS=2, row length n2), forced with ACLFFT_KERNEL_VARIANT."""
+    import torch
+    z = kind == 13
+    monkeypatch.setenv("ACLFFT_FOURSTEP_LEVELS", "2")
+    monkeypatch.setenv("ACLFFT_KERNEL_VARIANT", str(variant))
+    n = n2 * n2  # the balanced two-factor split: columns and rows of n2 points
+    batch = 2
+    x = crandn((batch, n), n2 + variant)
+    if not z:
+        x = x.astype(np.complex64)
+    p = aclfft.Plan.plan1d(n, aclfft.Z2Z if z else aclfft.C2C, batch)
+    desc = p.describe()
+    if ("kind13" if z else "kind6") + " n=%d " % n2 not in desc:
+        p.destroy()
+        pytest.skip("the four-step of %d does not end in a %d-point row step: %s" % (n, n2, desc))
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    (p.exec_z2z if z else p.exec_c2c)(xt, yt, aclfft.BACKWARD)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(yt.cpu().numpy(), oracle.c2c(x.astype(np.complex128), aclfft.BACKWARD))
+    print(desc, "rel_l2 %.3e" % err)
+    p.destroy()
+    tol = 1e-14 * np.log2(n) if z else fp32_tol(n)
+    assert err <= tol, err
