@@ -76,7 +76,6 @@ EXTRA = [
     ("R2C", 1500, 64, 2, (12, 5, 25)), ("R2C", 1500, 64, 1, (12, 5, 25)),
     ("C2R", 1500, 128, 2, (10, 10, 15)), ("C2R", 1500, 64, 2, (12, 5, 25)), ("C2R", 1500, 64, 1, (12, 5, 25)),
     ("R2C", 4096, 128, 1, (16, 16, 16)), ("C2R", 4096, 128, 1, (16, 16, 16)),
-    ("C2R", 4096, 512, 1, (8, 8, 8, 8)),
     ("R2C", 1500, 128, 2, (10, 15, 10)), ("R2C", 1500, 128, 2, (10, 10, 15)),
 ]
 
@@ -164,7 +163,7 @@ FIRST = [
     ("C2C", 1024, 64, 2, (8, 16, 8), 2),
     ("R2C", 2048, 256, 1, (8, 8, 8, 4), 1), ("C2R", 2048, 256, 1, (8, 8, 8, 4), 1),
     ("R2C", 1500, 256, 1, (10, 10, 15), 1), ("C2R", 1500, 128, 2, (12, 5, 25), 1),
-    ("R2C", 4096, 512, 1, (8, 8, 8, 8), 1),
+    ("R2C", 4096, 512, 1, (8, 8, 8, 8), 1), ("C2R", 4096, 512, 1, (8, 8, 8, 8), 1),
     # four-step row step of 1024 points: two radix-32 passes (2^20 x 64: 472 vs 482 us)
     ("ROWT", 1024, 32, 16, (32, 32), 1),
 ]
