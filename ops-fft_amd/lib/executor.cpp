@@ -694,8 +694,10 @@ struct Builder {
             s.tiles_table = e->add_table(tt);
             s.args.batch = (long long)tiles.size();
             s.conj_in = s.conj_out = true;
-            // longest-processing-time assignment of lengths to the side-stream lanes
-            const int lanes = std::max(0, std::min(8, env_int("ACLFFT_UNEVEN_LANES", 4)));
+            // longest-processing-time assignment of lengths to side-stream lanes, off by default:
+            // the fork / join of the side streams costs more than the overlap gains (2 to 8 lanes
+            // measured 1.2-3x slower than the plan's own stream for one to three lengths)
+            const int lanes = std::max(0, std::min(8, env_int("ACLFFT_UNEVEN_LANES", 0)));
             if (lanes > 1) {
                 lane_load.resize(lanes, 0.0);
                 const size_t k = std::min_element(lane_load.begin(), lane_load.end()) - lane_load.begin();
@@ -711,9 +713,12 @@ struct Builder {
 
     // Groups of (length, batch) stored one after the other.  ACLFFT_UNEVEN_MODE=1 selects the
     // persistent kernel (below), 2 the per-length launches (uneven_grouped).  By default the
-    // per-length launches are used when every length carries at least 2^22 points (each launch
-    // then amortises its ramp-up and tail, and runs with its own kernel's register and LDS
-    // budget), the persistent kernel otherwise (many lengths with little work each).
+    // per-length launches are used for a single length, or when every length carries at least
+    // 2^24 points (each launch then amortises its ramp-up and tail, and runs with its own
+    // kernel's register and LDS budget), the persistent kernel otherwise (several lengths with
+    // less work each, whose tiles share one launch).  Measured: one length of 5-134 M points,
+    // per-length 3-11% faster; 1024 + 4096 with 17 and 67 M points, 8% faster; 2048 + 4096 + 6561
+    // with 5-11 M points each, persistent 13% faster; 512 groups of 8 lengths, 29% faster.
     aclfftResult uneven(const std::vector<int>& len, const std::vector<int>& bat) {
         int mode = env_int("ACLFFT_UNEVEN_MODE", 0);
         if (mode != 1 && mode != 2) {
@@ -721,7 +726,7 @@ struct Builder {
             for (size_t g = 0; g < len.size(); ++g) pts[len[g]] += (long long)len[g] * bat[g];
             long long least = -1;
             for (auto& kv : pts) least = least < 0 ? kv.second : std::min(least, kv.second);
-            mode = least >= (1LL << 22) ? 2 : 1;
+            mode = (pts.size() == 1 || least >= (1LL << 24)) ? 2 : 1;
         }
         return mode == 2 ? uneven_grouped(len, bat) : uneven_persistent(len, bat);
     }

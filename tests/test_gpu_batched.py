@@ -33,16 +33,18 @@ def check_groups(out, x, groups, direction):
         off += n * b
 
 
-# (ACLFFT_UNEVEN_MODE, ACLFFT_PERSISTENT_SCHEDULE): persistent dynamic, persistent static, per-length
-FORMS = [(1, 0), (1, 1), (2, 1)]
+# (ACLFFT_UNEVEN_MODE, ACLFFT_PERSISTENT_SCHEDULE, ACLFFT_UNEVEN_LANES): persistent dynamic,
+# persistent static, per-length on the plan's stream, per-length over three side streams
+FORMS = [(1, 0, 0), (1, 1, 0), (2, 1, 0), (2, 1, 3)]
 
 
-@pytest.mark.parametrize("mode,sched", FORMS)
+@pytest.mark.parametrize("mode,sched,lanes", FORMS)
 @pytest.mark.parametrize("graph", [True, False])
-def test_uneven_plan_vs_oracle(cuda, monkeypatch, graph, mode, sched):
+def test_uneven_plan_vs_oracle(cuda, monkeypatch, graph, mode, sched, lanes):
     import torch
     monkeypatch.setenv("ACLFFT_UNEVEN_MODE", str(mode))
     monkeypatch.setenv("ACLFFT_PERSISTENT_SCHEDULE", str(sched))
+    monkeypatch.setenv("ACLFFT_UNEVEN_LANES", str(lanes))
     ln, bt = uneven_workload(40, 7)
     groups = list(zip(ln, bt))
     total = sum(n * b for n, b in groups)
@@ -63,12 +65,13 @@ def test_uneven_plan_vs_oracle(cuda, monkeypatch, graph, mode, sched):
     p.destroy()
 
 
-@pytest.mark.parametrize("mode,sched", FORMS)
-def test_uneven_plan_mixed_with_generic_lengths(cuda, monkeypatch, mode, sched):
+@pytest.mark.parametrize("mode,sched,lanes", FORMS)
+def test_uneven_plan_mixed_with_generic_lengths(cuda, monkeypatch, mode, sched, lanes):
     """Lengths without a compiled class run as extra launches inside the same plan/graph."""
     import torch
     monkeypatch.setenv("ACLFFT_UNEVEN_MODE", str(mode))
     monkeypatch.setenv("ACLFFT_PERSISTENT_SCHEDULE", str(sched))
+    monkeypatch.setenv("ACLFFT_UNEVEN_LANES", str(lanes))
     groups = [(60, 3), (420, 5), (1024, 7), (46200, 1), (7, 9), (6561, 2)]
     ln = [g[0] for g in groups]
     bt = [g[1] for g in groups]
