@@ -12,6 +12,8 @@
 // launched just before (no memset node, so the pair can be captured in a hipGraph and replayed).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "ofx_launch.h"
 #include "stockham.h"
 
@@ -192,12 +194,16 @@ extern "C" int ofx_batch_class_info_get(int cls, ofx_batch_class_info* c) {
 }
 
 extern "C" int ofx_batch_blocks_per_cu(int schedule) {
-    // cached per device and schedule (the query is a runtime call; plans ask on every execution)
-    static int cache[64][2] = {};
+    // cached per device and schedule (the query is a runtime call; plans ask on every execution,
+    // possibly from several host threads)
+    static std::atomic<int> cache[64][2] = {};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = -1;
     const int sc = schedule == 0 ? 0 : 1;
-    if (dev >= 0 && cache[dev][sc] > 0) return cache[dev][sc];
+    if (dev >= 0) {
+        const int c = cache[dev][sc].load(std::memory_order_relaxed);
+        if (c > 0) return c;
+    }
     int n = 0;
     const hipError_t e =
         sc == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_persistent<0>, kBatchBlock, 0)
@@ -206,7 +212,7 @@ extern "C" int ofx_batch_blocks_per_cu(int schedule) {
         (void)hipGetLastError();
         n = 1;
     }
-    if (dev >= 0) cache[dev][sc] = n;
+    if (dev >= 0) cache[dev][sc].store(n, std::memory_order_relaxed);
     return n;
 }
 

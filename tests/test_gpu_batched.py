@@ -1,5 +1,7 @@
-"""GPU: uneven-length batched plans (aclfftExtPlanManyUneven) - per-length tile launches (default)
-and the persistent kernel (ACLFFT_UNEVEN_MODE=1), each replayed from a hipGraph."""
+"""GPU: uneven-length batched plans (aclfftExtPlanManyUneven) - the persistent kernel
+(ACLFFT_UNEVEN_MODE=1; the default for several lengths) and per-length tile launches
+(ACLFFT_UNEVEN_MODE=2; the default for one length, or >= 2^24 points per length), each replayed
+from a hipGraph."""
 import numpy as np
 import pytest
 
@@ -140,3 +142,23 @@ def test_persistent_grid_sizes(cuda, monkeypatch, sched, blocks_per_cu):
     torch.cuda.synchronize()
     check_groups(yt.cpu().numpy(), x, groups, aclfft.FORWARD)
     p.destroy()
+
+
+def test_default_form(cuda, monkeypatch):
+    """Without ACLFFT_UNEVEN_MODE: one length, or every length with >= 2^24 points, runs as
+    per-length tile launches on the plan's stream; several lengths with less work share the
+    persistent kernel."""
+    monkeypatch.delenv("ACLFFT_UNEVEN_MODE", raising=False)
+    monkeypatch.delenv("ACLFFT_UNEVEN_LANES", raising=False)
+    cases = [(([4096, 4096], [3, 5]), "per-length"),
+             (([1024, 4096], [1 << 14, 1 << 12]), "per-length"),
+             (([1024, 4096], [1 << 14, (1 << 12) - 1]), "persistent"),
+             (([60, 1000, 6561], [2, 3, 4]), "persistent")]
+    for (ln, bt), want in cases:
+        p = aclfft.Plan.plan_many_uneven(ln, bt)
+        d = p.describe()
+        p.destroy()
+        if want == "persistent":
+            assert d.startswith("persistent"), (ln, bt, d)
+        else:
+            assert "persistent" not in d and "kind7" in d and "[lane" not in d, (ln, bt, d)
