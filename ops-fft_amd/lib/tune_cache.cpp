@@ -26,6 +26,21 @@ std::string key_of(const PlanDesc& d, size_t step) {
     std::ostringstream os;
     os << (int)d.type << ' ' << d.rank << ' ' << d.n0 << ' ' << d.n1 << ' ' << d.n2 << ' ' << d.batch << ' '
        << (d.vertical ? 1 : 0) << ' ' << step;
+    if (d.uneven) {
+        // uneven plans: the groups, as an FNV-1a hash of the (length, batch) list
+        unsigned long long h = 1469598103934665603ULL;
+        auto mix = [&h](long long v) {
+            for (int b = 0; b < 8; ++b) {
+                h ^= (unsigned long long)((v >> (8 * b)) & 0xff);
+                h *= 1099511628211ULL;
+            }
+        };
+        for (size_t g = 0; g < d.ulen.size(); ++g) {
+            mix(d.ulen[g]);
+            mix(d.ubatch[g]);
+        }
+        os << " u" << std::hex << h;
+    }
     return os.str();
 }
 
@@ -49,6 +64,7 @@ std::string kernel_signature(const ofx_kernel_info& k) {
     os << k.kind << ':' << k.n << ':' << k.threads_per_transform << ':' << k.transforms_per_block << ':' << k.staged
        << ':' << k.iov << ':';
     for (int p = 0; p < k.npass; ++p) os << (p ? "x" : "") << k.radices[p];
+    if (k.stream) os << ":s";
     return os.str();
 }
 
