@@ -126,9 +126,40 @@ def extra_copy(torch, aclfft, x, steps, stream, headline_ms):
     return out
 
 
+def sixty_db_signal(np, batch, seed):
+    """The precision workload's 60 dB case: per transform, 8 complex tones at random bins
+    (amplitudes 1 x4 and 1e-3 x4, random phases) plus complex Gaussian noise of deviation 1e-6."""
+    rng = np.random.default_rng(seed)
+    t = np.arange(N)
+    x = np.empty((batch, N), np.complex128)
+    amp = np.array([1, 1, 1, 1, 1e-3, 1e-3, 1e-3, 1e-3])
+    for b in range(batch):
+        f = rng.choice(N, 8, replace=False)
+        ph = rng.uniform(0, 2 * np.pi, 8)
+        x[b] = (amp * np.exp(1j * (2 * np.pi * np.outer(t, f) / N + ph))).sum(1)
+    x += 1e-6 * (rng.standard_normal((batch, N)) + 1j * rng.standard_normal((batch, N)))
+    return x.astype(np.complex64)
+
+
 def extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y, steps, stream, e32):
     out = {}
     bytes_ = 16.0 * N * BATCH
+    # the 60 dB case: 64 transforms, every path against the FP64 oracle
+    s60 = sixty_db_signal(np, 64, 3)
+    ref60 = oracle.c2c(s60.astype(np.complex128))
+    s60d = torch.from_numpy(s60).cuda()
+
+    def err60(mode):
+        q = aclfft.Plan.plan1d(N, aclfft.C2C, 64)
+        if mode:
+            q.set_precision(mode)
+        o = torch.empty_like(s60d)
+        q.exec_c2c(s60d, o, aclfft.FORWARD)
+        torch.cuda.synchronize()
+        q.destroy()
+        return oracle.rel_l2(o.cpu().numpy(), ref60)
+
+    out["sixty_db_fp32_rel_l2_vs_fp64"] = err60(0)
     for mode, name in ((aclfft.PRECISION_BF16, "bf16"), (aclfft.PRECISION_FP16X2, "fp16x2"),
                        (aclfft.PRECISION_BF16X3, "bf16x3")):
         p = aclfft.Plan.plan1d(N, aclfft.C2C, BATCH)
@@ -139,7 +170,7 @@ def extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y, steps, stream, e32):
         err = oracle.rel_l2(y[:256].cpu().numpy(), ref)
         out[name] = {"ms": round(ms, 5), "GFLOP/s": round(5.0 * N * math.log2(N) * BATCH / ms / 1e6, 1),
                      "hbm_frac": round(bytes_ / ms / 1e6 / HBM_PEAK_GBS, 4), "rel_l2_vs_fp64": err,
-                     "err_ratio_vs_fp32": round(err / e32, 3)}
+                     "err_ratio_vs_fp32": round(err / e32, 3), "sixty_db_rel_l2_vs_fp64": err60(mode)}
         p.destroy()
     return out
 
