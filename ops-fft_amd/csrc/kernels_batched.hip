@@ -120,6 +120,28 @@ __device__ __forceinline__ void run_class(float2* lds, const ofx_batch_tile& til
     }
 }
 
+// Tile descriptor `i` (uniform over the workgroup) through the scalar memory path: the list is
+// read-only while the kernel runs, and a scalar load is waited for with the LDS counter, so the
+// prefetched descriptor does not make the end-of-tile barrier wait for the tile's global stores.
+__device__ __forceinline__ ofx_batch_tile load_tile(const ofx_batch_tile* tiles, int i) {
+    typedef const __attribute__((address_space(4))) ofx_batch_tile* ConstTiles;
+    const ConstTiles p = ((ConstTiles)tiles) + __builtin_amdgcn_readfirstlane(i);
+    ofx_batch_tile t;
+    t.cls = p->cls;
+    t.count = p->count;
+    t.offset = p->offset;
+    return t;
+}
+
+// Workgroup barrier that orders LDS accesses only.  A tile ends with its global stores in flight;
+// __syncthreads() would wait for them (its fence covers global memory) before the next tile's
+// loads could issue, while the next tile only needs every wave done with the LDS tile.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Every loop below is uniform over the workgroup (tile indices come from blockIdx, the tile list
 // or LDS), so every wave leaves together once the index passes ntiles.
 template <int SCHED>
@@ -131,14 +153,14 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args
     if constexpr (SCHED == 1) {
         int t = b;
         ofx_batch_tile cur{};
-        if (t < a.ntiles) cur = a.tiles[t];
+        if (t < a.ntiles) cur = load_tile(a.tiles, t);
         for (int k = 1; t < a.ntiles; ++k) {
             const int tn = k * G + ((k & 1) ? G - 1 - b : b);
             ofx_batch_tile nxt{};
-            if (tn < a.ntiles) nxt = a.tiles[tn];  // in flight while this tile runs
+            if (tn < a.ntiles) nxt = load_tile(a.tiles, tn);  // in flight while this tile runs
             __asm__ volatile("" ::: "memory");      // no load of the tile bodies moves out of the loop
             run_class(lds, cur, a);
-            __syncthreads();  // LDS is reused by the next tile
+            lds_barrier();  // LDS is reused by the next tile
             cur = nxt;
             t = tn;
         }
@@ -148,12 +170,12 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args
         while (t < a.ntiles) {
             unsigned int nx = 0;
             if (threadIdx.x == 0) nx = atomicAdd(a.counter, 1u);  // the next index, used after this tile
-            const ofx_batch_tile cur = a.tiles[t];
+            const ofx_batch_tile cur = load_tile(a.tiles, t);
             __asm__ volatile("" ::: "memory");
             run_class(lds, cur, a);
             // two slots: a wave may still read this tile's slot while thread 0 writes the next one
             if (threadIdx.x == 0) s_next[parity] = (int)nx + G;
-            __syncthreads();  // publishes s_next; LDS is reused by the next tile
+            lds_barrier();  // publishes s_next; LDS is reused by the next tile
             t = s_next[parity];
             parity ^= 1;
         }

@@ -1419,7 +1419,7 @@ aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream
                 b.counter = static_cast<unsigned int*>(const_cast<void*>(table(st.counter_table)));
                 b.sign = dsign;
                 b.schedule = env_int("ACLFFT_PERSISTENT_SCHEDULE", 1) == 0 ? 0 : 1;
-                // every workgroup resident at once (4 per CU: 128 VGPRs, 34 KiB of LDS)
+                // every workgroup resident at once (4 per CU: 34 KiB of LDS each)
                 const int per_cu = std::max(1, env_int("ACLFFT_PERSISTENT_BLOCKS_PER_CU", ofx_batch_blocks_per_cu(b.schedule)));
                 err = (hipError_t)ofx_launch_batch_persistent(&b, num_cus_ * per_cu, s);
                 break;
