@@ -38,6 +38,9 @@ struct ColLoad {
 template <class C>
 struct FtFactored {
     static constexpr int E = C::N / C::T;
+    // per-column stride of the LDS table: one spare entry, so that the COLS columns' entries m
+    // fall in different banks (lanes of one column read the same m and share the access)
+    static constexpr int ES = E + 1;
     static constexpr bool OK = (C::N % C::T == 0) && ((C::N / C::radix(C::NP - 1)) % C::T == 0);
 };
 
@@ -84,12 +87,12 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
     const long long ccf = a.ft_div > 1 ? cc / a.ft_div : cc;  // the four-step twiddle's column index
     const S scale = (sizeof(S) == 8 && a.scale64 != 0.0) ? (S)a.scale64 : (S)a.scale;
     ColLoad<C> gin{static_cast<const V*>(a.in) + img * a.idist + cc, a.istride, a.in_sign};
-    __shared__ V ftw[FT == 3 ? C::COLS * FtFactored<C>::E : 1];
+    __shared__ V ftw[FT == 3 ? C::COLS * FtFactored<C>::ES : 1];
     V fbase = mkv(S(1), S(0));
     if constexpr (FT == 3) {
         constexpr int E = FtFactored<C>::E;
         for (int m = tid; m < E; m += C::T)
-            ftw[c * E + m] = twiddle_pow2_v((V*)nullptr, (unsigned long long)ccf * (unsigned)(C::T * m), ftlog2);
+            ftw[c * FtFactored<C>::ES + m] = twiddle_pow2_v((V*)nullptr, (unsigned long long)ccf * (unsigned)(C::T * m), ftlog2);
         fbase = twiddle_pow2_v((V*)nullptr, (unsigned long long)ccf * (unsigned)tid, ftlog2);
         if constexpr (C::NP == 1) __syncthreads();  // otherwise the passes' barriers order the table
     }
@@ -98,7 +101,7 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
     const long long oimg = a.cols_lo > 0 ? (img / a.cols_lo) * a.odist_hi + (img % a.cols_lo) * a.odist : img * a.odist;
     ColStore<C, FT> gout{static_cast<V*>(a.out) + oimg + cc, a.ostride, a.out_sign, scale, ok,
                          static_cast<const double2*>(a.tw_aux), ccf, ftlen, ftlog2,
-                         ftw + (FT == 3 ? c * FtFactored<C>::E : 0), fbase, tid};
+                         ftw + (FT == 3 ? c * FtFactored<C>::ES : 0), fbase, tid};
     if constexpr (C::SPLITX)
         split_fft<C>(lds + c, tid, static_cast<const V*>(a.tw), gin, gout);
     else
