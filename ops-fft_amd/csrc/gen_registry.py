@@ -170,8 +170,8 @@ FIRST = [
     ("C2C", 1024, 64, 2, (8, 16, 8), 2),
     ("R2C", 2048, 256, 1, (8, 8, 8, 4), 1), ("C2R", 2048, 256, 1, (8, 8, 8, 4), 1),
     # 1500 points (real 3000) unpadded: the radix-10/12 strides of these orders meet no LDS bank
-    # conflicts without the pad that power-of-two strides need (R2C 3000 x 8192: 35.2 vs 38.0 us,
-    # C2R 37.7 vs 40.0 us, against the padded (10, 10, 15) / (12, 5, 25) tiles)
+    # conflicts without the pad that power-of-two strides need (R2C 3000 x 8192: 35.8 vs 39.4-40.7
+    # us, C2R 38.9-39.2 vs 39.7-40.8 us on one box, against the padded (10, 10, 15) / (12, 5, 25))
     ("R2C", 1500, 256, 1, (10, 15, 10), 1, 30), ("C2R", 1500, 128, 2, (12, 5, 25), 1, 30),
     ("R2C", 1500, 256, 1, (10, 10, 15), 1), ("C2R", 1500, 128, 2, (12, 5, 25), 1),
     ("R2C", 4096, 512, 1, (8, 8, 8, 8), 1), ("C2R", 4096, 512, 1, (8, 8, 8, 8), 1),
