@@ -356,7 +356,8 @@ def extra_loss_grad(torch, steps, stream):
 
 def extra_batched(torch, aclfft, oracle, np, steps, stream):
     """512 C2C transforms, lengths log-uniform over the set, batches U{1..64}; one plan (persistent
-    kernel, static schedule, replayed from a hipGraph) vs one plan (and launch) per group."""
+    kernel, static schedule; launched directly, and replayed from the plan's hipGraph) vs one plan
+    (and launch) per group."""
     lengths = (60, 128, 243, 1000, 1024, 2048, 4096, 6561)
     rng = np.random.default_rng(4)
     logs = np.log(np.array(lengths, float))
@@ -369,7 +370,14 @@ def extra_batched(torch, aclfft, oracle, np, steps, stream):
     p = aclfft.Plan.plan_many_uneven(ln, bt)
     p.set_stream(stream)
     ms = time_steps(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
-    # correctness on a few groups
+    pg = aclfft.Plan.plan_many_uneven(ln, bt)
+    pg.set_stream(stream)
+    pg.set_graph_mode(True)
+    ms_graph = time_steps(torch, lambda: pg.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
+    pg.destroy()
+    # correctness on a few groups (output of the directly launched plan)
+    p.exec_c2c(x, y, aclfft.FORWARD)
+    torch.cuda.synchronize()
     off = 0
     worst = 0.0
     ycpu = y.cpu().numpy()
@@ -390,7 +398,7 @@ def extra_batched(torch, aclfft, oracle, np, steps, stream):
         q.set_stream(stream)
     ms_naive = time_steps(torch, per_group, max(3, steps // 5), 2, stream)
     flops = sum(5.0 * n * math.log2(n) * b for n, b in zip(ln, bt))
-    out = {"groups": 512, "points": int(total), "persistent_graph_ms": round(ms, 5),
+    out = {"groups": 512, "points": int(total), "persistent_ms": round(ms, 5), "persistent_hipgraph_ms": round(ms_graph, 5),
            "per_group_launches_ms": round(ms_naive, 5), "speedup": round(ms_naive / ms, 2),
            "GFLOP/s": round(flops / ms / 1e6, 1), "hbm_frac": round(16.0 * total / ms / 1e6 / HBM_PEAK_GBS, 4),
            "worst_rel_l2_vs_fp64": worst, "plan": p.describe()}
