@@ -77,9 +77,11 @@ const ofx_kernel_info* entry_of(const Step& st, int id, const Registries& reg) {
     return nullptr;
 }
 
+// the LDS footprint tells apart configurations that differ only in their LDS layout (the column
+// kernels' split exchange, the row kernels' pad shift)
 std::string step_signature(const Step& st, const ofx_kernel_info& k) {
     std::string s = kernel_signature(k);
-    if (st.type == Step::COLUMN) s += ":c" + std::to_string(k.lds_bytes);
+    s += (st.type == Step::COLUMN ? ":c" : ":l") + std::to_string(k.lds_bytes);
     return s;
 }
 

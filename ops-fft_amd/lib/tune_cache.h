@@ -3,10 +3,11 @@
 // File: $ACLFFT_TUNE_CACHE, default $HOME/.cache/ops_fft_amd/tune.txt.  One line per step:
 //   <type> <rank> <n0> <n1> <n2> <batch> <vertical> <step> [u<groups hash>] <signature>
 // (the hash of the (length, batch) list for uneven plans), where the signature names the kernel
-// configuration (kind, n, T, TPB, staged, I/O width, radices, ":s" for the streaming variants;
-// for column kernels also the LDS footprint, which tells the split-exchange variants apart), so a
-// cache stays valid across rebuilds that reorder the registries.  Row-kernel and column-kernel steps
-// are recorded.  The cache is the only state this library persists.
+// configuration (kind, n, T, TPB, staged, I/O width, radices, ":s" for the streaming variants,
+// and the LDS footprint, which tells apart the column kernels' split-exchange variants and the row
+// kernels' pad shifts), so a cache stays valid across rebuilds that reorder the registries.
+// Row-kernel and column-kernel steps are recorded.  The cache is the only state this library
+// persists.
 #ifndef OFX_TUNE_CACHE_H
 #define OFX_TUNE_CACHE_H
 

@@ -392,3 +392,25 @@ def test_double_precision_gates(monkeypatch):
     assert lib.aclfftPlan1d(ctypes.byref(h), 4096, aclfft.Z2D, 3, 0) == aclfft.SUCCESS
     assert lib.aclfftExecZ2D(h, p, p) != aclfft.INVALID_TYPE
     lib.aclfftDestroy(h)
+
+
+def test_tuning_signatures_tell_every_kernel_apart():
+    """The tuning cache records a step's choice by the kernel's configuration (kind, length, lanes,
+    transforms per block, staging, I/O width, radices, streaming, LDS footprint), not by registry
+    index: no two compiled kernels of either registry may share one, or a cached choice could
+    come back as another kernel."""
+    from aclfft._lib import KernelInfo
+
+    def sig(k):
+        return (k.kind, k.n, k.threads_per_transform, k.transforms_per_block, k.staged, k.iov,
+                tuple(k.radices[:k.npass]), k.stream, k.lds_bytes)
+
+    for count, get in ((lib.ofx_kernel_count, lib.ofx_kernel_info_get),
+                       (lib.ofx_col_kernel_count, lib.ofx_col_kernel_info_get)):
+        seen = {}
+        for i in range(count()):
+            k = KernelInfo()
+            assert get(i, ctypes.byref(k)) == 0
+            s = sig(k)
+            assert s not in seen, (i, seen.get(s), s)
+            seen[s] = i
