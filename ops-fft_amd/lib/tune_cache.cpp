@@ -8,6 +8,7 @@
 #include <mutex>
 #include <sstream>
 #include <sys/stat.h>
+#include <unistd.h>
 
 namespace ofx {
 
@@ -133,8 +134,18 @@ void store_tune_cache(const PlanDesc& d, const std::vector<Step>& steps, const R
         acc += (acc.empty() || acc.back() == '/' ? "" : "/") + part;
         mkdir(acc.c_str(), 0755);
     }
-    std::ofstream f(path, std::ios::trunc);
-    for (auto& kv : m) f << kv.first << ' ' << kv.second << '\n';
+    // write a private file and rename it over the cache, so that a reader (or another process
+    // writing at the same time) never sees a partly written file
+    const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+    {
+        std::ofstream f(tmp, std::ios::trunc);
+        for (auto& kv : m) f << kv.first << ' ' << kv.second << '\n';
+        if (!f.good()) {
+            std::remove(tmp.c_str());
+            return;
+        }
+    }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
 }
 
 }  // namespace ofx
