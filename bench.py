@@ -5,18 +5,29 @@ Workload (the headline metric's configuration): 1D C2C forward FFT, complex64, N
 batch = 65536, contiguous, re/im ~ N(0, 1/2) from seed 0 (synthetic), input resident in HBM.
 A step is one aclfftExecC2C over the whole batch (one kernel launch).
 
+Procedure of the headline: plan, then untimed calls for at least 0.25 s of device time (the clocks
+settle over tens of milliseconds of sustained load), then exactly K timed steps between HIP events
+recorded on the plan's stream, then the parity check against the FP64 oracle (after the timed
+region, so that host work never leaves the GPU idle just before it).
+
 Prints ONE JSON line:
-  value      = C2C GFLOP/s = 5 N log2 N batch / t
-  roofline   = algorithmic HBM bytes (16 B per point) / average kernel time vs 8 TB/s
-  cpu_baseline = the FP64 oracle (oracle/, a restatement of the reference's Stockham) timed on the
-                 host on a bounded sample of the same workload
-  cpu_golden_path = torch.fft.fft on host cores (the reference's golden path) on the full workload
+  value        = C2C GFLOP/s = 5 N log2 N batch / t
+  roofline     = algorithmic HBM bytes (16 B per point) / average kernel time vs 8 TB/s
+  cpu_baseline = the reference's CPU golden path (torch.fft.fft on the host cores, as its
+                 gen_data.py computes the goldens) on the same workload, warm median of >= 10
+  extras       = the other workloads (real transforms, MFMA split, uneven batch, 2D, adjoints,
+                 double precision, latency), each with its parity figure, and the CPU golden path
+                 timed on every workload
+
+--profile-workloads runs the secondary workloads back to back for a profiler instead (each plan is
+called until 0.1 s of device time has passed, then --reps more times) and prints no JSON line.
 """
 import argparse
 import ctypes
 import json
 import math
 import os
+import platform
 import sys
 import time
 
@@ -27,6 +38,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 N = 1024
 BATCH = 65536
+UNEVEN_LENGTHS = (60, 128, 243, 1000, 1024, 2048, 4096, 6561)
 
 
 def log(msg):
@@ -45,10 +57,71 @@ def read_traffic(name):
     return None
 
 
-def cpu_baseline(x_sample_np, budget_s=10.0):
-    """FP64 oracle on a bounded sample: transforms per second, scaled to GFLOP/s."""
-    from oracle import oracle
+def uneven_groups(count=512, seed=4):
+    """The uneven-batch workload: lengths log-uniform over the set, batches U{1..64}."""
     import numpy as np
+    rng = np.random.default_rng(seed)
+    logs = np.log(np.array(UNEVEN_LENGTHS, float))
+    u = rng.uniform(logs.min(), logs.max(), count)
+    ln = [int(UNEVEN_LENGTHS[int(np.argmin(np.abs(logs - v)))]) for v in u]
+    bt = [int(b) for b in rng.integers(1, 65, count)]
+    return ln, bt
+
+
+# ---------------------------------------------------------------------------------------------
+# CPU baselines
+# ---------------------------------------------------------------------------------------------
+def host_threads():
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return n
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_time(fn, reps=10, min_reps_seconds=0.0):
+    """Warm median of `reps` calls (one untimed call first), wall clock."""
+    fn()
+    ts = []
+    t_all = time.perf_counter()
+    while len(ts) < reps or time.perf_counter() - t_all < min_reps_seconds:
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    return ts[len(ts) // 2], len(ts)
+
+
+def cpu_golden_headline(torch, x_cpu):
+    """The reference's CPU golden path, torch.fft.fft (src/fft1_d/tests/fft1_d_data/gen_data.py:71-86),
+    on the whole headline workload with every host core."""
+    threads = host_threads()
+    torch.set_num_threads(threads)
+    t, reps = cpu_time(lambda: torch.fft.fft(x_cpu, dim=1), reps=10)
+    n, b = x_cpu.shape[1], x_cpu.shape[0]
+    return dict(value=round(5.0 * n * math.log2(n) * b / t / 1e9, 3), unit="GFLOP/s", cores=threads,
+                kind="reference",
+                sample="the reference's golden path torch.fft.fft (gen_data.py) on the full workload "
+                       "(65536 x 1024 complex64), warm median of %d reps, %.4f s per call" % (reps, t),
+                cpu_model=cpu_model(), torch=torch.__version__, mkl=bool(torch.backends.mkl.is_available()),
+                ms=round(t * 1e3, 3))
+
+
+def cpu_oracle_port(x_sample_np, budget_s=5.0):
+    """The FP64 oracle (a single-thread C restatement of the reference's Stockham) on a bounded
+    sample: transforms per second, scaled to GFLOP/s."""
+    from oracle import oracle
     n = x_sample_np.shape[-1]
     done = 0
     t0 = time.perf_counter()
@@ -63,20 +136,47 @@ def cpu_baseline(x_sample_np, budget_s=10.0):
                 sample="%d transforms of N=%d (FP64 oracle, single thread), %.1f s" % (done, n, el))
 
 
-def cpu_golden(x_cpu, torch):
-    threads = len(os.sched_getaffinity(0))
-    torch.set_num_threads(threads)
-    ts = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        torch.fft.fft(x_cpu, dim=1)
-        ts.append(time.perf_counter() - t0)
-    t = sorted(ts)[len(ts) // 2]
-    n, b = x_cpu.shape[1], x_cpu.shape[0]
-    return dict(value=round(5.0 * n * math.log2(n) * b / t / 1e9, 3), unit="GFLOP/s", cores=threads,
-                kind="torch.fft (reference golden path)", sample="full workload, median of 3, %.3f s" % t)
+def cpu_golden_configs(torch, np):
+    """torch.fft on the host cores (the reference's golden path: fft1_d, rfft1_d, irfft1_d and
+    fft2_d gen_data.py) for every other workload, warm median of >= 10 reps each."""
+    torch.set_num_threads(host_threads())
+    out = {"cores": host_threads(), "cpu_model": cpu_model(), "torch": torch.__version__,
+           "mkl": bool(torch.backends.mkl.is_available())}
+    g = torch.Generator().manual_seed(0)
+
+    def rec(name, fn, flops, nbytes):
+        t, reps = cpu_time(fn, reps=10)
+        out[name] = {"ms": round(t * 1e3, 4), "GFLOP/s": round(flops / t / 1e9, 2),
+                     "GB/s": round(nbytes / t / 1e9, 2), "reps": reps}
+
+    x = torch.randn(64, 1024, dtype=torch.complex64, generator=g)
+    rec("c2c_n1024_b64", lambda: torch.fft.fft(x, dim=1), 5.0 * 1024 * 10 * 64, 16.0 * 1024 * 64)
+    for n in (4096, 3000):
+        xr = torch.randn(8192, n, generator=g)
+        X = torch.fft.rfft(xr, dim=1)
+        byts = (4.0 * n + 8.0 * (n // 2 + 1)) * 8192
+        fl = 2.5 * n * math.log2(n) * 8192
+        rec("r2c_n%d_b8192" % n, lambda: torch.fft.rfft(xr, dim=1), fl, byts)
+        rec("c2r_n%d_b8192" % n, lambda: torch.fft.irfft(X, n=n, dim=1) * n, fl, byts)
+    ln, bt = uneven_groups()
+    segs = [torch.randn(b, n, dtype=torch.complex64, generator=g) for n, b in zip(ln, bt)]
+    fl = sum(5.0 * n * math.log2(n) * b for n, b in zip(ln, bt))
+    pts = sum(n * b for n, b in zip(ln, bt))
+    rec("uneven_512_groups", lambda: [torch.fft.fft(s, dim=1) for s in segs], fl, 16.0 * pts)
+    del segs
+    x2 = torch.randn(4, 2048, 2048, dtype=torch.complex64, generator=g)
+    pts = 4 * 2048 * 2048
+    rec("fft2_2048x2048_b4", lambda: torch.fft.fft2(x2), 5.0 * pts * math.log2(2048 * 2048), 16.0 * pts)
+    del x2
+    xr = torch.randn(1024, 8192, generator=g)
+    rec("r2c_n8192_b1024", lambda: torch.fft.rfft(xr, dim=1), 2.5 * 8192 * 13 * 1024,
+        (4.0 * 8192 + 8.0 * 4097) * 1024)
+    return out
 
 
+# ---------------------------------------------------------------------------------------------
+# GPU timing helpers
+# ---------------------------------------------------------------------------------------------
 def warm(torch, fn, warmup, min_seconds=0.1):
     """`warmup` untimed calls, continued until at least `min_seconds` of device time has passed:
     the clocks settle over tens of milliseconds of sustained load, so a few calls of a
@@ -91,8 +191,8 @@ def warm(torch, fn, warmup, min_seconds=0.1):
         torch.cuda.synchronize()
 
 
-def time_steps(torch, fn, steps, warmup, stream):
-    warm(torch, fn, warmup)
+def time_steps(torch, fn, steps, warmup, stream, min_seconds=0.1):
+    warm(torch, fn, warmup, min_seconds)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record(stream)
@@ -103,6 +203,13 @@ def time_steps(torch, fn, steps, warmup, stream):
     return e0.elapsed_time(e1) / steps
 
 
+def frac(nbytes, ms):
+    return round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)
+
+
+# ---------------------------------------------------------------------------------------------
+# extras
+# ---------------------------------------------------------------------------------------------
 def extra_copy(torch, aclfft, x, steps, stream, headline_ms):
     """Device-to-device copies of the same 1 GiB (512 MiB read + 512 MiB written): torch's copy and
     the library's streaming copy (16-byte non-temporal loads and stores, the access pattern no
@@ -113,32 +220,36 @@ def extra_copy(torch, aclfft, x, steps, stream, headline_ms):
     b = 2.0 * x.numel() * x.element_size()
     out = {}
     ms = time_steps(torch, lambda: y.copy_(x), steps, 3, stream)
-    out["torch_copy"] = {"ms": round(ms, 5), "GB/s": round(b / ms / 1e6, 1),
-                         "frac_of_peak": round(b / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    out["torch_copy"] = {"ms": round(ms, 5), "GB/s": round(b / ms / 1e6, 1), "frac_of_peak": frac(b, ms)}
     h = ctypes.c_void_p(stream.cuda_stream)
     nb = x.numel() * x.element_size()
     ms = time_steps(torch, lambda: stream_copy(x.data_ptr(), y.data_ptr(), nb, h), steps, 3, stream)
-    out["stream_copy"] = {"ms": round(ms, 5), "GB/s": round(b / ms / 1e6, 1),
-                          "frac_of_peak": round(b / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    out["stream_copy"] = {"ms": round(ms, 5), "GB/s": round(b / ms / 1e6, 1), "frac_of_peak": frac(b, ms)}
     out["headline_vs_stream_copy"] = round(ms / headline_ms, 4)
     if not torch.equal(x, y):
         out["error"] = "stream copy mismatch"
     return out
 
 
-def sixty_db_signal(np, batch, seed):
+def sixty_db_signal(np, batch, seed, n=N):
     """The precision workload's 60 dB case: per transform, 8 complex tones at random bins
     (amplitudes 1 x4 and 1e-3 x4, random phases) plus complex Gaussian noise of deviation 1e-6."""
     rng = np.random.default_rng(seed)
-    t = np.arange(N)
-    x = np.empty((batch, N), np.complex128)
+    t = np.arange(n)
+    x = np.empty((batch, n), np.complex128)
     amp = np.array([1, 1, 1, 1, 1e-3, 1e-3, 1e-3, 1e-3])
     for b in range(batch):
-        f = rng.choice(N, 8, replace=False)
+        f = rng.choice(n, 8, replace=False)
         ph = rng.uniform(0, 2 * np.pi, 8)
-        x[b] = (amp * np.exp(1j * (2 * np.pi * np.outer(t, f) / N + ph))).sum(1)
-    x += 1e-6 * (rng.standard_normal((batch, N)) + 1j * rng.standard_normal((batch, N)))
+        x[b] = (amp * np.exp(1j * (2 * np.pi * np.outer(t, f) / n + ph))).sum(1)
+    x += 1e-6 * (rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n)))
     return x.astype(np.complex64)
+
+
+def errs(oracle, np, got, ref):
+    """rel-L2 and max-abs error (relative to the reference's largest magnitude) vs FP64."""
+    got = np.asarray(got, np.complex128)
+    return oracle.rel_l2(got, ref), float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300))
 
 
 def extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y, steps, stream, e32):
@@ -157,20 +268,22 @@ def extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y, steps, stream, e32):
         q.exec_c2c(s60d, o, aclfft.FORWARD)
         torch.cuda.synchronize()
         q.destroy()
-        return oracle.rel_l2(o.cpu().numpy(), ref60)
+        return errs(oracle, np, o.cpu().numpy(), ref60)
 
-    out["sixty_db_fp32_rel_l2_vs_fp64"] = err60(0)
+    out["sixty_db_fp32_vs_fp64"] = dict(zip(("rel_l2", "max_abs_rel"), err60(0)))
+    ref = oracle.c2c(x_cpu[:256].numpy().astype(np.complex128))
     for mode, name in ((aclfft.PRECISION_BF16, "bf16"), (aclfft.PRECISION_FP16X2, "fp16x2"),
                        (aclfft.PRECISION_BF16X3, "bf16x3")):
         p = aclfft.Plan.plan1d(N, aclfft.C2C, BATCH)
         p.set_precision(mode)
         p.set_stream(stream)
         ms = time_steps(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
-        ref = oracle.c2c(x_cpu[:256].numpy().astype(np.complex128))
-        err = oracle.rel_l2(y[:256].cpu().numpy(), ref)
+        el2, emax = errs(oracle, np, y[:256].cpu().numpy(), ref)
+        e60 = err60(mode)
         out[name] = {"ms": round(ms, 5), "GFLOP/s": round(5.0 * N * math.log2(N) * BATCH / ms / 1e6, 1),
-                     "hbm_frac": round(bytes_ / ms / 1e6 / HBM_PEAK_GBS, 4), "rel_l2_vs_fp64": err,
-                     "err_ratio_vs_fp32": round(err / e32, 3), "sixty_db_rel_l2_vs_fp64": err60(mode)}
+                     "hbm_frac": frac(bytes_, ms), "rel_l2_vs_fp64": el2, "max_abs_rel_vs_fp64": emax,
+                     "err_ratio_vs_fp32": round(el2 / e32, 3), "sixty_db_rel_l2_vs_fp64": e60[0],
+                     "sixty_db_max_abs_rel_vs_fp64": e60[1], "plan": p.describe()}
         p.destroy()
     return out
 
@@ -193,10 +306,10 @@ def extra_real(torch, aclfft, oracle, np, steps, stream):
         err = oracle.rel_l2(X[:64].cpu().numpy(), oracle.r2c(xr[:64].cpu().numpy().astype(np.float64)))
         out["n%d_b%d" % (n, b)] = {
             "r2c_ms": round(ms_r, 5), "c2r_ms": round(ms_c, 5),
-            "r2c_hbm_frac": round(byts / ms_r / 1e6 / HBM_PEAK_GBS, 4),
-            "c2r_hbm_frac": round(byts / ms_c / 1e6 / HBM_PEAK_GBS, 4),
+            "r2c_hbm_frac": frac(byts, ms_r), "c2r_hbm_frac": frac(byts, ms_c),
             "r2c_GFLOP/s_2.5nlogn": round(2.5 * n * math.log2(n) * b / ms_r / 1e6, 1),
-            "roundtrip_rel_l2": rt, "r2c_rel_l2_vs_fp64": err, "plan_r2c": pr.describe()}
+            "roundtrip_rel_l2": rt, "r2c_rel_l2_vs_fp64": err, "plan_r2c": pr.describe(),
+            "plan_c2r": pc.describe()}
         pr.destroy()
         pc.destroy()
     return out
@@ -214,7 +327,7 @@ def extra_double(torch, aclfft, oracle, np, steps, stream):
     p.set_stream(stream)
     ms = time_steps(torch, lambda: p.exec_z2z(x, y, aclfft.FORWARD), steps, 3, stream)
     err = oracle.rel_l2(y[:64].cpu().numpy(), oracle.c2c(x[:64].cpu().numpy()))
-    out["z2z_n1024_b32768"] = {"ms": round(ms, 5), "hbm_frac": round(32.0 * n * b / ms / 1e6 / HBM_PEAK_GBS, 4),
+    out["z2z_n1024_b32768"] = {"ms": round(ms, 5), "hbm_frac": frac(32.0 * n * b, ms),
                                "GFLOP/s": round(5.0 * n * math.log2(n) * b / ms / 1e6, 1),
                                "rel_l2_vs_fp64_oracle": err, "plan": p.describe()}
     p.destroy()
@@ -233,8 +346,7 @@ def extra_double(torch, aclfft, oracle, np, steps, stream):
     err = oracle.rel_l2(X[:32].cpu().numpy(), oracle.r2c(xr[:32].cpu().numpy()))
     rt = float(((xb / n - xr).norm() / xr.norm()).item())
     out["d2z_z2d_n4096_b4096"] = {"d2z_ms": round(ms_r, 5), "z2d_ms": round(ms_c, 5),
-                                  "d2z_hbm_frac": round(byts / ms_r / 1e6 / HBM_PEAK_GBS, 4),
-                                  "z2d_hbm_frac": round(byts / ms_c / 1e6 / HBM_PEAK_GBS, 4),
+                                  "d2z_hbm_frac": frac(byts, ms_r), "z2d_hbm_frac": frac(byts, ms_c),
                                   "d2z_rel_l2_vs_fp64_oracle": err, "roundtrip_rel_l2": rt,
                                   "plan_d2z": pr.describe()}
     pr.destroy()
@@ -244,8 +356,8 @@ def extra_double(torch, aclfft, oracle, np, steps, stream):
 
 def extra_small_latency(torch, aclfft, oracle, np, stream):
     """The launch-bound configuration (C2C N=1024, batch 64, 512 KiB each way): time per call,
-    eager and replayed from a hipGraph, measured over many back-to-back calls (device time per
-    call with events, host time per call with the wall clock)."""
+    eager and replayed from the plan's hipGraph, measured over many back-to-back calls (device time
+    per call with events, host time per call with the wall clock)."""
     x = torch.randn(64, 1024, dtype=torch.complex64, device="cuda")
     y = torch.empty_like(x)
     out = {}
@@ -320,17 +432,18 @@ def extra_2d(torch, aclfft, oracle, np, steps, stream):
     err = oracle.rel_l2(y[:1].cpu().numpy(), oracle.c2c_2d(x[:1].cpu().numpy().astype(np.complex128)))
     pts = B * NX * NY
     out = {"ms": round(ms, 5), "GFLOP/s": round(5.0 * pts * math.log2(NX * NY) / ms / 1e6, 1),
-           "hbm_frac_compulsory": round(16.0 * pts / ms / 1e6 / HBM_PEAK_GBS, 4),
-           "hbm_frac_two_pass": round(32.0 * pts / ms / 1e6 / HBM_PEAK_GBS, 4),
+           "hbm_frac_compulsory": frac(16.0 * pts, ms), "hbm_frac_two_pass": frac(32.0 * pts, ms),
            "rel_l2_vs_fp64": err, "plan": p.describe()}
     p.destroy()
     return out
 
 
-def extra_loss_grad(torch, steps, stream):
+def extra_loss_grad(torch, aclfft, steps, stream):
     """The differentiable configuration: loss = sum |W * FFT(x)|^2 (W ~ U[0,1), fixed) and its
     gradient through torch.autograd over the library (forward transform, then the adjoint kernel),
-    for 2D C2C 2048x2048 batch 4 and 1D R2C n=8192 batch 1024.  One step = forward + backward."""
+    for 2D C2C 2048x2048 batch 4 and 1D R2C n=8192 batch 1024.  One step = forward + backward.
+    The adjoint launches are also timed alone: the 2D adjoint is the backward 2D transform, the
+    R2C adjoint is aclfftExtExecR2CAdjoint (C2R kernels with the Hermitian weights)."""
     from aclfft import autograd as AG
     out = {}
     g = torch.Generator(device="cuda").manual_seed(7)
@@ -349,43 +462,60 @@ def extra_loss_grad(torch, steps, stream):
             loss.backward()
         ms = time_steps(torch, step, steps, 3, stream)
         fft_ms = time_steps(torch, lambda: fn(x.detach()), steps, 3, stream)
-        out[name] = {"fwd_bwd_ms": round(ms, 5), "transform_ms": round(fft_ms, 5),
-                     "note": "step = transform + loss (torch elementwise) + adjoint transform"}
+        rec = {"fwd_bwd_ms": round(ms, 5), "transform_ms": round(fft_ms, 5),
+               "note": "step = transform + loss (torch elementwise) + adjoint transform"}
+        if name.startswith("fft2"):
+            p = aclfft.Plan.plan2d(4, 2048, 2048, aclfft.C2C)
+            p.set_stream(stream)
+            gi = torch.randn(shape, dtype=torch.complex64, device="cuda", generator=g)
+            go = torch.empty_like(gi)
+            ams = time_steps(torch, lambda: p.exec_c2c(gi, go, aclfft.BACKWARD), steps, 3, stream)
+            pts = 4 * 2048 * 2048
+            rec.update({"adjoint_ms": round(ams, 5), "adjoint_hbm_frac_two_pass": frac(32.0 * pts, ams),
+                        "adjoint_plan": p.describe()})
+        else:
+            p = aclfft.Plan.plan1d(8192, aclfft.R2C, 1024)
+            p.set_stream(stream)
+            gi = torch.randn(1024, 4097, dtype=torch.complex64, device="cuda", generator=g)
+            go = torch.empty(1024, 8192, device="cuda")
+            ams = time_steps(torch, lambda: p.exec_r2c_adjoint(gi, go), steps, 3, stream)
+            byts = (4.0 * 8192 + 8.0 * 4097) * 1024
+            rec.update({"adjoint_ms": round(ams, 5), "adjoint_hbm_frac": frac(byts, ams)})
+        p.destroy()
+        out[name] = rec
     return out
 
 
 def extra_batched(torch, aclfft, oracle, np, steps, stream):
     """512 C2C transforms, lengths log-uniform over the set, batches U{1..64}; one plan (persistent
     kernel, static schedule; launched directly, and replayed from the plan's hipGraph) vs one plan
-    (and launch) per group."""
-    lengths = (60, 128, 243, 1000, 1024, 2048, 4096, 6561)
-    rng = np.random.default_rng(4)
-    logs = np.log(np.array(lengths, float))
-    u = rng.uniform(logs.min(), logs.max(), 512)
-    ln = [int(lengths[int(np.argmin(np.abs(logs - v)))]) for v in u]
-    bt = [int(b) for b in rng.integers(1, 65, 512)]
+    (and launch) per group.  Every group's output of both the direct and the replayed plan is
+    checked against the FP64 oracle."""
+    ln, bt = uneven_groups()
     total = sum(a * b for a, b in zip(ln, bt))
     x = (torch.rand(total, dtype=torch.complex64, device="cuda") * 2 - (1 + 1j)).to(torch.complex64)
     y = torch.empty_like(x)
+    yg = torch.empty_like(x)
     p = aclfft.Plan.plan_many_uneven(ln, bt)
     p.set_stream(stream)
     ms = time_steps(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
     pg = aclfft.Plan.plan_many_uneven(ln, bt)
     pg.set_stream(stream)
     pg.set_graph_mode(True)
-    ms_graph = time_steps(torch, lambda: pg.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
+    ms_graph = time_steps(torch, lambda: pg.exec_c2c(x, yg, aclfft.FORWARD), steps, 3, stream)
     pg.destroy()
-    # correctness on a few groups (output of the directly launched plan)
     p.exec_c2c(x, y, aclfft.FORWARD)
     torch.cuda.synchronize()
     off = 0
     worst = 0.0
     ycpu = y.cpu().numpy()
+    ygcpu = yg.cpu().numpy()
     xcpu = x.cpu().numpy()
-    for gi, (n, b) in enumerate(zip(ln, bt)):
-        if gi % 64 == 0:
-            seg = xcpu[off:off + n * b].reshape(b, n).astype(np.complex128)
-            worst = max(worst, oracle.rel_l2(ycpu[off:off + n * b].reshape(b, n), oracle.c2c(seg)))
+    for n, b in zip(ln, bt):
+        seg = xcpu[off:off + n * b].reshape(b, n).astype(np.complex128)
+        ref = oracle.c2c(seg)
+        worst = max(worst, oracle.rel_l2(ycpu[off:off + n * b].reshape(b, n), ref),
+                    oracle.rel_l2(ygcpu[off:off + n * b].reshape(b, n), ref))
         off += n * b
     plans = [aclfft.Plan.plan1d(n, aclfft.C2C, b) for n, b in zip(ln, bt)]
     offs = np.cumsum([0] + [a * b for a, b in zip(ln, bt)])
@@ -400,12 +530,91 @@ def extra_batched(torch, aclfft, oracle, np, steps, stream):
     flops = sum(5.0 * n * math.log2(n) * b for n, b in zip(ln, bt))
     out = {"groups": 512, "points": int(total), "persistent_ms": round(ms, 5), "persistent_hipgraph_ms": round(ms_graph, 5),
            "per_group_launches_ms": round(ms_naive, 5), "speedup": round(ms_naive / ms, 2),
-           "GFLOP/s": round(flops / ms / 1e6, 1), "hbm_frac": round(16.0 * total / ms / 1e6 / HBM_PEAK_GBS, 4),
-           "worst_rel_l2_vs_fp64": worst, "plan": p.describe()}
+           "GFLOP/s": round(flops / ms / 1e6, 1), "hbm_frac": frac(16.0 * total, ms),
+           "worst_rel_l2_vs_fp64_all_groups": worst, "plan": p.describe()}
     for q in plans:
         q.destroy()
     p.destroy()
     return out
+
+
+# ---------------------------------------------------------------------------------------------
+# secondary workloads for a profiler (--profile-workloads)
+# ---------------------------------------------------------------------------------------------
+def profile_workloads(torch, aclfft, reps):
+    """R2C and C2R at n=4096 b=8192, n=3000 b=8192, n=8192 b=1024; 2D C2C 2048x2048 batch 4; the
+    512-group uneven batch; 1D C2C n=2^20 batch 64 (four-step); 3D C2C 256^3; the streaming copy of
+    1 GiB.  Each plan is called until 0.1 s of device time has passed (tables built, clocks
+    settled), then `reps` more times: a trace's last `reps` dispatches of a kernel are the measured
+    ones.  Every buffer is allocated at the size its plan reads or writes."""
+    s = torch.cuda.current_stream()
+
+    def rep(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < 0.1:
+            for _ in range(5):
+                fn()
+            torch.cuda.synchronize()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+
+    for n, b in ((4096, 8192), (3000, 8192), (8192, 1024)):
+        xr = torch.randn(b, n, device="cuda")
+        X = torch.empty(b, n // 2 + 1, dtype=torch.complex64, device="cuda")
+        pr = aclfft.Plan.plan1d(n, aclfft.R2C, b)
+        pc = aclfft.Plan.plan1d(n, aclfft.C2R, b)
+        pr.set_stream(s)
+        pc.set_stream(s)
+        rep(lambda: pr.exec_r2c(xr, X))
+        rep(lambda: pc.exec_c2r(X, xr))
+        log("real n=%d b=%d: %s | %s" % (n, b, pr.describe(), pc.describe()))
+        pr.destroy()
+        pc.destroy()
+        del xr, X
+    x = torch.randn(4, 2048, 2048, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    p = aclfft.Plan.plan2d(4, 2048, 2048, aclfft.C2C)
+    p.set_stream(s)
+    rep(lambda: p.exec_c2c(x, y, aclfft.FORWARD))
+    log("2d 2048x2048 b=4: %s" % p.describe())
+    p.destroy()
+    del x, y
+    ln, bt = uneven_groups()
+    total = sum(a * b for a, b in zip(ln, bt))
+    xs = torch.randn(total, dtype=torch.complex64, device="cuda")
+    ys = torch.empty_like(xs)
+    q = aclfft.Plan.plan_many_uneven(ln, bt)
+    q.set_stream(s)
+    rep(lambda: q.exec_c2c(xs, ys, aclfft.FORWARD))
+    log("uneven 512 groups, %d points: %s" % (total, q.describe()))
+    q.destroy()
+    del xs, ys
+    n, b = 1 << 20, 64
+    x = torch.randn(b, n, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    p = aclfft.Plan.plan1d(n, aclfft.C2C, b)
+    p.set_stream(s)
+    rep(lambda: p.exec_c2c(x, y, aclfft.FORWARD))
+    log("c2c n=2^20 b=64: %s" % p.describe())
+    p.destroy()
+    del x, y
+    x = torch.randn(256, 256, 256, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    p = aclfft.Plan.plan3d(256, 256, 256, aclfft.C2C)
+    p.set_stream(s)
+    rep(lambda: p.exec_c2c(x, y, aclfft.FORWARD))
+    log("3d 256^3: %s" % p.describe())
+    p.destroy()
+    del x, y
+    from aclfft._lib import stream_copy
+    a = torch.empty(1 << 29, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    h = s.cuda_stream
+    rep(lambda: stream_copy(a.data_ptr(), b.data_ptr(), a.numel(), h))
+    log("stream copy 512 MiB")
 
 
 def main():
@@ -419,6 +628,9 @@ def main():
                     help="arithmetic of the timed loop (profiling runs; the headline value is fp32)")
     ap.add_argument("--with-copy", action="store_true",
                     help="also run one device copy of the 1 GiB input (counter calibration for profiles)")
+    ap.add_argument("--profile-workloads", action="store_true",
+                    help="run the secondary workloads for a profiler, print no JSON line")
+    ap.add_argument("--reps", type=int, default=20, help="--profile-workloads: measured calls per plan")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the FFT path has no multi-GPU mode (a batch would be "
@@ -428,6 +640,11 @@ def main():
     import numpy as np
     import torch
     import aclfft
+
+    if args.profile_workloads:
+        profile_workloads(torch, aclfft, args.reps)
+        return
+
     from oracle import oracle
 
     dev = torch.device("cuda:0")
@@ -446,30 +663,26 @@ def main():
         ycopy.copy_(x)
         torch.cuda.synchronize()
         del ycopy
-    # parity on a subset of the output against the FP64 oracle (before the timed region, so that
-    # the CPU work does not leave the GPU idle between warmup and timing)
-    plan.exec_c2c(x, y, aclfft.FORWARD)
-    torch.cuda.synchronize()
-    sub = 256
-    ref = oracle.c2c(x_cpu[:sub].numpy().astype(np.complex128))
-    err = oracle.rel_l2(y[:sub].cpu().numpy(), ref)
-    log("bench: rel_l2 vs FP64 oracle on %d transforms = %.3e" % (sub, err))
 
-    log("bench: plan ready, warmup %d" % args.warmup)
-    for _ in range(args.warmup):
-        plan.exec_c2c(x, y, aclfft.FORWARD)
-    torch.cuda.synchronize()
-
+    fn = lambda: plan.exec_c2c(x, y, aclfft.FORWARD)  # noqa: E731
+    log("bench: plan ready, warmup %d calls + at least 0.25 s" % args.warmup)
+    warm(torch, fn, args.warmup, min_seconds=0.25)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
     e0.record(stream)
     for _ in range(args.steps):
-        plan.exec_c2c(x, y, aclfft.FORWARD)
+        fn()
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / args.steps
     log("bench: %.4f ms per step" % ms)
+
+    # parity after the timed region: 4096 transforms against the FP64 oracle (the full batch is
+    # checked by tests/test_gpu_fullsize.py)
+    sub = 4096
+    ref = oracle.c2c(x_cpu[:sub].numpy().astype(np.complex128))
+    err, emax = errs(oracle, np, y[:sub].cpu().numpy(), ref)
+    log("bench: rel_l2 vs FP64 oracle on %d transforms = %.3e (max %.3e)" % (sub, err, emax))
 
     flops = 5.0 * N * math.log2(N) * BATCH
     gflops = flops / (ms * 1e-3) / 1e9
@@ -495,30 +708,35 @@ def main():
                                                      "MFMA " + args.precision) + " (aclfftExecC2C, device pointers)",
                    "kernel": plan.describe()},
         "rel_l2_vs_fp64": err,
+        "max_abs_rel_vs_fp64": emax,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
     if not args.no_extras:
         extras = {}
-        for name, fn in (("copy_1GiB", lambda: extra_copy(torch, aclfft, x, 20, stream, ms)),
-                         ("mfma_split_n1024_b65536", lambda: extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y,
-                                                                        args.steps, stream, err)),
-                         ("real_r2c_c2r", lambda: extra_real(torch, aclfft, oracle, np, args.steps, stream)),
-                         ("c2c_n1024_b64_latency", lambda: extra_small_latency(torch, aclfft, oracle, np, stream)),
-                         ("double_precision", lambda: extra_double(torch, aclfft, oracle, np, args.steps, stream)),
-                         ("c2c_2d_2048x2048_b4", lambda: extra_2d(torch, aclfft, oracle, np, 10, stream)),
-                         ("uneven_batched_512", lambda: extra_batched(torch, aclfft, oracle, np, 20, stream)),
-                         ("loss_grad", lambda: extra_loss_grad(torch, 10, stream))):
+        for name, fn_ in (("copy_1GiB", lambda: extra_copy(torch, aclfft, x, 20, stream, ms)),
+                          ("mfma_split_n1024_b65536", lambda: extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y,
+                                                                         args.steps, stream, err)),
+                          ("real_r2c_c2r", lambda: extra_real(torch, aclfft, oracle, np, args.steps, stream)),
+                          ("c2c_n1024_b64_latency", lambda: extra_small_latency(torch, aclfft, oracle, np, stream)),
+                          ("double_precision", lambda: extra_double(torch, aclfft, oracle, np, args.steps, stream)),
+                          ("c2c_2d_2048x2048_b4", lambda: extra_2d(torch, aclfft, oracle, np, 10, stream)),
+                          ("uneven_batched_512", lambda: extra_batched(torch, aclfft, oracle, np, 20, stream)),
+                          ("loss_grad", lambda: extra_loss_grad(torch, aclfft, 10, stream))):
             try:
                 log("bench: extra %s" % name)
-                extras[name] = fn()
+                extras[name] = fn_()
             except Exception as ex:  # an extra must not hide the headline line
                 extras[name] = {"error": "%s: %s" % (type(ex).__name__, ex)}
         out["extras"] = extras
     if not args.no_cpu:
-        sample = x_cpu[:512].numpy().astype(np.complex128)
-        out["cpu_baseline"] = cpu_baseline(sample)
-        out["cpu_golden_path"] = cpu_golden(x_cpu, torch)
+        log("bench: CPU baselines")
+        out["cpu_baseline"] = cpu_golden_headline(torch, x_cpu)
+        out.setdefault("extras", {})["cpu_oracle_port"] = cpu_oracle_port(x_cpu[:512].numpy().astype(np.complex128))
+        try:
+            out["extras"]["cpu_golden_path_other_workloads"] = cpu_golden_configs(torch, np)
+        except Exception as ex:
+            out["extras"]["cpu_golden_path_other_workloads"] = {"error": "%s: %s" % (type(ex).__name__, ex)}
     plan.destroy()
     print(json.dumps(out), flush=True)
 

@@ -229,6 +229,7 @@ typedef struct ofx_batch_args {
                                        k*blocks + (k even ? b : blocks-1-b) of the list (the list
                                        is sorted by cost, so the rounds alternate direction),
                                        its descriptor loaded one tile ahead; no counter        */
+    float scale;                    /* multiplies every output value (0 is taken as 1)         */
 } ofx_batch_args;
 OFX_API int ofx_batch_class_count(void);
 OFX_API int ofx_batch_class_info_get(int cls, ofx_batch_class_info* info);

@@ -61,10 +61,11 @@ struct GLoadB {
 };
 struct GStoreB {
     float2* __restrict__ p;
-    float sg;
+    float sc;   // output scale
+    float sgs;  // sign * scale
     bool ok;
     __device__ __forceinline__ void operator()(int i, float2 v) const {
-        if (ok) p[i] = make_float2(v.x, v.y * sg);
+        if (ok) p[i] = make_float2(v.x * sc, v.y * sgs);
     }
 };
 
@@ -81,7 +82,7 @@ struct GStoreB {
 // ranges of the AMDGPU calling convention inflate each body's footprint.
 template <class C>
 __device__ __forceinline__ void run_tile(float2* lds, const ofx_batch_tile& tile, const float2* __restrict__ in,
-                                         float2* __restrict__ out, const float2* __restrict__ tw, float sg) {
+                                         float2* __restrict__ out, const float2* __restrict__ tw, float sg, float sc) {
     int tx;
     __asm__ volatile("v_mov_b32 %0, %1" : "=v"(tx) : "v"((int)threadIdx.x));
     const int tl = tx / C::T;
@@ -92,7 +93,7 @@ __device__ __forceinline__ void run_tile(float2* lds, const ofx_batch_tile& tile
     }
     const long long base = tile.offset + (long long)(ok ? tl : 0) * C::N;
     GLoadB gin{in + base, sg};
-    GStoreB gout{out + base, sg, ok};
+    GStoreB gout{out + base, sc, sg * sc, ok};
     if constexpr (C::SPLITX)
         split_fft<C>(reinterpret_cast<float*>(lds) + tl * C::LDS_N, tid, tw, gin, gout);
     else
@@ -105,17 +106,18 @@ __device__ __forceinline__ void run_class(float2* lds, const ofx_batch_tile& til
     const float2* __restrict__ in = static_cast<const float2*>(a.in);
     float2* __restrict__ out = static_cast<float2*>(a.out);
     const float2* tw = static_cast<const float2*>(a.tw) + a.tw_off[tile.cls];
+    const float sc = a.scale != 0.f ? a.scale : 1.f;
     switch (tile.cls) {
-        case OFX_BCLASS_60: run_tile<BC60>(lds, tile, in, out, tw, a.sign); break;
-        case OFX_BCLASS_128: run_tile<BC128>(lds, tile, in, out, tw, a.sign); break;
-        case OFX_BCLASS_243: run_tile<BC243>(lds, tile, in, out, tw, a.sign); break;
-        case OFX_BCLASS_256: run_tile<BC256>(lds, tile, in, out, tw, a.sign); break;
-        case OFX_BCLASS_512: run_tile<BC512>(lds, tile, in, out, tw, a.sign); break;
-        case OFX_BCLASS_1000: run_tile<BC1000>(lds, tile, in, out, tw, a.sign); break;
-        case OFX_BCLASS_1024: run_tile<BC1024>(lds, tile, in, out, tw, a.sign); break;
-        case OFX_BCLASS_2048: run_tile<BC2048>(lds, tile, in, out, tw, a.sign); break;
-        case OFX_BCLASS_4096: run_tile<BC4096>(lds, tile, in, out, tw, a.sign); break;
-        case OFX_BCLASS_6561: run_tile<BC6561>(lds, tile, in, out, tw, a.sign); break;
+        case OFX_BCLASS_60: run_tile<BC60>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_128: run_tile<BC128>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_243: run_tile<BC243>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_256: run_tile<BC256>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_512: run_tile<BC512>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_1000: run_tile<BC1000>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_1024: run_tile<BC1024>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_2048: run_tile<BC2048>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_4096: run_tile<BC4096>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_6561: run_tile<BC6561>(lds, tile, in, out, tw, a.sign, sc); break;
         default: break;
     }
 }

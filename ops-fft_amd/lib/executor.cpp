@@ -259,7 +259,8 @@ struct Builder {
     // Batch of contiguous C2C transforms of length n from buffer src to dst.  The direction's
     // conjugation is applied on the first load (conj_in) and the last store (conj_out).
     aclfftResult c2c_rows(long long n, long long batch, int src, int dst, bool conj_in, bool conj_out, bool z = false) {
-        if (n == 1) {
+        // (a copy cannot scale: with a scale, length 1 runs as the pass-free runtime-radix step)
+        if (n == 1 && e->desc_.scale == 1.f) {
             Step s;
             s.type = Step::COPY;
             s.src = src;
@@ -656,6 +657,7 @@ struct Builder {
                     const size_t first = e->steps_.size();
                     aclfftResult r = c2c_rows(n, bat[g], BUF_IN, BUF_OUT, true, true);
                     if (r != ACLFFT_SUCCESS) return r;
+                    e->steps_.back().final_out = true;
                     for (size_t k = first; k < e->steps_.size(); ++k) {
                         if (e->steps_[k].src == BUF_IN || e->steps_[k].src == BUF_OUT)
                             e->steps_[k].src_off = (size_t)offs[g] * 8;
@@ -694,6 +696,7 @@ struct Builder {
             s.tiles_table = e->add_table(tt);
             s.args.batch = (long long)tiles.size();
             s.conj_in = s.conj_out = true;
+            s.final_out = true;
             // longest-processing-time assignment of lengths to side-stream lanes, off by default:
             // the fork / join of the side streams costs more than the overlap gains (2 to 8 lanes
             // measured 1.2-3x slower than the plan's own stream for one to three lengths)
@@ -764,6 +767,7 @@ struct Builder {
                 const size_t first = e->steps_.size();
                 aclfftResult r = c2c_rows(n, b, BUF_IN, BUF_OUT, true, true);
                 if (r != ACLFFT_SUCCESS) return r;
+                e->steps_.back().final_out = true;
                 for (size_t k = first; k < e->steps_.size(); ++k) {
                     if (e->steps_[k].src == BUF_IN) e->steps_[k].src_off = (size_t)off * 8;
                     if (e->steps_[k].dst == BUF_OUT) e->steps_[k].dst_off = (size_t)off * 8;
@@ -782,6 +786,8 @@ struct Builder {
         Step s;
         s.type = Step::PERSISTENT;
         s.conj_in = s.conj_out = true;
+        s.final_out = true;
+        s.b.scale = 1.f;
         for (int c = 0; c < (int)cls.size(); ++c) {
             s.class_tables[c] = -1;
             if (cls[c].used) s.class_tables[c] = pass_table_for(cls[c].info.npass, cls[c].info.radices, cls[c].info.tw_len);
@@ -1047,7 +1053,7 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
     e->desc_ = d;
     Builder b{e};
     aclfftResult r = ACLFFT_NOT_IMPLEMENTED;
-    if (d.precision != 0 && !(d.rank == 1 && !d.vertical && d.type == ACLFFT_C2C)) {
+    if (d.precision != 0 && (d.uneven || !(d.rank == 1 && !d.vertical && d.type == ACLFFT_C2C))) {
         delete e;
         return ACLFFT_NOT_SUPPORTED;
     }
@@ -1125,10 +1131,34 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
         delete e;
         return r;
     }
+    const aclfftType t = d.type;
+    if (!d.hermitian_weight && (t == ACLFFT_R2C || t == ACLFFT_C2R || t == ACLFFT_D2Z || t == ACLFFT_Z2D)) {
+        // the adjoint execution (the opposite real type) runs on the same caller work area
+        PlanDesc a = d;
+        a.type = t == ACLFFT_R2C ? ACLFFT_C2R : t == ACLFFT_C2R ? ACLFFT_R2C : t == ACLFFT_D2Z ? ACLFFT_Z2D : ACLFFT_D2Z;
+        a.hermitian_weight = true;
+        Executor* twin = nullptr;
+        if (build(a, &twin) == ACLFFT_SUCCESS) {
+            e->twin_work_bytes_ = twin->work_bytes();
+            delete twin;
+        }
+    }
     if (d.hermitian_weight)
         for (Step& st : e->steps_)
             if (st.type != Step::COPY) st.args.flags |= OFX_FLAG_HERMITIAN_WEIGHT;
-    if (d.scale != 1.f) {
+    if (d.scale != 1.f && d.uneven) {
+        // every group's output region is written last by its own step (a tile step of its
+        // length, the persistent launch, or the last step of the group's own launch sequence)
+        for (Step& st : e->steps_) {
+            if (!st.final_out) continue;
+            if (st.type == Step::PERSISTENT) {
+                st.b.scale *= d.scale;
+            } else {
+                st.args.scale *= d.scale;
+                st.args.scale64 = (double)st.args.scale;
+            }
+        }
+    } else if (d.scale != 1.f) {
         // fold the normalisation into the last launch that writes the output
         for (auto it = e->steps_.rbegin(); it != e->steps_.rend(); ++it)
             if (it->dst == BUF_OUT && it->type != Step::COPY && it->type != Step::PERSISTENT) {
@@ -1178,7 +1208,7 @@ Executor::~Executor() {
 size_t Executor::work_bytes() const {
     size_t b = align256(work_bytes_) + align256(work1_bytes_);
     if (in_place_unsafe_) b += align256(in_bytes_);
-    return b;
+    return std::max(b, twin_work_bytes_);
 }
 
 void Executor::inherit_workspace(const Executor& old) {
@@ -1191,8 +1221,16 @@ void Executor::drop_graphs() {
     graphs_.clear();
 }
 
+void Executor::set_auto_allocation(bool on) {
+    auto_alloc_ = on;
+    delete adjoint_;  // hipFree in its destructor waits for work still using its buffers
+    adjoint_ = nullptr;
+}
+
 aclfftResult Executor::set_work_area(void* p) {
     if (p == nullptr || !is_device_pointer(p)) return ACLFFT_INVALID_VALUE;
+    delete adjoint_;
+    adjoint_ = nullptr;
     if (owns_work_) {
         // plan-allocated buffers (if any) are released; hipFree waits for work using them
         if (d_work_) (void)hipFree(d_work_);

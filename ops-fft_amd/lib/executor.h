@@ -51,6 +51,8 @@ struct Step {
     ofx_batch_args b{};            // PERSISTENT: launch arguments (pointers filled at execution)
     int class_tables[OFX_BCLASS_COUNT] = {0};  // PERSISTENT: pass table of each class (-1 none)
     int tiles_table = -1, counter_table = -1;
+    bool final_out = false;        // uneven plans: the last step writing its group's output region
+                                   // (the plan's scale is applied there)
     int lane = 0;                  // 0: the caller's stream; k > 0: forked onto the plan's k-th
                                    // side stream (independent steps run concurrently; joined
                                    // back before the next lane-0 step and at the end)
@@ -83,12 +85,15 @@ public:
     aclfftResult exec(const void* in, void* out, int direction, hipStream_t stream);
 
     // Device workspace of one execution: the four-step buffers, then (real plans) the copy an
-    // in-place execution on device memory reads from; 256-byte aligned parts.
+    // in-place execution on device memory reads from; 256-byte aligned parts.  For real plans it
+    // also covers the adjoint execution, which shares the caller's work area.
     size_t work_bytes() const;
     // Caller-managed workspace (aclfftSetWorkArea / aclfftSetAutoAllocation).  With automatic
     // allocation off and no work area, an execution that needs workspace returns NO_WORKSPACE.
+    // Both also drop the adjoint executor (it took the previous workspace settings when it was
+    // built), so the next adjoint execution rebuilds it with the current ones.
     aclfftResult set_work_area(void* p);
-    void set_auto_allocation(bool on) { auto_alloc_ = on; }
+    void set_auto_allocation(bool on);
     // a rebuilt plan (new precision or scale) keeps the caller's workspace settings; the work
     // area carries over when it is large enough for the new launch list
     void inherit_workspace(const Executor& old);
@@ -112,6 +117,7 @@ private:
     size_t work_bytes_ = 0;       // four-step intermediate
     size_t work1_bytes_ = 0;      // large real transforms: the packed complex data
     size_t in_bytes_ = 0, out_bytes_ = 0;
+    size_t twin_work_bytes_ = 0;  // real plans: workspace of the adjoint execution
     bool in_place_unsafe_ = false;  // in == out must go through the workspace
 
     int device_ = -1;

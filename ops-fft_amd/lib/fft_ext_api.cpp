@@ -27,6 +27,7 @@ ACLFFT_API aclfftResult aclfftExtSetPrecision(aclfftHandle plan, int precision) 
     ofx::Executor* e = nullptr;
     aclfftResult r = ofx::Executor::build(d, &e);
     if (r != ACLFFT_SUCCESS) return r == ACLFFT_NOT_IMPLEMENTED ? ACLFFT_NOT_SUPPORTED : r;
+    e->set_graphs(plan->exec->graphs());
     e->inherit_workspace(*plan->exec);
     delete plan->exec;
     plan->exec = e;

@@ -121,8 +121,9 @@ void store_tune_cache(const PlanDesc& d, const std::vector<Step>& steps, const R
         if (k != nullptr) m[key_of(d, i)] = step_signature(steps[i], *k);
     }
     const std::string path = cache_path();
-    // create the directory chain of the default location
-    std::string dir = path.substr(0, path.find_last_of('/'));
+    // create the directory chain of the location (none for a bare file name)
+    const size_t slash = path.find_last_of('/');
+    std::string dir = slash == std::string::npos ? std::string() : path.substr(0, slash);
     std::string acc;
     std::istringstream parts(dir);
     std::string part;

@@ -290,6 +290,13 @@ def test_uneven_plan_argument_gates():
     Bz = (ctypes.c_int * 2)(1, 0)
     assert lib.aclfftExtPlanManyUneven(ctypes.byref(h), 2, L, Bz, aclfft.C2C) == aclfft.INVALID_SIZE
     assert lib.aclfftExtPlanManyUneven(ctypes.byref(h), 2, L, B, aclfft.C2C) == aclfft.SUCCESS
+    # an MFMA precision on an uneven plan whose lengths have no MFMA class is refused (the plan
+    # keeps its FP32 launch list)
+    L3 = (ctypes.c_int * 2)(60, 243)
+    h3 = ctypes.c_void_p()
+    assert lib.aclfftExtPlanManyUneven(ctypes.byref(h3), 2, L3, B, aclfft.C2C) == aclfft.SUCCESS
+    assert lib.aclfftExtSetPrecision(h3, aclfft.PRECISION_FP16X2) == aclfft.NOT_SUPPORTED
+    assert lib.aclfftDestroy(h3) == aclfft.SUCCESS
     assert lib.aclfftDestroy(h) == aclfft.SUCCESS
 
 

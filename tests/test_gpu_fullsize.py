@@ -1,0 +1,187 @@
+"""GPU: parity at the benchmark configurations' full sizes, every transform checked against the FP64
+oracle with the reference's verdict (src/common/host/test_helper.py:41-71: any |o - g| > atol fails,
+then rel-L2 > rtol fails) at its C2C / R2C / C2R tolerances (the compare_data.py tables), and with
+this path's own bar, rel-L2 <= 1e-6 * log2(N).
+
+  - 1D C2C N=1024, batch 65536, complex64 ~ N(0,1) (the headline), and the 60 dB dynamic-range case
+    (8 tones at amplitudes 1 and 1e-3 plus 1e-6 noise) at the same batch; FP32 Stockham and the MFMA
+    split modes (FP16 hi/lo and BF16 x3 must also stay within 2x the FP32 path's error);
+  - R2C then C2R, N=4096 and N=3000, batch 8192, FP32 real ~ N(0,1) from seed 1, unnormalised
+    forward, 1/N applied to the inverse;
+  - the 512-group uneven batch (lengths log-uniform over {60 .. 6561}, batches U{1..64}, seed 4,
+    values U[-1,1)), launched directly and replayed from the plan's hipGraph.
+"""
+
+import numpy as np
+import pytest
+
+import aclfft
+from oracle import oracle
+from _util import fp32_tol
+
+pytestmark = pytest.mark.gpu
+
+
+def verdict(out, ref, atol, rtol):
+    ok, msg = oracle.comparison_details(out, ref, atol, rtol)
+    assert ok, msg
+
+
+def sixty_db(batch, n, seed):
+    """Per transform: 8 complex tones at distinct random bins (4 at amplitude 1, 4 at 1e-3, random
+    phases) plus complex Gaussian noise of deviation 1e-6."""
+    rng = np.random.default_rng(seed)
+    t = np.arange(n)
+    amp = np.array([1, 1, 1, 1, 1e-3, 1e-3, 1e-3, 1e-3])
+    f = np.argsort(rng.random((batch, n)), axis=1)[:, :8]
+    ph = rng.uniform(0, 2 * np.pi, (batch, 8))
+    x = np.zeros((batch, n), np.complex128)
+    for k in range(8):
+        x += amp[k] * np.exp(1j * (2 * np.pi * np.outer(f[:, k], t) / n + ph[:, k:k + 1]))
+    x += 1e-6 * (rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n)))
+    return x.astype(np.complex64)
+
+
+@pytest.fixture(scope="module")
+def headline(cuda):
+    import torch
+    g = torch.Generator().manual_seed(0)
+    x = torch.randn(65536, 1024, dtype=torch.complex64, generator=g)
+    ref = oracle.c2c(x.numpy())
+    return x, ref
+
+
+def run_c2c(x_cpu, precision=0):
+    import torch
+    xd = x_cpu.cuda() if hasattr(x_cpu, "cuda") else torch.from_numpy(x_cpu).cuda()
+    yd = torch.empty_like(xd)
+    p = aclfft.Plan.plan1d(xd.shape[1], aclfft.C2C, xd.shape[0])
+    if precision:
+        p.set_precision(precision)
+    print(p.describe(), flush=True)
+    p.exec_c2c(xd, yd, aclfft.FORWARD)
+    torch.cuda.synchronize()
+    p.destroy()
+    out = yd.cpu().numpy()
+    del xd, yd
+    return out
+
+
+def test_headline_full_batch_fp32(headline):
+    x, ref = headline
+    out = run_c2c(x)
+    err = oracle.rel_l2(out, ref)
+    print("C2C 1024 x 65536 FP32: rel-L2 %.3e" % err)
+    assert err <= fp32_tol(1024)
+    verdict(out, ref, 1e-2, 1e-3)
+
+
+@pytest.mark.parametrize("mode", [aclfft.PRECISION_FP16X2, aclfft.PRECISION_BF16X3])
+def test_headline_full_batch_mfma_split(headline, mode):
+    x, ref = headline
+    e32 = oracle.rel_l2(run_c2c(x), ref)
+    out = run_c2c(x, mode)
+    err = oracle.rel_l2(out, ref)
+    print("C2C 1024 x 65536 MFMA mode %d: rel-L2 %.3e (FP32 %.3e)" % (mode, err, e32))
+    assert err <= fp32_tol(1024)
+    assert err <= 2.0 * e32
+    verdict(out, ref, 1e-2, 1e-3)
+
+
+def test_sixty_db_full_batch(cuda):
+    x = sixty_db(65536, 1024, 11)
+    ref = oracle.c2c(x)
+    # every tone bin (the 1e-3 tones' bins too, |X| ~ 1) keeps its value to 1e-4 of its magnitude
+    tone = np.abs(ref) > 0.5
+
+    def tone_err(out):
+        return float(np.max(np.abs(out[tone] - ref[tone]) / np.abs(ref[tone])))
+
+    out = run_c2c(x)
+    e32 = oracle.rel_l2(out, ref)
+    print("60 dB case FP32: rel-L2 %.3e, worst tone bin %.3e" % (e32, tone_err(out)))
+    assert e32 <= fp32_tol(1024)
+    assert tone_err(out) < 1e-4
+    for mode in (aclfft.PRECISION_FP16X2, aclfft.PRECISION_BF16X3):
+        out = run_c2c(x, mode)
+        err = oracle.rel_l2(out, ref)
+        print("60 dB case MFMA mode %d: rel-L2 %.3e, worst tone bin %.3e" % (mode, err, tone_err(out)))
+        assert err <= fp32_tol(1024) and err <= 2.0 * e32
+        assert tone_err(out) < 1e-4
+
+
+@pytest.mark.parametrize("n", [4096, 3000])
+def test_real_round_trip_full_batch(cuda, n):
+    import torch
+    b = 8192
+    g = torch.Generator().manual_seed(1)
+    xr = torch.randn(b, n, generator=g)
+    ref = oracle.r2c(xr.numpy())
+    xd = xr.cuda()
+    X = torch.empty(b, n // 2 + 1, dtype=torch.complex64, device="cuda")
+    xb = torch.empty_like(xd)
+    pr = aclfft.Plan.plan1d(n, aclfft.R2C, b)
+    pc = aclfft.Plan.plan1d(n, aclfft.C2R, b)
+    pc.set_scale(1.0 / n)
+    print(pr.describe(), "|", pc.describe(), flush=True)
+    pr.exec_r2c(xd, X)
+    pc.exec_c2r(X, xb)
+    torch.cuda.synchronize()
+    Xh = X.cpu().numpy()
+    err = oracle.rel_l2(Xh, ref)
+    print("R2C %d x %d: rel-L2 %.3e" % (n, b, err))
+    assert err <= fp32_tol(n)
+    verdict(Xh, ref, 1e-2, 1e-3)
+    # the inverse of the GPU's own spectrum against the oracle's inverse of it, then the round trip
+    back = xb.cpu().numpy()
+    ref_back = oracle.c2r(Xh.astype(np.complex128), n) / n
+    err = oracle.rel_l2(back, ref_back)
+    print("C2R %d x %d (1/N): rel-L2 %.3e" % (n, b, err))
+    assert err <= fp32_tol(n)
+    verdict(back, ref_back, 1e-3, 1e-3)
+    rt = oracle.rel_l2(back, xr.numpy())
+    print("round trip rel-L2 %.3e" % rt)
+    assert rt <= 2 * fp32_tol(n)
+    pr.destroy()
+    pc.destroy()
+
+
+def uneven_workload(count=512, seed=4, lengths=(60, 128, 243, 1000, 1024, 2048, 4096, 6561)):
+    rng = np.random.default_rng(seed)
+    logs = np.log(np.array(lengths, float))
+    u = rng.uniform(logs.min(), logs.max(), count)
+    ln = [int(lengths[int(np.argmin(np.abs(logs - v)))]) for v in u]
+    bt = [int(b) for b in rng.integers(1, 65, count)]
+    return ln, bt
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_uneven_512_groups_every_group(cuda, graph):
+    import torch
+    ln, bt = uneven_workload()
+    total = sum(n * b for n, b in zip(ln, bt))
+    rng = np.random.default_rng(5)
+    x = (rng.uniform(-1, 1, total) + 1j * rng.uniform(-1, 1, total)).astype(np.complex64)
+    xd = torch.from_numpy(x).cuda()
+    yd = torch.empty_like(xd)
+    p = aclfft.Plan.plan_many_uneven(ln, bt)
+    p.set_graph_mode(graph)
+    print(p.describe(), flush=True)
+    for rep in range(2 if graph else 1):  # a replayed graph gives the same result every time
+        yd.zero_()
+        p.exec_c2c(xd, yd, aclfft.FORWARD)
+        torch.cuda.synchronize()
+        out = yd.cpu().numpy()
+        off = 0
+        worst = 0.0
+        for n, b in zip(ln, bt):
+            seg = x[off:off + n * b].reshape(b, n)
+            ref = oracle.c2c(seg)
+            got = out[off:off + n * b].reshape(b, n)
+            err = oracle.rel_l2(got, ref)
+            worst = max(worst, err)
+            assert err <= fp32_tol(n), (n, b, err)
+            verdict(got, ref, 1e-2, 1e-3)
+            off += n * b
+        print("512 groups (graph=%s, replay %d): worst rel-L2 %.3e" % (graph, rep, worst))
+    p.destroy()
