@@ -197,6 +197,24 @@ OFX_API int ofx_gen_mfma_consts(int mode, void* table, hipStream_t stream);
  * of the current one). */
 OFX_API int ofx_launch_mfma_c2c(int mode, int n, const ofx_args* args, int blocks, hipStream_t stream);
 
+/* Registry of the MFMA C2C kernels: n = r1 * r2 as two dense DFT stages (r1, r2 in {16, 32, 64}).
+ * Entry 0 is the n = 1024 kernel above (shape 32: v_mfma_f32_32x32x16); the others run on
+ * v_mfma_f32_16x16x32 (shape 16).  Same arguments as ofx_launch_mfma_c2c: contiguous transforms,
+ * args->tw = the entry's constant table for `mode` (ofx_gen_mfma_kernel_consts, bytes from
+ * ofx_mfma_kernel_const_bytes), args->tw_aux = W_n^k for k < n (ofx_gen_roots); blocks = grid
+ * (0 = default).  No reference counterpart beyond the matrix-DFT stage shapes of its Cube kernels. */
+typedef struct ofx_mfma_info {
+    int n;
+    int r1;      /* stage-2 DFT length (n = n1 + r1 * n2)                                         */
+    int r2;      /* stage-1 DFT length                                                            */
+    int shape;   /* 32 or 16: the MFMA tile                                                       */
+} ofx_mfma_info;
+OFX_API int ofx_mfma_kernel_count(void);
+OFX_API int ofx_mfma_kernel_info_get(int id, ofx_mfma_info* info);
+OFX_API long long ofx_mfma_kernel_const_bytes(int id, int mode);
+OFX_API int ofx_gen_mfma_kernel_consts(int id, int mode, void* table, hipStream_t stream);
+OFX_API int ofx_launch_mfma_kernel(int id, int mode, const ofx_args* args, int blocks, hipStream_t stream);
+
 /* Persistent batched C2C over uneven lengths: compiled length classes. */
 enum ofx_batch_class {
     OFX_BCLASS_60 = 0, OFX_BCLASS_128, OFX_BCLASS_243, OFX_BCLASS_256, OFX_BCLASS_512, OFX_BCLASS_1000,

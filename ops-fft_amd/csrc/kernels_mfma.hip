@@ -34,54 +34,11 @@
 
 #include "ofx_launch.h"
 #include "codelets.h"
+#include "mfma_split.h"
 
 namespace ofx {
 
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float floatx16 __attribute__((ext_vector_type(16)));
-
 constexpr int kMfmaBlock = 256;  // 4 waves, one transform per wave at a time
-
-// number of narrow parts per operand value
-template <int MODE>
-struct SplitTraits;
-template <>
-struct SplitTraits<OFX_MFMA_BF16> {
-    static constexpr int P = 1;
-    typedef bf16x8 vec;
-};
-template <>
-struct SplitTraits<OFX_MFMA_FP16X2> {
-    static constexpr int P = 2;
-    typedef half8 vec;
-};
-template <>
-struct SplitTraits<OFX_MFMA_BF16X3> {
-    static constexpr int P = 3;
-    typedef bf16x8 vec;
-};
-
-template <int MODE>
-__device__ __forceinline__ void split8(const float (&v)[8], typename SplitTraits<MODE>::vec (&out)[SplitTraits<MODE>::P]) {
-    constexpr int P = SplitTraits<MODE>::P;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        float rem = v[j];
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            if constexpr (MODE == OFX_MFMA_FP16X2) {
-                const _Float16 q = (_Float16)rem;
-                out[p][j] = q;
-                rem -= (float)q;
-            } else {
-                const __bf16 q = (__bf16)rem;
-                out[p][j] = q;
-                rem -= (float)q;
-            }
-        }
-    }
-}
 
 template <int MODE>
 __device__ __forceinline__ floatx16 mfma(typename SplitTraits<MODE>::vec a, typename SplitTraits<MODE>::vec b,
@@ -107,11 +64,6 @@ __device__ __forceinline__ floatx16 mfma_split(const typename SplitTraits<MODE>:
     return acc;
 }
 
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
 
 // Constant-operand table, generated at plan time in lane order:
 //   index ((((stage * 2 + tile) * 4 + kstep) * P + part) * 64 + lane) of 16-byte vectors

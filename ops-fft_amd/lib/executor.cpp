@@ -608,25 +608,89 @@ struct Builder {
         return ACLFFT_SUCCESS;
     }
 
-    // MFMA split-precision C2C (dense 32-point DFT stages); only n = 1024 has this path.
-    aclfftResult c2c_mfma(long long n, long long batch, int mode) {
-        if (n != 1024 || ofx_mfma_const_bytes(mode) <= 0) return ACLFFT_NOT_SUPPORTED;
+    // MFMA split-precision C2C: n = r1 * r2 as two dense DFT stages (r1, r2 in {16, 32, 64}; the
+    // registry's lengths 256 ... 4096).  Every registry entry of length n is a candidate (the
+    // autotuner times them); the first one is the default (for 1024 the 32 x 32 kernel on
+    // v_mfma_f32_32x32x16).  ACLFFT_MFMA_VARIANT picks another candidate.
+    static std::vector<int> mfma_kernels_for(long long n) {
+        std::vector<int> ids;
+        for (int i = 0; i < ofx_mfma_kernel_count(); ++i) {
+            ofx_mfma_info mi;
+            if (ofx_mfma_kernel_info_get(i, &mi) == 0 && mi.n == n) ids.push_back(i);
+        }
+        return ids;
+    }
+    int mfma_table(int id, int mode) {
+        TableSpec t;
+        t.kind = TableSpec::MFMA_CONST;
+        t.npass = mode;
+        t.radices[0] = id;
+        t.bytes = (size_t)ofx_mfma_kernel_const_bytes(id, mode);
+        return e->add_table(t);
+    }
+    aclfftResult c2c_mfma(long long n, long long batch, int mode, int src = BUF_IN, int dst = BUF_OUT,
+                          bool conj = true) {
+        const std::vector<int> ids = mfma_kernels_for(n);
+        if (ids.empty() || ofx_mfma_kernel_const_bytes(ids[0], mode) <= 0) return ACLFFT_NOT_SUPPORTED;
         Step s;
         s.type = Step::MFMA;
         s.n = (int)n;
         s.mfma_mode = mode;
-        TableSpec t;
-        t.kind = TableSpec::MFMA_CONST;
-        t.npass = mode;
-        t.bytes = (size_t)ofx_mfma_const_bytes(mode);
-        s.tw_table = e->add_table(t);
+        s.candidates = ids;
+        for (int id : ids) s.candidate_tables.push_back(mfma_table(id, mode));
+        const int pick = std::min<int>(std::max(0, env_int("ACLFFT_MFMA_VARIANT", 0)), (int)ids.size() - 1);
+        s.kernel = ids[pick];
+        s.tw_table = s.candidate_tables[pick];
         s.aux_table = roots_table(n, n);
+        s.src = src;
+        s.dst = dst;
         s.args.batch = batch;
         s.args.idist = n;
         s.args.odist = n;
         s.args.in_sign = s.args.out_sign = s.args.scale = 1.f;
-        s.conj_in = s.conj_out = true;
+        s.conj_in = s.conj_out = conj;
         e->steps_.push_back(s);
+        return ACLFFT_SUCCESS;
+    }
+
+    // Real transforms in an MFMA precision (even n, n/2 with an MFMA kernel): R2C as the n/2-point
+    // MFMA transform of the real samples viewed as complex, into the workspace, then the Hermitian
+    // unpack; C2R as the pre-pack into the workspace (conjugated), then the forward MFMA transform
+    // with a conjugated output.
+    aclfftResult r2c_mfma(long long n, long long batch, int mode) {
+        const long long m = n / 2;
+        if (n % 2 || mfma_kernels_for(m).empty()) return ACLFFT_NOT_SUPPORTED;
+        e->in_place_unsafe_ = true;
+        aclfftResult r = c2c_mfma(m, batch, mode, BUF_IN, BUF_WORK1, false);
+        if (r != ACLFFT_SUCCESS) return r;
+        Step u;
+        u.type = Step::R2C_UNPACK;
+        u.src = BUF_WORK1;
+        u.dst = BUF_OUT;
+        u.m = m;
+        u.args.batch = batch;
+        u.args.scale = 1.f;
+        u.aux_table = roots_table(n, n / 2 + 1);
+        e->steps_.push_back(u);
+        e->work1_bytes_ = std::max(e->work1_bytes_, (size_t)(m * batch * 8));
+        return ACLFFT_SUCCESS;
+    }
+    aclfftResult c2r_mfma(long long n, long long batch, int mode) {
+        const long long m = n / 2;
+        if (n % 2 || mfma_kernels_for(m).empty()) return ACLFFT_NOT_SUPPORTED;
+        e->in_place_unsafe_ = true;
+        Step pk;
+        pk.type = Step::C2R_PACK;
+        pk.src = BUF_IN;
+        pk.dst = BUF_WORK1;
+        pk.m = m;
+        pk.args.batch = batch;
+        pk.aux_table = roots_table(n, n / 2 + 1);
+        e->steps_.push_back(pk);
+        e->work1_bytes_ = std::max(e->work1_bytes_, (size_t)(m * batch * 8));
+        aclfftResult r = c2c_mfma(m, batch, mode, BUF_WORK1, BUF_OUT, false);
+        if (r != ACLFFT_SUCCESS) return r;
+        e->steps_.back().args.out_sign = -1.f;
         return ACLFFT_SUCCESS;
     }
 
@@ -1053,7 +1117,9 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
     e->desc_ = d;
     Builder b{e};
     aclfftResult r = ACLFFT_NOT_IMPLEMENTED;
-    if (d.precision != 0 && (d.uneven || !(d.rank == 1 && !d.vertical && d.type == ACLFFT_C2C))) {
+    if (d.precision != 0 &&
+        (d.uneven || !(d.rank == 1 && !d.vertical &&
+                       (d.type == ACLFFT_C2C || d.type == ACLFFT_R2C || d.type == ACLFFT_C2R)))) {
         delete e;
         return ACLFFT_NOT_SUPPORTED;
     }
@@ -1072,12 +1138,12 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
             case ACLFFT_R2C:
                 e->in_bytes_ = (size_t)n * d.batch * 4;
                 e->out_bytes_ = (size_t)(n / 2 + 1) * d.batch * 8;
-                r = b.r2c(n, d.batch);
+                r = d.precision ? b.r2c_mfma(n, d.batch, d.precision) : b.r2c(n, d.batch);
                 break;
             case ACLFFT_C2R:
                 e->in_bytes_ = (size_t)(n / 2 + 1) * d.batch * 8;
                 e->out_bytes_ = (size_t)n * d.batch * 4;
-                r = b.c2r(n, d.batch);
+                r = d.precision ? b.c2r_mfma(n, d.batch, d.precision) : b.c2r(n, d.batch);
                 break;
             case ACLFFT_Z2Z:
                 e->in_bytes_ = e->out_bytes_ = (size_t)n * d.batch * 16;
@@ -1283,7 +1349,10 @@ std::string Executor::describe() const {
             continue;
         }
         if (s.type == Step::MFMA) {
-            os << "mfma n=" << s.n << " mode=" << (s.mfma_mode == OFX_MFMA_BF16 ? "bf16" : s.mfma_mode == OFX_MFMA_FP16X2 ? "fp16x2" : "bf16x3");
+            ofx_mfma_info mi{};
+            ofx_mfma_kernel_info_get(s.kernel, &mi);
+            os << "mfma n=" << s.n << " " << mi.r1 << "x" << mi.r2 << " shape=" << mi.shape
+               << " mode=" << (s.mfma_mode == OFX_MFMA_BF16 ? "bf16" : s.mfma_mode == OFX_MFMA_FP16X2 ? "fp16x2" : "bf16x3");
             continue;
         }
         if (s.type == Step::GENERIC) {
@@ -1331,7 +1400,7 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
         else if (t.kind == TableSpec::ROOTS)
             rc = ofx_gen_roots(p, t.len, t.count, s);
         else if (t.kind == TableSpec::MFMA_CONST)
-            rc = ofx_gen_mfma_consts(t.npass, p, s);
+            rc = ofx_gen_mfma_kernel_consts(t.radices[0], t.npass, p, s);
         else if (t.kind == TableSpec::PASS_Z)
             rc = ofx_gen_pass_twiddles_z(p, t.npass, t.radices, s);
         else if (t.kind == TableSpec::ROOTS_Z)
@@ -1470,7 +1539,7 @@ aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream
                 a.tw_aux = table(st.aux_table);
                 if (st.conj_in) a.in_sign *= dsign;
                 if (st.conj_out) a.out_sign *= dsign;
-                err = (hipError_t)ofx_launch_mfma_c2c(st.mfma_mode, st.n, &a, env_int("ACLFFT_MFMA_BLOCKS", 0), s);
+                err = (hipError_t)ofx_launch_mfma_kernel(st.kernel, st.mfma_mode, &a, env_int("ACLFFT_MFMA_BLOCKS", 0), s);
                 break;
             }
             case Step::GENERIC: {
@@ -1614,7 +1683,8 @@ aclfftResult Executor::run_graph(const void* din, void* dout, int direction, hip
 aclfftResult Executor::autotune(hipStream_t s) {
     bool any = false;
     for (const Step& st : steps_)
-        if ((st.type == Step::KERNEL || st.type == Step::COLUMN) && st.candidates.size() > 1) any = true;
+        if ((st.type == Step::KERNEL || st.type == Step::COLUMN || st.type == Step::MFMA) && st.candidates.size() > 1)
+            any = true;
     if (!any) return ACLFFT_SUCCESS;
     void* a = nullptr;
     void* b = nullptr;
@@ -1631,8 +1701,10 @@ aclfftResult Executor::autotune(hipStream_t s) {
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     for (Step& st : steps_) {
-        if ((st.type != Step::KERNEL && st.type != Step::COLUMN) || st.candidates.size() < 2) continue;
+        if ((st.type != Step::KERNEL && st.type != Step::COLUMN && st.type != Step::MFMA) || st.candidates.size() < 2)
+            continue;
         auto launch_cand = [&](int id, const ofx_args* a) {
+            if (st.type == Step::MFMA) return ofx_launch_mfma_kernel(id, st.mfma_mode, a, 0, s);
             return st.type == Step::COLUMN ? ofx_launch_col(id, a, s) : ofx_launch(id, a, s);
         };
         float best = 1e30f;

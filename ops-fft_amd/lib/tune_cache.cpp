@@ -78,16 +78,23 @@ const ofx_kernel_info* entry_of(const Step& st, int id, const Registries& reg) {
     return nullptr;
 }
 
-// the LDS footprint tells apart configurations that differ only in their LDS layout (the column
-// kernels' split exchange, the row kernels' pad shift)
-std::string step_signature(const Step& st, const ofx_kernel_info& k) {
-    std::string s = kernel_signature(k);
-    s += (st.type == Step::COLUMN ? ":c" : ":l") + std::to_string(k.lds_bytes);
-    return s;
+// Signature of candidate `id` of a step ("" when it names no entry).  The LDS footprint tells
+// apart configurations that differ only in their LDS layout (the column kernels' split exchange,
+// the row kernels' pad shift); MFMA steps are named by their factorisation and MFMA tile.
+std::string candidate_signature(const Step& st, int id, const Registries& reg) {
+    if (st.type == Step::MFMA) {
+        ofx_mfma_info mi;
+        if (ofx_mfma_kernel_info_get(id, &mi) != 0) return std::string();
+        return "mfma:" + std::to_string(mi.n) + ":" + std::to_string(mi.r1) + "x" + std::to_string(mi.r2) + ":" +
+               std::to_string(mi.shape) + ":m" + std::to_string(st.mfma_mode);
+    }
+    const ofx_kernel_info* k = entry_of(st, id, reg);
+    if (k == nullptr) return std::string();
+    return kernel_signature(*k) + (st.type == Step::COLUMN ? ":c" : ":l") + std::to_string(k->lds_bytes);
 }
 
 bool tunable(const Step& st) {
-    return (st.type == Step::KERNEL || st.type == Step::COLUMN) && st.candidates.size() >= 2;
+    return (st.type == Step::KERNEL || st.type == Step::COLUMN || st.type == Step::MFMA) && st.candidates.size() >= 2;
 }
 
 }  // namespace
@@ -102,8 +109,8 @@ void apply_tune_cache(const PlanDesc& d, std::vector<Step>& steps, const Registr
         auto it = m.find(key_of(d, i));
         if (it == m.end()) continue;
         for (size_t c = 0; c < st.candidates.size(); ++c) {
-            const ofx_kernel_info* k = entry_of(st, st.candidates[c], reg);
-            if (k != nullptr && step_signature(st, *k) == it->second) {
+            const std::string sig = candidate_signature(st, st.candidates[c], reg);
+            if (!sig.empty() && sig == it->second) {
                 st.kernel = st.candidates[c];
                 st.tw_table = st.candidate_tables[c];
                 break;
@@ -117,8 +124,8 @@ void store_tune_cache(const PlanDesc& d, const std::vector<Step>& steps, const R
     auto m = load();
     for (size_t i = 0; i < steps.size(); ++i) {
         if (!tunable(steps[i])) continue;
-        const ofx_kernel_info* k = entry_of(steps[i], steps[i].kernel, reg);
-        if (k != nullptr) m[key_of(d, i)] = step_signature(steps[i], *k);
+        const std::string sig = candidate_signature(steps[i], steps[i].kernel, reg);
+        if (!sig.empty()) m[key_of(d, i)] = sig;
     }
     const std::string path = cache_path();
     // create the directory chain of the location (none for a bare file name)

@@ -1,25 +1,52 @@
 """GPU: the MFMA split-precision C2C path (aclfftExtSetPrecision) against the FP64 oracle.
 
-Bars (the precision workload, C2C n=1024 with the 60 dB case): the FP16 hi/lo and BF16 hi/mid/lo paths must be no worse than
-twice the FP32 Stockham path's error against FP64 and within rel-L2 1e-6; single-BF16 is reported
-(about 3 significant digits) with a loose bound.
+Every MFMA kernel of the registry (n = r1 * r2 with 16-, 32- and 64-point dense DFT stages:
+256 = 16x16, 512 = 16x32 / 32x16, 1024 = 32x32 on both MFMA tiles / 16x64 / 64x16, 2048 = 32x64 /
+64x32, 4096 = 64x64), forced one by one with ACLFFT_MFMA_VARIANT, in every operand precision.
+
+Bars (the precision workload): the FP16 hi/lo and BF16 hi/mid/lo paths must be no worse than twice
+the FP32 Stockham path's error against FP64 and within rel-L2 1e-6 * log2(n); single-BF16 is
+reported (about 3 significant digits) with a loose bound.  The 60 dB dynamic-range case (8 tones
+at amplitudes 1 and 1e-3 plus 1e-6 noise) is held to the same bars.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
 import aclfft
 from oracle import oracle
-from _util import crandn
+from _util import crandn, fp32_tol
 
 pytestmark = pytest.mark.gpu
 
-N = 1024
+MODES = (aclfft.PRECISION_BF16, aclfft.PRECISION_FP16X2, aclfft.PRECISION_BF16X3)
+
+
+class MfmaInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("r1", ctypes.c_int), ("r2", ctypes.c_int), ("shape", ctypes.c_int)]
+
+
+def mfma_variants():
+    """(n, variant index among the entries of length n, r1, r2, shape) of every registry entry."""
+    lib = aclfft.lib
+    out, seen = [], {}
+    for i in range(lib.ofx_mfma_kernel_count()):
+        mi = MfmaInfo()
+        assert lib.ofx_mfma_kernel_info_get(i, ctypes.byref(mi)) == 0
+        k = seen.get(mi.n, 0)
+        seen[mi.n] = k + 1
+        out.append((mi.n, k, mi.r1, mi.r2, mi.shape))
+    return out
+
+
+VARIANTS = mfma_variants()
 
 
 def run(torch, x, mode, direction=aclfft.FORWARD):
     xt = torch.from_numpy(x.astype(np.complex64)).cuda()
     yt = torch.empty_like(xt)
-    p = aclfft.Plan.plan1d(N, aclfft.C2C, x.shape[0])
+    p = aclfft.Plan.plan1d(x.shape[1], aclfft.C2C, x.shape[0])
     if mode:
         p.set_precision(mode)
     p.exec_c2c(xt, yt, direction)
@@ -29,67 +56,126 @@ def run(torch, x, mode, direction=aclfft.FORWARD):
     return yt.cpu().numpy(), d
 
 
-def sixty_db(batch, seed):
+def sixty_db(batch, n, seed):
     """Sum of 8 random-frequency complex tones (amplitudes 1 x4, 1e-3 x4) + 1e-6 complex noise."""
     rng = np.random.default_rng(seed)
-    n = np.arange(N)
-    x = np.zeros((batch, N), complex)
+    t = np.arange(n)
+    x = np.zeros((batch, n), complex)
     for b in range(batch):
-        f = rng.choice(N, 8, replace=False)
+        f = rng.choice(n, 8, replace=False)
         amp = np.array([1, 1, 1, 1, 1e-3, 1e-3, 1e-3, 1e-3])
         ph = rng.uniform(0, 2 * np.pi, 8)
-        x[b] = (amp * np.exp(1j * (2 * np.pi * np.outer(n, f) / N + ph))).sum(1)
-    return x + 1e-6 * crandn((batch, N), seed + 1)
+        x[b] = (amp * np.exp(1j * (2 * np.pi * np.outer(t, f) / n + ph))).sum(1)
+    return x + 1e-6 * crandn((batch, n), seed + 1)
 
 
+def test_registry_covers_the_reference_stage_shapes():
+    shapes = {(n, r1, r2) for n, _, r1, r2, _ in VARIANTS}
+    # the radix pairs of the reference's matrix-DFT kernel (fft_b/fft1_d_b.cpp:27-48), in both orders
+    for n, a, b in ((256, 16, 16), (512, 16, 32), (1024, 16, 64), (2048, 32, 64), (4096, 64, 64)):
+        assert (n, a, b) in shapes or (n, b, a) in shapes, (n, a, b)
+
+
+@pytest.mark.parametrize("n,variant,r1,r2,shape", VARIANTS)
 @pytest.mark.parametrize("direction", [aclfft.FORWARD, aclfft.BACKWARD])
-def test_mfma_modes_vs_fp64(cuda, direction):
+def test_mfma_kernels_vs_fp64(cuda, monkeypatch, n, variant, r1, r2, shape, direction):
     import torch
-    x = crandn((300, N), 11).astype(np.complex64)
+    monkeypatch.setenv("ACLFFT_MFMA_VARIANT", str(variant))
+    x = crandn((37, n), 11 + n).astype(np.complex64)
     ref = oracle.c2c(x.astype(np.complex128), direction)
-    y32, d32 = run(torch, x, 0, direction)
+    y32, _ = run(torch, x, 0, direction)
     e32 = oracle.rel_l2(y32, ref)
     errs = {}
-    for mode in (aclfft.PRECISION_BF16, aclfft.PRECISION_FP16X2, aclfft.PRECISION_BF16X3):
+    for mode in MODES:
         y, d = run(torch, x, mode, direction)
-        assert d.startswith("mfma"), d
+        assert d.startswith("mfma n=%d %dx%d shape=%d" % (n, r1, r2, shape)), d
         errs[mode] = oracle.rel_l2(y, ref)
-    print("rel_l2 vs FP64: fp32 %.3e  bf16 %.3e  fp16x2 %.3e  bf16x3 %.3e" % (
-        e32, errs[1], errs[2], errs[3]))
+    print("n=%d %dx%d/%d rel_l2 vs FP64: fp32 %.3e  bf16 %.3e  fp16x2 %.3e  bf16x3 %.3e" % (
+        n, r1, r2, shape, e32, errs[1], errs[2], errs[3]))
     assert errs[aclfft.PRECISION_BF16] < 2e-2
     for mode in (aclfft.PRECISION_FP16X2, aclfft.PRECISION_BF16X3):
         assert errs[mode] <= max(2 * e32, 1e-7), (mode, errs[mode], e32)
-        assert errs[mode] <= 1e-6
+        assert errs[mode] <= fp32_tol(n)
 
 
-def test_mfma_sixty_db(cuda):
+@pytest.mark.parametrize("n", sorted({v[0] for v in VARIANTS}))
+def test_mfma_sixty_db(cuda, n):
     import torch
-    x = sixty_db(64, 3).astype(np.complex64)
+    x = sixty_db(16, n, 3).astype(np.complex64)
     ref = oracle.c2c(x.astype(np.complex128))
     e32 = oracle.rel_l2(run(torch, x, 0)[0], ref)
     for mode in (aclfft.PRECISION_FP16X2, aclfft.PRECISION_BF16X3):
         y, _ = run(torch, x, mode)
         e = oracle.rel_l2(y, ref)
-        print("60 dB: mode %d rel_l2 %.3e (fp32 %.3e)" % (mode, e, e32))
+        print("60 dB n=%d: mode %d rel_l2 %.3e (fp32 %.3e)" % (n, mode, e, e32))
         assert e <= max(2 * e32, 1e-7)
 
 
-def test_mfma_ragged_batch_and_in_place(cuda):
+@pytest.mark.parametrize("n", [256, 1024, 4096])
+def test_mfma_ragged_batch_and_in_place(cuda, n):
     import torch
     for batch in (1, 3, 5, 4099):
-        x = crandn((batch, N), batch).astype(np.complex64)
+        x = crandn((batch, n), batch).astype(np.complex64)
         xt = torch.from_numpy(x).cuda()
-        p = aclfft.Plan.plan1d(N, aclfft.C2C, batch)
+        p = aclfft.Plan.plan1d(n, aclfft.C2C, batch)
         p.set_precision(aclfft.PRECISION_FP16X2)
         p.exec_c2c(xt, xt, aclfft.FORWARD)
         torch.cuda.synchronize()
-        assert oracle.rel_l2(xt.cpu().numpy(), oracle.c2c(x.astype(np.complex128))) < 1e-6
+        assert oracle.rel_l2(xt.cpu().numpy(), oracle.c2c(x.astype(np.complex128))) < fp32_tol(n)
         p.destroy()
 
 
+def test_mfma_extreme_magnitudes(cuda):
+    """FP16 operands are formed from data scaled per transform by a power of two: transforms of
+    magnitude 1e-30 and 1e30 next to ordinary ones keep their accuracy; zero transforms stay zero."""
+    import torch
+    n = 2048
+    x = crandn((6, n), 41)
+    x[1] *= 1e-30
+    x[2] *= 1e30
+    x[3] = 0
+    x[4] *= 1e-20
+    x = x.astype(np.complex64)
+    ref = oracle.c2c(x.astype(np.complex128))
+    for mode in (aclfft.PRECISION_FP16X2, aclfft.PRECISION_BF16X3):
+        y, _ = run(torch, x, mode)
+        assert np.all(y[3] == 0)
+        for b in (0, 1, 2, 4, 5):
+            assert oracle.rel_l2(y[b], ref[b]) <= fp32_tol(n), (mode, b)
+
+
+@pytest.mark.parametrize("n", [512, 2048, 4096, 8192])
+def test_mfma_real_transforms(cuda, n):
+    """R2C / C2R in an MFMA precision: the n/2-point MFMA transform with the Hermitian unpack /
+    pre-pack kernels."""
+    import torch
+    b = 9
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal((b, n)).astype(np.float32)
+    xt = torch.from_numpy(x).cuda()
+    X = torch.empty(b, n // 2 + 1, dtype=torch.complex64, device="cuda")
+    back = torch.empty_like(xt)
+    ref = oracle.r2c(x.astype(np.float64))
+    for mode in (aclfft.PRECISION_FP16X2, aclfft.PRECISION_BF16X3):
+        pr = aclfft.Plan.plan1d(n, aclfft.R2C, b)
+        pc = aclfft.Plan.plan1d(n, aclfft.C2R, b)
+        pr.set_precision(mode)
+        pc.set_precision(mode)
+        assert "mfma" in pr.describe() and "mfma" in pc.describe()
+        pr.exec_r2c(xt, X)
+        pc.exec_c2r(X, back)
+        torch.cuda.synchronize()
+        Xh = X.cpu().numpy()
+        assert oracle.rel_l2(Xh, ref) <= fp32_tol(n)
+        assert oracle.rel_l2(back.cpu().numpy(), oracle.c2r(Xh.astype(np.complex128), n)) <= fp32_tol(n)
+        pr.destroy()
+        pc.destroy()
+
+
 def test_mfma_precision_gate(cuda):
-    p = aclfft.Plan.plan1d(512, aclfft.C2C, 4)
+    p = aclfft.Plan.plan1d(1000, aclfft.C2C, 4)
     with pytest.raises(aclfft.AclfftError) as e:
         p.set_precision(aclfft.PRECISION_FP16X2)
     assert e.value.code == aclfft.NOT_SUPPORTED
     assert p.describe().startswith("kind0")
+    p.destroy()
