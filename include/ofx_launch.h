@@ -208,11 +208,15 @@ typedef struct ofx_mfma_info {
     int r1;      /* stage-2 DFT length (n = n1 + r1 * n2)                                         */
     int r2;      /* stage-1 DFT length                                                            */
     int shape;   /* 32 or 16: the MFMA tile                                                       */
+    int tiled;   /* 1: the input is read one stage-1 row tile at a time (fewer registers)          */
 } ofx_mfma_info;
 OFX_API int ofx_mfma_kernel_count(void);
 OFX_API int ofx_mfma_kernel_info_get(int id, ofx_mfma_info* info);
 OFX_API long long ofx_mfma_kernel_const_bytes(int id, int mode);
 OFX_API int ofx_gen_mfma_kernel_consts(int id, int mode, void* table, hipStream_t stream);
+/* With args->tiles set (entries other than 0): a tile-list launch for uneven batches, `batch` = the
+ * number of ofx_batch_tile entries, workgroup b transforms tiles[b].count contiguous transforms at
+ * complex64 offset tiles[b].offset of in and out (offsets may be odd). */
 OFX_API int ofx_launch_mfma_kernel(int id, int mode, const ofx_args* args, int blocks, hipStream_t stream);
 
 /* Persistent batched C2C over uneven lengths: compiled length classes. */

@@ -20,8 +20,12 @@ ACLFFT_API aclfftResult aclfftExtDescribe(aclfftHandle plan, char* buf, size_t l
  *   ACLFFT_EXT_PRECISION_BF16     (1) MFMA dense-DFT stages, one BF16 part per operand
  *   ACLFFT_EXT_PRECISION_FP16X2   (2) MFMA, FP16 hi + lo split operands (about FP32 accuracy)
  *   ACLFFT_EXT_PRECISION_BF16X3   (3) MFMA, BF16 hi + mid + lo split operands
- * The MFMA modes exist for 1D C2C plans of length 1024; other plans return
- * ACLFFT_NOT_SUPPORTED and keep their current precision.  Rebuilds the plan's launch list. */
+ * The MFMA modes run n = r1 * r2 as two dense DFT stages of 16, 32 or 64 points: 1D C2C plans of
+ * length 256, 512, 1024, 2048 or 4096; 1D R2C / C2R plans of length 512, 1024, 2048, 4096 or 8192
+ * (the half-length complex transform on MFMA, the Hermitian unpack / pre-pack in FP32); uneven
+ * plans (aclfftExtPlanManyUneven) with at least one such length (its groups run on MFMA, the
+ * other lengths stay on the FP32 kernels).  Other plans return ACLFFT_NOT_SUPPORTED and keep their
+ * current precision.  Rebuilds the plan's launch list. */
 #define ACLFFT_EXT_PRECISION_FP32 0
 #define ACLFFT_EXT_PRECISION_BF16 1
 #define ACLFFT_EXT_PRECISION_FP16X2 2

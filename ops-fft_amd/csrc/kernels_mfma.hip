@@ -78,7 +78,7 @@ __device__ __forceinline__ void mfma_transform(const float2 (&v)[16], float2* __
 #pragma unroll
     for (int q = 0; q < 16; ++q) m = fmaxf(m, fmaxf(fabsf(v[q].x), fabsf(v[q].y)));
     m = wave_max(m);
-    const float s1 = (m > 0.f && m < 3.0e38f) ? ldexpf(1.f, 11 - ilogbf(m)) : 1.f;
+    const float s1 = fp16_scale<OFX_MFMA_FP16X2>(m);  // every mode: the power-of-two scale is exact
     const float si = s1 * in_sign;
 
     // ---- stage 1 ------------------------------------------------------------------------------

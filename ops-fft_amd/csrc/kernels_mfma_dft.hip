@@ -133,6 +133,9 @@ __device__ __forceinline__ void dft_transform(const float2 (&v)[SH::S2][SH::S1][
 
 #pragma unroll
     for (int mt = 0; mt < S2; ++mt) {
+        // the constant operands are re-read from LDS for every row tile instead of being held in
+        // VGPRs across tiles (for 64-point stages they would take 128 VGPRs)
+        __asm__ volatile("" ::: "memory");
         // ---- stage 1, rows n1 = 16 mt + (0..15) ---------------------------------------------
         floatx4 acc1[S1][2];
 #pragma unroll
@@ -209,6 +212,152 @@ __device__ __forceinline__ void dft_transform(const float2 (&v)[SH::S2][SH::S1][
             }
 }
 
+// Tiled form: the transform is read one stage-1 row tile (16 rows n1, all R2 columns: R2/4
+// complex values per lane) at a time, the next tile's loads issued before the current tile's MFMA
+// work, so only one tile of input is held in registers besides the stage-2 accumulators.  FP16
+// operands then take a power-of-two scale per tile; stage 2 keeps one running scale (the smallest
+// so far): a tile with a smaller scale rescales the accumulators, one with a larger scale has its
+// stage-2 input scaled down to the running one.  Every scale is a power of two, so this is exact.
+template <class SH, bool NT>
+__device__ __forceinline__ void dft_load_tile(float2 (&v)[SH::S1][4], const float2* __restrict__ x, int mt, int r,
+                                              int g) {
+#pragma unroll
+    for (int s = 0; s < SH::S1; ++s)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[s][q] = ldx<NT>(x + (16 * mt + r) + SH::R1 * (16 * s + 4 * g + q));
+}
+
+template <int MODE, class SH, bool NT>
+__device__ __forceinline__ void dft_transform_tiled(const float2* __restrict__ x, float2* __restrict__ y,
+                                                    const typename SplitTraits<MODE>::vec* cst, const float2* tw,
+                                                    int lane, int r, int g, float in_sign, float out_sign,
+                                                    float scale) {
+    typedef typename SplitTraits<MODE>::vec vec;
+    constexpr int P = SplitTraits<MODE>::P;
+    constexpr int S1 = SH::S1, S2 = SH::S2, R2 = SH::R2;
+    constexpr int NC1 = S1 * S1 * P * 64;
+    constexpr float zs = 1.f / (2 * R2);
+
+    floatx4 acc2[S2][S1][2];
+#pragma unroll
+    for (int b = 0; b < S2; ++b)
+#pragma unroll
+        for (int c = 0; c < S1; ++c) acc2[b][c][0] = acc2[b][c][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+    float srun = 1.f;  // running stage-2 scale (FP16 operands)
+
+    float2 cur[S1][4];
+    dft_load_tile<SH, NT>(cur, x, 0, r, g);
+#pragma unroll
+    for (int mt = 0; mt < S2; ++mt) {
+        __asm__ volatile("" ::: "memory");  // constants re-read per tile; loads stay one tile ahead
+        float2 nxt[S1][4];
+        if (mt + 1 < S2) dft_load_tile<SH, NT>(nxt, x, mt + 1, r, g);
+        float st = 1.f;
+        if constexpr (MODE == OFX_MFMA_FP16X2) {
+            float m = 0.f;
+#pragma unroll
+            for (int s = 0; s < S1; ++s)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) m = fmaxf(m, fmaxf(fabsf(cur[s][q].x), fabsf(cur[s][q].y)));
+            st = fp16_scale<MODE>(wave_max(m));
+        }
+        const float si = st * in_sign;
+        floatx4 acc1[S1][2];
+#pragma unroll
+        for (int c = 0; c < S1; ++c) acc1[c][0] = acc1[c][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < S1; ++s) {
+            float av[8];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                av[q] = cur[s][q].x * st;
+                av[4 + q] = cur[s][q].y * si;
+            }
+            vec ap[P];
+            split8<MODE>(av, ap);
+#pragma unroll
+            for (int c = 0; c < S1; ++c) {
+                vec bre[P], bim[P];
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    bre[p] = cst[((c * S1 + s) * P + p) * 64 + lane];
+                    bim[p] = im_tile(bre[p]);
+                }
+                acc1[c][0] = mfma16_split<MODE>(ap, bre, acc1[c][0]);
+                acc1[c][1] = mfma16_split<MODE>(ap, bim, acc1[c][1]);
+            }
+        }
+        float zt = zs;
+        if constexpr (MODE == OFX_MFMA_FP16X2) {
+            if (mt == 0) {
+                srun = st;
+            } else if (st < srun) {  // wave-uniform
+                const float ratio = st / srun;
+#pragma unroll
+                for (int b = 0; b < S2; ++b)
+#pragma unroll
+                    for (int c = 0; c < S1; ++c) {
+                        acc2[b][c][0] *= ratio;
+                        acc2[b][c][1] *= ratio;
+                    }
+                srun = st;
+            }
+            zt = zs * (srun / st);
+        }
+#pragma unroll
+        for (int c = 0; c < S1; ++c) {
+            float bv[8];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int e = (16 * mt + 4 * g + q) * (16 * c + r);
+                float2 w;
+                if constexpr (SH::TW2)
+                    w = cmul(tw[e >> 6], tw[64 + (e & 63)]);
+                else
+                    w = tw[e];
+                const float2 z = cmul(make_float2(acc1[c][0][q] * zt, acc1[c][1][q] * zt), w);
+                bv[q] = z.x;
+                bv[4 + q] = z.y;
+            }
+            vec bp[P];
+            split8<MODE>(bv, bp);
+#pragma unroll
+            for (int b = 0; b < S2; ++b) {
+                vec are[P], aim[P];
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    are[p] = cst[NC1 + ((b * S2 + mt) * P + p) * 64 + lane];
+                    aim[p] = im_tile(are[p]);
+                }
+                acc2[b][c][0] = mfma16_split<MODE>(are, bp, acc2[b][c][0]);
+                acc2[b][c][1] = mfma16_split<MODE>(aim, bp, acc2[b][c][1]);
+            }
+        }
+        if (mt + 1 < S2) {
+#pragma unroll
+            for (int s = 0; s < S1; ++s)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) cur[s][q] = nxt[s][q];
+        }
+    }
+
+    const float inv = (2.f * R2) / srun * scale;
+    const float invi = inv * out_sign;
+#pragma unroll
+    for (int b = 0; b < S2; ++b)
+#pragma unroll
+        for (int c = 0; c < S1; ++c)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float2* o = y + R2 * (16 * b + 4 * g + q) + 16 * c + r;
+                const float2 val = make_float2(acc2[b][c][0][q] * inv, acc2[b][c][1][q] * invi);
+                if constexpr (NT)
+                    st_nt(o, val);
+                else
+                    *o = val;
+            }
+}
+
 template <class SH, bool NT>
 __device__ __forceinline__ void dft_load(float2 (&v)[SH::S2][SH::S1][4], const float2* __restrict__ x, int r, int g) {
 #pragma unroll
@@ -220,7 +369,7 @@ __device__ __forceinline__ void dft_load(float2 (&v)[SH::S2][SH::S1][4], const f
 }
 
 // args: tw = constant table (ofx_gen_mfma_dft_consts), tw_aux = W_n^k for k < n (ofx_gen_roots)
-template <int MODE, int R1, int R2, bool NT>
+template <int MODE, int R1, int R2, bool NT, bool TILED>
 __global__ __launch_bounds__(kDftBlock) void k_mfma_dft(ofx_args a) {
     typedef DftShape<R1, R2> SH;
     typedef typename SplitTraits<MODE>::vec vec;
@@ -244,15 +393,31 @@ __global__ __launch_bounds__(kDftBlock) void k_mfma_dft(ofx_args a) {
     const int wave = threadIdx.x >> 6;
     const int r = lane & 15;
     const int g = lane >> 4;
-    const long long stride = (long long)gridDim.x * (kDftBlock / 64);
     const float2* __restrict__ in = static_cast<const float2*>(a.in);
     float2* __restrict__ out = static_cast<float2*>(a.out);
-    for (long long t = (long long)blockIdx.x * (kDftBlock / 64) + wave; t < a.batch; t += stride) {
+    // plain launch: the grid strides over the batch; tile-list launch (uneven batches): workgroup b
+    // runs the tile's `count` contiguous transforms at element offset `offset`, one wave each
+    long long t = (long long)blockIdx.x * (kDftBlock / 64) + wave, stride = (long long)gridDim.x * (kDftBlock / 64);
+    long long end = a.batch;
+    if (a.tiles != nullptr) {
+        const ofx_batch_tile tile = static_cast<const ofx_batch_tile*>(a.tiles)[blockIdx.x];
+        in += tile.offset;
+        out += tile.offset;
+        t = wave;
+        stride = kDftBlock / 64;
+        end = tile.count;
+    }
+    for (; t < end; t += stride) {
         // constants are re-read from LDS for every transform rather than held in VGPRs
         __asm__ volatile("" ::: "memory");
-        float2 v[SH::S2][SH::S1][4];
-        dft_load<SH, NT>(v, in + t * a.idist, r, g);
-        dft_transform<MODE, SH, NT>(v, out + t * a.odist, cst, tw, lane, r, g, a.in_sign, a.out_sign, a.scale);
+        if constexpr (TILED) {
+            dft_transform_tiled<MODE, SH, NT>(in + t * a.idist, out + t * a.odist, cst, tw, lane, r, g, a.in_sign,
+                                              a.out_sign, a.scale);
+        } else {
+            float2 v[SH::S2][SH::S1][4];
+            dft_load<SH, NT>(v, in + t * a.idist, r, g);
+            dft_transform<MODE, SH, NT>(v, out + t * a.odist, cst, tw, lane, r, g, a.in_sign, a.out_sign, a.scale);
+        }
     }
 }
 
@@ -301,12 +466,13 @@ __global__ void k_gen_dft_consts(void* out, int R1, int R2) {
 // two-stage kernels above
 // ---------------------------------------------------------------------------------------------
 struct DftEntry {
-    int n, r1, r2;
+    int n, r1, r2, tiled;
 };
 constexpr DftEntry kDft[] = {
-    {1024, 32, 32},  // 32x32x16 kernel
-    {256, 16, 16},  {512, 16, 32},  {512, 32, 16},  {1024, 32, 32}, {1024, 16, 64},
-    {1024, 64, 16}, {2048, 32, 64}, {2048, 64, 32}, {4096, 64, 64},
+    {1024, 32, 32, 0},  // 32x32x16 kernel
+    {256, 16, 16, 0},   {512, 16, 32, 0},   {512, 32, 16, 0},   {1024, 32, 32, 0},  {1024, 16, 64, 0},
+    {1024, 64, 16, 0},  {2048, 32, 64, 0},  {2048, 64, 32, 0},  {4096, 64, 64, 0},  {1024, 32, 32, 1},
+    {1024, 64, 16, 1},  {2048, 32, 64, 1},  {2048, 64, 32, 1},  {4096, 64, 64, 1},
 };
 constexpr int kDftCount = (int)(sizeof(kDft) / sizeof(kDft[0]));
 
@@ -318,18 +484,23 @@ typedef void (*dft_kernel_t)(ofx_args);
 
 template <int MODE>
 dft_kernel_t dft_kernel_for(int id, bool nt) {
-#define OFX_DFT_CASE(I, A, B) \
-    case I: return nt ? k_mfma_dft<MODE, A, B, true> : k_mfma_dft<MODE, A, B, false>
+#define OFX_DFT_CASE(I, A, B, T) \
+    case I: return nt ? k_mfma_dft<MODE, A, B, true, T> : k_mfma_dft<MODE, A, B, false, T>
     switch (id) {
-        OFX_DFT_CASE(1, 16, 16);
-        OFX_DFT_CASE(2, 16, 32);
-        OFX_DFT_CASE(3, 32, 16);
-        OFX_DFT_CASE(4, 32, 32);
-        OFX_DFT_CASE(5, 16, 64);
-        OFX_DFT_CASE(6, 64, 16);
-        OFX_DFT_CASE(7, 32, 64);
-        OFX_DFT_CASE(8, 64, 32);
-        OFX_DFT_CASE(9, 64, 64);
+        OFX_DFT_CASE(1, 16, 16, false);
+        OFX_DFT_CASE(2, 16, 32, false);
+        OFX_DFT_CASE(3, 32, 16, false);
+        OFX_DFT_CASE(4, 32, 32, false);
+        OFX_DFT_CASE(5, 16, 64, false);
+        OFX_DFT_CASE(6, 64, 16, false);
+        OFX_DFT_CASE(7, 32, 64, false);
+        OFX_DFT_CASE(8, 64, 32, false);
+        OFX_DFT_CASE(9, 64, 64, false);
+        OFX_DFT_CASE(10, 32, 32, true);
+        OFX_DFT_CASE(11, 64, 16, true);
+        OFX_DFT_CASE(12, 32, 64, true);
+        OFX_DFT_CASE(13, 64, 32, true);
+        OFX_DFT_CASE(14, 64, 64, true);
         default: return nullptr;
     }
 #undef OFX_DFT_CASE
@@ -347,6 +518,7 @@ extern "C" int ofx_mfma_kernel_info_get(int id, ofx_mfma_info* info) {
     info->r1 = kDft[id].r1;
     info->r2 = kDft[id].r2;
     info->shape = id == 0 ? 32 : 16;
+    info->tiled = kDft[id].tiled;
     return 0;
 }
 
@@ -380,15 +552,18 @@ extern "C" int ofx_gen_mfma_kernel_consts(int id, int mode, void* table, hipStre
 
 extern "C" int ofx_launch_mfma_kernel(int id, int mode, const ofx_args* a, int blocks, hipStream_t stream) {
     if (a == nullptr || id < 0 || id >= kDftCount || parts_of(mode) == 0) return (int)hipErrorInvalidValue;
-    if (id == 0) return ofx_launch_mfma_c2c(mode, 1024, a, blocks, stream);
+    if (id == 0) return a->tiles != nullptr ? (int)hipErrorInvalidValue : ofx_launch_mfma_c2c(mode, 1024, a, blocks, stream);
     if (a->batch <= 0) return 0;
     const int n = kDft[id].n;
     const long long need = (a->batch + 3) / 4;
-    // one wave per transform; by default a grid of 2048 workgroups strides over the batch
-    const unsigned grid = (unsigned)std::min<long long>(need, blocks > 0 ? blocks : 2048);
+    // one wave per transform; by default a grid of 2048 workgroups strides over the batch (with a
+    // tile list: one workgroup per tile, `batch` tiles)
+    if (a->tiles != nullptr && a->batch > 0x7fffffffLL) return (int)hipErrorInvalidConfiguration;
+    const unsigned grid =
+        a->tiles != nullptr ? (unsigned)a->batch : (unsigned)std::min<long long>(need, blocks > 0 ? blocks : 2048);
     // streaming HBM access when input and output together exceed the 256 MB last-level cache by
-    // half again (the threshold of the FP32 kernels' streaming variants)
-    const bool nt = a->batch * (long long)n * 16 >= (384LL << 20);
+    // half again (the threshold of the FP32 kernels' streaming variants; tile lists never stream)
+    const bool nt = a->tiles == nullptr && a->batch * (long long)n * 16 >= (384LL << 20);
     dft_kernel_t k = nullptr;
     switch (mode) {
         case OFX_MFMA_BF16: k = dft_kernel_for<OFX_MFMA_BF16>(id, nt); break;

@@ -95,7 +95,8 @@ __device__ __forceinline__ float fp16_scale(float wave_max_abs) {
         return 1.f;
     } else {
         const float m = wave_max_abs;
-        return (m > 0.f && m < 3.0e38f) ? ldexpf(1.f, 11 - ilogbf(m)) : 1.f;
+        // (the exponent is capped so that the scale stays finite for subnormal data)
+        return (m > 0.f && m < 3.0e38f) ? ldexpf(1.f, min(11 - ilogbf(m), 126)) : 1.f;
     }
 }
 

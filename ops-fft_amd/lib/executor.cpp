@@ -614,9 +614,22 @@ struct Builder {
     // v_mfma_f32_32x32x16).  ACLFFT_MFMA_VARIANT picks another candidate.
     static std::vector<int> mfma_kernels_for(long long n) {
         std::vector<int> ids;
+        std::vector<int> tiled;
         for (int i = 0; i < ofx_mfma_kernel_count(); ++i) {
             ofx_mfma_info mi;
             if (ofx_mfma_kernel_info_get(i, &mi) == 0 && mi.n == n) ids.push_back(i);
+            if (ofx_mfma_kernel_info_get(i, &mi) == 0 && mi.n == n) tiled.push_back(mi.tiled);
+        }
+        // 4096 (64 x 64): the tiled form first (one row tile of input in registers instead of
+        // the whole transform: measured 202 vs 246 us (BF16) and 301 vs 339 us (FP16 hi/lo) for
+        // 64 Mi points)
+        if (n > 2048) {
+            std::vector<int> order(ids.size());
+            for (size_t k = 0; k < ids.size(); ++k) order[k] = (int)k;
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tiled[a] > tiled[b]; });
+            std::vector<int> sorted;
+            for (int k : order) sorted.push_back(ids[k]);
+            ids = sorted;
         }
         return ids;
     }
@@ -800,7 +813,81 @@ struct Builder {
 
     // Lengths with a compiled batch class become tiles of the persistent kernel (one launch); any
     // other length runs as its own step.
-    aclfftResult uneven_persistent(const std::vector<int>& len, const std::vector<int>& bat) {
+    // Uneven plan in an MFMA precision: every length with a tile-capable MFMA kernel (registry
+    // entries other than 0) gets one MFMA launch over its groups' tiles (up to 16 contiguous
+    // transforms each, one wave per transform); the other lengths run as in the FP32 plan (the
+    // persistent kernel and, for lengths without a class, their own steps).
+    aclfftResult uneven_mfma(const std::vector<int>& len, const std::vector<int>& bat, int mode) {
+        constexpr int kTile = 16;
+        std::vector<long long> offs(len.size());
+        long long off = 0;
+        std::vector<char> on_mfma(len.size(), 0);
+        std::map<int, std::vector<int>> ids_of;
+        for (size_t g = 0; g < len.size(); ++g) {
+            offs[g] = off;
+            off += (long long)len[g] * bat[g];
+            if (ids_of.count(len[g]) == 0) {
+                std::vector<int> ids;
+                for (int id : mfma_kernels_for(len[g]))
+                    if (id != 0) ids.push_back(id);
+                ids_of[len[g]] = ids;
+            }
+            on_mfma[g] = ids_of[len[g]].empty() ? 0 : 1;
+        }
+        bool any_fp32 = false;
+        for (char c : on_mfma) any_fp32 = any_fp32 || !c;
+        if (any_fp32) {
+            aclfftResult r = uneven_persistent(len, bat, &on_mfma);
+            if (r != ACLFFT_SUCCESS) return r;
+        }
+        e->in_bytes_ = e->out_bytes_ = (size_t)off * 8;
+        // one step per MFMA length, largest total work first
+        std::map<int, double> work;
+        for (size_t g = 0; g < len.size(); ++g)
+            if (on_mfma[g]) work[len[g]] += (double)len[g] * bat[g];
+        std::vector<int> lens;
+        for (auto& kv : work) lens.push_back(kv.first);
+        std::stable_sort(lens.begin(), lens.end(), [&](int a, int b) { return work[a] > work[b]; });
+        for (int n : lens) {
+            std::vector<ofx_batch_tile> tiles;
+            for (size_t g = 0; g < len.size(); ++g) {
+                if (!on_mfma[g] || len[g] != n) continue;
+                for (long long i = 0; i < bat[g]; i += kTile) {
+                    ofx_batch_tile t;
+                    t.cls = 0;
+                    t.count = (int)std::min<long long>(kTile, bat[g] - i);
+                    t.offset = offs[g] + i * n;
+                    tiles.push_back(t);
+                }
+            }
+            Step s;
+            s.type = Step::MFMA;
+            s.n = n;
+            s.mfma_mode = mode;
+            s.candidates = ids_of[n];
+            for (int id : s.candidates) s.candidate_tables.push_back(mfma_table(id, mode));
+            const int pick = std::min<int>(std::max(0, env_int("ACLFFT_MFMA_VARIANT", 0)), (int)s.candidates.size() - 1);
+            s.kernel = s.candidates[pick];
+            s.tw_table = s.candidate_tables[pick];
+            s.aux_table = roots_table(n, n);
+            TableSpec tt;
+            tt.kind = TableSpec::BLOB;
+            tt.bytes = tiles.size() * sizeof(ofx_batch_tile);
+            tt.blob.resize(tt.bytes);
+            std::memcpy(tt.blob.data(), tiles.data(), tt.bytes);
+            s.tiles_table = e->add_table(tt);
+            s.args.batch = (long long)tiles.size();
+            s.args.idist = s.args.odist = n;
+            s.args.in_sign = s.args.out_sign = s.args.scale = 1.f;
+            s.conj_in = s.conj_out = true;
+            s.final_out = true;
+            e->steps_.push_back(s);
+        }
+        return ACLFFT_SUCCESS;
+    }
+
+    aclfftResult uneven_persistent(const std::vector<int>& len, const std::vector<int>& bat,
+                                   const std::vector<char>* skip = nullptr) {
         struct ClassInfo {
             ofx_batch_class_info info;
             bool used = false;
@@ -813,6 +900,11 @@ struct Builder {
         size_t total = 0;
         for (size_t g = 0; g < len.size(); ++g) {
             const long long n = len[g], b = bat[g];
+            if (skip != nullptr && (*skip)[g]) {  // planned elsewhere; only its offset counts here
+                off += n * b;
+                total += (size_t)(n * b);
+                continue;
+            }
             int c = -1;
             for (int k = 0; k < (int)cls.size(); ++k)
                 if (cls[k].info.n == n) c = k;
@@ -1117,14 +1209,22 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
     e->desc_ = d;
     Builder b{e};
     aclfftResult r = ACLFFT_NOT_IMPLEMENTED;
-    if (d.precision != 0 &&
-        (d.uneven || !(d.rank == 1 && !d.vertical &&
-                       (d.type == ACLFFT_C2C || d.type == ACLFFT_R2C || d.type == ACLFFT_C2R)))) {
+    if (d.precision != 0 && !(d.rank == 1 && !d.vertical &&
+                              (d.type == ACLFFT_C2C || d.type == ACLFFT_R2C || d.type == ACLFFT_C2R))) {
         delete e;
         return ACLFFT_NOT_SUPPORTED;
     }
     if (d.uneven) {
-        r = (d.type == ACLFFT_C2C) ? b.uneven(d.ulen, d.ubatch) : ACLFFT_NOT_SUPPORTED;
+        if (d.type != ACLFFT_C2C) {
+            r = ACLFFT_NOT_SUPPORTED;
+        } else if (d.precision != 0) {
+            // an MFMA precision needs at least one length with an MFMA kernel
+            bool any = false;
+            for (int n : d.ulen) any = any || !Builder::mfma_kernels_for(n).empty();
+            r = any ? b.uneven_mfma(d.ulen, d.ubatch, d.precision) : ACLFFT_NOT_SUPPORTED;
+        } else {
+            r = b.uneven(d.ulen, d.ubatch);
+        }
         // replaying a graph costs a few microseconds per launch on the device: worth it only when
         // it replaces several launches (e.g. the ordinary steps of lengths without a class)
         if (r == ACLFFT_SUCCESS) e->use_graphs_ = e->steps_.size() > 3;
@@ -1351,8 +1451,9 @@ std::string Executor::describe() const {
         if (s.type == Step::MFMA) {
             ofx_mfma_info mi{};
             ofx_mfma_kernel_info_get(s.kernel, &mi);
-            os << "mfma n=" << s.n << " " << mi.r1 << "x" << mi.r2 << " shape=" << mi.shape
+            os << "mfma n=" << s.n << " " << mi.r1 << "x" << mi.r2 << " shape=" << mi.shape << (mi.tiled ? " tiled" : "")
                << " mode=" << (s.mfma_mode == OFX_MFMA_BF16 ? "bf16" : s.mfma_mode == OFX_MFMA_FP16X2 ? "fp16x2" : "bf16x3");
+            if (s.tiles_table >= 0) os << " tiles=" << s.args.batch;
             continue;
         }
         if (s.type == Step::GENERIC) {
@@ -1539,6 +1640,7 @@ aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream
                 a.tw_aux = table(st.aux_table);
                 if (st.conj_in) a.in_sign *= dsign;
                 if (st.conj_out) a.out_sign *= dsign;
+                if (st.tiles_table >= 0) a.tiles = table(st.tiles_table);
                 err = (hipError_t)ofx_launch_mfma_kernel(st.kernel, st.mfma_mode, &a, env_int("ACLFFT_MFMA_BLOCKS", 0), s);
                 break;
             }

@@ -86,7 +86,7 @@ std::string candidate_signature(const Step& st, int id, const Registries& reg) {
         ofx_mfma_info mi;
         if (ofx_mfma_kernel_info_get(id, &mi) != 0) return std::string();
         return "mfma:" + std::to_string(mi.n) + ":" + std::to_string(mi.r1) + "x" + std::to_string(mi.r2) + ":" +
-               std::to_string(mi.shape) + ":m" + std::to_string(st.mfma_mode);
+               std::to_string(mi.shape) + (mi.tiled ? ":t" : "") + ":m" + std::to_string(st.mfma_mode);
     }
     const ofx_kernel_info* k = entry_of(st, id, reg);
     if (k == nullptr) return std::string();

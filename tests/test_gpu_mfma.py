@@ -24,7 +24,8 @@ MODES = (aclfft.PRECISION_BF16, aclfft.PRECISION_FP16X2, aclfft.PRECISION_BF16X3
 
 
 class MfmaInfo(ctypes.Structure):
-    _fields_ = [("n", ctypes.c_int), ("r1", ctypes.c_int), ("r2", ctypes.c_int), ("shape", ctypes.c_int)]
+    _fields_ = [("n", ctypes.c_int), ("r1", ctypes.c_int), ("r2", ctypes.c_int), ("shape", ctypes.c_int),
+                ("tiled", ctypes.c_int)]
 
 
 def mfma_variants():
@@ -178,4 +179,59 @@ def test_mfma_precision_gate(cuda):
         p.set_precision(aclfft.PRECISION_FP16X2)
     assert e.value.code == aclfft.NOT_SUPPORTED
     assert p.describe().startswith("kind0")
+    p.destroy()
+
+
+def uneven_workload(count, seed, lengths=(60, 128, 243, 256, 512, 1000, 1024, 2048, 4096, 6561)):
+    rng = np.random.default_rng(seed)
+    logs = np.log(np.array(lengths, float))
+    u = rng.uniform(logs.min(), logs.max(), count)
+    ln = [int(lengths[int(np.argmin(np.abs(logs - v)))]) for v in u]
+    bt = [int(b) for b in rng.integers(1, 40, count)]
+    return ln, bt
+
+
+@pytest.mark.parametrize("mode", [aclfft.PRECISION_FP16X2, aclfft.PRECISION_BF16X3])
+@pytest.mark.parametrize("graph", [False, True])
+def test_mfma_uneven_plan(cuda, mode, graph):
+    """Uneven plans in an MFMA precision: lengths with MFMA kernels run as MFMA tile launches,
+    the others on the FP32 kernels; every group against the FP64 oracle, both directions, with an
+    output scale, odd group offsets included ((243, b) groups shift the following ones by 8 B)."""
+    import torch
+    ln, bt = uneven_workload(30, 7)
+    ln += [243, 1024, 7, 4096]
+    bt += [1, 3, 2, 5]
+    groups = list(zip(ln, bt))
+    total = sum(n * b for n, b in groups)
+    x = crandn(total, 8).astype(np.complex64)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    p = aclfft.Plan.plan_many_uneven(ln, bt)
+    p.set_precision(mode)
+    p.set_graph_mode(graph)
+    p.set_scale(0.5)
+    d = p.describe()
+    print(d)
+    assert "mfma" in d
+    for direction in (aclfft.FORWARD, aclfft.BACKWARD):
+        for rep in range(2):
+            yt.zero_()
+            p.exec_c2c(xt, yt, direction)
+            torch.cuda.synchronize()
+            out = yt.cpu().numpy()
+            off = 0
+            for n, b in groups:
+                ref = 0.5 * oracle.c2c(x[off:off + n * b].reshape(b, n).astype(np.complex128), direction)
+                err = oracle.rel_l2(out[off:off + n * b].reshape(b, n), ref)
+                assert err <= fp32_tol(n), (n, b, err)
+                off += n * b
+    p.destroy()
+
+
+def test_mfma_uneven_gate(cuda):
+    p = aclfft.Plan.plan_many_uneven([60, 243, 1000], [2, 3, 4])
+    with pytest.raises(aclfft.AclfftError) as e:
+        p.set_precision(aclfft.PRECISION_FP16X2)
+    assert e.value.code == aclfft.NOT_SUPPORTED
+    assert p.describe().startswith("persistent")
     p.destroy()
