@@ -288,6 +288,34 @@ def extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y, steps, stream, e32):
     return out
 
 
+def extra_mfma_lengths(torch, aclfft, oracle, np, x, y, steps, stream):
+    """The MFMA split path at the other lengths of its registry (16-, 32- and 64-point dense DFT
+    stages), each at 64 Mi points (1 GiB moved) against the FP32 Stockham kernel: time, HBM
+    fraction and the error of 64 transforms against the FP64 oracle, per operand precision."""
+    out = {}
+    pts = x.numel()
+    for n in (256, 512, 2048, 4096):
+        b = pts // n
+        xv, yv = x.view(-1)[:b * n].view(b, n), y.view(-1)[:b * n].view(b, n)
+        ref = oracle.c2c(xv[:64].cpu().numpy().astype(np.complex128))
+        rec = {}
+        for mode, name in ((0, "fp32"), (aclfft.PRECISION_BF16, "bf16"), (aclfft.PRECISION_FP16X2, "fp16x2"),
+                           (aclfft.PRECISION_BF16X3, "bf16x3")):
+            p = aclfft.Plan.plan1d(n, aclfft.C2C, b)
+            if mode:
+                p.set_precision(mode)
+            p.set_stream(stream)
+            ms = time_steps(torch, lambda: p.exec_c2c(xv, yv, aclfft.FORWARD), steps, 3, stream)
+            el2, emax = errs(oracle, np, yv[:64].cpu().numpy(), ref)
+            rec[name] = {"ms": round(ms, 5), "hbm_frac": frac(16.0 * pts, ms), "rel_l2_vs_fp64": el2,
+                         "max_abs_rel_vs_fp64": emax, "plan": p.describe()}
+            p.destroy()
+        for name in ("bf16", "fp16x2", "bf16x3"):
+            rec[name]["vs_fp32"] = round(rec["fp32"]["ms"] / rec[name]["ms"], 3)
+        out["n%d_b%d" % (n, b)] = rec
+    return out
+
+
 def extra_real(torch, aclfft, oracle, np, steps, stream):
     out = {}
     for n, b in ((4096, 8192), (3000, 8192), (8192, 1024)):
@@ -304,12 +332,34 @@ def extra_real(torch, aclfft, oracle, np, steps, stream):
         byts = (4.0 * n + 8.0 * (n // 2 + 1)) * b
         rt = float(((xb / n - xr).norm() / xr.norm()).item())
         err = oracle.rel_l2(X[:64].cpu().numpy(), oracle.r2c(xr[:64].cpu().numpy().astype(np.float64)))
+        rec_mfma = {}
+        if n == 8192:
+            # the MFMA precisions: the 4096-point MFMA transform with the Hermitian unpack / pre-pack
+            for mode, name in ((aclfft.PRECISION_BF16, "bf16"), (aclfft.PRECISION_FP16X2, "fp16x2"),
+                               (aclfft.PRECISION_BF16X3, "bf16x3")):
+                qr = aclfft.Plan.plan1d(n, aclfft.R2C, b)
+                qc = aclfft.Plan.plan1d(n, aclfft.C2R, b)
+                qr.set_precision(mode)
+                qc.set_precision(mode)
+                qr.set_stream(stream)
+                qc.set_stream(stream)
+                Xm = torch.empty_like(X)
+                mr = time_steps(torch, lambda: qr.exec_r2c(xr, Xm), steps, 3, stream)
+                mc = time_steps(torch, lambda: qc.exec_c2r(Xm, xb), steps, 3, stream)
+                em = oracle.rel_l2(Xm[:64].cpu().numpy(), oracle.r2c(xr[:64].cpu().numpy().astype(np.float64)))
+                rec_mfma[name] = {"r2c_ms": round(mr, 5), "c2r_ms": round(mc, 5), "r2c_hbm_frac": frac(byts, mr),
+                                  "c2r_hbm_frac": frac(byts, mc), "r2c_vs_fp32": round(ms_r / mr, 3),
+                                  "r2c_rel_l2_vs_fp64": em, "plan_r2c": qr.describe()}
+                qr.destroy()
+                qc.destroy()
         out["n%d_b%d" % (n, b)] = {
             "r2c_ms": round(ms_r, 5), "c2r_ms": round(ms_c, 5),
             "r2c_hbm_frac": frac(byts, ms_r), "c2r_hbm_frac": frac(byts, ms_c),
             "r2c_GFLOP/s_2.5nlogn": round(2.5 * n * math.log2(n) * b / ms_r / 1e6, 1),
             "roundtrip_rel_l2": rt, "r2c_rel_l2_vs_fp64": err, "plan_r2c": pr.describe(),
             "plan_c2r": pc.describe()}
+        if rec_mfma:
+            out["n%d_b%d" % (n, b)]["mfma"] = rec_mfma
         pr.destroy()
         pc.destroy()
     return out
@@ -535,6 +585,45 @@ def extra_batched(torch, aclfft, oracle, np, steps, stream):
     for q in plans:
         q.destroy()
     p.destroy()
+
+    def worst_err(yy):
+        yc = yy.cpu().numpy()
+        o, w = 0, 0.0
+        for n, b in zip(ln, bt):
+            ref = oracle.c2c(xcpu[o:o + n * b].reshape(b, n).astype(np.complex128))
+            w = max(w, oracle.rel_l2(yc[o:o + n * b].reshape(b, n), ref))
+            o += n * b
+        return w
+
+    # the tuned plan (aclfftExtTune: form, schedule, workgroups per CU, class set, step kernels)
+    # and the MFMA precisions (MFMA tile launches for 1024 / 2048 / 4096, the FP32 persistent
+    # kernel for the other lengths; "auto" = the tuner's choice, MFMA kept only within 1e-6 of the
+    # FP32 result); the tuning cache goes to a private temporary file
+    import tempfile
+    old_cache = os.environ.get("ACLFFT_TUNE_CACHE")
+    with tempfile.TemporaryDirectory() as td:
+        os.environ["ACLFFT_TUNE_CACHE"] = os.path.join(td, "tune.txt")
+        try:
+            for name, mode, tune in (("tuned", 0, True), ("mfma_fp16x2", 2, False), ("mfma_bf16x3", 3, False),
+                                     ("mfma_bf16", 1, False), ("auto_tuned", 4, True)):
+                q = aclfft.Plan.plan_many_uneven(ln, bt)
+                q.set_stream(stream)
+                if mode:
+                    q.set_precision(mode)
+                if tune:
+                    q.tune()
+                mq = time_steps(torch, lambda: q.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
+                q.exec_c2c(x, y, aclfft.FORWARD)
+                torch.cuda.synchronize()
+                out[name] = {"ms": round(mq, 5), "hbm_frac": frac(16.0 * total, mq),
+                             "vs_fp32_default": round(ms / mq, 3), "worst_rel_l2_vs_fp64_all_groups": worst_err(y),
+                             "plan": q.describe()}
+                q.destroy()
+        finally:
+            if old_cache is None:
+                os.environ.pop("ACLFFT_TUNE_CACHE", None)
+            else:
+                os.environ["ACLFFT_TUNE_CACHE"] = old_cache
     return out
 
 
@@ -717,6 +806,8 @@ def main():
         for name, fn_ in (("copy_1GiB", lambda: extra_copy(torch, aclfft, x, 20, stream, ms)),
                           ("mfma_split_n1024_b65536", lambda: extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y,
                                                                          args.steps, stream, err)),
+                          ("mfma_split_other_lengths", lambda: extra_mfma_lengths(torch, aclfft, oracle, np, x, y,
+                                                                                   args.steps, stream)),
                           ("real_r2c_c2r", lambda: extra_real(torch, aclfft, oracle, np, args.steps, stream)),
                           ("c2c_n1024_b64_latency", lambda: extra_small_latency(torch, aclfft, oracle, np, stream)),
                           ("double_precision", lambda: extra_double(torch, aclfft, oracle, np, args.steps, stream)),

@@ -252,9 +252,15 @@ typedef struct ofx_batch_args {
                                        is sorted by cost, so the rounds alternate direction),
                                        its descriptor loaded one tile ahead; no counter        */
     float scale;                    /* multiplies every output value (0 is taken as 1)         */
+    int class_set;                  /* compiled class set (ofx_batch_class_set_count; 0 default)  */
 } ofx_batch_args;
 OFX_API int ofx_batch_class_count(void);
-OFX_API int ofx_batch_class_info_get(int cls, ofx_batch_class_info* info);
+OFX_API int ofx_batch_class_info_get(int cls, ofx_batch_class_info* info);   /* class set 0 */
+/* The persistent kernel is compiled for several class sets (different radix orders / threads per
+ * transform of some length classes); a launch names its set in ofx_batch_args.class_set. */
+OFX_API int ofx_batch_class_set_count(void);
+OFX_API int ofx_batch_class_info_get_set(int set, int cls, ofx_batch_class_info* info);
+OFX_API int ofx_batch_blocks_per_cu_set(int set, int schedule);
 /* Launches `blocks` persistent workgroups (dynamic schedule: after a one-thread kernel that
  * resets the counter). */
 OFX_API int ofx_launch_batch_persistent(const ofx_batch_args* args, int blocks, hipStream_t stream);

@@ -30,7 +30,23 @@ ACLFFT_API aclfftResult aclfftExtDescribe(aclfftHandle plan, char* buf, size_t l
 #define ACLFFT_EXT_PRECISION_BF16 1
 #define ACLFFT_EXT_PRECISION_FP16X2 2
 #define ACLFFT_EXT_PRECISION_BF16X3 3
+/*   ACLFFT_EXT_PRECISION_AUTO     (4) FP32 until the plan is tuned (aclfftExtTune, or ACLFFT_TUNE=1 at
+ *                                     the first execution); the tuner then keeps the fastest of FP32
+ *                                     and the MFMA modes whose result is within rel-L2 1e-6 of the
+ *                                     FP32 result.  Plans without an MFMA mode: NOT_SUPPORTED. */
+#define ACLFFT_EXT_PRECISION_AUTO 4
 ACLFFT_API aclfftResult aclfftExtSetPrecision(aclfftHandle plan, int precision);
+
+/* Measure now, on the plan's stream, the alternatives of the plan's launch list and keep the
+ * fastest (like FFTW's measured planning): for uneven plans the form (one persistent launch or one
+ * launch per length), the persistent kernel's schedule, workgroups per CU and compiled class set;
+ * with ACLFFT_EXT_PRECISION_AUTO the operand precision; for every plan each step's compiled kernel
+ * variants.  Synchronous; allocates temporary buffers of the plan's input and output size; the
+ * choices are stored in the tuning cache ($ACLFFT_TUNE_CACHE, default
+ * $HOME/.cache/ops_fft_amd/tune.txt), from which later plans of the same shape take them on their
+ * first execution.  No counterpart in the reference, whose kernel choice is a static table
+ * (src/fft1_d/arch32/fft1_d_exec_api.cpp:29-40, src/common/tiling/fft_tile_common.h:55-102). */
+ACLFFT_API aclfftResult aclfftExtTune(aclfftHandle plan);
 
 /* One plan for many C2C transforms of uneven lengths: group g holds batches[g] transforms of
  * length lengths[g], stored contiguously, groups one after the other (group g starts at the

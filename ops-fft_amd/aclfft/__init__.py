@@ -12,7 +12,7 @@ from ._lib import (LIB_PATH, AclfftError, check, error_string, kernel_registry, 
                    SUCCESS, INVALID_PLAN, ALLOC_FAILED, INVALID_TYPE, INVALID_VALUE, INTERNAL_ERROR,
                    EXEC_FAILED, SETUP_FAILED, INVALID_SIZE, NOT_IMPLEMENTED, NOT_SUPPORTED, INVALID_DEVICE,
                    NO_WORKSPACE, UNALIGNED_DATA,
-                   PRECISION_FP32, PRECISION_BF16, PRECISION_FP16X2, PRECISION_BF16X3)
+                   PRECISION_FP32, PRECISION_BF16, PRECISION_FP16X2, PRECISION_BF16X3, PRECISION_AUTO)
 from .plan import Plan
 
 __all__ = ["Plan", "AclfftError", "check", "error_string", "kernel_registry", "lib", "LIB_PATH",

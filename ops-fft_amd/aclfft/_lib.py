@@ -40,6 +40,7 @@ PRECISION_FP32 = 0
 PRECISION_BF16 = 1
 PRECISION_FP16X2 = 2
 PRECISION_BF16X3 = 3
+PRECISION_AUTO = 4
 
 FORWARD = -1
 BACKWARD = 1

@@ -186,8 +186,18 @@ class Plan:
         check(lib.aclfftExtSetGraphMode(self._h, 1 if on else 0), "aclfftExtSetGraphMode")
 
     def set_precision(self, mode):
-        """0 FP32 Stockham; 1 MFMA BF16; 2 MFMA FP16 hi/lo; 3 MFMA BF16 hi/mid/lo."""
+        """0 FP32 Stockham; 1 MFMA BF16; 2 MFMA FP16 hi/lo; 3 MFMA BF16 hi/mid/lo; 4 automatic (chosen
+        by tune())."""
         check(lib.aclfftExtSetPrecision(self._h, mode), "aclfftExtSetPrecision")
+
+    def tune(self):
+        """aclfftExtTune: measure the plan's alternatives now (on its stream: torch's current stream
+        unless set_stream chose one) and keep the fastest."""
+        if self._stream is None:
+            import torch
+            if torch.cuda.is_available():
+                check(lib.aclfftSetStream(self._h, torch.cuda.current_stream().cuda_stream), "aclfftSetStream")
+        check(lib.aclfftExtTune(self._h), "aclfftExtTune")
 
     # ---- execution ----------------------------------------------------------------------------
     def _sizes(self):

@@ -21,35 +21,58 @@ namespace ofx {
 
 constexpr int kBatchBlock = 256;
 
-// length classes (all with T * TPB = 256)
-typedef StockhamCfg<60, 4, 64, 4, false, Radices<4, 3, 5>> BC60;
-typedef StockhamCfg<128, 8, 32, 4, false, Radices<8, 16>> BC128;
-typedef StockhamCfg<243, 16, 16, 30, false, Radices<27, 9>> BC243;  // odd: no LDS pad needed
-typedef StockhamCfg<1000, 128, 2, 4, false, Radices<10, 10, 10>> BC1000;
-typedef StockhamCfg<1024, 64, 4, 4, false, Radices<8, 8, 16>> BC1024;
-typedef StockhamCfg<2048, 128, 2, 4, false, Radices<16, 16, 8>> BC2048;
-typedef StockhamCfg<4096, 256, 1, 4, false, Radices<16, 16, 16>> BC4096;
-// 6561: split exchange (26 KiB of LDS instead of 52), so the kernel's LDS budget is set by the
-// 35 KiB classes and four workgroups fit on a CU
-typedef StockhamCfg<6561, 256, 1, 30, false, Radices<27, 27, 9>, 1, 0, 1, float2, false, true> BC6561;
-typedef StockhamCfg<512, 32, 8, 4, false, Radices<8, 8, 8>> BC512;
-typedef StockhamCfg<256, 16, 16, 4, false, Radices<16, 16>> BC256;
+// Length classes, all with T * TPB = 256.  Two compiled class sets (the tuner times both):
+// set 0 (the default) and set 1, which differs in the radix order / threads per transform of the
+// 1000-, 1024-, 2048-, 4096- and 6561-point classes.
+struct ClassSet0 {
+    typedef StockhamCfg<60, 4, 64, 4, false, Radices<4, 3, 5>> C60;
+    typedef StockhamCfg<128, 8, 32, 4, false, Radices<8, 16>> C128;
+    typedef StockhamCfg<243, 16, 16, 30, false, Radices<27, 9>> C243;  // odd: no LDS pad needed
+    typedef StockhamCfg<256, 16, 16, 4, false, Radices<16, 16>> C256;
+    typedef StockhamCfg<512, 32, 8, 4, false, Radices<8, 8, 8>> C512;
+    typedef StockhamCfg<1000, 128, 2, 4, false, Radices<10, 10, 10>> C1000;
+    typedef StockhamCfg<1024, 64, 4, 4, false, Radices<8, 8, 16>> C1024;
+    typedef StockhamCfg<2048, 128, 2, 4, false, Radices<16, 16, 8>> C2048;
+    typedef StockhamCfg<4096, 256, 1, 4, false, Radices<16, 16, 16>> C4096;
+    // 6561: split exchange (26 KiB of LDS instead of 52), so the kernel's LDS budget is set by the
+    // 35 KiB classes and four workgroups fit on a CU
+    typedef StockhamCfg<6561, 256, 1, 30, false, Radices<27, 27, 9>, 1, 0, 1, float2, false, true> C6561;
+};
+struct ClassSet1 {
+    typedef ClassSet0::C60 C60;
+    typedef ClassSet0::C128 C128;
+    typedef ClassSet0::C243 C243;
+    typedef ClassSet0::C256 C256;
+    typedef ClassSet0::C512 C512;
+    typedef StockhamCfg<1000, 128, 2, 4, false, Radices<8, 5, 25>> C1000;
+    typedef StockhamCfg<1024, 64, 4, 4, false, Radices<16, 8, 8>> C1024;
+    typedef StockhamCfg<2048, 256, 1, 4, false, Radices<8, 8, 8, 4>> C2048;
+    typedef StockhamCfg<4096, 256, 1, 4, false, Radices<8, 8, 8, 8>> C4096;
+    typedef StockhamCfg<6561, 256, 1, 30, false, Radices<9, 27, 27>, 1, 0, 1, float2, false, true> C6561;
+};
+constexpr int kClassSets = 2;
 
 template <class C>
 constexpr int lds_of() {  // in complex64 units
     return C::SPLITX ? (C::TPB * C::LDS_N + 1) / 2 : C::TPB * C::LDS_N;
 }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
-constexpr int kBatchLds =
-    cmax(cmax(cmax(lds_of<BC60>(), lds_of<BC128>()), cmax(lds_of<BC243>(), lds_of<BC1000>())),
-         cmax(cmax(cmax(lds_of<BC1024>(), lds_of<BC2048>()), cmax(lds_of<BC4096>(), lds_of<BC6561>())),
-              cmax(lds_of<BC512>(), lds_of<BC256>())));
-
-static_assert(BC60::BLOCK == kBatchBlock && BC128::BLOCK == kBatchBlock && BC243::BLOCK == kBatchBlock &&
-                  BC1000::BLOCK == kBatchBlock && BC1024::BLOCK == kBatchBlock && BC2048::BLOCK == kBatchBlock &&
-                  BC4096::BLOCK == kBatchBlock && BC6561::BLOCK == kBatchBlock && BC512::BLOCK == kBatchBlock &&
-                  BC256::BLOCK == kBatchBlock,
-              "every class runs with the same workgroup size");
+template <class CS>
+constexpr int batch_lds() {
+    return cmax(cmax(cmax(lds_of<typename CS::C60>(), lds_of<typename CS::C128>()),
+                     cmax(lds_of<typename CS::C243>(), lds_of<typename CS::C1000>())),
+                cmax(cmax(cmax(lds_of<typename CS::C1024>(), lds_of<typename CS::C2048>()),
+                          cmax(lds_of<typename CS::C4096>(), lds_of<typename CS::C6561>())),
+                     cmax(lds_of<typename CS::C512>(), lds_of<typename CS::C256>())));
+}
+template <class CS>
+constexpr bool same_block() {
+    return CS::C60::BLOCK == kBatchBlock && CS::C128::BLOCK == kBatchBlock && CS::C243::BLOCK == kBatchBlock &&
+           CS::C1000::BLOCK == kBatchBlock && CS::C1024::BLOCK == kBatchBlock && CS::C2048::BLOCK == kBatchBlock &&
+           CS::C4096::BLOCK == kBatchBlock && CS::C6561::BLOCK == kBatchBlock && CS::C512::BLOCK == kBatchBlock &&
+           CS::C256::BLOCK == kBatchBlock;
+}
+static_assert(same_block<ClassSet0>() && same_block<ClassSet1>(), "every class runs with the same workgroup size");
 
 struct GLoadB {
     const float2* __restrict__ p;
@@ -102,22 +125,23 @@ __device__ __forceinline__ void run_tile(float2* lds, const ofx_batch_tile& tile
 
 __global__ __launch_bounds__(kBatchBlock) void k_batch_reset(unsigned int* counter) { *counter = 0u; }
 
+template <class CS>
 __device__ __forceinline__ void run_class(float2* lds, const ofx_batch_tile& tile, const ofx_batch_args& a) {
     const float2* __restrict__ in = static_cast<const float2*>(a.in);
     float2* __restrict__ out = static_cast<float2*>(a.out);
     const float2* tw = static_cast<const float2*>(a.tw) + a.tw_off[tile.cls];
     const float sc = a.scale != 0.f ? a.scale : 1.f;
     switch (tile.cls) {
-        case OFX_BCLASS_60: run_tile<BC60>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_128: run_tile<BC128>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_243: run_tile<BC243>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_256: run_tile<BC256>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_512: run_tile<BC512>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_1000: run_tile<BC1000>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_1024: run_tile<BC1024>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_2048: run_tile<BC2048>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_4096: run_tile<BC4096>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_6561: run_tile<BC6561>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_60: run_tile<typename CS::C60>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_128: run_tile<typename CS::C128>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_243: run_tile<typename CS::C243>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_256: run_tile<typename CS::C256>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_512: run_tile<typename CS::C512>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_1000: run_tile<typename CS::C1000>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_1024: run_tile<typename CS::C1024>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_2048: run_tile<typename CS::C2048>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_4096: run_tile<typename CS::C4096>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_6561: run_tile<typename CS::C6561>(lds, tile, in, out, tw, a.sign, sc); break;
         default: break;
     }
 }
@@ -146,9 +170,9 @@ __device__ __forceinline__ void lds_barrier() {
 
 // Every loop below is uniform over the workgroup (tile indices come from blockIdx, the tile list
 // or LDS), so every wave leaves together once the index passes ntiles.
-template <int SCHED>
+template <class CS, int SCHED>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args a) {
-    __shared__ float2 lds[kBatchLds];
+    __shared__ float2 lds[batch_lds<CS>()];
     __shared__ int s_next[2];
     const int G = (int)gridDim.x;
     const int b = (int)blockIdx.x;
@@ -161,7 +185,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args
             ofx_batch_tile nxt{};
             if (tn < a.ntiles) nxt = load_tile(a.tiles, tn);  // in flight while this tile runs
             __asm__ volatile("" ::: "memory");      // no load of the tile bodies moves out of the loop
-            run_class(lds, cur, a);
+            run_class<CS>(lds, cur, a);
             lds_barrier();  // LDS is reused by the next tile
             cur = nxt;
             t = tn;
@@ -174,7 +198,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args
             if (threadIdx.x == 0) nx = atomicAdd(a.counter, 1u);  // the next index, used after this tile
             const ofx_batch_tile cur = load_tile(a.tiles, t);
             __asm__ volatile("" ::: "memory");
-            run_class(lds, cur, a);
+            run_class<CS>(lds, cur, a);
             // two slots: a wave may still read this tile's slot while thread 0 writes the next one
             if (threadIdx.x == 0) s_next[parity] = (int)nx + G;
             lds_barrier();  // publishes s_next; LDS is reused by the next tile
@@ -198,60 +222,76 @@ void fill_info(ofx_batch_class_info* c) {
 using namespace ofx;
 
 extern "C" int ofx_batch_class_count(void) { return OFX_BCLASS_COUNT; }
+extern "C" int ofx_batch_class_set_count(void) { return kClassSets; }
 
-extern "C" int ofx_batch_class_info_get(int cls, ofx_batch_class_info* c) {
-    if (c == nullptr) return (int)hipErrorInvalidValue;
+template <class CS>
+static int class_info(int cls, ofx_batch_class_info* c) {
     switch (cls) {
-        case OFX_BCLASS_60: fill_info<BC60>(c); break;
-        case OFX_BCLASS_128: fill_info<BC128>(c); break;
-        case OFX_BCLASS_243: fill_info<BC243>(c); break;
-        case OFX_BCLASS_256: fill_info<BC256>(c); break;
-        case OFX_BCLASS_512: fill_info<BC512>(c); break;
-        case OFX_BCLASS_1000: fill_info<BC1000>(c); break;
-        case OFX_BCLASS_1024: fill_info<BC1024>(c); break;
-        case OFX_BCLASS_2048: fill_info<BC2048>(c); break;
-        case OFX_BCLASS_4096: fill_info<BC4096>(c); break;
-        case OFX_BCLASS_6561: fill_info<BC6561>(c); break;
+        case OFX_BCLASS_60: fill_info<typename CS::C60>(c); break;
+        case OFX_BCLASS_128: fill_info<typename CS::C128>(c); break;
+        case OFX_BCLASS_243: fill_info<typename CS::C243>(c); break;
+        case OFX_BCLASS_256: fill_info<typename CS::C256>(c); break;
+        case OFX_BCLASS_512: fill_info<typename CS::C512>(c); break;
+        case OFX_BCLASS_1000: fill_info<typename CS::C1000>(c); break;
+        case OFX_BCLASS_1024: fill_info<typename CS::C1024>(c); break;
+        case OFX_BCLASS_2048: fill_info<typename CS::C2048>(c); break;
+        case OFX_BCLASS_4096: fill_info<typename CS::C4096>(c); break;
+        case OFX_BCLASS_6561: fill_info<typename CS::C6561>(c); break;
         default: return (int)hipErrorInvalidValue;
     }
     return 0;
 }
 
-extern "C" int ofx_batch_blocks_per_cu(int schedule) {
-    // cached per device and schedule (the query is a runtime call; plans ask on every execution,
-    // possibly from several host threads)
-    static std::atomic<int> cache[64][2] = {};
+extern "C" int ofx_batch_class_info_get_set(int set, int cls, ofx_batch_class_info* c) {
+    if (c == nullptr) return (int)hipErrorInvalidValue;
+    if (set == 0) return class_info<ClassSet0>(cls, c);
+    if (set == 1) return class_info<ClassSet1>(cls, c);
+    return (int)hipErrorInvalidValue;
+}
+
+extern "C" int ofx_batch_class_info_get(int cls, ofx_batch_class_info* c) { return ofx_batch_class_info_get_set(0, cls, c); }
+
+typedef void (*batch_kernel_t)(ofx_batch_args);
+static batch_kernel_t batch_kernel(int set, int schedule) {
+    if (set == 0) return schedule == 0 ? k_batch_persistent<ClassSet0, 0> : k_batch_persistent<ClassSet0, 1>;
+    if (set == 1) return schedule == 0 ? k_batch_persistent<ClassSet1, 0> : k_batch_persistent<ClassSet1, 1>;
+    return nullptr;
+}
+
+extern "C" int ofx_batch_blocks_per_cu_set(int set, int schedule) {
+    // cached per device, class set and schedule (the query is a runtime call; plans ask on every
+    // execution, possibly from several host threads)
+    static std::atomic<int> cache[64][kClassSets][2] = {};
+    if (set < 0 || set >= kClassSets) return 1;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = -1;
     const int sc = schedule == 0 ? 0 : 1;
     if (dev >= 0) {
-        const int c = cache[dev][sc].load(std::memory_order_relaxed);
+        const int c = cache[dev][set][sc].load(std::memory_order_relaxed);
         if (c > 0) return c;
     }
     int n = 0;
-    const hipError_t e =
-        sc == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_persistent<0>, kBatchBlock, 0)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_persistent<1>, kBatchBlock, 0);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel(set, sc), kBatchBlock, 0);
     if (e != hipSuccess || n <= 0) {
         (void)hipGetLastError();
         n = 1;
     }
-    if (dev >= 0) cache[dev][sc].store(n, std::memory_order_relaxed);
+    if (dev >= 0) cache[dev][set][sc].store(n, std::memory_order_relaxed);
     return n;
 }
 
+extern "C" int ofx_batch_blocks_per_cu(int schedule) { return ofx_batch_blocks_per_cu_set(0, schedule); }
+
 extern "C" int ofx_launch_batch_persistent(const ofx_batch_args* a, int blocks, hipStream_t stream) {
-    if (a == nullptr || a->tiles == nullptr || blocks <= 0 || (a->schedule == 0 && a->counter == nullptr))
+    if (a == nullptr || a->tiles == nullptr || blocks <= 0 || (a->schedule == 0 && a->counter == nullptr) ||
+        a->class_set < 0 || a->class_set >= kClassSets || (a->schedule != 0 && a->schedule != 1))
         return (int)hipErrorInvalidValue;
     if (a->ntiles <= 0) return 0;
-    if (a->schedule == 1) {
-        hipLaunchKernelGGL(k_batch_persistent<1>, dim3(blocks), dim3(kBatchBlock), 0, stream, *a);
-        return (int)hipGetLastError();
+    if (a->schedule == 0) {
+        hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, stream, a->counter);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return (int)e;
     }
-    if (a->schedule != 0) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, stream, a->counter);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL(k_batch_persistent<0>, dim3(blocks), dim3(kBatchBlock), 0, stream, *a);
+    hipLaunchKernelGGL(batch_kernel(a->class_set, a->schedule), dim3(blocks), dim3(kBatchBlock), 0, stream, *a);
     return (int)hipGetLastError();
 }

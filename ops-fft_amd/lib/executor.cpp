@@ -800,7 +800,7 @@ struct Builder {
     // per-length 3-11% faster; 1024 + 4096 with 17 and 67 M points, 8% faster; 2048 + 4096 + 6561
     // with 5-11 M points each, persistent 13% faster; 512 groups of 8 lengths, 29% faster.
     aclfftResult uneven(const std::vector<int>& len, const std::vector<int>& bat) {
-        int mode = env_int("ACLFFT_UNEVEN_MODE", 0);
+        int mode = e->desc_.uneven_form ? e->desc_.uneven_form : env_int("ACLFFT_UNEVEN_MODE", 0);
         if (mode != 1 && mode != 2) {
             std::map<int, long long> pts;
             for (size_t g = 0; g < len.size(); ++g) pts[len[g]] += (long long)len[g] * bat[g];
@@ -892,8 +892,10 @@ struct Builder {
             ofx_batch_class_info info;
             bool used = false;
         };
+        const int cset = e->desc_.class_set;
         std::vector<ClassInfo> cls(ofx_batch_class_count());
-        for (int c = 0; c < (int)cls.size(); ++c) ofx_batch_class_info_get(c, &cls[c].info);
+        for (int c = 0; c < (int)cls.size(); ++c)
+            if (ofx_batch_class_info_get_set(cset, c, &cls[c].info) != 0) return ACLFFT_INVALID_VALUE;
         std::vector<ofx_batch_tile> tiles;
         std::vector<double> cost;
         long long off = 0;
@@ -944,6 +946,11 @@ struct Builder {
         s.conj_in = s.conj_out = true;
         s.final_out = true;
         s.b.scale = 1.f;
+        s.b.class_set = cset;
+        s.b.schedule = e->desc_.sched >= 0 ? (e->desc_.sched ? 1 : 0)
+                                           : (env_int("ACLFFT_PERSISTENT_SCHEDULE", 1) == 0 ? 0 : 1);
+        s.blocks_per_cu = e->desc_.blocks_per_cu > 0 ? e->desc_.blocks_per_cu
+                                                     : std::max(0, env_int("ACLFFT_PERSISTENT_BLOCKS_PER_CU", 0));
         for (int c = 0; c < (int)cls.size(); ++c) {
             s.class_tables[c] = -1;
             if (cls[c].used) s.class_tables[c] = pass_table_for(cls[c].info.npass, cls[c].info.radices, cls[c].info.tw_len);
@@ -1437,12 +1444,14 @@ std::string Executor::describe() const {
             continue;
         }
         if (s.type == Step::PERSISTENT) {
-            os << "persistent tiles=" << s.b.ntiles << " classes=";
+            os << "persistent tiles=" << s.b.ntiles << " set=" << s.b.class_set << " sched=" << s.b.schedule;
+            if (s.blocks_per_cu > 0) os << " bpc=" << s.blocks_per_cu;
+            os << " classes=";
             bool first = true;
             for (int c = 0; c < OFX_BCLASS_COUNT; ++c)
                 if (s.class_tables[c] >= 0) {
                     ofx_batch_class_info ci;
-                    ofx_batch_class_info_get(c, &ci);
+                    ofx_batch_class_info_get_set(s.b.class_set, c, &ci);
                     os << (first ? "" : ",") << ci.n;
                     first = false;
                 }
@@ -1479,6 +1488,24 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
     int dev = 0;
     if (hip_log(hipGetDevice(&dev), "hipGetDevice") != hipSuccess) return ACLFFT_SETUP_FAILED;
     if (device_ >= 0) return dev == device_ ? ACLFFT_SUCCESS : ACLFFT_INVALID_DEVICE;
+    if (!plan_checked_ && has_plan_alternatives()) {
+        // plan-level choice: measured now (ACLFFT_TUNE=1 or aclfftExtTune), else from the cache
+        plan_checked_ = true;
+        if (force_tune_ || env_int("ACLFFT_TUNE", 0) != 0) {
+            aclfftResult r = tune_plan(s);
+            if (r != ACLFFT_SUCCESS) return r;
+        } else {
+            PlanDesc k;
+            Executor* w = nullptr;
+            if (load_plan_choice(desc_, &k) && build(k, &w) == ACLFFT_SUCCESS) {
+                if (owns_work_ || w->work_bytes() <= work_bytes())
+                    adopt(w);
+                else
+                    delete w;
+            }
+        }
+    }
+    plan_checked_ = true;
     if (owns_work_ && !auto_alloc_ && (work_bytes_ > 0 || work1_bytes_ > 0))
         return ACLFFT_NO_WORKSPACE;  // caller-managed workspace not set yet
     if (table_bytes_ > 0 && hipMalloc(&d_tables_, table_bytes_) != hipSuccess) {
@@ -1538,7 +1565,7 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
             (void)hipGetLastError();
     }
     device_ = dev;
-    if (!tuned_ && env_int("ACLFFT_TUNE", 0) != 0) {
+    if (!tuned_ && (force_tune_ || env_int("ACLFFT_TUNE", 0) != 0)) {
         aclfftResult r = autotune(s);
         if (r != ACLFFT_SUCCESS) return r;
     } else if (!tuned_) {
@@ -1626,9 +1653,9 @@ aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream
                 b.tiles = static_cast<const ofx_batch_tile*>(table(st.tiles_table));
                 b.counter = static_cast<unsigned int*>(const_cast<void*>(table(st.counter_table)));
                 b.sign = dsign;
-                b.schedule = env_int("ACLFFT_PERSISTENT_SCHEDULE", 1) == 0 ? 0 : 1;
-                // every workgroup resident at once (4 per CU: 34 KiB of LDS each)
-                const int per_cu = std::max(1, env_int("ACLFFT_PERSISTENT_BLOCKS_PER_CU", ofx_batch_blocks_per_cu(b.schedule)));
+                // by default every workgroup resident at once (4 per CU: 34 KiB of LDS each)
+                const int per_cu =
+                    st.blocks_per_cu > 0 ? st.blocks_per_cu : std::max(1, ofx_batch_blocks_per_cu_set(b.class_set, b.schedule));
                 err = (hipError_t)ofx_launch_batch_persistent(&b, num_cus_ * per_cu, s);
                 break;
             }
@@ -1777,6 +1804,206 @@ aclfftResult Executor::run_graph(const void* din, void* dout, int direction, hip
     graphs_.push_back(GraphEntry{din, dout, direction, ge});
     if (hip_log(hipGraphLaunch(ge, s), "hipGraphLaunch") != hipSuccess) return ACLFFT_EXEC_FAILED;
     return ACLFFT_SUCCESS;
+}
+
+// -------------------------------------------------------------------------------------------
+// plan-level tuning: alternative launch lists of the same plan, built host-only, timed on scratch
+// buffers; the fastest (among the accurate ones) replaces this executor's launch list
+// -------------------------------------------------------------------------------------------
+bool Executor::has_plan_alternatives() const { return desc_.uneven || desc_.auto_precision; }
+
+void Executor::release_device() {
+    drop_graphs();
+    if (d_tables_) (void)hipFree(d_tables_);
+    d_tables_ = nullptr;
+    if (owns_work_) {
+        if (d_work_) (void)hipFree(d_work_);
+        if (d_work1_) (void)hipFree(d_work1_);
+        if (d_inplace_) (void)hipFree(d_inplace_);
+        d_work_ = d_work1_ = d_inplace_ = nullptr;
+    }
+    if (d_stage_in_) (void)hipFree(d_stage_in_);
+    if (d_stage_out_) (void)hipFree(d_stage_out_);
+    d_stage_in_ = d_stage_out_ = nullptr;
+    for (hipStream_t ls : lane_streams_) (void)hipStreamDestroy(ls);
+    for (hipEvent_t le : lane_events_) (void)hipEventDestroy(le);
+    lane_streams_.clear();
+    lane_events_.clear();
+    delete adjoint_;
+    adjoint_ = nullptr;
+    device_ = -1;
+    tuned_ = false;
+}
+
+void Executor::adopt(Executor* w) {
+    release_device();
+    desc_ = w->desc_;
+    steps_.swap(w->steps_);
+    tables_.swap(w->tables_);
+    table_bytes_ = w->table_bytes_;
+    work_bytes_ = w->work_bytes_;
+    work1_bytes_ = w->work1_bytes_;
+    in_bytes_ = w->in_bytes_;
+    out_bytes_ = w->out_bytes_;
+    in_place_unsafe_ = w->in_place_unsafe_;
+    twin_work_bytes_ = w->twin_work_bytes_;
+    delete w;
+    if (!owns_work_ && user_work_ != nullptr) {
+        // the caller's area keeps serving the plan (candidates needing more were not considered)
+        char* area = user_work_;
+        owns_work_ = true;
+        d_work_ = d_work1_ = d_inplace_ = nullptr;
+        (void)set_work_area(area);
+    }
+}
+
+namespace {
+
+// deterministic pseudo-random fill of the tuning input, values in [-1, 1)
+void fill_probe(std::vector<float>& h) {
+    unsigned long long x = 0x9E3779B97F4A7C15ULL;
+    for (float& v : h) {
+        x = x * 6364136223846793005ULL + 1442695040888963407ULL;
+        v = (float)((double)(x >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0);
+    }
+}
+
+double rel_l2_host(const std::vector<float>& a, const std::vector<float>& ref) {
+    double num = 0, den = 0;
+    for (size_t i = 0; i < a.size() && i < ref.size(); ++i) {
+        const double d = (double)a[i] - (double)ref[i];
+        num += d * d;
+        den += (double)ref[i] * ref[i];
+    }
+    return den > 0 ? std::sqrt(num / den) : std::sqrt(num);
+}
+
+}  // namespace
+
+aclfftResult Executor::tune_plan(hipStream_t s) {
+    if (!has_plan_alternatives()) return ACLFFT_SUCCESS;
+    // candidate descriptions
+    std::vector<PlanDesc> cands;
+    std::vector<int> precs;
+    if (desc_.auto_precision) {
+        precs.push_back(0);
+        for (int m : {OFX_MFMA_BF16, OFX_MFMA_FP16X2, OFX_MFMA_BF16X3}) precs.push_back(m);
+    } else {
+        precs.push_back(desc_.precision);
+    }
+    for (int prec : precs) {
+        PlanDesc c = desc_;
+        c.precision = prec;
+        c.uneven_form = 0;
+        c.sched = -1;
+        c.blocks_per_cu = 0;
+        c.class_set = 0;
+        if (!desc_.uneven) {
+            cands.push_back(c);
+            continue;
+        }
+        // ACLFFT_UNEVEN_MODE=1|2 restricts the comparison to that form
+        const int env_form = env_int("ACLFFT_UNEVEN_MODE", 0);
+        if (prec == 0 && env_form != 1) {
+            PlanDesc t = c;
+            t.uneven_form = 2;
+            cands.push_back(t);
+        }
+        if (env_form == 2 && prec == 0) continue;
+        for (int set = 0; set < ofx_batch_class_set_count(); ++set)
+            for (int sched = 0; sched <= 1; ++sched) {
+                const int maxb = std::max(1, ofx_batch_blocks_per_cu_set(set, sched));
+                for (int bpc = 1; bpc <= maxb; ++bpc) {
+                    // MFMA precisions: the FP32 part (lengths without MFMA kernels) at its defaults
+                    if (prec != 0 && !(sched == 1 && bpc == maxb)) continue;
+                    PlanDesc t = c;
+                    t.uneven_form = 1;
+                    t.class_set = set;
+                    t.sched = sched;
+                    t.blocks_per_cu = bpc;
+                    cands.push_back(t);
+                }
+            }
+    }
+    // scratch input (pseudo-random) and output, sized for the plan's buffers
+    const size_t ib = std::max<size_t>(in_bytes_, 16), ob = std::max<size_t>(out_bytes_, 16);
+    void* din = nullptr;
+    void* dout = nullptr;
+    if (hipMalloc(&din, ib) != hipSuccess || hipMalloc(&dout, ob) != hipSuccess) {
+        (void)hipGetLastError();
+        if (din) (void)hipFree(din);
+        return ACLFFT_ALLOC_FAILED;
+    }
+    std::vector<float> host(ib / 4);
+    fill_probe(host);
+    aclfftResult status = ACLFFT_SUCCESS;
+    if (hipMemcpy(din, host.data(), ib, hipMemcpyHostToDevice) != hipSuccess) status = ACLFFT_EXEC_FAILED;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    std::vector<float> ref, got;  // the FP32 candidate's output and a candidate's, for the accuracy gate
+    const PlanDesc* best = nullptr;
+    float best_ms = 1e30f;
+    const size_t my_work = work_bytes();
+    for (const PlanDesc& c : cands) {
+        if (status != ACLFFT_SUCCESS) break;
+        Executor* x = nullptr;
+        if (build(c, &x) != ACLFFT_SUCCESS) continue;
+        if (!owns_work_ && x->work_bytes() > my_work) {  // must fit the caller's work area
+            delete x;
+            continue;
+        }
+        x->plan_checked_ = true;
+        x->tuned_ = true;  // its step kernels stay at their defaults while plans are compared
+        x->use_graphs_ = false;
+        bool ok = x->exec(din, dout, ACLFFT_FORWARD, s) == ACLFFT_SUCCESS &&
+                  x->exec(din, dout, ACLFFT_FORWARD, s) == ACLFFT_SUCCESS && hipStreamSynchronize(s) == hipSuccess;
+        float ms = 0.f;
+        if (ok) {
+            (void)hipEventRecord(e0, s);
+            for (int r = 0; ok && r < 5; ++r) ok = x->exec(din, dout, ACLFFT_FORWARD, s) == ACLFFT_SUCCESS;
+            (void)hipEventRecord(e1, s);
+            ok = ok && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        }
+        if (ok && desc_.auto_precision) {
+            // accuracy gate: an MFMA split is kept only within rel-L2 1e-6 of the FP32 result
+            std::vector<float>& dst = c.precision == 0 ? ref : got;
+            dst.resize(ob / 4);
+            ok = hipMemcpy(dst.data(), dout, ob, hipMemcpyDeviceToHost) == hipSuccess;
+            if (ok && c.precision != 0) ok = !ref.empty() && rel_l2_host(got, ref) <= 1e-6;
+        }
+        delete x;
+        if (!ok) {
+            (void)hipGetLastError();
+            continue;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = &c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    if (status != ACLFFT_SUCCESS) return status;
+    if (best == nullptr) return ACLFFT_SUCCESS;  // keep the current launch list
+    Executor* w = nullptr;
+    if (build(*best, &w) != ACLFFT_SUCCESS) return ACLFFT_SUCCESS;
+    const bool graphs = use_graphs_;
+    adopt(w);
+    use_graphs_ = graphs;
+    store_plan_choice(desc_);
+    return ACLFFT_SUCCESS;
+}
+
+aclfftResult Executor::tune(hipStream_t s) {
+    release_device();
+    plan_checked_ = false;
+    force_tune_ = true;
+    const aclfftResult r = prepare_device(s);
+    force_tune_ = false;
+    return r;
 }
 
 // -------------------------------------------------------------------------------------------

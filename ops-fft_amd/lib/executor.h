@@ -51,6 +51,7 @@ struct Step {
     ofx_batch_args b{};            // PERSISTENT: launch arguments (pointers filled at execution)
     int class_tables[OFX_BCLASS_COUNT] = {0};  // PERSISTENT: pass table of each class (-1 none)
     int tiles_table = -1, counter_table = -1;
+    int blocks_per_cu = 0;         // PERSISTENT: workgroups per CU (0: all that fit at once)
     bool final_out = false;        // uneven plans: the last step writing its group's output region
                                    // (the plan's scale is applied there)
     int lane = 0;                  // 0: the caller's stream; k > 0: forked onto the plan's k-th
@@ -68,10 +69,18 @@ struct PlanDesc {
     aclfftType type = ACLFFT_C2C;
     bool vertical = false;         // 1D VERTICAL layout: n0 rows x `batch` columns
     int precision = 0;             // 0: FP32 Stockham; else enum ofx_mfma_mode (MFMA split path)
+    bool auto_precision = false;   // the tuner chooses `precision` (FP32 or an MFMA split within
+                                   // 1e-6 of the FP32 result); FP32 until tuned
     bool hermitian_weight = false; // adjoint real transforms: OFX_FLAG_HERMITIAN_WEIGHT on real steps
     float scale = 1.f;             // multiplies the output (normalisation), fused in the last step
     bool uneven = false;           // aclfftExtPlanManyUneven: groups of (length, batch)
     std::vector<int> ulen, ubatch;
+    // knobs of uneven plans, set by the autotuner (zero / negative: the default rule or the
+    // ACLFFT_* environment overrides)
+    int uneven_form = 0;           // 1 persistent kernel, 2 one tile launch per length
+    int sched = -1;                // persistent schedule: 0 dynamic, 1 static
+    int blocks_per_cu = 0;         // persistent workgroups per CU (0: all that fit at once)
+    int class_set = 0;             // compiled class set of the persistent kernel
 };
 
 class Executor {
@@ -103,12 +112,27 @@ public:
     const PlanDesc& desc() const { return desc_; }
     std::string describe() const;
 
+    // Measure the alternatives of this plan on scratch buffers on stream s and keep the fastest:
+    // for uneven plans the form (persistent kernel or per-length launches), schedule, workgroups
+    // per CU and class set; with automatic precision the operand split (MFMA modes only when their
+    // result is within rel-L2 1e-6 of the FP32 path's); for every plan each step's kernel
+    // candidates.  Synchronous; allocates temporary buffers; the choices go to the tuning cache,
+    // from which later plans of the same shape take them on their first execution.
+    aclfftResult tune(hipStream_t s);
+
 private:
     Executor() = default;
     aclfftResult prepare_device(hipStream_t s);
     aclfftResult run(const void* din, void* dout, int direction, hipStream_t s);
     aclfftResult autotune(hipStream_t s);
+    aclfftResult tune_plan(hipStream_t s);
+    bool has_plan_alternatives() const;
+    // take over w's launch list (this executor's device state is released first); deletes w
+    void adopt(Executor* w);
+    void release_device();
     int add_table(const TableSpec& t);
+    bool plan_checked_ = false;   // plan-level choice looked up (cache) or measured
+    bool force_tune_ = false;
 
     PlanDesc desc_{};
     std::vector<Step> steps_;

@@ -20,10 +20,20 @@ ACLFFT_API aclfftResult aclfftExtDescribe(aclfftHandle plan, char* buf, size_t l
 ACLFFT_API aclfftResult aclfftExtSetPrecision(aclfftHandle plan, int precision) {
     if (plan == nullptr) return ACLFFT_INVALID_VALUE;
     if (!plan->is_initialized) return ACLFFT_INVALID_PLAN;
-    if (precision < ACLFFT_EXT_PRECISION_FP32 || precision > ACLFFT_EXT_PRECISION_BF16X3) return ACLFFT_INVALID_VALUE;
+    if (precision < ACLFFT_EXT_PRECISION_FP32 || precision > ACLFFT_EXT_PRECISION_AUTO) return ACLFFT_INVALID_VALUE;
     if (plan->exec == nullptr) return ACLFFT_NOT_SUPPORTED;
     ofx::PlanDesc d = plan->exec->desc();
-    d.precision = precision;
+    d.auto_precision = precision == ACLFFT_EXT_PRECISION_AUTO;
+    d.precision = d.auto_precision ? ACLFFT_EXT_PRECISION_FP32 : precision;
+    if (d.auto_precision) {
+        // automatic precision needs an MFMA alternative to choose from
+        ofx::PlanDesc probe = d;
+        probe.precision = ACLFFT_EXT_PRECISION_FP16X2;
+        ofx::Executor* t = nullptr;
+        const aclfftResult rp = ofx::Executor::build(probe, &t);
+        delete t;
+        if (rp != ACLFFT_SUCCESS) return rp == ACLFFT_NOT_IMPLEMENTED ? ACLFFT_NOT_SUPPORTED : rp;
+    }
     ofx::Executor* e = nullptr;
     aclfftResult r = ofx::Executor::build(d, &e);
     if (r != ACLFFT_SUCCESS) return r == ACLFFT_NOT_IMPLEMENTED ? ACLFFT_NOT_SUPPORTED : r;
@@ -112,6 +122,13 @@ ACLFFT_API aclfftResult aclfftExtExecR2CAdjoint(aclfftHandle plan, aclfftComplex
 
 ACLFFT_API aclfftResult aclfftExtExecC2RAdjoint(aclfftHandle plan, aclfftReal* grad_out, aclfftComplex* grad_in) {
     return exec_adjoint(plan, grad_out, grad_in, ACLFFT_C2R);
+}
+
+ACLFFT_API aclfftResult aclfftExtTune(aclfftHandle plan) {
+    if (plan == nullptr) return ACLFFT_INVALID_VALUE;
+    if (!plan->is_initialized) return ACLFFT_INVALID_PLAN;
+    if (plan->exec == nullptr) return ACLFFT_NOT_IMPLEMENTED;
+    return plan->exec->tune(plan->stream);
 }
 
 ACLFFT_API aclfftResult aclfftExtSetGraphMode(aclfftHandle plan, int on) {

@@ -99,6 +99,37 @@ bool tunable(const Step& st) {
 
 }  // namespace
 
+// write the whole cache to a private file, then rename it over the cache (called with g_mu held)
+static void write_cache(const std::map<std::string, std::string>& m) {
+    const std::string path = cache_path();
+    // create the directory chain of the location (none for a bare file name)
+    const size_t slash = path.find_last_of('/');
+    std::string dir = slash == std::string::npos ? std::string() : path.substr(0, slash);
+    std::string acc;
+    std::istringstream parts(dir);
+    std::string part;
+    while (std::getline(parts, part, '/')) {
+        if (part.empty()) {
+            acc = "/";
+            continue;
+        }
+        acc += (acc.empty() || acc.back() == '/' ? "" : "/") + part;
+        mkdir(acc.c_str(), 0755);
+    }
+    // write a private file and rename it over the cache, so that a reader (or another process
+    // writing at the same time) never sees a partly written file
+    const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+    {
+        std::ofstream f(tmp, std::ios::trunc);
+        for (auto& kv : m) f << kv.first << ' ' << kv.second << '\n';
+        if (!f.good()) {
+            std::remove(tmp.c_str());
+            return;
+        }
+    }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+}
+
 void apply_tune_cache(const PlanDesc& d, std::vector<Step>& steps, const Registries& reg) {
     std::lock_guard<std::mutex> lk(g_mu);
     auto m = load();
@@ -127,33 +158,43 @@ void store_tune_cache(const PlanDesc& d, const std::vector<Step>& steps, const R
         const std::string sig = candidate_signature(steps[i], steps[i].kernel, reg);
         if (!sig.empty()) m[key_of(d, i)] = sig;
     }
-    const std::string path = cache_path();
-    // create the directory chain of the location (none for a bare file name)
-    const size_t slash = path.find_last_of('/');
-    std::string dir = slash == std::string::npos ? std::string() : path.substr(0, slash);
-    std::string acc;
-    std::istringstream parts(dir);
-    std::string part;
-    while (std::getline(parts, part, '/')) {
-        if (part.empty()) {
-            acc = "/";
-            continue;
-        }
-        acc += (acc.empty() || acc.back() == '/' ? "" : "/") + part;
-        mkdir(acc.c_str(), 0755);
-    }
-    // write a private file and rename it over the cache, so that a reader (or another process
-    // writing at the same time) never sees a partly written file
-    const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
-    {
-        std::ofstream f(tmp, std::ios::trunc);
-        for (auto& kv : m) f << kv.first << ' ' << kv.second << '\n';
-        if (!f.good()) {
-            std::remove(tmp.c_str());
-            return;
-        }
-    }
-    if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+    write_cache(m);
+}
+
+namespace {
+
+// the plan-level key: the shape (as for steps, the group hash for uneven plans) and the requested
+// precision (automatic or a fixed mode)
+std::string plan_key(const PlanDesc& d) {
+    return key_of(d, 0) + (d.auto_precision ? std::string(" plan auto") : " plan p" + std::to_string(d.precision));
+}
+
+}  // namespace
+
+bool load_plan_choice(const PlanDesc& d, PlanDesc* out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto m = load();
+    auto it = m.find(plan_key(d));
+    if (it == m.end()) return false;
+    // "form:sched:bpc:set:precision"
+    int v[5] = {0, -1, 0, 0, 0};
+    if (std::sscanf(it->second.c_str(), "%d:%d:%d:%d:%d", &v[0], &v[1], &v[2], &v[3], &v[4]) != 5) return false;
+    *out = d;
+    out->uneven_form = v[0];
+    out->sched = v[1];
+    out->blocks_per_cu = v[2];
+    out->class_set = v[3];
+    out->precision = v[4];
+    return true;
+}
+
+void store_plan_choice(const PlanDesc& c) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto m = load();
+    m[plan_key(c)] = std::to_string(c.uneven_form) + ":" + std::to_string(c.sched) + ":" +
+                     std::to_string(c.blocks_per_cu) + ":" + std::to_string(c.class_set) + ":" +
+                     std::to_string(c.precision);
+    write_cache(m);
 }
 
 }  // namespace ofx

@@ -29,6 +29,10 @@ struct Registries {
 void apply_tune_cache(const PlanDesc& d, std::vector<Step>& steps, const Registries& reg);
 // Record the steps' current choices.
 void store_tune_cache(const PlanDesc& d, const std::vector<Step>& steps, const Registries& reg);
+// Plan-level choices (uneven-plan knobs, automatic precision): the chosen knobs are written into
+// *out (a copy of d) when the cache holds an entry for d's shape.
+bool load_plan_choice(const PlanDesc& d, PlanDesc* out);
+void store_plan_choice(const PlanDesc& chosen);
 
 }  // namespace ofx
 

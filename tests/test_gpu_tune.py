@@ -115,4 +115,98 @@ def test_autotune_uneven_per_length(cuda, tmp_path, monkeypatch):
                 off += n * b
         assert described[0] == described[1], described
     text = (tmp_path / "tune.txt").read_text()
-    assert len({line.split()[-2] for line in text.splitlines()}) == 2, text  # two group hashes
+    hashes = {tok for line in text.splitlines() for tok in line.split() if tok.startswith("u")}
+    assert len(hashes) == 2, text  # two group hashes
+    assert all(" plan p0 2:" in line for line in text.splitlines() if " plan " in line), text  # the forced form
+
+
+def uneven_groups(count, seed, lengths=(60, 128, 243, 1000, 1024, 2048, 4096, 6561)):
+    rng = np.random.default_rng(seed)
+    logs = np.log(np.array(lengths, float))
+    u = rng.uniform(logs.min(), logs.max(), count)
+    ln = [int(lengths[int(np.argmin(np.abs(logs - v)))]) for v in u]
+    bt = [int(b) for b in rng.integers(1, 65, count)]
+    return ln, bt
+
+
+def check_uneven(p, ln, bt, seed):
+    import torch
+    groups = list(zip(ln, bt))
+    total = sum(n * b for n, b in groups)
+    x = crandn(total, seed).astype(np.complex64)
+    y = torch.empty(total, dtype=torch.complex64, device="cuda")
+    p.exec_c2c(torch.from_numpy(x).cuda(), y, aclfft.FORWARD)
+    torch.cuda.synchronize()
+    out = y.cpu().numpy()
+    off = 0
+    for n, b in groups:
+        ref = oracle.c2c(x[off:off + n * b].reshape(b, n).astype(np.complex128))
+        assert oracle.rel_l2(out[off:off + n * b].reshape(b, n), ref) <= fp32_tol(n), (n, b)
+        off += n * b
+
+
+@pytest.mark.parametrize("auto", [False, True])
+def test_tuned_uneven_plan(cuda, tmp_path, monkeypatch, auto):
+    """aclfftExtTune on an uneven plan: the plan-level choice (form, schedule, workgroups per CU,
+    class set; with automatic precision also FP32 or an MFMA split) and the step kernels; the
+    tuned plan matches the oracle, and a new plan of the same groups takes the cached choice."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("ACLFFT_TUNE_CACHE", str(cache))
+    monkeypatch.delenv("ACLFFT_TUNE", raising=False)
+    ln, bt = uneven_groups(60, 3)
+    p = aclfft.Plan.plan_many_uneven(ln, bt)
+    if auto:
+        p.set_precision(aclfft.PRECISION_AUTO)
+    before = p.describe()
+    p.tune()
+    after = p.describe()
+    print("default:", before)
+    print("tuned:  ", after)
+    check_uneven(p, ln, bt, 5)
+    text = cache.read_text()
+    assert (" plan auto " if auto else " plan p0 ") in text
+    q = aclfft.Plan.plan_many_uneven(ln, bt)
+    if auto:
+        q.set_precision(aclfft.PRECISION_AUTO)
+    check_uneven(q, ln, bt, 6)  # first execution applies the cached plan choice
+    assert q.describe() == after
+    p.destroy()
+    q.destroy()
+
+
+@pytest.mark.parametrize("kind,n", [("c2c", 512), ("c2c", 1024), ("c2c", 4096), ("r2c", 8192)])
+def test_automatic_precision(cuda, tmp_path, monkeypatch, kind, n):
+    """ACLFFT_EXT_PRECISION_AUTO: FP32 until tuned; the tuner keeps FP32 or an MFMA split within
+    1e-6 of the FP32 result (single BF16 never qualifies); the result meets the FP32 bar."""
+    import torch
+    monkeypatch.setenv("ACLFFT_TUNE_CACHE", str(tmp_path / "tune.txt"))
+    b = 512
+    t = aclfft.C2C if kind == "c2c" else aclfft.R2C
+    p = aclfft.Plan.plan1d(n, t, b)
+    p.set_precision(aclfft.PRECISION_AUTO)
+    assert "mfma" not in p.describe()
+    p.tune()
+    d = p.describe()
+    print(kind, n, "->", d)
+    assert "mode=bf16 " not in d + " "
+    if kind == "c2c":
+        x = crandn((b, n), 9).astype(np.complex64)
+        y = torch.empty((b, n), dtype=torch.complex64, device="cuda")
+        p.exec_c2c(torch.from_numpy(x).cuda(), y, aclfft.FORWARD)
+        ref = oracle.c2c(x.astype(np.complex128))
+    else:
+        x = np.random.default_rng(9).standard_normal((b, n)).astype(np.float32)
+        y = torch.empty((b, n // 2 + 1), dtype=torch.complex64, device="cuda")
+        p.exec_r2c(torch.from_numpy(x).cuda(), y)
+        ref = oracle.r2c(x.astype(np.float64))
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(y.cpu().numpy(), ref) <= fp32_tol(n)
+    p.destroy()
+
+
+def test_automatic_precision_gate(cuda):
+    p = aclfft.Plan.plan1d(1000, aclfft.C2C, 4)
+    with pytest.raises(aclfft.AclfftError) as e:
+        p.set_precision(aclfft.PRECISION_AUTO)
+    assert e.value.code == aclfft.NOT_SUPPORTED
+    p.destroy()
