@@ -466,6 +466,22 @@ def extra_small_latency(torch, aclfft, oracle, np, stream):
     out["captured_graph"] = {"device_us_per_call": round(e0.elapsed_time(e1) * 1e3 / (replays * per_graph), 2),
                              "calls_per_graph": per_graph}
     del g
+    # the runtime's own cost of replaying a one-kernel graph: the same call captured alone in a torch
+    # graph, replayed back to back (what the plan's per-call hipGraph pays on top of the kernel)
+    g1 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g1, stream=side, capture_error_mode="relaxed"):
+        p.exec_c2c(x, y, aclfft.FORWARD)
+    for _ in range(20):
+        g1.replay()
+    torch.cuda.synchronize()
+    reps1 = 2000
+    e0.record(cur)
+    for _ in range(reps1):
+        g1.replay()
+    e1.record(cur)
+    torch.cuda.synchronize()
+    out["one_call_graph_replay"] = {"device_us_per_replay": round(e0.elapsed_time(e1) * 1e3 / reps1, 2)}
+    del g1
     p.destroy()
     out["ideal_us_at_8TBs"] = round(2 * x.numel() * 8 / 8e12 * 1e6, 3)
     return out

@@ -215,8 +215,9 @@ OFX_API int ofx_mfma_kernel_info_get(int id, ofx_mfma_info* info);
 OFX_API long long ofx_mfma_kernel_const_bytes(int id, int mode);
 OFX_API int ofx_gen_mfma_kernel_consts(int id, int mode, void* table, hipStream_t stream);
 /* With args->tiles set (entries other than 0): a tile-list launch for uneven batches, `batch` = the
- * number of ofx_batch_tile entries, workgroup b transforms tiles[b].count contiguous transforms at
- * complex64 offset tiles[b].offset of in and out (offsets may be odd). */
+ * number of ofx_batch_tile entries; workgroup b transforms tiles b, b + grid, ... (grid = blocks,
+ * default min(batch, 1024)), each of tiles[i].count contiguous transforms at complex64 offset
+ * tiles[i].offset of in and out (offsets may be odd). */
 OFX_API int ofx_launch_mfma_kernel(int id, int mode, const ofx_args* args, int blocks, hipStream_t stream);
 
 /* Persistent batched C2C over uneven lengths: compiled length classes. */

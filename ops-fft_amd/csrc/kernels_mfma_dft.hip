@@ -396,27 +396,35 @@ __global__ __launch_bounds__(kDftBlock) void k_mfma_dft(ofx_args a) {
     const float2* __restrict__ in = static_cast<const float2*>(a.in);
     float2* __restrict__ out = static_cast<float2*>(a.out);
     // plain launch: the grid strides over the batch; tile-list launch (uneven batches): workgroup b
-    // runs the tile's `count` contiguous transforms at element offset `offset`, one wave each
-    long long t = (long long)blockIdx.x * (kDftBlock / 64) + wave, stride = (long long)gridDim.x * (kDftBlock / 64);
-    long long end = a.batch;
-    if (a.tiles != nullptr) {
-        const ofx_batch_tile tile = static_cast<const ofx_batch_tile*>(a.tiles)[blockIdx.x];
-        in += tile.offset;
-        out += tile.offset;
-        t = wave;
-        stride = kDftBlock / 64;
-        end = tile.count;
-    }
-    for (; t < end; t += stride) {
-        // constants are re-read from LDS for every transform rather than held in VGPRs
-        __asm__ volatile("" ::: "memory");
-        if constexpr (TILED) {
-            dft_transform_tiled<MODE, SH, NT>(in + t * a.idist, out + t * a.odist, cst, tw, lane, r, g, a.in_sign,
-                                              a.out_sign, a.scale);
-        } else {
-            float2 v[SH::S2][SH::S1][4];
-            dft_load<SH, NT>(v, in + t * a.idist, r, g);
-            dft_transform<MODE, SH, NT>(v, out + t * a.odist, cst, tw, lane, r, g, a.in_sign, a.out_sign, a.scale);
+    // runs tiles b, b + grid, ... (`batch` tiles), each of `count` contiguous transforms at element
+    // offset `offset`, one wave per transform, so the constant tables are loaded once per
+    // workgroup, not once per tile
+    const bool tiled_list = a.tiles != nullptr;
+    const long long ntile = tiled_list ? a.batch : 1;
+    for (long long ti = tiled_list ? blockIdx.x : 0; ti < ntile; ti += tiled_list ? gridDim.x : 1) {
+        const float2* __restrict__ tin = in;
+        float2* __restrict__ tout = out;
+        long long t = (long long)blockIdx.x * (kDftBlock / 64) + wave, stride = (long long)gridDim.x * (kDftBlock / 64);
+        long long end = a.batch;
+        if (tiled_list) {
+            const ofx_batch_tile tile = static_cast<const ofx_batch_tile*>(a.tiles)[ti];
+            tin += tile.offset;
+            tout += tile.offset;
+            t = wave;
+            stride = kDftBlock / 64;
+            end = tile.count;
+        }
+        for (; t < end; t += stride) {
+            // constants are re-read from LDS for every transform rather than held in VGPRs
+            __asm__ volatile("" ::: "memory");
+            if constexpr (TILED) {
+                dft_transform_tiled<MODE, SH, NT>(tin + t * a.idist, tout + t * a.odist, cst, tw, lane, r, g, a.in_sign,
+                                                  a.out_sign, a.scale);
+            } else {
+                float2 v[SH::S2][SH::S1][4];
+                dft_load<SH, NT>(v, tin + t * a.idist, r, g);
+                dft_transform<MODE, SH, NT>(v, tout + t * a.odist, cst, tw, lane, r, g, a.in_sign, a.out_sign, a.scale);
+            }
         }
     }
 }
@@ -559,8 +567,8 @@ extern "C" int ofx_launch_mfma_kernel(int id, int mode, const ofx_args* a, int b
     // one wave per transform; by default a grid of 2048 workgroups strides over the batch (with a
     // tile list: one workgroup per tile, `batch` tiles)
     if (a->tiles != nullptr && a->batch > 0x7fffffffLL) return (int)hipErrorInvalidConfiguration;
-    const unsigned grid =
-        a->tiles != nullptr ? (unsigned)a->batch : (unsigned)std::min<long long>(need, blocks > 0 ? blocks : 2048);
+    const unsigned grid = a->tiles != nullptr ? (unsigned)std::min<long long>(a->batch, blocks > 0 ? blocks : 1024)
+                                              : (unsigned)std::min<long long>(need, blocks > 0 ? blocks : 2048);
     // streaming HBM access when input and output together exceed the 256 MB last-level cache by
     // half again (the threshold of the FP32 kernels' streaming variants; tile lists never stream)
     const bool nt = a->tiles == nullptr && a->batch * (long long)n * 16 >= (384LL << 20);

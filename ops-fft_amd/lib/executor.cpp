@@ -1797,6 +1797,11 @@ aclfftResult Executor::run_graph(const void* din, void* dout, int direction, hip
     e = hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (hip_log(e, "hipGraphInstantiate") != hipSuccess) return ACLFFT_EXEC_FAILED;
+    // move the executable graph's resources to the device now, not on its first launch
+    if (hip_log(hipGraphUpload(ge, s), "hipGraphUpload") != hipSuccess) {
+        (void)hipGraphExecDestroy(ge);
+        return ACLFFT_EXEC_FAILED;
+    }
     if (graphs_.size() >= 8) {  // bounded cache: drop the oldest
         (void)hipGraphExecDestroy(static_cast<hipGraphExec_t>(graphs_.front().exec));
         graphs_.erase(graphs_.begin());
