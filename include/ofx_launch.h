@@ -273,6 +273,13 @@ OFX_API int ofx_batch_blocks_per_cu(int schedule);
  * (adjoint of R2C).  Used by the hand-written adjoint kernels. */
 #define OFX_FLAG_HERMITIAN_WEIGHT 1
 
+/* Device scratch area of the allocation-free execution entry point (aclfftExtExecC2CNoAlloc):
+ * OFX_SCRATCH_BYTES on every GPU, a file-scope device array held from the library's load until the
+ * process ends.  *ptr = its address on the current device (looked up once per device), *bytes =
+ * its size (optional). */
+#define OFX_SCRATCH_BYTES ((size_t)64 << 20)
+OFX_API int ofx_scratch(void** ptr, size_t* bytes);
+
 OFX_API int ofx_kernel_count(void);
 OFX_API int ofx_kernel_info_get(int id, ofx_kernel_info* info);
 

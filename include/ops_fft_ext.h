@@ -78,6 +78,19 @@ ACLFFT_API aclfftResult aclfftExtSetScale(aclfftHandle plan, float scale);
 ACLFFT_API aclfftResult aclfftExtExecR2CAdjoint(aclfftHandle plan, aclfftComplex* grad_out, aclfftReal* grad_in);
 ACLFFT_API aclfftResult aclfftExtExecC2RAdjoint(aclfftHandle plan, aclfftReal* grad_out, aclfftComplex* grad_in);
 
+/* aclfftExecC2C that allocates nothing, for callers that cannot allocate device memory (code under
+ * hipGraph stream capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in
+ * the reference.  Instead of the plan's own device tables and workspace it uses the library's
+ * device scratch area: OFX_SCRATCH_BYTES (64 MiB) per GPU, a device array held from the library's
+ * load until the process ends; the twiddle tables are regenerated into it on every call (a few small
+ * kernels), the four-step workspace follows them.  Device pointers only (host pointers ->
+ * INVALID_VALUE); a plan whose tables plus workspace exceed the area -> INVALID_SIZE (with a
+ * message on stderr); uneven plans -> NOT_SUPPORTED.  Asynchronous on the plan's stream.  Every call
+ * on one GPU uses the same area, so calls that overlap on one GPU (several streams, several host
+ * threads) race: serialise them.  The result is the plan's aclfftExecC2C result. */
+ACLFFT_API aclfftResult aclfftExtExecC2CNoAlloc(aclfftHandle plan, aclfftComplex* idata, aclfftComplex* odata,
+                                                int direction);
+
 #ifdef __cplusplus
 }
 #endif

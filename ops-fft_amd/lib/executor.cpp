@@ -1520,27 +1520,9 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
         (void)hipGetLastError();
         return ACLFFT_ALLOC_FAILED;
     }
-    for (const TableSpec& t : tables_) {
-        void* p = static_cast<char*>(d_tables_) + t.offset;
-        int rc = 0;
-        if (t.kind == TableSpec::PASS)
-            rc = ofx_gen_pass_twiddles(p, t.npass, t.radices, s);
-        else if (t.kind == TableSpec::ROOTS)
-            rc = ofx_gen_roots(p, t.len, t.count, s);
-        else if (t.kind == TableSpec::MFMA_CONST)
-            rc = ofx_gen_mfma_kernel_consts(t.radices[0], t.npass, p, s);
-        else if (t.kind == TableSpec::PASS_Z)
-            rc = ofx_gen_pass_twiddles_z(p, t.npass, t.radices, s);
-        else if (t.kind == TableSpec::ROOTS_Z)
-            rc = ofx_gen_roots_z(p, t.len, t.count, s);
-        else if (t.kind == TableSpec::BLOB)
-            rc = (int)hipMemcpyAsync(p, t.blob.data(), t.bytes, hipMemcpyHostToDevice, s);
-        else
-            rc = ofx_gen_ft_table(p, t.len, s);
-        if (rc != 0) {
-            (void)hip_log((hipError_t)rc, "twiddle-table generation");
-            return ACLFFT_SETUP_FAILED;
-        }
+    {
+        const aclfftResult rt = gen_tables(d_tables_, s);
+        if (rt != ACLFFT_SUCCESS) return rt;
     }
     if (hip_log(hipStreamSynchronize(s), "hipStreamSynchronize (plan setup)") != hipSuccess) return ACLFFT_SETUP_FAILED;
     int lanes = 0;
@@ -1576,11 +1558,81 @@ aclfftResult Executor::prepare_device(hipStream_t s) {
     return ACLFFT_SUCCESS;
 }
 
+aclfftResult Executor::gen_tables(void* base, hipStream_t s) {
+    for (const TableSpec& t : tables_) {
+        void* p = static_cast<char*>(base) + t.offset;
+        int rc = 0;
+        if (t.kind == TableSpec::PASS)
+            rc = ofx_gen_pass_twiddles(p, t.npass, t.radices, s);
+        else if (t.kind == TableSpec::ROOTS)
+            rc = ofx_gen_roots(p, t.len, t.count, s);
+        else if (t.kind == TableSpec::MFMA_CONST)
+            rc = ofx_gen_mfma_kernel_consts(t.radices[0], t.npass, p, s);
+        else if (t.kind == TableSpec::PASS_Z)
+            rc = ofx_gen_pass_twiddles_z(p, t.npass, t.radices, s);
+        else if (t.kind == TableSpec::ROOTS_Z)
+            rc = ofx_gen_roots_z(p, t.len, t.count, s);
+        else if (t.kind == TableSpec::BLOB)
+            rc = (int)hipMemcpyAsync(p, t.blob.data(), t.bytes, hipMemcpyHostToDevice, s);
+        else
+            rc = ofx_gen_ft_table(p, t.len, s);
+        if (rc != 0) {
+            (void)hip_log((hipError_t)rc, "twiddle-table generation");
+            return ACLFFT_SETUP_FAILED;
+        }
+    }
+    return ACLFFT_SUCCESS;
+}
+
 aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream_t s) {
-    void* bufs[4] = {const_cast<void*>(din), dout, d_work_, d_work1_};
+    return run_on(din, dout, direction, s, d_tables_, d_work_, d_work1_);
+}
+
+aclfftResult Executor::exec_noalloc(const void* in, void* out, int direction, hipStream_t s) {
+    if (!is_device_pointer(in) || !is_device_pointer(out)) return ACLFFT_INVALID_VALUE;
+    for (const TableSpec& t : tables_)
+        if (t.kind == TableSpec::BLOB) return ACLFFT_NOT_SUPPORTED;  // host-side tables (tile lists)
+    for (const Step& st : steps_)
+        if (st.lane > 0 || st.type == Step::PERSISTENT) return ACLFFT_NOT_SUPPORTED;
+    const bool inplace = in == out && in_place_unsafe_;
+    // layout: [tables | work0 | work1 | the in-place copy], 256-byte aligned parts
+    const size_t need = align256(table_bytes_) + align256(work_bytes_) + align256(work1_bytes_) +
+                        (inplace ? align256(in_bytes_) : 0);
+    void* base = nullptr;
+    size_t cap = 0;
+    if (hip_log((hipError_t)ofx_scratch(&base, &cap), "device scratch lookup") != hipSuccess) return ACLFFT_SETUP_FAILED;
+    if (need > cap) {
+        std::fprintf(stderr, "[ops-fft] aclfftExtExecC2CNoAlloc: the plan needs %zu bytes of device scratch, "
+                             "the library holds %zu per device\n", need, cap);
+        return ACLFFT_INVALID_SIZE;
+    }
+    char* b = static_cast<char*>(base);
+    char* tables = b;
+    char* w0 = b + align256(table_bytes_);
+    char* w1 = w0 + align256(work_bytes_);
+    char* cp = w1 + align256(work1_bytes_);
+    aclfftResult r = gen_tables(tables, s);
+    if (r != ACLFFT_SUCCESS) return r;
+    if (inplace) {
+        if (hipMemcpyAsync(cp, in, in_bytes_, hipMemcpyDeviceToDevice, s) != hipSuccess) return ACLFFT_EXEC_FAILED;
+        in = cp;
+    }
+    int cus = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        cus > 0)
+        num_cus_ = cus;
+    else
+        (void)hipGetLastError();
+    return run_on(in, out, direction, s, tables, work_bytes_ > 0 ? w0 : nullptr, work1_bytes_ > 0 ? w1 : nullptr);
+}
+
+aclfftResult Executor::run_on(const void* din, void* dout, int direction, hipStream_t s, void* tables, void* work0,
+                              void* work1) {
+    void* bufs[4] = {const_cast<void*>(din), dout, work0, work1};
     const float dsign = (direction == ACLFFT_BACKWARD) ? -1.f : 1.f;
     auto table = [&](int idx) -> const void* {
-        return idx >= 0 ? static_cast<const void*>(static_cast<char*>(d_tables_) + tables_[idx].offset) : nullptr;
+        return idx >= 0 ? static_cast<const void*>(static_cast<char*>(tables) + tables_[idx].offset) : nullptr;
     };
     // fork / join of the side-stream lanes: the side streams wait for everything issued so far on
     // `s`, and `s` waits for every side stream before the next lane-0 step and at the end (under
@@ -1647,7 +1699,7 @@ aclfftResult Executor::run(const void* din, void* dout, int direction, hipStream
                 ofx_batch_args b = st.b;
                 b.in = src;
                 b.out = dst;
-                b.tw = d_tables_;
+                b.tw = tables;
                 for (int c = 0; c < OFX_BCLASS_COUNT; ++c)
                     b.tw_off[c] = st.class_tables[c] >= 0 ? (int)(tables_[st.class_tables[c]].offset / 8) : 0;
                 b.tiles = static_cast<const ofx_batch_tile*>(table(st.tiles_table));

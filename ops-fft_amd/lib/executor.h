@@ -120,10 +120,20 @@ public:
     // from which later plans of the same shape take them on their first execution.
     aclfftResult tune(hipStream_t s);
 
+    // The same execution with no allocation: the device tables and the workspace are placed in
+    // the library's device scratch area (ofx_scratch), the tables regenerated on every call.
+    // Device pointers only; plans whose tables and workspace exceed the area -> INVALID_SIZE; plans
+    // with side-stream lanes or host-side tables (uneven plans) -> NOT_SUPPORTED.
+    aclfftResult exec_noalloc(const void* in, void* out, int direction, hipStream_t stream);
+
 private:
     Executor() = default;
     aclfftResult prepare_device(hipStream_t s);
     aclfftResult run(const void* din, void* dout, int direction, hipStream_t s);
+    // run with explicit device state: the table allocation and the two workspace buffers
+    aclfftResult run_on(const void* din, void* dout, int direction, hipStream_t s, void* tables, void* work0,
+                        void* work1);
+    aclfftResult gen_tables(void* base, hipStream_t s);
     aclfftResult autotune(hipStream_t s);
     aclfftResult tune_plan(hipStream_t s);
     bool has_plan_alternatives() const;

@@ -124,6 +124,17 @@ ACLFFT_API aclfftResult aclfftExtExecC2RAdjoint(aclfftHandle plan, aclfftReal* g
     return exec_adjoint(plan, grad_out, grad_in, ACLFFT_C2R);
 }
 
+ACLFFT_API aclfftResult aclfftExtExecC2CNoAlloc(aclfftHandle plan, aclfftComplex* idata, aclfftComplex* odata,
+                                                int direction) {
+    if (plan == nullptr) return ACLFFT_INVALID_VALUE;
+    if (!plan->is_initialized) return ACLFFT_INVALID_PLAN;
+    if (idata == nullptr || odata == nullptr) return ACLFFT_INVALID_VALUE;
+    if (plan->type != ACLFFT_C2C) return ACLFFT_INVALID_TYPE;
+    if (direction != ACLFFT_FORWARD && direction != ACLFFT_BACKWARD) return ACLFFT_INVALID_VALUE;
+    if (plan->exec == nullptr) return ACLFFT_NOT_IMPLEMENTED;
+    return plan->exec->exec_noalloc(idata, odata, direction, plan->stream);
+}
+
 ACLFFT_API aclfftResult aclfftExtTune(aclfftHandle plan) {
     if (plan == nullptr) return ACLFFT_INVALID_VALUE;
     if (!plan->is_initialized) return ACLFFT_INVALID_PLAN;
