@@ -272,6 +272,10 @@ OFX_API int ofx_batch_blocks_per_cu(int schedule);
 /* R2C: multiply bins 1..n/2-1 by 2 (adjoint of C2R).  C2R: multiply input bins 1..n/2-1 by 1/2
  * (adjoint of R2C).  Used by the hand-written adjoint kernels. */
 #define OFX_FLAG_HERMITIAN_WEIGHT 1
+/* COL kernels with a power-of-two ft_len (complex64): form the four-step twiddle in FP64 (the
+ * factored form's per-lane and per-column factors from FP64 sincospi, their product in FP64, one
+ * rounding to FP32) instead of from FP32 sincospif factors (about 2 ulp). */
+#define OFX_FLAG_FT_FP64 2
 
 /* Device scratch area of the allocation-free execution entry point (aclfftExtExecC2CNoAlloc):
  * OFX_SCRATCH_BYTES on every GPU, a file-scope device array held from the library's load until the

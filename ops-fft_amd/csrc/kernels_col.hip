@@ -46,7 +46,8 @@ struct FtFactored {
 
 // FT: 0 = none; 1 = four-step twiddle W_ftlen^(col * i) on output i from the FP64 two-level table;
 // 2 = the same twiddle computed in registers (ftlen = 2^ftlog2); 3 = the factored register form
-// (FtFactored).  In the element type's precision.
+// (FtFactored) in the element type's precision; 4 = the factored form in FP64 for complex64 data
+// (FP64 factors, FP64 product, one rounding: OFX_FLAG_FT_FP64).
 template <class C, int FT>
 struct ColStore {
     using V = typename C::V;
@@ -60,10 +61,16 @@ struct ColStore {
     const V* ftab;  // FT 3: this column's W^(col*T*m), m < N/T
     V fbase;        // FT 3: W^(col*lane)
     int lane;
+    const double2* ftab64;  // FT 4: the same in FP64
+    double2 fbase64;
     __device__ __forceinline__ void operator()(int i, V v) const {
         if constexpr (FT == 1) v = cmul(v, ft_twiddle_v((V*)nullptr, ft, col * (long long)i, ftlen));
         if constexpr (FT == 2) v = cmul(v, twiddle_pow2_v((V*)nullptr, (unsigned long long)col * (unsigned)i, ftlog2));
         if constexpr (FT == 3) v = cmul(v, cmul(fbase, ftab[(i - lane) / C::T]));
+        if constexpr (FT == 4) {
+            const double2 w = cmul(fbase64, ftab64[(i - lane) / C::T]);
+            v = cmul(v, mkv((scalar_t<V>)w.x, (scalar_t<V>)w.y));
+        }
         if (ok) p[(long long)i * stride] = mkv(v.x * scale, v.y * (sg * scale));
     }
 };
@@ -88,7 +95,16 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
     const S scale = (sizeof(S) == 8 && a.scale64 != 0.0) ? (S)a.scale64 : (S)a.scale;
     ColLoad<C> gin{static_cast<const V*>(a.in) + img * a.idist + cc, a.istride, a.in_sign};
     __shared__ V ftw[FT == 3 ? C::COLS * FtFactored<C>::ES : 1];
+    __shared__ double2 ftw64[FT == 4 ? C::COLS * FtFactored<C>::ES : 1];
     V fbase = mkv(S(1), S(0));
+    double2 fbase64 = make_double2(1.0, 0.0);
+    if constexpr (FT == 4) {
+        constexpr int E = FtFactored<C>::E;
+        for (int m = tid; m < E; m += C::T)
+            ftw64[c * FtFactored<C>::ES + m] = twiddle_pow2_z((unsigned long long)ccf * (unsigned)(C::T * m), ftlog2);
+        fbase64 = twiddle_pow2_z((unsigned long long)ccf * (unsigned)tid, ftlog2);
+        if constexpr (C::NP == 1) __syncthreads();  // otherwise the passes' barriers order the table
+    }
     if constexpr (FT == 3) {
         constexpr int E = FtFactored<C>::E;
         for (int m = tid; m < E; m += C::T)
@@ -101,7 +117,8 @@ __global__ __launch_bounds__(C::BLOCK) void k_col(ofx_args a, long long cols, lo
     const long long oimg = a.cols_lo > 0 ? (img / a.cols_lo) * a.odist_hi + (img % a.cols_lo) * a.odist : img * a.odist;
     ColStore<C, FT> gout{static_cast<V*>(a.out) + oimg + cc, a.ostride, a.out_sign, scale, ok,
                          static_cast<const double2*>(a.tw_aux), ccf, ftlen, ftlog2,
-                         ftw + (FT == 3 ? c * FtFactored<C>::ES : 0), fbase, tid};
+                         ftw + (FT == 3 ? c * FtFactored<C>::ES : 0), fbase, tid,
+                         ftw64 + (FT == 4 ? c * FtFactored<C>::ES : 0), fbase64};
     if constexpr (C::SPLITX)
         split_fft<C>(lds + c, tid, static_cast<const V*>(a.tw), gin, gout);
     else
@@ -119,10 +136,17 @@ hipError_t launch_col(const ofx_args* a, hipStream_t s) {
         const long long f = a->ft_len;
         int lg = 0;
         while ((1LL << lg) < f) ++lg;
-        if ((1LL << lg) == f && lg >= 3 && lg <= 27) {
+        const bool fp64 = (a->flags & OFX_FLAG_FT_FP64) != 0 && sizeof(typename C::V) == 8;
+        if ((1LL << lg) == f && lg >= 3 && lg <= 27 && !(fp64 && !FtFactored<C>::OK)) {
             constexpr int FT = FtFactored<C>::OK ? 3 : 2;
-            hipLaunchKernelGGL((k_col<C, FT>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, f, lg);
+            constexpr int FT64 = (FtFactored<C>::OK && sizeof(typename C::V) == 8) ? 4 : FT;
+            if (fp64)
+                hipLaunchKernelGGL((k_col<C, FT64>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, f, lg);
+            else
+                hipLaunchKernelGGL((k_col<C, FT>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, f, lg);
         } else {
+            // FP64 two-level table (any length; power-of-two lengths without the factored form
+            // when FP64 twiddles are asked for)
             if (a->tw_aux == nullptr) return hipErrorInvalidValue;
             hipLaunchKernelGGL((k_col<C, 1>), dim3((unsigned)blocks), dim3(C::BLOCK), 0, s, *a, a->cols, gpi, f, 0);
         }

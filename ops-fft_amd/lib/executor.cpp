@@ -363,6 +363,14 @@ struct Builder {
         return best > 0;
     }
 
+    // Four-step twiddles of transforms of n >= 2^20 points are formed in FP64 and rounded once
+    // (OFX_FLAG_FT_FP64); shorter ones from FP32 factors (about 2 ulp).  ACLFFT_FOURSTEP_FP64=0|1
+    // forces either form.
+    static void ft_precision(Step& st, long long n) {
+        const int f = env_int("ACLFFT_FOURSTEP_FP64", -1);
+        if (f == 1 || (f < 0 && n >= (1LL << 20))) st.args.flags |= OFX_FLAG_FT_FP64;
+    }
+
     // Three-level four-step, n = N1*N2*N3, index n = n1*N2*N3 + n2*N3 + n3, k = k1 + N1*k2 + N1*N2*k3:
     //   A: columns over n1 (stride N2*N3), twiddle W_n^(k1*(n2*N3 + n3)), src -> dst (in place safe)
     //   B: per (b, k1), columns over n2 (stride N3), twiddle W_(N2*N3)^(k2*n3), stored to the
@@ -377,6 +385,7 @@ struct Builder {
         if (r != ACLFFT_SUCCESS) return r;
         e->steps_.back().args.ft_len = n;
         e->steps_.back().aux_table = fourstep_table(n);
+        ft_precision(e->steps_.back(), n);
         r = c2c_columns(N2, N3, batch * N1, dst, scratch, false, false, z);
         if (r != ACLFFT_SUCCESS) return r;
         Step& b = e->steps_.back();
@@ -384,6 +393,7 @@ struct Builder {
         b.args.ostride = batch * N1 * N3;
         b.args.ft_len = N2 * N3;
         b.aux_table = fourstep_table(N2 * N3);
+        ft_precision(b, n);
         Step c = kernel_step(rowt_kind(z), N3, kernels_for(rowt_kind(z), N3, true));
         c.src = scratch;
         c.dst = dst;
@@ -432,6 +442,7 @@ struct Builder {
             a.args.odist = T1 * P;
             a.args.ostride = P;
             a.aux_table = fourstep_table(n);
+            ft_precision(a, n);
             // row step: rows of length N2, transposed store X[k1 + N1*k2]
             auto ids = kernels_for(rowt_kind(z), T2, true);
             Step b = kernel_step(rowt_kind(z), T2, ids);
@@ -523,6 +534,7 @@ struct Builder {
         a.args.ft_len = n;
         a.args.ft_div = cols;
         a.aux_table = fourstep_table(n);
+        ft_precision(a, n);
         r = c2c_columns(N2, cols, N1 * images, ws, dst, false, conj_out, z);
         if (r != ACLFFT_SUCCESS) return r;
         Step& b = e->steps_.back();
@@ -1440,7 +1452,7 @@ std::string Executor::describe() const {
             os << (k.kind == OFX_KIND_COL_Z ? "column-z n=" : "column n=") << k.n << " T=" << k.threads_per_transform << " cols=" << k.transforms_per_block
                << " radices=";
             for (int p = 0; p < k.npass; ++p) os << (p ? "x" : "") << k.radices[p];
-            if (s.args.ft_len) os << " fourstep-twiddle";
+            if (s.args.ft_len) os << ((s.args.flags & OFX_FLAG_FT_FP64) ? " fourstep-twiddle-fp64" : " fourstep-twiddle");
             continue;
         }
         if (s.type == Step::PERSISTENT) {

@@ -144,3 +144,32 @@ def test_every_rowt_kernel(cuda, monkeypatch, kind, n2, variant):
     p.destroy()
     tol = 1e-14 * np.log2(n) if z else fp32_tol(n)
     assert err <= tol, err
+
+
+@pytest.mark.parametrize("n,batch", [(1 << 20, 2), (1 << 21, 2), (1 << 22, 1), (1 << 24, 1), (1 << 19, 3)])
+def test_fourstep_twiddle_precision(cuda, monkeypatch, n, batch):
+    """Four-step twiddles: formed in FP64 and rounded once (the default for n >= 2^20) or from
+    FP32 factors (about 2 ulp; ACLFFT_FOURSTEP_FP64=0); both meet the FP32 bar against the FP64
+    oracle, and the FP64 form is no worse than the FP32 form beyond noise."""
+    import torch
+    x = crandn((batch, n), seed=n % 7919).astype(np.complex64)
+    ref = oracle.c2c(x.astype(np.complex128))
+    xt = torch.from_numpy(x).cuda()
+    errs = {}
+    for f in ("0", "1"):
+        monkeypatch.setenv("ACLFFT_FOURSTEP_FP64", f)
+        p = aclfft.Plan.plan1d(n, aclfft.C2C, batch)
+        desc = p.describe()
+        assert ("fourstep-twiddle-fp64" in desc) == (f == "1"), desc
+        yt = torch.empty_like(xt)
+        p.exec_c2c(xt, yt, aclfft.FORWARD)
+        torch.cuda.synchronize()
+        p.destroy()
+        errs[f] = oracle.rel_l2(yt.cpu().numpy(), ref)
+        assert errs[f] <= fp32_tol(n), (f, errs[f])
+    monkeypatch.delenv("ACLFFT_FOURSTEP_FP64")
+    p = aclfft.Plan.plan1d(n, aclfft.C2C, batch)
+    assert ("fourstep-twiddle-fp64" in p.describe()) == (n >= (1 << 20))
+    p.destroy()
+    print("n=%d: rel-L2 FP32 factored twiddles %.4e, FP64 twiddles %.4e" % (n, errs["0"], errs["1"]))
+    assert errs["1"] <= errs["0"] * 1.05
