@@ -343,11 +343,15 @@ OFX_DFT_SPLIT(8, 2, 4)
 OFX_DFT_SPLIT(9, 3, 3)
 OFX_DFT_SPLIT(10, 2, 5)
 OFX_DFT_SPLIT(12, 4, 3)
+OFX_DFT_SPLIT(14, 2, 7)
 OFX_DFT_SPLIT(15, 3, 5)
 OFX_DFT_SPLIT(16, 4, 4)
+OFX_DFT_SPLIT(20, 4, 5)
+OFX_DFT_SPLIT(21, 3, 7)
 OFX_DFT_SPLIT(25, 5, 5)
 OFX_DFT_SPLIT(27, 3, 9)
 OFX_DFT_SPLIT(32, 4, 8)
+OFX_DFT_SPLIT(35, 5, 7)
 OFX_DFT_SPLIT(49, 7, 7)
 OFX_DFT_SPLIT(64, 8, 8)
 #undef OFX_DFT_ODD

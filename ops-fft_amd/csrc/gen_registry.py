@@ -29,6 +29,11 @@ PLANS = {
     60: [(4, 3, 5)], 243: [(27, 9)], 1000: [(10, 10, 10)], 6561: [(27, 27, 9)],
     1500: [(12, 5, 25), (10, 10, 15)],
     3000: [(8, 15, 25)],
+    # the reference's own case lengths (its mixed-radix and real-transform tests) and the complex
+    # halves of its real lengths 1536, 2100, 5040 and 25200
+    6: [(6,)], 105: [(15, 7)], 210: [(14, 15)], 420: [(20, 21)], 768: [(16, 16, 3)], 1050: [(10, 15, 7), (6, 25, 7)],
+    1536: [(16, 16, 6)], 2100: [(12, 25, 7)], 2310: [(10, 21, 11)], 2520: [(12, 14, 15)], 5040: [(12, 20, 21)],
+    12600: [(8, 9, 25, 7)],
 }
 
 
@@ -37,7 +42,10 @@ def elems(n, t, rad):
 
 
 def choose_t(n, rad):
-    """Threads per transform: about 16 elements per thread, wave-contained when possible."""
+    """Threads per transform: about 16 elements per thread, wave-contained when possible; lengths
+    beyond 8192 take a whole 1024-thread workgroup."""
+    if n > 8192:
+        return 1024
     best = None
     for t in (1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024):
         e = elems(n, t, rad)
@@ -202,7 +210,8 @@ def lines():
             tpb //= 2
         out.append(("ROWT", n, t, tpb, 4, False, rad))
     # real transforms: even n = 2N uses the N-point complex kernel with fused pack/unpack
-    for n in (1024, 2048, 1500, 4096, 512, 256, 128, 64, 32, 16, 8, 4, 2, 8192, 3000, 1000):
+    for n in (1024, 2048, 1500, 4096, 512, 256, 128, 64, 32, 16, 8, 4, 2, 8192, 3000, 1000,
+              768, 1050, 2520, 12600):
         rad = PLANS[n][0]
         t = choose_t(n, rad)
         tpb = tpb_for(n, t)
