@@ -166,8 +166,10 @@ ACLFFT_API aclfftResult aclfftExecR2C(aclfftHandle plan, aclfftReal* idata, aclf
  * (reference include/cann_ops_fft.h:376-378, lib/fft_exec_api.cpp:70-85). */
 ACLFFT_API aclfftResult aclfftExecC2R(aclfftHandle plan, aclfftComplex* idata, aclfftReal* odata);
 
-/* Double-precision placeholders: ACLFFT_NOT_IMPLEMENTED (reference lib/fft_exec_api.cpp:87-116;
- * such plans are already refused at plan time). */
+/* Double precision: complex128 / float64 counterparts of ExecC2C / ExecR2C / ExecC2R on plans of
+ * type Z2Z / D2Z / Z2D.  The reference ships these as NOT_IMPLEMENTED placeholders and refuses such
+ * plans (lib/fft_exec_api.cpp:87-116, lib/fft_plan_api.cpp:58-62); here they run on the complex128
+ * kernels (ACLFFT_STRICT_REFERENCE_DOMAIN=1 restores the refusal). */
 ACLFFT_API aclfftResult aclfftExecZ2Z(aclfftHandle plan, aclfftDoubleComplex* idata, aclfftDoubleComplex* odata,
                                       int direction);
 ACLFFT_API aclfftResult aclfftExecD2Z(aclfftHandle plan, aclfftDoubleReal* idata, aclfftDoubleComplex* odata);

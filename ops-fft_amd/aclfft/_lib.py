@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_PKG_DIR), "libcann_ops_fft.so")
+# ACLFFT_LIBRARY selects another build of the library (the sanitizer build of the host layer)
+LIB_PATH = os.environ.get("ACLFFT_LIBRARY") or os.path.join(os.path.dirname(_PKG_DIR), "libcann_ops_fft.so")
 
 # aclfftResult values (include/cann_ops_fft.h)
 SUCCESS = 0

@@ -41,7 +41,8 @@ ACLFFT_API const char* aclfftGetErrorString(aclfftResult result) {
 //   aclfftGetSize1d / 2d    the workspace a plan of these parameters would need (host-only build of
 //                           the launch list; same validation and codes as MakePlan1d / 2d; the
 //                           2D query is for one transform, as the call has no batch argument)
-//   aclfftGetSize3d         NOT_IMPLEMENTED (no 3D plans, as in the reference)
+//   aclfftGetSize3d         the same for 3D plans (the reference's is a placeholder, kept under
+//                           ACLFFT_STRICT_REFERENCE_DOMAIN=1)
 //   aclfftSetAutoAllocation 0: the plan never allocates its workspace; the caller provides it
 //   aclfftSetWorkArea       device memory of at least aclfftGetSize bytes, used from then on
 //                           (several plans may share one area when they run on one stream)
