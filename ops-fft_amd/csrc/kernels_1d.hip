@@ -292,6 +292,43 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2c(ofx_args a) {
     }
 }
 
+// The weight of R2C bins 1..M-1: 2 for the adjoint of C2R (each stands for itself and its
+// conjugate partner), 1 otherwise; the output scale included.
+template <class S>
+__device__ __forceinline__ S r2c_weight(const ofx_args& a, S ascale) {
+    return ((a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(2) : S(1)) * ascale;
+}
+
+// Hermitian unpack of Z = FFT_M(z) held in LDS (lt[pad(k)]) into the M+1 bins at `out`, bins k and
+// M-k together: with E = (Z[k] + conj Z[M-k]) / 2, O = (Z[k] - conj Z[M-k]) / 2 and P = W^k O,
+//     X[k] = E - iP,   X[M-k] = conj(E + iP)      (W^(M-k) = -conj(W^k), since W^M = -1)
+// so each pair costs two LDS reads and one root of unity (k = 0 gives X[0] and X[M]).  DC and
+// Nyquist imaginary parts are written as exact zeros.
+template <class C>
+__device__ __forceinline__ void r2c_unpack(const typename C::V* lt, int tid, const typename C::V* __restrict__ W,
+                                           typename C::V* __restrict__ out, scalar_t<typename C::V> ascale,
+                                           scalar_t<typename C::V> wgt) {
+    using V = typename C::V;
+    using S = scalar_t<V>;
+    constexpr int M = C::N;
+    for (int k = tid; k <= M / 2; k += C::T) {
+        const V zk = lt[C::pad(k)];
+        const V zc = cconj(lt[C::pad(k == 0 ? 0 : M - k)]);
+        const V e = cscale(cadd(zk, zc), S(0.5));
+        const V o = cscale(csub(zk, zc), S(0.5));
+        const V pk = cmul(W[k], o);
+        const V lo = cadd(e, mul_negi(pk));           // X[k]
+        const V hi = cconj(csub(e, mul_negi(pk)));     // X[M-k] = conj(E + iP)
+        if (k == 0) {
+            out[0] = mkv(lo.x * ascale, S(0));
+            out[M] = mkv(hi.x * ascale, S(0));
+        } else {
+            out[k] = mkv(lo.x * wgt, lo.y * wgt);
+            if (2 * k != M) out[M - k] = mkv(hi.x * wgt, hi.y * wgt);
+        }
+    }
+}
+
 // R2C, even n = 2*C::N.  in: real (idist floats), out: complex (odist complex, >= C::N+1).
 template <class C>
 __global__ __launch_bounds__(C::BLOCK) void k_r2c(ofx_args a) {
@@ -324,30 +361,8 @@ __global__ __launch_bounds__(C::BLOCK) void k_r2c(ofx_args a) {
     }
     if constexpr (!C::WAVE_SYNC || C::STAGED) __syncthreads();
     if (!ok) return;
-    // fused Hermitian unpack, bins k and M-k together: with E = (Z[k] + conj Z[M-k]) / 2,
-    // O = (Z[k] - conj Z[M-k]) / 2 and P = W^k O,
-    //     X[k] = E - iP,   X[M-k] = conj(E + iP)      (W^(M-k) = -conj(W^k), since W^M = -1)
-    // so each pair costs two LDS reads and one root of unity (k = 0 gives X[0] and X[M]).
-    constexpr int M = C::N;
-    const V* __restrict__ W = static_cast<const V*>(a.tw_aux);
-    V* __restrict__ out = static_cast<V*>(a.out) + tt * a.odist;
-    const S wgt = ((a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(2) : S(1)) * ascale;
-    for (int k = tid; k <= M / 2; k += C::T) {
-        const V zk = lt[C::pad(k)];
-        const V zc = cconj(lt[C::pad(k == 0 ? 0 : M - k)]);
-        const V e = cscale(cadd(zk, zc), S(0.5));
-        const V o = cscale(csub(zk, zc), S(0.5));
-        const V pk = cmul(W[k], o);
-        const V lo = cadd(e, mul_negi(pk));           // X[k]
-        const V hi = cconj(csub(e, mul_negi(pk)));     // X[M-k] = conj(E + iP)
-        if (k == 0) {
-            out[0] = mkv(lo.x * ascale, S(0));
-            out[M] = mkv(hi.x * ascale, S(0));
-        } else {
-            out[k] = mkv(lo.x * wgt, lo.y * wgt);
-            if (2 * k != M) out[M - k] = mkv(hi.x * wgt, hi.y * wgt);
-        }
-    }
+    r2c_unpack<C>(lt, tid, static_cast<const V*>(a.tw_aux), static_cast<V*>(a.out) + tt * a.odist, ascale,
+                  r2c_weight<S>(a, ascale));
 }
 
 // C2R, even n = 2*C::N.  in: complex (idist complex, >= C::N+1), out: real (odist floats).

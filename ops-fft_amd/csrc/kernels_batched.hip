@@ -159,15 +159,6 @@ __device__ __forceinline__ ofx_batch_tile load_tile(const ofx_batch_tile* tiles,
     return t;
 }
 
-// Workgroup barrier that orders LDS accesses only.  A tile ends with its global stores in flight;
-// __syncthreads() would wait for them (its fence covers global memory) before the next tile's
-// loads could issue, while the next tile only needs every wave done with the LDS tile.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Every loop below is uniform over the workgroup (tile indices come from blockIdx, the tile list
 // or LDS), so every wave leaves together once the index passes ntiles.
 template <class CS, int SCHED>

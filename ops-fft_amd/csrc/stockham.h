@@ -25,6 +25,15 @@
 
 namespace ofx {
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also waits for the wave's
+// outstanding global loads and stores (its fence covers global memory); a kernel that keeps the
+// next tile's loads or the last tile's stores in flight across a barrier uses this one instead.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <int... Rs>
 struct Radices {
     static constexpr int count = sizeof...(Rs);
@@ -101,6 +110,7 @@ struct StockhamCfg {
     static_assert(N == ns(NP), "radix product must equal N");
     static_assert(IOV_ == 1 || sizeof(V_) == 8, "paired 16-byte I/O is for complex64 only");
 };
+
 
 // Reads element i of the current transform from LDS.
 template <class C>
