@@ -19,8 +19,10 @@ Prints ONE JSON line:
                  double precision, latency), each with its parity figure, and the CPU golden path
                  timed on every workload
 
---profile-workloads runs the secondary workloads back to back for a profiler instead (each plan is
-called until 0.1 s of device time has passed, then --reps more times) and prints no JSON line.
+--profile-workloads [secondary|reflen|mfma] runs a set of workloads back to back for a profiler
+instead (each plan is called until 0.1 s of device time has passed, then --reps more times) and
+prints no JSON line: the secondary workloads, the reference's own test-case lengths at batch
+>= 1024, or the MFMA split-precision kernels in every mode.
 """
 import argparse
 import ctypes
@@ -646,25 +648,126 @@ def extra_batched(torch, aclfft, oracle, np, steps, stream):
 # ---------------------------------------------------------------------------------------------
 # secondary workloads for a profiler (--profile-workloads)
 # ---------------------------------------------------------------------------------------------
+def _profile_rep(torch, fn, reps):
+    """Call fn until 0.1 s of device time has passed (tables built, clocks settled), then `reps`
+    more times: a trace's last `reps` dispatches of a kernel are the measured ones."""
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 0.1:
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+    for _ in range(reps):
+        fn()
+    torch.cuda.synchronize()
+
+
+# the reference's own test-case lengths (its fft1_d / rfft1_d / irfft1_d case lists) that the
+# compiled kernels cover at batch >= 1024: C2C lengths, and even real lengths (R2C then C2R)
+REFLEN_C2C = (6, 105, 210, 420, 1536, 2100, 2310, 5040)
+REFLEN_REAL = (1536, 2100, 5040, 25200)
+
+
+def profile_reflen(torch, aclfft, reps):
+    """C2C at each of REFLEN_C2C and R2C + C2R at each of REFLEN_REAL, 2^24 points per call (batch
+    at least 1024), then the streaming copy of 512 MiB (counter calibration)."""
+    s = torch.cuda.current_stream()
+    pts = 1 << 24
+    x = torch.randn(2 * pts, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    for n in REFLEN_C2C:
+        b = max(1024, pts // n)
+        p = aclfft.Plan.plan1d(n, aclfft.C2C, b)
+        p.set_stream(s)
+        _profile_rep(torch, lambda: p.exec_c2c(x[:n * b], y[:n * b], aclfft.FORWARD), reps)
+        log("c2c n=%d b=%d: %s" % (n, b, p.describe()))
+        p.destroy()
+    del x, y
+    for n in REFLEN_REAL:
+        b = max(1024, pts // n)
+        xr = torch.randn(b, n, device="cuda")
+        X = torch.empty(b, n // 2 + 1, dtype=torch.complex64, device="cuda")
+        pr = aclfft.Plan.plan1d(n, aclfft.R2C, b)
+        pc = aclfft.Plan.plan1d(n, aclfft.C2R, b)
+        pr.set_stream(s)
+        pc.set_stream(s)
+        _profile_rep(torch, lambda: pr.exec_r2c(xr, X), reps)
+        _profile_rep(torch, lambda: pc.exec_c2r(X, xr), reps)
+        log("real n=%d b=%d: %s | %s" % (n, b, pr.describe(), pc.describe()))
+        pr.destroy()
+        pc.destroy()
+        del xr, X
+    _profile_copy(torch, reps)
+
+
+def profile_mfma(torch, aclfft, reps):
+    """The MFMA split-precision path in each mode (BF16, FP16 hi/lo, BF16 hi/mid/lo): C2C at
+    n = 256, 512, 1024, 2048, 4096 over 2^26 points, R2C and C2R at n = 8192 batch 1024, and the
+    512-group uneven batch; then the streaming copy of 512 MiB."""
+    s = torch.cuda.current_stream()
+    modes = ((aclfft.PRECISION_BF16, "bf16"), (aclfft.PRECISION_FP16X2, "fp16x2"),
+             (aclfft.PRECISION_BF16X3, "bf16x3"))
+    pts = 1 << 26
+    x = torch.randn(pts, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    for n in (256, 512, 1024, 2048, 4096):
+        for mode, name in modes:
+            p = aclfft.Plan.plan1d(n, aclfft.C2C, pts // n)
+            p.set_precision(mode)
+            p.set_stream(s)
+            _profile_rep(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), reps)
+            log("c2c n=%d %s: %s" % (n, name, p.describe()))
+            p.destroy()
+    del x, y
+    n, b = 8192, 1024
+    xr = torch.randn(b, n, device="cuda")
+    X = torch.empty(b, n // 2 + 1, dtype=torch.complex64, device="cuda")
+    for mode, name in modes:
+        pr = aclfft.Plan.plan1d(n, aclfft.R2C, b)
+        pc = aclfft.Plan.plan1d(n, aclfft.C2R, b)
+        for p in (pr, pc):
+            p.set_precision(mode)
+            p.set_stream(s)
+        _profile_rep(torch, lambda: pr.exec_r2c(xr, X), reps)
+        _profile_rep(torch, lambda: pc.exec_c2r(X, xr), reps)
+        log("real n=%d b=%d %s: %s | %s" % (n, b, name, pr.describe(), pc.describe()))
+        pr.destroy()
+        pc.destroy()
+    del xr, X
+    ln, bt = uneven_groups()
+    total = sum(a * c for a, c in zip(ln, bt))
+    xs = torch.randn(total, dtype=torch.complex64, device="cuda")
+    ys = torch.empty_like(xs)
+    for mode, name in modes:
+        q = aclfft.Plan.plan_many_uneven(ln, bt)
+        q.set_precision(mode)
+        q.set_stream(s)
+        _profile_rep(torch, lambda: q.exec_c2c(xs, ys, aclfft.FORWARD), reps)
+        log("uneven 512 groups %s: %s" % (name, q.describe()))
+        q.destroy()
+    del xs, ys
+    _profile_copy(torch, reps)
+
+
+def _profile_copy(torch, reps):
+    from aclfft._lib import stream_copy
+    s = torch.cuda.current_stream()
+    a = torch.empty(1 << 29, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    h = s.cuda_stream
+    _profile_rep(torch, lambda: stream_copy(a.data_ptr(), b.data_ptr(), a.numel(), h), reps)
+    log("stream copy 512 MiB")
+
+
 def profile_workloads(torch, aclfft, reps):
     """R2C and C2R at n=4096 b=8192, n=3000 b=8192, n=8192 b=1024; 2D C2C 2048x2048 batch 4; the
     512-group uneven batch; 1D C2C n=2^20 batch 64 (four-step); 3D C2C 256^3; the streaming copy of
-    1 GiB.  Each plan is called until 0.1 s of device time has passed (tables built, clocks
-    settled), then `reps` more times: a trace's last `reps` dispatches of a kernel are the measured
-    ones.  Every buffer is allocated at the size its plan reads or writes."""
+    512 MiB.  Every buffer is allocated at the size its plan reads or writes."""
     s = torch.cuda.current_stream()
 
     def rep(fn):
-        fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        while time.perf_counter() - t0 < 0.1:
-            for _ in range(5):
-                fn()
-            torch.cuda.synchronize()
-        for _ in range(reps):
-            fn()
-        torch.cuda.synchronize()
+        _profile_rep(torch, fn, reps)
 
     for n, b in ((4096, 8192), (3000, 8192), (8192, 1024)):
         xr = torch.randn(b, n, device="cuda")
@@ -714,12 +817,7 @@ def profile_workloads(torch, aclfft, reps):
     log("3d 256^3: %s" % p.describe())
     p.destroy()
     del x, y
-    from aclfft._lib import stream_copy
-    a = torch.empty(1 << 29, dtype=torch.uint8, device="cuda")
-    b = torch.empty_like(a)
-    h = s.cuda_stream
-    rep(lambda: stream_copy(a.data_ptr(), b.data_ptr(), a.numel(), h))
-    log("stream copy 512 MiB")
+    _profile_copy(torch, reps)
 
 
 def main():
@@ -733,8 +831,9 @@ def main():
                     help="arithmetic of the timed loop (profiling runs; the headline value is fp32)")
     ap.add_argument("--with-copy", action="store_true",
                     help="also run one device copy of the 1 GiB input (counter calibration for profiles)")
-    ap.add_argument("--profile-workloads", action="store_true",
-                    help="run the secondary workloads for a profiler, print no JSON line")
+    ap.add_argument("--profile-workloads", nargs="?", const="secondary", default=None,
+                    choices=["secondary", "reflen", "mfma"],
+                    help="run a set of workloads for a profiler, print no JSON line")
     ap.add_argument("--reps", type=int, default=20, help="--profile-workloads: measured calls per plan")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -747,7 +846,8 @@ def main():
     import aclfft
 
     if args.profile_workloads:
-        profile_workloads(torch, aclfft, args.reps)
+        {"secondary": profile_workloads, "reflen": profile_reflen,
+         "mfma": profile_mfma}[args.profile_workloads](torch, aclfft, args.reps)
         return
 
     from oracle import oracle
