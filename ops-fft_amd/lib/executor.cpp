@@ -904,7 +904,10 @@ struct Builder {
             ofx_batch_class_info info;
             bool used = false;
         };
-        const int cset = e->desc_.class_set;
+        const int cset = e->desc_.class_set >= 0
+                             ? e->desc_.class_set
+                             : std::min(std::max(0, env_int("ACLFFT_PERSISTENT_CLASS_SET", 0)),
+                                        ofx_batch_class_set_count() - 1);
         std::vector<ClassInfo> cls(ofx_batch_class_count());
         for (int c = 0; c < (int)cls.size(); ++c)
             if (ofx_batch_class_info_get_set(cset, c, &cls[c].info) != 0) return ACLFFT_INVALID_VALUE;

@@ -80,7 +80,8 @@ struct PlanDesc {
     int uneven_form = 0;           // 1 persistent kernel, 2 one tile launch per length
     int sched = -1;                // persistent schedule: 0 dynamic, 1 static
     int blocks_per_cu = 0;         // persistent workgroups per CU (0: all that fit at once)
-    int class_set = 0;             // compiled class set of the persistent kernel
+    int class_set = -1;            // compiled class set of the persistent kernel (negative:
+                                   // ACLFFT_PERSISTENT_CLASS_SET, default 0)
 };
 
 class Executor {

@@ -67,6 +67,32 @@ def test_uneven_plan_vs_oracle(cuda, monkeypatch, graph, mode, sched, lanes):
     p.destroy()
 
 
+@pytest.mark.parametrize("cset", [0, 1])
+@pytest.mark.parametrize("sched", [0, 1])
+def test_every_class_set(cuda, monkeypatch, cset, sched):
+    """Each compiled class set of the persistent kernel (ACLFFT_PERSISTENT_CLASS_SET), under both
+    schedules, on the benchmark's length mix: every group against the FP64 oracle."""
+    import torch
+    monkeypatch.setenv("ACLFFT_UNEVEN_MODE", "1")
+    monkeypatch.setenv("ACLFFT_PERSISTENT_SCHEDULE", str(sched))
+    monkeypatch.setenv("ACLFFT_PERSISTENT_CLASS_SET", str(cset))
+    ln, bt = uneven_workload(48, 11 + cset)
+    groups = list(zip(ln, bt))
+    total = sum(n * b for n, b in groups)
+    x = crandn(total, 5 + cset).astype(np.complex64)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    p = aclfft.Plan.plan_many_uneven(ln, bt)
+    d = p.describe()
+    print(d)
+    assert "set=%d sched=%d" % (cset, sched) in d
+    for direction in (aclfft.FORWARD, aclfft.BACKWARD):
+        p.exec_c2c(xt, yt, direction)
+        torch.cuda.synchronize()
+        check_groups(yt.cpu().numpy(), x, groups, direction)
+    p.destroy()
+
+
 @pytest.mark.parametrize("mode,sched,lanes", FORMS)
 def test_uneven_plan_mixed_with_generic_lengths(cuda, monkeypatch, mode, sched, lanes):
     """Lengths without a compiled class run as extra launches inside the same plan/graph."""
