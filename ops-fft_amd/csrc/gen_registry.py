@@ -88,6 +88,16 @@ EXTRA = [
 ]
 
 
+# split-exchange C2C configurations (real and imaginary parts through LDS in turn: half the LDS, so
+# more workgroups per CU for the long transforms): (kind, N, T, TPB, radices)
+# SPLIT_FIRST lead their length's candidates: 6561 points take 52 KiB of complex LDS, so only three
+# workgroups share a CU; split, four do (C2C 6561 x 1024: 20.6 vs 23.8 us, one round of workgroups
+# instead of two; x 8192: 172.4 vs 173.3 us).  At 4096 and 2048 points the split exchange measured
+# slower (197.6 vs 195.6 us, 189.7 vs 187.3 us at 2^26 points) and is not built.
+SPLIT_FIRST = [("C2C", 6561, 256, 1, (27, 27, 9)), ("C2C_TILES", 6561, 256, 1, (27, 27, 9))]
+SPLIT = [("C2C", 6561, 256, 1, (9, 27, 27))]
+
+
 # tile-list C2C kernels of the uneven-batch lengths (one launch per length): (N, T, TPB, radices)
 TILES = [
     (60, 4, 64, (4, 3, 5)), (128, 8, 32, (8, 16)), (243, 16, 16, (27, 9)), (256, 16, 16, (16, 16)),
@@ -229,6 +239,8 @@ def main():
     with open(os.path.join(here, "registry_1d.inc"), "w") as f:
         f.write("// generated by gen_registry.py - do not edit\n")
         f.write("static const Entry kRegistry[] = {\n")
+        for kind, n, t, tpb, rad in SPLIT_FIRST:
+            f.write("    OFX_EX(%s, %d, %d, %d, %d, %s),\n" % (KINDS[kind], n, t, tpb, padsh(n), ", ".join(str(r) for r in rad)))
         for kind, n, t, tpb, rad, iov, *psh in FIRST:
             assert t * tpb <= 1024
             psh = psh[0] if psh else 4
@@ -257,12 +269,15 @@ def main():
             assert t * tpb <= 1024
             mac = "OFX_EZS" if kind == "Z2Z" else ("OFX_EVS" if iov == 2 else "OFX_ES")
             f.write("    %s(%s, %d, %d, %d, %d, %s),\n" % (mac, KINDS[kind], n, t, tpb, padsh(n), ", ".join(str(r) for r in rad)))
+        for kind, n, t, tpb, rad in SPLIT:
+            assert t * tpb <= 1024
+            f.write("    OFX_EX(%s, %d, %d, %d, %d, %s),\n" % (KINDS[kind], n, t, tpb, padsh(n), ", ".join(str(r) for r in rad)))
         for kind, n, t, tpb, rad, psh in PADX:
             assert t * tpb <= 1024
             f.write("    OFX_E(%s, %d, %d, %d, %d, false, %s),\n" % (KINDS[kind], n, t, tpb, psh, ", ".join(str(r) for r in rad)))
         f.write("};\n")
     print("wrote %d registry entries" % (len(FIRST) + len(rows) + len(TILES) + len(EXTRA) + len(PAIRED) +
-                                         len(z_lines()) + len(STREAM) + len(PADX)))
+                                         len(z_lines()) + len(STREAM) + len(PADX) + len(SPLIT) + len(SPLIT_FIRST)))
 
 
 if __name__ == "__main__":

@@ -255,17 +255,32 @@ __device__ __forceinline__ void stage_out(const typename C::V* lds, typename C::
 // ---------------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------------
+// SPLITX configurations (never staged) exchange between passes through a scalar LDS tile per
+// transform, real parts then imaginary parts (split_fft): half the LDS of the complex tile.
 template <class C>
 __global__ __launch_bounds__(C::BLOCK) void k_c2c(ofx_args a) {
     using V = typename C::V;
     using S = scalar_t<V>;
     const S ascale = arg_scale<S>(a);
-    __shared__ V lds[C::TPB * C::LDS_N];
+    static_assert(!(C::SPLITX && (C::STAGED || C::IOV != 1)), "split exchange is for direct single-element I/O");
+    __shared__ std::conditional_t<C::SPLITX, S, V> lds[C::TPB * C::LDS_N];
     const int tl = threadIdx.x / C::T;
     const int tid = threadIdx.x - tl * C::T;
     const long long t0 = (long long)blockIdx.x * C::TPB;
     const long long t = t0 + tl;
     const V* __restrict__ tw = static_cast<const V*>(a.tw);
+    if constexpr (C::SPLITX) {
+        const bool ok = t < a.batch;
+        const long long tt = ok ? t : 0;
+        constexpr bool NT = C::STREAM && rows_whole_lines<V, C::N>();
+        GLoadC<V, NT> gin{static_cast<const V*>(a.in) + tt * a.idist, a.in_sign};
+        GStoreC<V, NT> gout{static_cast<V*>(a.out) + tt * a.odist, a.out_sign, ascale, ok};
+        if constexpr (C::WAVE_SYNC) {
+            if (!ok) return;
+        }
+        split_fft<C>(lds + tl * C::LDS_N, tid, tw, gin, gout);
+        return;
+    } else {
     V* lt = lds + tl * C::LDS_N;
     if constexpr (C::STAGED) {
         // staged tiles require contiguous transforms (idist == odist == N): checked by the launcher
@@ -289,6 +304,7 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2c(ofx_args a) {
             if (!ok) return;  // whole transforms per wave: no block-wide barrier follows
         }
         stockham_passes<C, false, false>(lt, tid, tw, gin, gout);
+    }
     }
 }
 
@@ -512,7 +528,7 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2c_tiles(ofx_args a) {
     using S = scalar_t<V>;
     const S ascale = arg_scale<S>(a);
     static_assert(C::IOV == 1 && !C::STAGED, "tile offsets carry no 16-byte alignment");
-    __shared__ V lds[C::TPB * C::LDS_N];
+    __shared__ std::conditional_t<C::SPLITX, S, V> lds[C::TPB * C::LDS_N];
     const int tl = threadIdx.x / C::T;
     const int tid = threadIdx.x - tl * C::T;
     const ofx_batch_tile tile = static_cast<const ofx_batch_tile*>(a.tiles)[blockIdx.x];
@@ -523,7 +539,10 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2c_tiles(ofx_args a) {
     const long long base = tile.offset + (long long)(ok ? tl : 0) * C::N;
     GLoadC<V> gin{static_cast<const V*>(a.in) + base, a.in_sign};
     GStoreC<V> gout{static_cast<V*>(a.out) + base, a.out_sign, ascale, ok};
-    stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, static_cast<const V*>(a.tw), gin, gout);
+    if constexpr (C::SPLITX)
+        split_fft<C>(lds + tl * C::LDS_N, tid, static_cast<const V*>(a.tw), gin, gout);
+    else
+        stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, static_cast<const V*>(a.tw), gin, gout);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -595,7 +614,9 @@ Entry make_entry() {
     for (int p = 0; p < C::NP; ++p) e.info.radices[p] = C::radix(p);
     e.info.staged = (C::STAGED || C::IOV == 2) ? 1 : 0;
     e.info.iov = C::IOV;
-    e.info.lds_bytes = (int)(C::TPB * (C::LDS_N + (KIND == OFX_KIND_ROWT ? 1 : 0)) * sizeof(typename C::V));
+    e.info.lds_bytes = (int)(C::TPB * (C::LDS_N + (KIND == OFX_KIND_ROWT ? 1 : 0)) *
+                             (C::SPLITX ? sizeof(scalar_t<typename C::V>) : sizeof(typename C::V)));
+    static_assert(!C::SPLITX || KIND == OFX_KIND_C2C || KIND == OFX_KIND_C2C_TILES, "split exchange: C2C kernels only");
     e.info.tw_len = C::tw_len();
     e.info.stream = C::STREAM ? 1 : 0;
     e.fn = &launch<C, KIND>;
@@ -608,6 +629,9 @@ Entry make_entry() {
 #define OFX_EV(KIND, N, T, TPB, PADSH, ...) make_entry<OFX_CFGV(N, T, TPB, PADSH, __VA_ARGS__), KIND>()
 #define OFX_CFGZ(N, T, TPB, PADSH, ...) StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, double2>
 #define OFX_EZ(KIND, N, T, TPB, PADSH, ...) make_entry<OFX_CFGZ(N, T, TPB, PADSH, __VA_ARGS__), KIND>()
+// split-exchange variant of a direct complex64 configuration
+#define OFX_EX(KIND, N, T, TPB, PADSH, ...) \
+    make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, float2, false, true>, KIND>()
 // streaming variants (STREAM_ = true) of a direct, a paired and a complex128 configuration
 #define OFX_ES(KIND, N, T, TPB, PADSH, ...) \
     make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, float2, true>, KIND>()
