@@ -731,9 +731,22 @@ struct Builder {
             off += (long long)len[g] * bat[g];
         }
         e->in_bytes_ = e->out_bytes_ = (size_t)off * 8;
+        aclfftResult r = per_length_steps(len, bat, offs, nullptr, true);
+        if (r != ACLFFT_SUCCESS) return r;
+        // steps of lengths without a tile kernel (lane 0) first, so that the lanes fork once
+        std::stable_partition(e->steps_.begin(), e->steps_.end(), [](const Step& a) { return a.lane == 0; });
+        return ACLFFT_SUCCESS;
+    }
+
+    // One tile launch per distinct length over its groups (groups g with take[g] set, or all),
+    // largest total work first; lengths without a tile kernel run one step per group.  With
+    // `lanes`, ACLFFT_UNEVEN_LANES spreads the launches over side streams.
+    aclfftResult per_length_steps(const std::vector<int>& len, const std::vector<int>& bat,
+                                  const std::vector<long long>& offs, const std::vector<char>* take, bool lanes_ok) {
         // distinct lengths, largest total work first
         std::map<int, double> work;
-        for (size_t g = 0; g < len.size(); ++g) work[len[g]] += (double)len[g] * bat[g];
+        for (size_t g = 0; g < len.size(); ++g)
+            if (take == nullptr || (*take)[g]) work[len[g]] += (double)len[g] * bat[g];
         std::vector<int> lens;
         for (auto& kv : work) lens.push_back(kv.first);
         std::stable_sort(lens.begin(), lens.end(), [&](int a, int b) { return work[a] > work[b]; });
@@ -742,7 +755,7 @@ struct Builder {
             auto ids = kernels_for(OFX_KIND_C2C_TILES, n, true);
             if (ids.empty()) {
                 for (size_t g = 0; g < len.size(); ++g) {
-                    if (len[g] != n) continue;
+                    if (len[g] != n || (take != nullptr && !(*take)[g])) continue;
                     const size_t first = e->steps_.size();
                     aclfftResult r = c2c_rows(n, bat[g], BUF_IN, BUF_OUT, true, true);
                     if (r != ACLFFT_SUCCESS) return r;
@@ -768,7 +781,7 @@ struct Builder {
             s.candidate_tables = tabs;
             std::vector<ofx_batch_tile> tiles;
             for (size_t g = 0; g < len.size(); ++g) {
-                if (len[g] != n) continue;
+                if (len[g] != n || (take != nullptr && !(*take)[g])) continue;
                 for (long long i = 0; i < bat[g]; i += tpb) {
                     ofx_batch_tile t;
                     t.cls = 0;
@@ -789,7 +802,7 @@ struct Builder {
             // longest-processing-time assignment of lengths to side-stream lanes, off by default:
             // the fork / join of the side streams costs more than the overlap gains (2 to 8 lanes
             // measured 1.2-3x slower than the plan's own stream for one to three lengths)
-            const int lanes = std::max(0, std::min(8, env_int("ACLFFT_UNEVEN_LANES", 0)));
+            const int lanes = lanes_ok ? std::max(0, std::min(8, env_int("ACLFFT_UNEVEN_LANES", 0))) : 0;
             if (lanes > 1) {
                 lane_load.resize(lanes, 0.0);
                 const size_t k = std::min_element(lane_load.begin(), lane_load.end()) - lane_load.begin();
@@ -798,8 +811,6 @@ struct Builder {
             }
             e->steps_.push_back(s);
         }
-        // steps of lengths without a tile kernel (lane 0) first, so that the lanes fork once
-        std::stable_partition(e->steps_.begin(), e->steps_.end(), [](const Step& a) { return a.lane == 0; });
         return ACLFFT_SUCCESS;
     }
 
@@ -915,8 +926,11 @@ struct Builder {
         std::vector<double> cost;
         long long off = 0;
         size_t total = 0;
+        std::vector<long long> offs(len.size());
+        std::vector<char> own(len.size(), 0);  // groups of lengths without a class in this set
         for (size_t g = 0; g < len.size(); ++g) {
             const long long n = len[g], b = bat[g];
+            offs[g] = off;
             if (skip != nullptr && (*skip)[g]) {  // planned elsewhere; only its offset counts here
                 off += n * b;
                 total += (size_t)(n * b);
@@ -937,20 +951,16 @@ struct Builder {
                     cost.push_back((double)t.count * n * std::log2((double)n + 1.0));
                 }
             } else {
-                const size_t first = e->steps_.size();
-                aclfftResult r = c2c_rows(n, b, BUF_IN, BUF_OUT, true, true);
-                if (r != ACLFFT_SUCCESS) return r;
-                e->steps_.back().final_out = true;
-                for (size_t k = first; k < e->steps_.size(); ++k) {
-                    if (e->steps_[k].src == BUF_IN) e->steps_[k].src_off = (size_t)off * 8;
-                    if (e->steps_[k].dst == BUF_OUT) e->steps_[k].dst_off = (size_t)off * 8;
-                    if (e->steps_[k].src == BUF_OUT) e->steps_[k].src_off = (size_t)off * 8;
-                }
+                own[g] = 1;
             }
             off += n * b;
             total += (size_t)(n * b);
         }
         e->in_bytes_ = e->out_bytes_ = total * 8;
+        {
+            aclfftResult r = per_length_steps(len, bat, offs, &own, false);
+            if (r != ACLFFT_SUCCESS) return r;
+        }
         if (tiles.empty()) return ACLFFT_SUCCESS;
         // longest-processing-time order
         std::vector<size_t> order(tiles.size());
