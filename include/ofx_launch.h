@@ -297,6 +297,24 @@ OFX_API int ofx_col_kernel_count(void);
 OFX_API int ofx_col_kernel_info_get(int id, ofx_kernel_info* info);
 OFX_API int ofx_launch_col(int id, const ofx_args* args, hipStream_t stream);
 
+/* One-pass 2D C2C kernels (complex64) for images that fit in LDS, nx, ny in {32, 64, 128} - the
+ * reference's 2D domain (src/fft2_d/arch32/fft2_d_exec_api.cpp:18-60, dd/fft2_d_dd.cpp:17-174):
+ * `batch` images of nx*ny complex64 (row-major, ny fastest), image m at in + m*idist and
+ * out + m*odist; tw = the row passes' twiddle table (row radices),
+ * tw_aux = the column passes' (column radices), both from ofx_gen_pass_twiddles; in_sign
+ * conjugates on load, out_sign and scale apply on store.  In place is allowed (each workgroup reads
+ * its whole image before writing it). */
+typedef struct ofx_2d_kernel_info {
+    int nx, ny;                          /* image rows (slow axis), columns (fast axis)        */
+    int block_threads;
+    int lds_bytes;
+    int row_npass, row_radices[OFX_MAX_PASSES], row_tw_len;   /* length-ny row transforms    */
+    int col_npass, col_radices[OFX_MAX_PASSES], col_tw_len;   /* length-nx column transforms */
+} ofx_2d_kernel_info;
+OFX_API int ofx_2d_kernel_count(void);
+OFX_API int ofx_2d_kernel_info_get(int id, ofx_2d_kernel_info* info);
+OFX_API int ofx_launch_2d(int id, const ofx_args* args, hipStream_t stream);
+
 /* Fill `tw` (device, tw_len complex64) with the pass-twiddle table of a radix sequence:
  * FP64 sincospi on the GPU, rounded once to FP32. */
 OFX_API int ofx_gen_pass_twiddles(void* tw, int npass, const int* radices, hipStream_t stream);

@@ -49,6 +49,15 @@ hipError_t hip_log(hipError_t e, const char* what) {
     return e;
 }
 
+const std::vector<ofx_2d_kernel_info>& reg_2d() {
+    static std::vector<ofx_2d_kernel_info> reg = [] {
+        std::vector<ofx_2d_kernel_info> r(ofx_2d_kernel_count());
+        for (int i = 0; i < (int)r.size(); ++i) ofx_2d_kernel_info_get(i, &r[i]);
+        return r;
+    }();
+    return reg;
+}
+
 const std::vector<ofx_kernel_info>& col_registry() {
     static std::vector<ofx_kernel_info> reg = [] {
         std::vector<ofx_kernel_info> r(ofx_col_kernel_count());
@@ -123,6 +132,29 @@ struct Builder {
 
     // All registry entries computing `kind` at complex length n, in preference order; staged
     // entries only when the transforms are contiguous.
+    // One-pass 2D C2C (complex64) of `images` nx x ny images when a compiled LDS-resident kernel
+    // covers the size (and ACLFFT_2D_FUSED is not 0); false otherwise.
+    bool fused_2d(long long nx, long long ny, long long images) {
+        if (env_int("ACLFFT_2D_FUSED", 1) == 0 || env_int("ACLFFT_DISABLE_TEMPLATED", 0) != 0) return false;
+        const auto& reg = reg_2d();
+        for (int i = 0; i < (int)reg.size(); ++i) {
+            if (reg[i].nx != nx || reg[i].ny != ny) continue;
+            Step s;
+            s.type = Step::FUSED2D;
+            s.kernel = i;
+            s.n = (int)(nx * ny);
+            s.tw_table = pass_table_for(reg[i].row_npass, reg[i].row_radices, reg[i].row_tw_len);
+            s.aux_table = pass_table_for(reg[i].col_npass, reg[i].col_radices, reg[i].col_tw_len);
+            s.args.batch = images;
+            s.args.idist = s.args.odist = nx * ny;
+            s.args.in_sign = s.args.out_sign = s.args.scale = 1.f;
+            s.conj_in = s.conj_out = true;
+            e->steps_.push_back(s);
+            return true;
+        }
+        return false;
+    }
+
     std::vector<int> kernels_for(int kind, long long n, bool contiguous) {
         std::vector<int> ids;
         if (env_int("ACLFFT_DISABLE_TEMPLATED", 0) != 0) return ids;
@@ -1318,11 +1350,16 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
         // outer = batch * product of the slower axes images
         const int k = (int)dims.size();
         long long inner = dims[k - 1];
+        if (k == 2 && !z && b.fused_2d(dims[0], dims[1], total / (dims[0] * dims[1]) * d.batch)) {
+            // images that fit in LDS: rows and columns in one pass over HBM
+            r = ACLFFT_SUCCESS;
+        } else {
         r = b.c2c_rows(dims[k - 1], total / dims[k - 1] * d.batch, BUF_IN, BUF_OUT, true, k == 1, z);
         for (int i = k - 2; i >= 0 && r == ACLFFT_SUCCESS; --i) {
             const long long outer = total / (inner * dims[i]) * d.batch;
             r = b.c2c_columns(dims[i], inner, outer, BUF_OUT, BUF_OUT, false, i == 0, z);
             inner *= dims[i];
+        }
         }
     }
     if (r != ACLFFT_SUCCESS) {
@@ -1458,6 +1495,14 @@ std::string Executor::describe() const {
         }
         if (s.type == Step::R2C_UNPACK || s.type == Step::C2R_PACK) {
             os << (s.type == Step::R2C_UNPACK ? "r2c_unpack" : "c2r_pack") << " M=" << s.m;
+            continue;
+        }
+        if (s.type == Step::FUSED2D) {
+            const ofx_2d_kernel_info& k = reg_2d()[s.kernel];
+            os << "fused2d " << k.nx << "x" << k.ny << " rows=";
+            for (int p = 0; p < k.row_npass; ++p) os << (p ? "x" : "") << k.row_radices[p];
+            os << " cols=";
+            for (int p = 0; p < k.col_npass; ++p) os << (p ? "x" : "") << k.col_radices[p];
             continue;
         }
         if (s.type == Step::COLUMN) {
@@ -1709,7 +1754,8 @@ aclfftResult Executor::run_on(const void* din, void* dout, int direction, hipStr
                           ? (hipError_t)ofx_c2r_pack_z(src, dst, table(st.aux_table), st.m, st.args.batch, st.args.flags, s)
                           : (hipError_t)ofx_c2r_pack(src, dst, table(st.aux_table), st.m, st.args.batch, st.args.flags, s);
                 break;
-            case Step::COLUMN: {
+            case Step::COLUMN:
+            case Step::FUSED2D: {
                 ofx_args a = st.args;
                 a.in = src;
                 a.out = dst;
@@ -1717,7 +1763,8 @@ aclfftResult Executor::run_on(const void* din, void* dout, int direction, hipStr
                 a.tw_aux = table(st.aux_table);
                 if (st.conj_in) a.in_sign *= dsign;
                 if (st.conj_out) a.out_sign *= dsign;
-                err = (hipError_t)ofx_launch_col(st.kernel, &a, s);
+                err = (hipError_t)(st.type == Step::COLUMN ? ofx_launch_col(st.kernel, &a, s)
+                                                           : ofx_launch_2d(st.kernel, &a, s));
                 break;
             }
             case Step::PERSISTENT: {

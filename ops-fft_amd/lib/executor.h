@@ -34,8 +34,8 @@ struct TableSpec {
 
 // One launch.
 struct Step {
-    enum Type { KERNEL, GENERIC, COPY, R2C_UNPACK, C2R_PACK, MFMA, PERSISTENT, COLUMN } type = KERNEL;
-    int kernel = -1;               // registry id (KERNEL)
+    enum Type { KERNEL, GENERIC, COPY, R2C_UNPACK, C2R_PACK, MFMA, PERSISTENT, COLUMN, FUSED2D } type = KERNEL;
+    int kernel = -1;               // registry id (KERNEL; FUSED2D: ofx_2d_kernel_* id)
     int kind = OFX_KIND_C2C;
     int n = 0;                     // complex length the kernel computes (for autotuning)
     ofx_args args{};               // in/out/tw/tw_aux and signs filled at execution

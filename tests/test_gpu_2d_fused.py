@@ -1,0 +1,98 @@
+"""GPU: the one-pass 2D C2C kernels (images that fit in LDS, nx, ny in {32, 64, 128}) against the
+FP64 oracle and against the two-pass form (row kernel + column kernel, ACLFFT_2D_FUSED=0)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import aclfft
+from oracle import oracle
+from _util import crandn, fp32_tol
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(nx, ny) for nx in (32, 64, 128) for ny in (32, 64, 128)]
+
+
+def _fused_sizes():
+    lib = aclfft.lib
+    lib.ofx_2d_kernel_count.restype = ctypes.c_int
+    return lib.ofx_2d_kernel_count()
+
+
+def test_every_size_has_a_kernel():
+    assert _fused_sizes() == len(SIZES)
+
+
+@pytest.mark.parametrize("nx,ny", SIZES)
+@pytest.mark.parametrize("direction", [aclfft.FORWARD, aclfft.BACKWARD])
+def test_fused_2d_vs_oracle(cuda, nx, ny, direction):
+    """Odd batch, both directions; the plan must run one fused launch."""
+    import torch
+    batch = 5
+    x = crandn((batch, nx, ny), nx * 3 + ny + (direction > 0)).astype(np.complex64)
+    p = aclfft.Plan.plan2d(batch, nx, ny, aclfft.C2C)
+    d = p.describe()
+    assert d.startswith("fused2d %dx%d" % (nx, ny)) and "|" not in d, d
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    p.exec_c2c(xt, yt, direction)
+    torch.cuda.synchronize()
+    err = oracle.rel_l2(yt.cpu().numpy(), oracle.c2c_2d(x.astype(np.complex128), direction))
+    print(d, "rel_l2 %.3e" % err)
+    assert err <= fp32_tol(nx * ny)
+    p.destroy()
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 64), (32, 128), (128, 32), (128, 128)])
+def test_fused_2d_matches_two_pass(cuda, monkeypatch, nx, ny):
+    """The fused kernel and the row + column kernels compute the same transform (to FP32
+    rounding), in place and out of place."""
+    import torch
+    batch = 7
+    x = crandn((batch, nx, ny), nx + ny).astype(np.complex64)
+    xt = torch.from_numpy(x).cuda()
+    p = aclfft.Plan.plan2d(batch, nx, ny, aclfft.C2C)
+    y1 = torch.empty_like(xt)
+    p.exec_c2c(xt, y1, aclfft.FORWARD)
+    z = xt.clone()
+    p.exec_c2c(z, z, aclfft.FORWARD)  # in place
+    torch.cuda.synchronize()
+    p.destroy()
+    monkeypatch.setenv("ACLFFT_2D_FUSED", "0")
+    q = aclfft.Plan.plan2d(batch, nx, ny, aclfft.C2C)
+    assert "fused2d" not in q.describe()
+    y2 = torch.empty_like(xt)
+    q.exec_c2c(xt, y2, aclfft.FORWARD)
+    torch.cuda.synchronize()
+    q.destroy()
+    assert torch.equal(y1, z)
+    a, b = y1.cpu().numpy(), y2.cpu().numpy()
+    assert oracle.rel_l2(a, b) <= fp32_tol(nx * ny)
+
+
+def test_fused_2d_unaligned_scaled_and_3d(cuda):
+    """Images that start off a 16-byte boundary (a one-element offset view), the fused output
+    scale, and a 3D plan with a unit axis (a 1 x 64 x 64 volume is one 64 x 64 image)."""
+    import torch
+    nx, ny, batch = 64, 64, 3
+    x = crandn((batch * nx * ny + 1,), 9).astype(np.complex64)
+    base = torch.from_numpy(x).cuda()
+    view = base[1:]
+    assert view.data_ptr() % 16 == 8
+    y = torch.empty(batch * nx * ny, dtype=torch.complex64, device=cuda)
+    p = aclfft.Plan.plan2d(batch, nx, ny, aclfft.C2C)
+    p.set_scale(1.0 / (nx * ny))
+    p.exec_c2c(view, y, aclfft.BACKWARD)
+    torch.cuda.synchronize()
+    ref = oracle.c2c_2d(x[1:].reshape(batch, nx, ny).astype(np.complex128), aclfft.BACKWARD) / (nx * ny)
+    assert oracle.rel_l2(y.cpu().numpy().reshape(batch, nx, ny), ref) <= fp32_tol(nx * ny)
+    p.destroy()
+    v = crandn((1, nx, ny), 10).astype(np.complex64)
+    vt = torch.from_numpy(v).cuda()
+    q = aclfft.Plan.plan3d(1, nx, ny, aclfft.C2C)
+    assert q.describe().startswith("fused2d")
+    q.exec_c2c(vt, vt, aclfft.FORWARD)
+    torch.cuda.synchronize()
+    assert oracle.rel_l2(vt.cpu().numpy(), oracle.c2c_2d(v.astype(np.complex128))) <= fp32_tol(nx * ny)
+    q.destroy()
