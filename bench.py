@@ -506,6 +506,41 @@ def extra_2d(torch, aclfft, oracle, np, steps, stream):
     return out
 
 
+def extra_2d_small(torch, aclfft, oracle, np, x, y, steps, stream):
+    """The reference's 2D sizes ({32, 64, 128}^2, its DD domain) over 2^26 points (the headline's
+    1 GiB buffers): the one-pass kernel (image in LDS, 16 B/point) against the row + column kernel
+    pair (ACLFFT_2D_FUSED=0, 32 B/point), with parity of the first 16 images against the FP64
+    oracle."""
+    out = {}
+    pts = x.numel()
+    for n in (32, 64, 128):
+        b = pts // (n * n)
+        rec = {}
+        for fused in (True, False):
+            old = os.environ.get("ACLFFT_2D_FUSED")
+            if not fused:
+                os.environ["ACLFFT_2D_FUSED"] = "0"
+            try:
+                p = aclfft.Plan.plan2d(b, n, n, aclfft.C2C)
+            finally:
+                if old is None:
+                    os.environ.pop("ACLFFT_2D_FUSED", None)
+                else:
+                    os.environ["ACLFFT_2D_FUSED"] = old
+            p.set_stream(stream)
+            ms = time_steps(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
+            k = 16 * n * n
+            err = oracle.rel_l2(y[:k].cpu().numpy().reshape(16, n, n),
+                                oracle.c2c_2d(x[:k].cpu().numpy().reshape(16, n, n).astype(np.complex128)))
+            rec["one_pass" if fused else "two_pass"] = {
+                "ms": round(ms, 5), "GFLOP/s": round(5.0 * pts * math.log2(n * n) / ms / 1e6, 1),
+                "hbm_frac_compulsory": frac(16.0 * pts, ms), "rel_l2_vs_fp64": err, "plan": p.describe()}
+            p.destroy()
+        rec["speedup"] = round(rec["two_pass"]["ms"] / rec["one_pass"]["ms"], 3)
+        out["%dx%d_b%d" % (n, n, b)] = rec
+    return out
+
+
 def extra_loss_grad(torch, aclfft, steps, stream):
     """The differentiable configuration: loss = sum |W * FFT(x)|^2 (W ~ U[0,1), fixed) and its
     gradient through torch.autograd over the library (forward transform, then the adjoint kernel),
@@ -671,7 +706,8 @@ REFLEN_REAL = (1536, 2100, 5040, 25200)
 
 def profile_reflen(torch, aclfft, reps):
     """C2C at each of REFLEN_C2C and R2C + C2R at each of REFLEN_REAL, 2^24 points per call (batch
-    at least 1024), then the streaming copy of 512 MiB (counter calibration)."""
+    at least 1024), 2D C2C of the reference's sizes 32^2, 64^2, 128^2 over 2^24 points (one-pass
+    kernel), then the streaming copy of 512 MiB (counter calibration)."""
     s = torch.cuda.current_stream()
     pts = 1 << 24
     x = torch.randn(2 * pts, dtype=torch.complex64, device="cuda")
@@ -682,6 +718,13 @@ def profile_reflen(torch, aclfft, reps):
         p.set_stream(s)
         _profile_rep(torch, lambda: p.exec_c2c(x[:n * b], y[:n * b], aclfft.FORWARD), reps)
         log("c2c n=%d b=%d: %s" % (n, b, p.describe()))
+        p.destroy()
+    for n in (32, 64, 128):
+        b = pts // (n * n)
+        p = aclfft.Plan.plan2d(b, n, n, aclfft.C2C)
+        p.set_stream(s)
+        _profile_rep(torch, lambda: p.exec_c2c(x[:pts], y[:pts], aclfft.FORWARD), reps)
+        log("2d %dx%d b=%d: %s" % (n, n, b, p.describe()))
         p.destroy()
     del x, y
     for n in REFLEN_REAL:
@@ -928,6 +971,8 @@ def main():
                           ("c2c_n1024_b64_latency", lambda: extra_small_latency(torch, aclfft, oracle, np, stream)),
                           ("double_precision", lambda: extra_double(torch, aclfft, oracle, np, args.steps, stream)),
                           ("c2c_2d_2048x2048_b4", lambda: extra_2d(torch, aclfft, oracle, np, 10, stream)),
+                          ("c2c_2d_reference_sizes", lambda: extra_2d_small(torch, aclfft, oracle, np, x.view(-1),
+                                                                            y.view(-1), 20, stream)),
                           ("uneven_batched_512", lambda: extra_batched(torch, aclfft, oracle, np, 20, stream)),
                           ("loss_grad", lambda: extra_loss_grad(torch, aclfft, 10, stream))):
             try:
