@@ -680,6 +680,72 @@ def extra_batched(torch, aclfft, oracle, np, steps, stream):
     return out
 
 
+def extra_vendor_fft(torch, aclfft, stream):
+    """The vendor FFT on the same GPU, as a yardstick beside this library: torch.fft on device
+    tensors runs hipFFT / rocFFT (ROCm 7.2).  Each benchmark workload is timed both ways on the same
+    device-resident input, with the same warm-up and event timing; the uneven batch has no rocFFT
+    form, so it is one torch.fft call per group there."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator().manual_seed(11)
+    out = {"library": "torch.fft on cuda:0 (hipFFT / rocFFT), torch %s" % torch.__version__}
+
+    def both(name, nbytes, lib_fn, vendor_fn, steps):
+        ml = time_steps(torch, lib_fn, steps, 3, stream)
+        mv = time_steps(torch, vendor_fn, steps, 3, stream)
+        out[name] = {"this_ms": round(ml, 5), "rocfft_ms": round(mv, 5), "this_hbm_frac": frac(nbytes, ml),
+                     "rocfft_hbm_frac": frac(nbytes, mv), "speedup_vs_rocfft": round(mv / ml, 2)}
+
+    for n, b, steps in ((1024, 65536, 20), (1024, 64, 100)):
+        x = torch.randn(b, n, dtype=torch.complex64, generator=g).to(dev)
+        y = torch.empty_like(x)
+        p = aclfft.Plan.plan1d(n, aclfft.C2C, b)
+        p.set_stream(stream)
+        both("c2c_n%d_b%d" % (n, b), 16.0 * n * b, lambda: p.exec_c2c(x, y, aclfft.FORWARD),
+             lambda: torch.fft.fft(x), steps)
+        p.destroy()
+        del x, y
+    for n, b in ((4096, 8192), (3000, 8192), (8192, 1024)):
+        x = torch.randn(b, n, generator=g).to(dev)
+        xf = torch.empty(b, n // 2 + 1, dtype=torch.complex64, device=dev)
+        z = torch.empty(b, n, device=dev)
+        pr = aclfft.Plan.plan1d(n, aclfft.R2C, b)
+        pc = aclfft.Plan.plan1d(n, aclfft.C2R, b)
+        pr.set_stream(stream)
+        pc.set_stream(stream)
+        nb = b * (4.0 * n + 8.0 * (n // 2 + 1))
+        both("r2c_n%d_b%d" % (n, b), nb, lambda: pr.exec_r2c(x, xf), lambda: torch.fft.rfft(x), 50)
+        both("c2r_n%d_b%d" % (n, b), nb, lambda: pc.exec_c2r(xf, z), lambda: torch.fft.irfft(xf, n=n), 50)
+        pr.destroy()
+        pc.destroy()
+        del x, xf, z
+    x = torch.randn(4, 2048, 2048, dtype=torch.complex64, generator=g).to(dev)
+    y = torch.empty_like(x)
+    p = aclfft.Plan.plan2d(4, 2048, 2048, aclfft.C2C)
+    p.set_stream(stream)
+    both("fft2_2048x2048_b4", 16.0 * x.numel(), lambda: p.exec_c2c(x, y, aclfft.FORWARD),
+         lambda: torch.fft.fft2(x), 20)
+    p.destroy()
+    del x, y
+    ln, bt = uneven_groups()
+    total = sum(a * b for a, b in zip(ln, bt))
+    x = torch.randn(total, dtype=torch.complex64, generator=g).to(dev)
+    y = torch.empty_like(x)
+    offs = [0]
+    for a, b in zip(ln, bt):
+        offs.append(offs[-1] + a * b)
+    views = [x[offs[i]:offs[i + 1]].view(bt[i], ln[i]) for i in range(len(ln))]
+    p = aclfft.Plan.plan_many_uneven(ln, bt)
+    p.set_stream(stream)
+
+    def vendor_groups():
+        for v in views:
+            torch.fft.fft(v)
+
+    both("uneven_512_groups", 16.0 * total, lambda: p.exec_c2c(x, y, aclfft.FORWARD), vendor_groups, 5)
+    p.destroy()
+    return out
+
+
 # ---------------------------------------------------------------------------------------------
 # secondary workloads for a profiler (--profile-workloads)
 # ---------------------------------------------------------------------------------------------
@@ -974,7 +1040,8 @@ def main():
                           ("c2c_2d_reference_sizes", lambda: extra_2d_small(torch, aclfft, oracle, np, x.view(-1),
                                                                             y.view(-1), 20, stream)),
                           ("uneven_batched_512", lambda: extra_batched(torch, aclfft, oracle, np, 20, stream)),
-                          ("loss_grad", lambda: extra_loss_grad(torch, aclfft, 10, stream))):
+                          ("loss_grad", lambda: extra_loss_grad(torch, aclfft, 10, stream)),
+                          ("vendor_fft_same_gpu", lambda: extra_vendor_fft(torch, aclfft, stream))):
             try:
                 log("bench: extra %s" % name)
                 extras[name] = fn_()

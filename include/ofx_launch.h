@@ -75,7 +75,9 @@ typedef struct ofx_kernel_info {
                                        needs 16-byte aligned rows                            */
     int lds_bytes;                  /* static LDS per workgroup                             */
     int tw_len;                     /* complex64 entries of the pass-twiddle table          */
-    int iov;                        /* 2: paired butterflies in the first/last pass         */
+    int iov;                        /* 2: paired butterflies in the first/last pass;
+                                       3: R2C / C2R with the Hermitian partner butterflies of
+                                       the last / first pass in one thread                  */
     int stream;                     /* 1: streaming (non-temporal) HBM input and output, for
                                        working sets well beyond the last-level cache          */
 } ofx_kernel_info;

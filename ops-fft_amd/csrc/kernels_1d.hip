@@ -253,6 +253,201 @@ __device__ __forceinline__ void stage_out(const typename C::V* lds, typename C::
 }
 
 // ---------------------------------------------------------------------------------------------
+// Hermitian partners in one thread (IOV = 3: real transforms of even n = 2M)
+// ---------------------------------------------------------------------------------------------
+// The R2C unpack combines Z[i] with Z[M-i]; the C2R pre-pack combines X[i] with X[M-i].  In the
+// last Stockham pass (Ns = M/R butterflies) butterfly j produces Z[j + k*Ns], k = 0..R-1, and the
+// partner of j + k*Ns is (Ns - j) + (R-1-k)*Ns: output R-1-k of butterfly Ns - j.  The first pass
+// (NBF = M/R butterflies, Ns = 1) reads its inputs at j + k*NBF, whose partners sit the same way
+// in butterfly NBF - j.  So when one thread owns butterflies j and Ns - j (the "unit" j), the
+// unpack / pre-pack runs in registers: the R2C stores its bins straight from the last pass, the C2R
+// reads every bin from HBM once, in its first pass, and neither moves the spectrum through LDS an
+// extra time.  Unit 0 owns butterfly 0, whose outputs pair among themselves (k with R-k), and for
+// an even butterfly count butterfly Ns/2 (k with R-1-k).  (Ns + 1) / 2 units cover a pass.
+template <class V>
+__device__ __forceinline__ void r2c_pair_out(int i, int M, V zi, V zp, V w, V* __restrict__ out, scalar_t<V> ascale,
+                                             scalar_t<V> wgt, bool ok) {
+    // X[i] and X[M-i] from Z[i] and its partner Z[M-i] (Z[0] for i = 0, giving X[0] and X[M])
+    using S = scalar_t<V>;
+    const V zc = cconj(zp);
+    const V e = cscale(cadd(zi, zc), S(0.5));
+    const V o = cscale(csub(zi, zc), S(0.5));
+    const V pk = cmul(w, o);
+    const V lo = cadd(e, mul_negi(pk));
+    const V hi = cconj(csub(e, mul_negi(pk)));
+    if (!ok) return;
+    if (i == 0) {
+        out[0] = mkv(lo.x * ascale, S(0));
+        out[M] = mkv(hi.x * ascale, S(0));
+    } else {
+        out[i] = mkv(lo.x * wgt, lo.y * wgt);
+        if (2 * i != M) out[M - i] = mkv(hi.x * wgt, hi.y * wgt);
+    }
+}
+
+// C2R pre-pack value of bin k from X[k] (a) and X[M-k] (b), conjugated (backward via the forward
+// kernel); the same formula as GLoadC2RPre.
+template <class V>
+__device__ __forceinline__ V c2r_prepack(int k, V a, V b, V w, scalar_t<V> half_weight) {
+    using S = scalar_t<V>;
+    if (k == 0) {
+        a.y = S(0);
+        b.y = S(0);
+    } else {
+        a = cscale(a, half_weight);
+        b = cscale(b, half_weight);
+    }
+    const V bc = cconj(b);
+    const V t = mul_posi(cmulc(csub(a, bc), w));
+    return cconj(cadd(cadd(a, bc), t));
+}
+
+// passes 0 .. NP-2, the first reading `gin`, all writing this transform's LDS tile
+template <class C, int P = 0, class GIn>
+__device__ __forceinline__ void passes_into_lds(typename C::V* lt, int tid, const typename C::V* __restrict__ tw,
+                                                const GIn& gin) {
+    LdsRead<C> lr{lt};
+    LdsWrite<C> lw{lt};
+    if constexpr (P == 0)
+        stockham_pass<C, 0, false, true>(tid, tw, gin, lw);
+    else
+        stockham_pass<C, P, true, true>(tid, tw, lr, lw);
+    if constexpr (P + 2 < C::NP) passes_into_lds<C, P + 1>(lt, tid, tw, gin);
+}
+
+// R2C last pass with the unpack in registers: reads LDS, writes the M+1 bins to HBM.
+template <class C>
+__device__ __forceinline__ void r2c_last_pass_paired(const typename C::V* lt, int tid,
+                                                     const typename C::V* __restrict__ tw,
+                                                     const typename C::V* __restrict__ W, typename C::V* __restrict__ out,
+                                                     scalar_t<typename C::V> ascale, scalar_t<typename C::V> wgt,
+                                                     bool ok) {
+    using V = typename C::V;
+    constexpr int P = C::NP - 1;
+    constexpr int R = C::radix(P);
+    constexpr int Ns = C::ns(P);  // = M / R, the pass's butterfly count
+    constexpr int M = C::N;
+    constexpr int U = (Ns + 1) / 2;
+    constexpr int NU = (U + C::T - 1) / C::T;
+    static_assert(P > 0, "paired real kernels have at least two passes");
+    const V* __restrict__ t0 = tw + C::tw_off(P);
+#pragma unroll
+    for (int r = 0; r < NU; ++r) {
+        const int u = tid + r * C::T;
+        if (U % C::T == 0 || u < U) {
+            const bool hasb = (u != 0) || (Ns % 2 == 0);
+            const int ja = u;
+            const int jb = u ? Ns - u : Ns / 2;
+            V va[R], vb[R];
+#pragma unroll
+            for (int k = 0; k < R; ++k) va[k] = lt[C::pad(ja + k * Ns)];
+            if (hasb) {
+#pragma unroll
+                for (int k = 0; k < R; ++k) vb[k] = lt[C::pad(jb + k * Ns)];
+            }
+#pragma unroll
+            for (int k = 1; k < R; ++k) va[k] = cmul(va[k], t0[ja * (R - 1) + k - 1]);
+            Dft<R, V>::run(va);
+            if (hasb) {
+#pragma unroll
+                for (int k = 1; k < R; ++k) vb[k] = cmul(vb[k], t0[jb * (R - 1) + k - 1]);
+                Dft<R, V>::run(vb);
+            }
+            if (u != 0) {
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const int i = ja + k * Ns;
+                    r2c_pair_out(i, M, va[k], vb[R - 1 - k], W[i], out, ascale, wgt, ok);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; 2 * k <= R; ++k) r2c_pair_out(k * Ns, M, va[k], va[(R - k) % R], W[k * Ns], out, ascale, wgt, ok);
+                if constexpr (Ns % 2 == 0) {
+#pragma unroll
+                    for (int k = 0; 2 * k <= R - 1; ++k) {
+                        const int i = Ns / 2 + k * Ns;
+                        r2c_pair_out(i, M, vb[k], vb[R - 1 - k], W[i], out, ascale, wgt, ok);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// C2R first pass with the pre-pack in registers: reads the M+1 bins from HBM, writes LDS.
+template <class C>
+__device__ __forceinline__ void c2r_first_pass_paired(typename C::V* lt, int tid, const typename C::V* __restrict__ X,
+                                                      const typename C::V* __restrict__ W,
+                                                      scalar_t<typename C::V> hw) {
+    using V = typename C::V;
+    constexpr int R = C::radix(0);
+    constexpr int M = C::N;
+    constexpr int NBF = M / R;
+    constexpr int U = (NBF + 1) / 2;
+    constexpr int NU = (U + C::T - 1) / C::T;
+    static_assert(C::NP > 1, "paired real kernels have at least two passes");
+#pragma unroll
+    for (int r = 0; r < NU; ++r) {
+        const int u = tid + r * C::T;
+        if (U % C::T == 0 || u < U) {
+            const bool hasb = (u != 0) || (NBF % 2 == 0);
+            const int ja = u;
+            const int jb = u ? NBF - u : NBF / 2;
+            // the pre-pack replaces each partner pair in place: (xa[k], xb[R-1-k]) for u >= 1,
+            // (xa[k], xa[R-k]) and (xb[k], xb[R-1-k]) for unit 0.  The partner's root is
+            // W[M-i] = -conj(W[i]) (W = W_2M: W^M = -1), exactly so in the rounded table too, so
+            // a pair loads one root.
+            V xa[R], xb[R];
+#pragma unroll
+            for (int k = 0; k < R; ++k) xa[k] = X[ja + k * NBF];
+            if (hasb) {
+#pragma unroll
+                for (int k = 0; k < R; ++k) xb[k] = X[jb + k * NBF];
+            }
+            if (u != 0) {
+#pragma unroll
+                for (int k = 0; k < R; ++k) {
+                    const int ia = ja + k * NBF, ib = jb + (R - 1 - k) * NBF;  // ib = M - ia
+                    const V a = xa[k], b = xb[R - 1 - k];
+                    const V w = W[ia];
+                    xa[k] = c2r_prepack(ia, a, b, w, hw);
+                    xb[R - 1 - k] = c2r_prepack(ib, b, a, mkv(-w.x, w.y), hw);
+                }
+            } else {
+                const V a0 = xa[0];
+                xa[0] = c2r_prepack(0, a0, X[M], W[0], hw);
+#pragma unroll
+                for (int k = 1; 2 * k <= R; ++k) {
+                    const V a = xa[k], b = xa[R - k];
+                    const V w = W[k * NBF];
+                    xa[k] = c2r_prepack(k * NBF, a, b, w, hw);
+                    if (2 * k != R) xa[R - k] = c2r_prepack((R - k) * NBF, b, a, mkv(-w.x, w.y), hw);
+                }
+                if constexpr (NBF % 2 == 0) {
+#pragma unroll
+                    for (int k = 0; 2 * k <= R - 1; ++k) {
+                        const int ia = NBF / 2 + k * NBF, ib = NBF / 2 + (R - 1 - k) * NBF;
+                        const V a = xb[k], b = xb[R - 1 - k];
+                        const V w = W[ia];
+                        xb[k] = c2r_prepack(ia, a, b, w, hw);
+                        if (2 * k != R - 1) xb[R - 1 - k] = c2r_prepack(ib, b, a, mkv(-w.x, w.y), hw);
+                    }
+                }
+            }
+            Dft<R, V>::run(xa);
+#pragma unroll
+            for (int k = 0; k < R; ++k) lt[C::pad(ja * R + k)] = xa[k];
+            if (hasb) {
+                Dft<R, V>::run(xb);
+#pragma unroll
+                for (int k = 0; k < R; ++k) lt[C::pad(jb * R + k)] = xb[k];
+            }
+        }
+    }
+    C::sync();
+}
+
+// ---------------------------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------------------------
 // SPLITX configurations (never staged) exchange between passes through a scalar LDS tile per
@@ -347,7 +542,7 @@ __device__ __forceinline__ void r2c_unpack(const typename C::V* lt, int tid, con
 
 // R2C, even n = 2*C::N.  in: real (idist floats), out: complex (odist complex, >= C::N+1).
 template <class C>
-__global__ __launch_bounds__(C::BLOCK) void k_r2c(ofx_args a) {
+__global__ __launch_bounds__(C::BLOCK, C::MINW) void k_r2c(ofx_args a) {
     using V = typename C::V;
     using S = scalar_t<V>;
     const S ascale = arg_scale<S>(a);
@@ -361,7 +556,16 @@ __global__ __launch_bounds__(C::BLOCK) void k_r2c(ofx_args a) {
     const V* __restrict__ tw = static_cast<const V*>(a.tw);
     V* lt = lds + tl * C::LDS_N;
     LdsWrite<C> lw{lt};
-    if constexpr (C::STAGED) {
+    if constexpr (C::IOV == 3) {
+        if constexpr (C::WAVE_SYNC) {
+            if (!ok) return;
+        }
+        GLoadC<V> gin{reinterpret_cast<const V*>(static_cast<const S*>(a.in) + tt * a.idist), 1.f};
+        passes_into_lds<C>(lt, tid, tw, gin);
+        r2c_last_pass_paired<C>(lt, tid, tw, static_cast<const V*>(a.tw_aux), static_cast<V*>(a.out) + tt * a.odist,
+                                ascale, r2c_weight<S>(a, ascale), ok);
+        return;
+    } else if constexpr (C::STAGED) {
         const int nvalid = (int)min((long long)C::TPB, a.batch - t0);
         const V* src = reinterpret_cast<const V*>(static_cast<const S*>(a.in)) + t0 * C::N;
         stage_in<C>(lds, src, nvalid, S(1));
@@ -383,7 +587,7 @@ __global__ __launch_bounds__(C::BLOCK) void k_r2c(ofx_args a) {
 
 // C2R, even n = 2*C::N.  in: complex (idist complex, >= C::N+1), out: real (odist floats).
 template <class C>
-__global__ __launch_bounds__(C::BLOCK) void k_c2r(ofx_args a) {
+__global__ __launch_bounds__(C::BLOCK, C::MINW) void k_c2r(ofx_args a) {
     using V = typename C::V;
     using S = scalar_t<V>;
     const S ascale = arg_scale<S>(a);
@@ -398,7 +602,16 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2r(ofx_args a) {
     V* lt = lds + tl * C::LDS_N;
     const S hw = (a.flags & OFX_FLAG_HERMITIAN_WEIGHT) ? S(0.5) : S(1);
     GLoadC2RPre<V> gin{static_cast<const V*>(a.in) + tt * a.idist, static_cast<const V*>(a.tw_aux), C::N, hw};
-    if constexpr (C::STAGED) {
+    if constexpr (C::IOV == 3) {
+        if constexpr (C::WAVE_SYNC) {
+            if (!ok) return;
+        }
+        GStoreC<V> gout{reinterpret_cast<V*>(static_cast<S*>(a.out) + tt * a.odist), S(-1), ascale, ok};
+        c2r_first_pass_paired<C>(lt, tid, static_cast<const V*>(a.in) + tt * a.idist, static_cast<const V*>(a.tw_aux),
+                                 hw);
+        LdsRead<C> lr{lt};
+        stockham_passes<C, true, false, 1>(lt, tid, tw, lr, gout);
+    } else if constexpr (C::STAGED) {
         const int nvalid = (int)min((long long)C::TPB, a.batch - t0);
         V* dst = reinterpret_cast<V*>(static_cast<S*>(a.out)) + t0 * C::N;
         LdsWrite<C> gout{lt};
@@ -630,6 +843,8 @@ Entry make_entry() {
 #define OFX_CFGZ(N, T, TPB, PADSH, ...) StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, double2>
 #define OFX_EZ(KIND, N, T, TPB, PADSH, ...) make_entry<OFX_CFGZ(N, T, TPB, PADSH, __VA_ARGS__), KIND>()
 // split-exchange variant of a direct complex64 configuration
+// real transforms with the Hermitian partners in one thread (IOV = 3)
+#define OFX_EH(KIND, N, T, TPB, PADSH, ...) make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 3>, KIND>()
 #define OFX_EX(KIND, N, T, TPB, PADSH, ...) \
     make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, float2, false, true>, KIND>()
 // streaming variants (STREAM_ = true) of a direct, a paired and a complex128 configuration

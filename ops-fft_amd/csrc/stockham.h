@@ -42,6 +42,9 @@ struct Radices {
 
 // IOV_ = 2: in the first and last pass a thread owns adjacent butterflies j, j+1, so its HBM loads
 // and stores move two complex64 values (16 bytes) per instruction.
+// IOV_ = 3 (R2C / C2R kernels only): in the last (R2C) or first (C2R) pass a thread owns the
+// butterflies j and Ns - j that hold Hermitian partners, so the unpack / pre-pack runs in registers
+// (kernels_1d.hip).
 // COLS_ > 0: column kernels.  The workgroup holds COLS_ transforms (columns of a row-major matrix)
 // interleaved in LDS, element i of column c at lds[c + i*(COLS_ + COLPAD_)], and consecutive
 // lanes take consecutive columns, so each HBM row segment of COLS_ values is one contiguous access.
@@ -83,6 +86,19 @@ struct StockhamCfg {
     static constexpr int LDS_N = COLS_ > 0 ? N * COLSTRIDE : N + (N >> PADSH);
     static constexpr bool USES_LDS = STAGED || NP > 1;
     static constexpr int LDS_ELEMS = USES_LDS ? TPB * LDS_N : 1;
+    // Waves per SIMD that the register allocation must leave room for (the real kernels' launch
+    // bounds): for the Hermitian-pair kernels (IOV 3), as many as their LDS tiles let a CU hold
+    // (160 KiB of LDS, 32 waves), so the pair pass's extra registers cannot cost occupancy; 1 (no
+    // bound) otherwise.
+    static constexpr int wgs_per_cu() {
+        const int by_lds = 163840 / (LDS_ELEMS * (int)sizeof(V_));
+        const int by_waves = 32 / ((BLOCK + 63) / 64);
+        return by_lds < by_waves ? by_lds : by_waves;
+    }
+    static constexpr int MINW = IOV_ != 3 ? 1
+                                : (wgs_per_cu() * ((BLOCK + 63) / 64) / 4 > 8 ? 8
+                                   : wgs_per_cu() * ((BLOCK + 63) / 64) / 4 < 1 ? 1
+                                                                             : wgs_per_cu() * ((BLOCK + 63) / 64) / 4);
     static constexpr int max_elems_per_thread() {
         int m = 0;
         for (int p = 0; p < NP; ++p) {

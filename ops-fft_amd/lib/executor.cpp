@@ -1543,7 +1543,7 @@ std::string Executor::describe() const {
         }
         const ofx_kernel_info& k = reg[s.kernel];
         os << "kind" << k.kind << " n=" << k.n << " T=" << k.threads_per_transform << " TPB=" << k.transforms_per_block
-           << (k.iov == 2 ? " paired" : (k.staged ? " staged" : " direct")) << (k.stream ? " stream" : "")
+           << (k.iov == 2 ? " paired" : k.iov == 3 ? " hpair" : (k.staged ? " staged" : " direct")) << (k.stream ? " stream" : "")
            << " radices=";
         for (int p = 0; p < k.npass; ++p) os << (p ? "x" : "") << k.radices[p];
         os << " id=" << s.kernel;
