@@ -256,6 +256,9 @@ typedef struct ofx_batch_args {
                                        its descriptor loaded one tile ahead; no counter        */
     float scale;                    /* multiplies every output value (0 is taken as 1)         */
     int class_set;                  /* compiled class set (ofx_batch_class_set_count; 0 default)  */
+    int stream;                     /* 1: streaming (non-temporal) HBM accesses for the classes
+                                       whose transforms are whole 128-byte lines (n % 16 == 0),
+                                       for working sets well beyond the last-level cache      */
 } ofx_batch_args;
 OFX_API int ofx_batch_class_count(void);
 OFX_API int ofx_batch_class_info_get(int cls, ofx_batch_class_info* info);   /* class set 0 */
@@ -264,6 +267,8 @@ OFX_API int ofx_batch_class_info_get(int cls, ofx_batch_class_info* info);   /* 
 OFX_API int ofx_batch_class_set_count(void);
 OFX_API int ofx_batch_class_info_get_set(int set, int cls, ofx_batch_class_info* info);
 OFX_API int ofx_batch_blocks_per_cu_set(int set, int schedule);
+/* The same for the streaming (stream = 1) or cached (0) variant of the kernel. */
+OFX_API int ofx_batch_blocks_per_cu_stream(int set, int schedule, int stream);
 /* Launches `blocks` persistent workgroups (dynamic schedule: after a one-thread kernel that
  * resets the counter). */
 OFX_API int ofx_launch_batch_persistent(const ofx_batch_args* args, int blocks, hipStream_t stream);

@@ -74,21 +74,33 @@ constexpr bool same_block() {
 }
 static_assert(same_block<ClassSet0>() && same_block<ClassSet1>(), "every class runs with the same workgroup size");
 
+// NT: streaming (non-temporal) HBM accesses
+template <bool NT>
 struct GLoadB {
     const float2* __restrict__ p;
     float sg;
     __device__ __forceinline__ float2 operator()(int i) const {
-        float2 v = p[i];
+        float2 v;
+        if constexpr (NT)
+            v = ld_nt(p + i);
+        else
+            v = p[i];
         return make_float2(v.x, v.y * sg);
     }
 };
+template <bool NT>
 struct GStoreB {
     float2* __restrict__ p;
     float sc;   // output scale
     float sgs;  // sign * scale
     bool ok;
     __device__ __forceinline__ void operator()(int i, float2 v) const {
-        if (ok) p[i] = make_float2(v.x * sc, v.y * sgs);
+        if (ok) {
+            if constexpr (NT)
+                st_nt(p + i, make_float2(v.x * sc, v.y * sgs));
+            else
+                p[i] = make_float2(v.x * sc, v.y * sgs);
+        }
     }
 };
 
@@ -103,7 +115,23 @@ struct GStoreB {
 // the loop has a memory barrier per tile (twiddle loads are not hoisted out of the loop).  Calls
 // instead of inlining cost more: call arguments travel in VGPRs and the callee-saved register
 // ranges of the AMDGPU calling convention inflate each body's footprint.
-template <class C>
+template <class C, bool LNT, bool SNT>
+__device__ __forceinline__ void tile_body(float2* lds, int tl, int tid, bool ok, long long base, const float2* __restrict__ in,
+                                          float2* __restrict__ out, const float2* __restrict__ tw, float sg, float sc) {
+    GLoadB<LNT> gin{in + base, sg};
+    GStoreB<SNT> gout{out + base, sc, sg * sc, ok};
+    if constexpr (C::SPLITX)
+        split_fft<C>(reinterpret_cast<float*>(lds) + tl * C::LDS_N, tid, tw, gin, gout);
+    else
+        stockham_passes<C, false, false>(lds + tl * C::LDS_N, tid, tw, gin, gout);
+}
+
+// STREAM: streaming (non-temporal) HBM accesses for the classes whose transforms are whole
+// 128-byte lines (n a multiple of 16): for plans whose working set is well beyond the last-level
+// cache (uneven batch of 27 M points: 89.8 vs 99.8 us).  The odd classes (60, 243, 1000, 6561) keep
+// cached accesses: their passes' wave accesses straddle lines, and streaming them measured slower
+// (loads 94.6 us, stores 98.1 us, both 102.4 us).
+template <class C, bool STREAM>
 __device__ __forceinline__ void run_tile(float2* lds, const ofx_batch_tile& tile, const float2* __restrict__ in,
                                          float2* __restrict__ out, const float2* __restrict__ tw, float sg, float sc) {
     int tx;
@@ -115,8 +143,9 @@ __device__ __forceinline__ void run_tile(float2* lds, const ofx_batch_tile& tile
         if (!ok) return;
     }
     const long long base = tile.offset + (long long)(ok ? tl : 0) * C::N;
-    GLoadB gin{in + base, sg};
-    GStoreB gout{out + base, sc, sg * sc, ok};
+    constexpr bool NT = STREAM && C::N % 16 == 0;
+    GLoadB<NT> gin{in + base, sg};
+    GStoreB<NT> gout{out + base, sc, sg * sc, ok};
     if constexpr (C::SPLITX)
         split_fft<C>(reinterpret_cast<float*>(lds) + tl * C::LDS_N, tid, tw, gin, gout);
     else
@@ -125,23 +154,23 @@ __device__ __forceinline__ void run_tile(float2* lds, const ofx_batch_tile& tile
 
 __global__ __launch_bounds__(kBatchBlock) void k_batch_reset(unsigned int* counter) { *counter = 0u; }
 
-template <class CS>
+template <class CS, bool STREAM>
 __device__ __forceinline__ void run_class(float2* lds, const ofx_batch_tile& tile, const ofx_batch_args& a) {
     const float2* __restrict__ in = static_cast<const float2*>(a.in);
     float2* __restrict__ out = static_cast<float2*>(a.out);
     const float2* tw = static_cast<const float2*>(a.tw) + a.tw_off[tile.cls];
     const float sc = a.scale != 0.f ? a.scale : 1.f;
     switch (tile.cls) {
-        case OFX_BCLASS_60: run_tile<typename CS::C60>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_128: run_tile<typename CS::C128>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_243: run_tile<typename CS::C243>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_256: run_tile<typename CS::C256>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_512: run_tile<typename CS::C512>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_1000: run_tile<typename CS::C1000>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_1024: run_tile<typename CS::C1024>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_2048: run_tile<typename CS::C2048>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_4096: run_tile<typename CS::C4096>(lds, tile, in, out, tw, a.sign, sc); break;
-        case OFX_BCLASS_6561: run_tile<typename CS::C6561>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_60: run_tile<typename CS::C60, STREAM>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_128: run_tile<typename CS::C128, STREAM>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_243: run_tile<typename CS::C243, STREAM>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_256: run_tile<typename CS::C256, STREAM>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_512: run_tile<typename CS::C512, STREAM>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_1000: run_tile<typename CS::C1000, STREAM>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_1024: run_tile<typename CS::C1024, STREAM>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_2048: run_tile<typename CS::C2048, STREAM>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_4096: run_tile<typename CS::C4096, STREAM>(lds, tile, in, out, tw, a.sign, sc); break;
+        case OFX_BCLASS_6561: run_tile<typename CS::C6561, STREAM>(lds, tile, in, out, tw, a.sign, sc); break;
         default: break;
     }
 }
@@ -161,7 +190,7 @@ __device__ __forceinline__ ofx_batch_tile load_tile(const ofx_batch_tile* tiles,
 
 // Every loop below is uniform over the workgroup (tile indices come from blockIdx, the tile list
 // or LDS), so every wave leaves together once the index passes ntiles.
-template <class CS, int SCHED>
+template <class CS, int SCHED, bool STREAM>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args a) {
     __shared__ float2 lds[batch_lds<CS>()];
     __shared__ int s_next[2];
@@ -176,7 +205,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args
             ofx_batch_tile nxt{};
             if (tn < a.ntiles) nxt = load_tile(a.tiles, tn);  // in flight while this tile runs
             __asm__ volatile("" ::: "memory");      // no load of the tile bodies moves out of the loop
-            run_class<CS>(lds, cur, a);
+            run_class<CS, STREAM>(lds, cur, a);
             lds_barrier();  // LDS is reused by the next tile
             cur = nxt;
             t = tn;
@@ -189,7 +218,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args
             if (threadIdx.x == 0) nx = atomicAdd(a.counter, 1u);  // the next index, used after this tile
             const ofx_batch_tile cur = load_tile(a.tiles, t);
             __asm__ volatile("" ::: "memory");
-            run_class<CS>(lds, cur, a);
+            run_class<CS, STREAM>(lds, cur, a);
             // two slots: a wave may still read this tile's slot while thread 0 writes the next one
             if (threadIdx.x == 0) s_next[parity] = (int)nx + G;
             lds_barrier();  // publishes s_next; LDS is reused by the next tile
@@ -243,39 +272,49 @@ extern "C" int ofx_batch_class_info_get_set(int set, int cls, ofx_batch_class_in
 extern "C" int ofx_batch_class_info_get(int cls, ofx_batch_class_info* c) { return ofx_batch_class_info_get_set(0, cls, c); }
 
 typedef void (*batch_kernel_t)(ofx_batch_args);
-static batch_kernel_t batch_kernel(int set, int schedule) {
-    if (set == 0) return schedule == 0 ? k_batch_persistent<ClassSet0, 0> : k_batch_persistent<ClassSet0, 1>;
-    if (set == 1) return schedule == 0 ? k_batch_persistent<ClassSet1, 0> : k_batch_persistent<ClassSet1, 1>;
+template <class CS>
+static batch_kernel_t batch_kernel_of(int schedule, int stream) {
+    if (stream)
+        return schedule == 0 ? k_batch_persistent<CS, 0, true> : k_batch_persistent<CS, 1, true>;
+    return schedule == 0 ? k_batch_persistent<CS, 0, false> : k_batch_persistent<CS, 1, false>;
+}
+static batch_kernel_t batch_kernel(int set, int schedule, int stream) {
+    if (set == 0) return batch_kernel_of<ClassSet0>(schedule, stream);
+    if (set == 1) return batch_kernel_of<ClassSet1>(schedule, stream);
     return nullptr;
 }
 
-extern "C" int ofx_batch_blocks_per_cu_set(int set, int schedule) {
-    // cached per device, class set and schedule (the query is a runtime call; plans ask on every
-    // execution, possibly from several host threads)
-    static std::atomic<int> cache[64][kClassSets][2] = {};
+extern "C" int ofx_batch_blocks_per_cu_stream(int set, int schedule, int stream) {
+    // cached per device, class set, schedule and access kind (the query is a runtime call; plans
+    // ask on every execution, possibly from several host threads)
+    static std::atomic<int> cache[64][kClassSets][2][2] = {};
     if (set < 0 || set >= kClassSets) return 1;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = -1;
     const int sc = schedule == 0 ? 0 : 1;
+    const int st = stream ? 1 : 0;
     if (dev >= 0) {
-        const int c = cache[dev][set][sc].load(std::memory_order_relaxed);
+        const int c = cache[dev][set][sc][st].load(std::memory_order_relaxed);
         if (c > 0) return c;
     }
     int n = 0;
-    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel(set, sc), kBatchBlock, 0);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel(set, sc, st), kBatchBlock, 0);
     if (e != hipSuccess || n <= 0) {
         (void)hipGetLastError();
         n = 1;
     }
-    if (dev >= 0) cache[dev][set][sc].store(n, std::memory_order_relaxed);
+    if (dev >= 0) cache[dev][set][sc][st].store(n, std::memory_order_relaxed);
     return n;
 }
+
+extern "C" int ofx_batch_blocks_per_cu_set(int set, int schedule) { return ofx_batch_blocks_per_cu_stream(set, schedule, 0); }
 
 extern "C" int ofx_batch_blocks_per_cu(int schedule) { return ofx_batch_blocks_per_cu_set(0, schedule); }
 
 extern "C" int ofx_launch_batch_persistent(const ofx_batch_args* a, int blocks, hipStream_t stream) {
     if (a == nullptr || a->tiles == nullptr || blocks <= 0 || (a->schedule == 0 && a->counter == nullptr) ||
-        a->class_set < 0 || a->class_set >= kClassSets || (a->schedule != 0 && a->schedule != 1))
+        a->class_set < 0 || a->class_set >= kClassSets || (a->schedule != 0 && a->schedule != 1) ||
+        (a->stream != 0 && a->stream != 1))
         return (int)hipErrorInvalidValue;
     if (a->ntiles <= 0) return 0;
     if (a->schedule == 0) {
@@ -283,6 +322,7 @@ extern "C" int ofx_launch_batch_persistent(const ofx_batch_args* a, int blocks, 
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return (int)e;
     }
-    hipLaunchKernelGGL(batch_kernel(a->class_set, a->schedule), dim3(blocks), dim3(kBatchBlock), 0, stream, *a);
+    const batch_kernel_t k = batch_kernel(a->class_set, a->schedule, a->stream);
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(kBatchBlock), 0, stream, *a);
     return (int)hipGetLastError();
 }

@@ -1008,6 +1008,12 @@ struct Builder {
                                            : (env_int("ACLFFT_PERSISTENT_SCHEDULE", 1) == 0 ? 0 : 1);
         s.blocks_per_cu = e->desc_.blocks_per_cu > 0 ? e->desc_.blocks_per_cu
                                                      : std::max(0, env_int("ACLFFT_PERSISTENT_BLOCKS_PER_CU", 0));
+        {
+            const int env_stream = env_int("ACLFFT_PERSISTENT_STREAM", -1);
+            s.b.stream = e->desc_.stream >= 0 ? (e->desc_.stream ? 1 : 0)
+                         : env_stream >= 0    ? (env_stream ? 1 : 0)
+                                              : (total * 16 >= kStreamBytes ? 1 : 0);
+        }
         for (int c = 0; c < (int)cls.size(); ++c) {
             s.class_tables[c] = -1;
             if (cls[c].used) s.class_tables[c] = pass_table_for(cls[c].info.npass, cls[c].info.radices, cls[c].info.tw_len);
@@ -1514,7 +1520,8 @@ std::string Executor::describe() const {
             continue;
         }
         if (s.type == Step::PERSISTENT) {
-            os << "persistent tiles=" << s.b.ntiles << " set=" << s.b.class_set << " sched=" << s.b.schedule;
+            os << "persistent tiles=" << s.b.ntiles << " set=" << s.b.class_set << " sched=" << s.b.schedule
+               << (s.b.stream ? " stream" : "");
             if (s.blocks_per_cu > 0) os << " bpc=" << s.blocks_per_cu;
             os << " classes=";
             bool first = true;
@@ -1778,8 +1785,9 @@ aclfftResult Executor::run_on(const void* din, void* dout, int direction, hipStr
                 b.counter = static_cast<unsigned int*>(const_cast<void*>(table(st.counter_table)));
                 b.sign = dsign;
                 // by default every workgroup resident at once (4 per CU: 34 KiB of LDS each)
-                const int per_cu =
-                    st.blocks_per_cu > 0 ? st.blocks_per_cu : std::max(1, ofx_batch_blocks_per_cu_set(b.class_set, b.schedule));
+                const int per_cu = st.blocks_per_cu > 0
+                                       ? st.blocks_per_cu
+                                       : std::max(1, ofx_batch_blocks_per_cu_stream(b.class_set, b.schedule, b.stream));
                 err = (hipError_t)ofx_launch_batch_persistent(&b, num_cus_ * per_cu, s);
                 break;
             }
@@ -2027,6 +2035,7 @@ aclfftResult Executor::tune_plan(hipStream_t s) {
         c.sched = -1;
         c.blocks_per_cu = 0;
         c.class_set = 0;
+        c.stream = -1;
         if (!desc_.uneven) {
             cands.push_back(c);
             continue;
@@ -2042,16 +2051,19 @@ aclfftResult Executor::tune_plan(hipStream_t s) {
         for (int set = 0; set < ofx_batch_class_set_count(); ++set)
             for (int sched = 0; sched <= 1; ++sched) {
                 const int maxb = std::max(1, ofx_batch_blocks_per_cu_set(set, sched));
-                for (int bpc = 1; bpc <= maxb; ++bpc) {
-                    // MFMA precisions: the FP32 part (lengths without MFMA kernels) at its defaults
-                    if (prec != 0 && !(sched == 1 && bpc == maxb)) continue;
-                    PlanDesc t = c;
-                    t.uneven_form = 1;
-                    t.class_set = set;
-                    t.sched = sched;
-                    t.blocks_per_cu = bpc;
-                    cands.push_back(t);
-                }
+                for (int bpc = 1; bpc <= maxb; ++bpc)
+                    for (int stream = 0; stream <= 1; ++stream) {
+                        // MFMA precisions: the FP32 part (lengths without MFMA kernels) at its
+                        // defaults
+                        if (prec != 0 && !(sched == 1 && bpc == maxb && stream == 0)) continue;
+                        PlanDesc t = c;
+                        t.uneven_form = 1;
+                        t.class_set = set;
+                        t.sched = sched;
+                        t.blocks_per_cu = bpc;
+                        t.stream = prec != 0 ? -1 : stream;
+                        cands.push_back(t);
+                    }
             }
     }
     // scratch input (pseudo-random) and output, sized for the plan's buffers

@@ -82,6 +82,9 @@ struct PlanDesc {
     int blocks_per_cu = 0;         // persistent workgroups per CU (0: all that fit at once)
     int class_set = -1;            // compiled class set of the persistent kernel (negative:
                                    // ACLFFT_PERSISTENT_CLASS_SET, default 0)
+    int stream = -1;               // persistent kernel: 1 streaming accesses for its whole-line
+                                   // classes, 0 cached; negative: ACLFFT_PERSISTENT_STREAM, else
+                                   // streaming when the plan's input plus output is >= 384 MiB
 };
 
 class Executor {

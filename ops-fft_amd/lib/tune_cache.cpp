@@ -176,15 +176,17 @@ bool load_plan_choice(const PlanDesc& d, PlanDesc* out) {
     auto m = load();
     auto it = m.find(plan_key(d));
     if (it == m.end()) return false;
-    // "form:sched:bpc:set:precision"
-    int v[5] = {0, -1, 0, 0, 0};
-    if (std::sscanf(it->second.c_str(), "%d:%d:%d:%d:%d", &v[0], &v[1], &v[2], &v[3], &v[4]) != 5) return false;
+    // "form:sched:bpc:set:precision:stream" (entries without the last field: stream by default)
+    int v[6] = {0, -1, 0, 0, 0, -1};
+    const int got = std::sscanf(it->second.c_str(), "%d:%d:%d:%d:%d:%d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]);
+    if (got != 5 && got != 6) return false;
     *out = d;
     out->uneven_form = v[0];
     out->sched = v[1];
     out->blocks_per_cu = v[2];
     out->class_set = v[3];
     out->precision = v[4];
+    out->stream = v[5];
     return true;
 }
 
@@ -193,7 +195,7 @@ void store_plan_choice(const PlanDesc& c) {
     auto m = load();
     m[plan_key(c)] = std::to_string(c.uneven_form) + ":" + std::to_string(c.sched) + ":" +
                      std::to_string(c.blocks_per_cu) + ":" + std::to_string(c.class_set) + ":" +
-                     std::to_string(c.precision);
+                     std::to_string(c.precision) + ":" + std::to_string(c.stream);
     write_cache(m);
 }
 

@@ -67,15 +67,18 @@ def test_uneven_plan_vs_oracle(cuda, monkeypatch, graph, mode, sched, lanes):
     p.destroy()
 
 
+@pytest.mark.parametrize("stream", [0, 1])
 @pytest.mark.parametrize("cset", [0, 1])
 @pytest.mark.parametrize("sched", [0, 1])
-def test_every_class_set(cuda, monkeypatch, cset, sched):
+def test_every_class_set(cuda, monkeypatch, cset, sched, stream):
     """Each compiled class set of the persistent kernel (ACLFFT_PERSISTENT_CLASS_SET), under both
-    schedules, on the benchmark's length mix: every group against the FP64 oracle."""
+    schedules, with cached and streaming accesses (ACLFFT_PERSISTENT_STREAM), on the benchmark's
+    length mix: every group against the FP64 oracle."""
     import torch
     monkeypatch.setenv("ACLFFT_UNEVEN_MODE", "1")
     monkeypatch.setenv("ACLFFT_PERSISTENT_SCHEDULE", str(sched))
     monkeypatch.setenv("ACLFFT_PERSISTENT_CLASS_SET", str(cset))
+    monkeypatch.setenv("ACLFFT_PERSISTENT_STREAM", str(stream))
     ln, bt = uneven_workload(48, 11 + cset)
     groups = list(zip(ln, bt))
     total = sum(n * b for n, b in groups)
@@ -86,6 +89,7 @@ def test_every_class_set(cuda, monkeypatch, cset, sched):
     d = p.describe()
     print(d)
     assert "set=%d sched=%d" % (cset, sched) in d
+    assert ("stream" in d) == bool(stream)
     for direction in (aclfft.FORWARD, aclfft.BACKWARD):
         p.exec_c2c(xt, yt, direction)
         torch.cuda.synchronize()
