@@ -116,6 +116,23 @@ TILES = [
 HPAIR_FIRST = [("R2C", 2048, 256, 1, (8, 8, 8, 4)), ("C2R", 2048, 256, 1, (4, 8, 8, 8))]
 HPAIR = [("R2C", 2048, 128, 1, (16, 16, 8)), ("C2R", 2048, 128, 1, (8, 16, 16))]
 
+# streaming variants by access (mode 1 loads and stores, 2 stores only, 3 loads only):
+# (kind, N, T, TPB, radices, iov, mode[, pad shift]).  The mode-2 twins of the default kernels are
+# what single-launch plans take for outputs of >= 16 MiB (the plan's store rule); the rest are
+# tuner candidates.
+NTVAR = [
+    ("R2C", 2048, 256, 1, (8, 8, 8, 4), 3, 2), ("R2C", 2048, 256, 1, (8, 8, 8, 4), 3, 3),
+    ("C2R", 2048, 256, 1, (4, 8, 8, 8), 3, 2),
+    ("R2C", 4096, 512, 1, (8, 8, 8, 8), 1, 2), ("R2C", 4096, 512, 1, (8, 8, 8, 8), 1, 3),
+    ("R2C", 4096, 512, 1, (8, 8, 8, 8), 1, 1), ("C2R", 4096, 512, 1, (8, 8, 8, 8), 1, 2),
+    ("R2C", 1500, 256, 1, (10, 15, 10), 1, 2, 30), ("C2R", 1500, 128, 2, (12, 5, 25), 1, 2, 30),
+    ("R2C", 1024, 64, 4, (16, 16, 4), 1, 2), ("C2R", 1024, 64, 4, (16, 16, 4), 1, 2),
+    ("C2C", 2048, 128, 2, (16, 16, 8), 1, 2), ("C2C", 4096, 256, 1, (16, 16, 16), 1, 2),
+    ("C2C", 1024, 64, 2, (8, 16, 8), 2, 2), ("C2C", 1024, 64, 2, (8, 8, 16), 2, 2),
+    ("C2C", 256, 16, 16, (16, 16), 1, 2), ("C2C", 128, 8, 32, (8, 16), 1, 2), ("C2C", 512, 32, 8, (8, 8, 8), 1, 2),
+    ("C2C", 64, 4, 64, (8, 8), 1, 2), ("C2C", 8192, 512, 1, (16, 16, 16, 2), 1, 2),
+]
+
 
 # LDS pad variants (pad shift other than the default): (kind, N, T, TPB, radices, pad shift)
 PADX = [
@@ -290,9 +307,13 @@ def main():
         for kind, n, t, tpb, rad in HPAIR:
             assert t * tpb <= 1024
             f.write("    OFX_EH(%s, %d, %d, %d, %d, %s),\n" % (KINDS[kind], n, t, tpb, padsh(n), ", ".join(str(r) for r in rad)))
+        for kind, n, t, tpb, rad, iov, mode, *psh in NTVAR:
+            assert t * tpb <= 1024
+            f.write("    OFX_EM(%s, %d, %d, %d, %d, %d, %d, %s),\n" % (KINDS[kind], n, t, tpb, psh[0] if psh else padsh(n),
+                                                                iov, mode, ", ".join(str(r) for r in rad)))
         f.write("};\n")
     print("wrote %d registry entries" % (len(FIRST) + len(rows) + len(TILES) + len(EXTRA) + len(PAIRED) +
-                                         len(z_lines()) + len(STREAM) + len(PADX) + len(SPLIT) + len(SPLIT_FIRST) + len(HPAIR) + len(HPAIR_FIRST)))
+                                         len(z_lines()) + len(STREAM) + len(PADX) + len(SPLIT) + len(SPLIT_FIRST) + len(HPAIR) + len(HPAIR_FIRST) + len(NTVAR)))
 
 
 if __name__ == "__main__":

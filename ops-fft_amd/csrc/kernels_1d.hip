@@ -213,7 +213,7 @@ __device__ __forceinline__ void stage_in(typename C::V* lds, const typename C::V
     if constexpr ((C::N & 1) == 0 && sizeof(V) == 8) {
         const float4* s4 = reinterpret_cast<const float4*>(src);
         for (int l = threadIdx.x; l < total / 2; l += C::BLOCK) {
-            float4 v = ldx<C::STREAM && rows_whole_lines<V, C::N>()>(s4 + l);
+            float4 v = ldx<C::NT_LOAD && rows_whole_lines<V, C::N>()>(s4 + l);
             const int e = 2 * l;
             const int t = e / C::N, i = e - t * C::N;
             V* d = lds + t * C::LDS_N;
@@ -222,7 +222,7 @@ __device__ __forceinline__ void stage_in(typename C::V* lds, const typename C::V
         }
     } else {
         for (int l = threadIdx.x; l < total; l += C::BLOCK) {
-            V v = ldx<C::STREAM && rows_whole_lines<V, C::N>()>(src + l);
+            V v = ldx<C::NT_LOAD && rows_whole_lines<V, C::N>()>(src + l);
             const int t = l / C::N, i = l - t * C::N;
             lds[t * C::LDS_N + C::pad(i)] = mkv(v.x, v.y * sg);
         }
@@ -241,13 +241,13 @@ __device__ __forceinline__ void stage_out(const typename C::V* lds, typename C::
             const int t = e / C::N, i = e - t * C::N;
             const V* s = lds + t * C::LDS_N;
             V a = s[C::pad(i)], b = s[C::pad(i + 1)];
-            stx<C::STREAM && rows_whole_lines<V, C::N>()>(d4 + l, make_float4(a.x * scale, a.y * si, b.x * scale, b.y * si));
+            stx<C::NT_STORE && rows_whole_lines<V, C::N>()>(d4 + l, make_float4(a.x * scale, a.y * si, b.x * scale, b.y * si));
         }
     } else {
         for (int l = threadIdx.x; l < total; l += C::BLOCK) {
             const int t = l / C::N, i = l - t * C::N;
             V a = lds[t * C::LDS_N + C::pad(i)];
-            stx<C::STREAM && rows_whole_lines<V, C::N>()>(dst + l, mkv(a.x * scale, a.y * si));
+            stx<C::NT_STORE && rows_whole_lines<V, C::N>()>(dst + l, mkv(a.x * scale, a.y * si));
         }
     }
 }
@@ -264,7 +264,7 @@ __device__ __forceinline__ void stage_out(const typename C::V* lds, typename C::
 // reads every bin from HBM once, in its first pass, and neither moves the spectrum through LDS an
 // extra time.  Unit 0 owns butterfly 0, whose outputs pair among themselves (k with R-k), and for
 // an even butterfly count butterfly Ns/2 (k with R-1-k).  (Ns + 1) / 2 units cover a pass.
-template <class V>
+template <bool NTS, class V>
 __device__ __forceinline__ void r2c_pair_out(int i, int M, V zi, V zp, V w, V* __restrict__ out, scalar_t<V> ascale,
                                              scalar_t<V> wgt, bool ok) {
     // X[i] and X[M-i] from Z[i] and its partner Z[M-i] (Z[0] for i = 0, giving X[0] and X[M])
@@ -277,11 +277,11 @@ __device__ __forceinline__ void r2c_pair_out(int i, int M, V zi, V zp, V w, V* _
     const V hi = cconj(csub(e, mul_negi(pk)));
     if (!ok) return;
     if (i == 0) {
-        out[0] = mkv(lo.x * ascale, S(0));
-        out[M] = mkv(hi.x * ascale, S(0));
+        stx<NTS>(out, mkv(lo.x * ascale, S(0)));
+        stx<NTS>(out + M, mkv(hi.x * ascale, S(0)));
     } else {
-        out[i] = mkv(lo.x * wgt, lo.y * wgt);
-        if (2 * i != M) out[M - i] = mkv(hi.x * wgt, hi.y * wgt);
+        stx<NTS>(out + i, mkv(lo.x * wgt, lo.y * wgt));
+        if (2 * i != M) stx<NTS>(out + (M - i), mkv(hi.x * wgt, hi.y * wgt));
     }
 }
 
@@ -357,16 +357,16 @@ __device__ __forceinline__ void r2c_last_pass_paired(const typename C::V* lt, in
 #pragma unroll
                 for (int k = 0; k < R; ++k) {
                     const int i = ja + k * Ns;
-                    r2c_pair_out(i, M, va[k], vb[R - 1 - k], W[i], out, ascale, wgt, ok);
+                    r2c_pair_out<C::NT_STORE>(i, M, va[k], vb[R - 1 - k], W[i], out, ascale, wgt, ok);
                 }
             } else {
 #pragma unroll
-                for (int k = 0; 2 * k <= R; ++k) r2c_pair_out(k * Ns, M, va[k], va[(R - k) % R], W[k * Ns], out, ascale, wgt, ok);
+                for (int k = 0; 2 * k <= R; ++k) r2c_pair_out<C::NT_STORE>(k * Ns, M, va[k], va[(R - k) % R], W[k * Ns], out, ascale, wgt, ok);
                 if constexpr (Ns % 2 == 0) {
 #pragma unroll
                     for (int k = 0; 2 * k <= R - 1; ++k) {
                         const int i = Ns / 2 + k * Ns;
-                        r2c_pair_out(i, M, vb[k], vb[R - 1 - k], W[i], out, ascale, wgt, ok);
+                        r2c_pair_out<C::NT_STORE>(i, M, vb[k], vb[R - 1 - k], W[i], out, ascale, wgt, ok);
                     }
                 }
             }
@@ -467,9 +467,10 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2c(ofx_args a) {
     if constexpr (C::SPLITX) {
         const bool ok = t < a.batch;
         const long long tt = ok ? t : 0;
-        constexpr bool NT = C::STREAM && rows_whole_lines<V, C::N>();
-        GLoadC<V, NT> gin{static_cast<const V*>(a.in) + tt * a.idist, a.in_sign};
-        GStoreC<V, NT> gout{static_cast<V*>(a.out) + tt * a.odist, a.out_sign, ascale, ok};
+        constexpr bool NTL = C::NT_LOAD && rows_whole_lines<V, C::N>();
+        constexpr bool NTS = C::NT_STORE && rows_whole_lines<V, C::N>();
+        GLoadC<V, NTL> gin{static_cast<const V*>(a.in) + tt * a.idist, a.in_sign};
+        GStoreC<V, NTS> gout{static_cast<V*>(a.out) + tt * a.odist, a.out_sign, ascale, ok};
         if constexpr (C::WAVE_SYNC) {
             if (!ok) return;
         }
@@ -492,9 +493,10 @@ __global__ __launch_bounds__(C::BLOCK) void k_c2c(ofx_args a) {
     } else {
         const bool ok = t < a.batch;
         const long long tt = ok ? t : 0;
-        constexpr bool NT = C::STREAM && rows_whole_lines<V, C::N>();
-        GLoadC<V, NT> gin{static_cast<const V*>(a.in) + tt * a.idist, a.in_sign};
-        GStoreC<V, NT> gout{static_cast<V*>(a.out) + tt * a.odist, a.out_sign, ascale, ok};
+        constexpr bool NTL = C::NT_LOAD && rows_whole_lines<V, C::N>();
+        constexpr bool NTS = C::NT_STORE && rows_whole_lines<V, C::N>();
+        GLoadC<V, NTL> gin{static_cast<const V*>(a.in) + tt * a.idist, a.in_sign};
+        GStoreC<V, NTS> gout{static_cast<V*>(a.out) + tt * a.odist, a.out_sign, ascale, ok};
         if constexpr (C::WAVE_SYNC) {
             if (!ok) return;  // whole transforms per wave: no block-wide barrier follows
         }
@@ -531,11 +533,11 @@ __device__ __forceinline__ void r2c_unpack(const typename C::V* lt, int tid, con
         const V lo = cadd(e, mul_negi(pk));           // X[k]
         const V hi = cconj(csub(e, mul_negi(pk)));     // X[M-k] = conj(E + iP)
         if (k == 0) {
-            out[0] = mkv(lo.x * ascale, S(0));
-            out[M] = mkv(hi.x * ascale, S(0));
+            stx<C::NT_STORE>(out, mkv(lo.x * ascale, S(0)));
+            stx<C::NT_STORE>(out + M, mkv(hi.x * ascale, S(0)));
         } else {
-            out[k] = mkv(lo.x * wgt, lo.y * wgt);
-            if (2 * k != M) out[M - k] = mkv(hi.x * wgt, hi.y * wgt);
+            stx<C::NT_STORE>(out + k, mkv(lo.x * wgt, lo.y * wgt));
+            if (2 * k != M) stx<C::NT_STORE>(out + (M - k), mkv(hi.x * wgt, hi.y * wgt));
         }
     }
 }
@@ -560,7 +562,10 @@ __global__ __launch_bounds__(C::BLOCK, C::MINW) void k_r2c(ofx_args a) {
         if constexpr (C::WAVE_SYNC) {
             if (!ok) return;
         }
-        GLoadC<V> gin{reinterpret_cast<const V*>(static_cast<const S*>(a.in) + tt * a.idist), 1.f};
+        // streaming variants: the real input rows (2N values) are read with streaming loads when
+        // they are whole 128-byte lines; the bins are stored with streaming stores when NT_STORE
+        constexpr bool NTI = C::NT_LOAD && rows_whole_lines<S, 2 * C::N>();
+        GLoadC<V, NTI> gin{reinterpret_cast<const V*>(static_cast<const S*>(a.in) + tt * a.idist), 1.f};
         passes_into_lds<C>(lt, tid, tw, gin);
         r2c_last_pass_paired<C>(lt, tid, tw, static_cast<const V*>(a.tw_aux), static_cast<V*>(a.out) + tt * a.odist,
                                 ascale, r2c_weight<S>(a, ascale), ok);
@@ -576,7 +581,8 @@ __global__ __launch_bounds__(C::BLOCK, C::MINW) void k_r2c(ofx_args a) {
         if constexpr (C::WAVE_SYNC) {
             if (!ok) return;
         }
-        GLoadC<V> gin{reinterpret_cast<const V*>(static_cast<const S*>(a.in) + tt * a.idist), 1.f};
+        constexpr bool NTI = C::NT_LOAD && rows_whole_lines<S, 2 * C::N>();
+        GLoadC<V, NTI> gin{reinterpret_cast<const V*>(static_cast<const S*>(a.in) + tt * a.idist), 1.f};
         stockham_passes<C, false, true>(lt, tid, tw, gin, lw);
     }
     if constexpr (!C::WAVE_SYNC || C::STAGED) __syncthreads();
@@ -606,7 +612,9 @@ __global__ __launch_bounds__(C::BLOCK, C::MINW) void k_c2r(ofx_args a) {
         if constexpr (C::WAVE_SYNC) {
             if (!ok) return;
         }
-        GStoreC<V> gout{reinterpret_cast<V*>(static_cast<S*>(a.out) + tt * a.odist), S(-1), ascale, ok};
+        // streaming-store variants: the real output rows are written with streaming stores
+        constexpr bool NTO = C::NT_STORE;
+        GStoreC<V, NTO> gout{reinterpret_cast<V*>(static_cast<S*>(a.out) + tt * a.odist), S(-1), ascale, ok};
         c2r_first_pass_paired<C>(lt, tid, static_cast<const V*>(a.in) + tt * a.idist, static_cast<const V*>(a.tw_aux),
                                  hw);
         LdsRead<C> lr{lt};
@@ -622,7 +630,9 @@ __global__ __launch_bounds__(C::BLOCK, C::MINW) void k_c2r(ofx_args a) {
         if constexpr (C::WAVE_SYNC) {
             if (!ok) return;
         }
-        GStoreC<V> gout{reinterpret_cast<V*>(static_cast<S*>(a.out) + tt * a.odist), S(-1), ascale, ok};
+        // streaming-store variants: the real output rows are written with streaming stores
+        constexpr bool NTO = C::NT_STORE;
+        GStoreC<V, NTO> gout{reinterpret_cast<V*>(static_cast<S*>(a.out) + tt * a.odist), S(-1), ascale, ok};
         if constexpr (C::NP > 1) {
             // bins into LDS (coalesced), then the pre-pack reads both partners there
             const V* __restrict__ X = static_cast<const V*>(a.in) + tt * a.idist;
@@ -831,7 +841,7 @@ Entry make_entry() {
                              (C::SPLITX ? sizeof(scalar_t<typename C::V>) : sizeof(typename C::V)));
     static_assert(!C::SPLITX || KIND == OFX_KIND_C2C || KIND == OFX_KIND_C2C_TILES, "split exchange: C2C kernels only");
     e.info.tw_len = C::tw_len();
-    e.info.stream = C::STREAM ? 1 : 0;
+    e.info.stream = C::STREAM;
     e.fn = &launch<C, KIND>;
     return e;
 }
@@ -843,8 +853,11 @@ Entry make_entry() {
 #define OFX_CFGZ(N, T, TPB, PADSH, ...) StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, double2>
 #define OFX_EZ(KIND, N, T, TPB, PADSH, ...) make_entry<OFX_CFGZ(N, T, TPB, PADSH, __VA_ARGS__), KIND>()
 // split-exchange variant of a direct complex64 configuration
-// real transforms with the Hermitian partners in one thread (IOV = 3)
+// real transforms with the Hermitian partners in one thread (IOV = 3), and their streaming variants
 #define OFX_EH(KIND, N, T, TPB, PADSH, ...) make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 3>, KIND>()
+// a configuration with I/O form IOV and streaming mode MODE (StockhamCfg::STREAM)
+#define OFX_EM(KIND, N, T, TPB, PADSH, IOV, MODE, ...) \
+    make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, IOV, 0, 1, float2, MODE>, KIND>()
 #define OFX_EX(KIND, N, T, TPB, PADSH, ...) \
     make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, float2, false, true>, KIND>()
 // streaming variants (STREAM_ = true) of a direct, a paired and a complex128 configuration

@@ -49,15 +49,18 @@ struct Radices {
 // interleaved in LDS, element i of column c at lds[c + i*(COLS_ + COLPAD_)], and consecutive
 // lanes take consecutive columns, so each HBM row segment of COLS_ values is one contiguous access.
 // V_: the element type, float2 (complex64) or double2 (complex128).
-// STREAM_: the kernel's HBM input and output use streaming (non-temporal) accesses; for working
-// sets well beyond the last-level cache (chosen by the plan).
+// STREAM_: the kernel's HBM input and/or output use streaming (non-temporal) accesses: 1 both,
+// 2 the stores only, 3 the loads only (0 neither); chosen by the plan by working-set size, or
+// where measured faster.
 // SPLITX_: the LDS tile holds one scalar per element and each exchange between passes moves the
 // real parts, then the imaginary parts (split_fft below).
 template <int N_, int T_, int TPB_, int PADSH_, bool STAGED_, class RL, int IOV_ = 1, int COLS_ = 0, int COLPAD_ = 1,
-          class V_ = float2, bool STREAM_ = false, bool SPLITX_ = false>
+          class V_ = float2, int STREAM_ = 0, bool SPLITX_ = false>
 struct StockhamCfg {
     using V = V_;
-    static constexpr bool STREAM = STREAM_;
+    static constexpr int STREAM = STREAM_;                       // 0 none, 1 both, 2 stores, 3 loads
+    static constexpr bool NT_LOAD = STREAM_ == 1 || STREAM_ == 3;   // streaming HBM loads
+    static constexpr bool NT_STORE = STREAM_ == 1 || STREAM_ == 2;  // streaming HBM stores
     static constexpr bool SPLITX = SPLITX_;
     static constexpr int N = N_;
     static constexpr int T = T_;

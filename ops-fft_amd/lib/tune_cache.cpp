@@ -65,7 +65,7 @@ std::string kernel_signature(const ofx_kernel_info& k) {
     os << k.kind << ':' << k.n << ':' << k.threads_per_transform << ':' << k.transforms_per_block << ':' << k.staged
        << ':' << k.iov << ':';
     for (int p = 0; p < k.npass; ++p) os << (p ? "x" : "") << k.radices[p];
-    if (k.stream) os << ":s";
+    if (k.stream) os << ":s" << (k.stream == 1 ? std::string() : std::to_string(k.stream));
     return os.str();
 }
 

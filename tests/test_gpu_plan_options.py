@@ -184,3 +184,47 @@ def test_uneven_groups_at_odd_offsets(cuda, monkeypatch, mode):
     torch.cuda.synchronize()
     check_groups(yt.cpu().numpy(), x, groups, aclfft.FORWARD)
     p.destroy()
+
+
+# (type, n, batch): single-launch plans whose output is at least 16 MiB take the streaming-store
+# twin of their default kernel
+STORE_CASES = [(aclfft.C2C, 4096, 1024), (aclfft.C2C, 1024, 4096), (aclfft.R2C, 8192, 1024),
+               (aclfft.C2R, 8192, 1024), (aclfft.R2C, 4096, 2048), (aclfft.C2R, 3000, 2048)]
+
+
+@pytest.mark.parametrize("kind,n,batch", STORE_CASES)
+def test_streaming_store_rule(cuda, monkeypatch, kind, n, batch):
+    """Plans of one launch with >= 16 MiB of output write it with streaming stores (the kernel's
+    description names "ntstore"); ACLFFT_STREAM_STORES=0 keeps cached stores.  Both against the
+    FP64 oracle, and the two outputs bit-identical (only the store instruction differs)."""
+    import torch
+    outs = []
+    for rule in ("1", "0"):
+        monkeypatch.setenv("ACLFFT_STREAM_STORES", rule)
+        p = aclfft.Plan.plan1d(n, kind, batch)
+        d = p.describe()
+        assert ("ntstore" in d) == (rule == "1"), d
+        if kind == aclfft.C2C:
+            x = crandn((batch, n), 3).astype(np.complex64)
+            xt = torch.from_numpy(x).cuda()
+            yt = torch.empty_like(xt)
+            p.exec_c2c(xt, yt, aclfft.FORWARD)
+            ref = oracle.c2c(x[:64].astype(np.complex128))
+        elif kind == aclfft.R2C:
+            x = np.random.default_rng(3).standard_normal((batch, n)).astype(np.float32)
+            xt = torch.from_numpy(x).cuda()
+            yt = torch.empty((batch, n // 2 + 1), dtype=torch.complex64, device=cuda)
+            p.exec_r2c(xt, yt)
+            ref = oracle.r2c(x[:64].astype(np.float64))
+        else:
+            X = crandn((batch, n // 2 + 1), 3).astype(np.complex64)
+            xt = torch.from_numpy(X).cuda()
+            yt = torch.empty((batch, n), dtype=torch.float32, device=cuda)
+            p.exec_c2r(xt, yt)
+            ref = oracle.c2r(X[:64].astype(np.complex128), n)
+        torch.cuda.synchronize()
+        y = yt.cpu().numpy()
+        assert oracle.rel_l2(y[:64], ref) <= fp32_tol(n)
+        outs.append(y)
+        p.destroy()
+    assert np.array_equal(outs[0], outs[1])
