@@ -283,6 +283,10 @@ OFX_API int ofx_batch_blocks_per_cu(int schedule);
  * factored form's per-lane and per-column factors from FP64 sincospi, their product in FP64, one
  * rounding to FP32) instead of from FP32 sincospif factors (about 2 ulp). */
 #define OFX_FLAG_FT_FP64 2
+/* Streaming (non-temporal) HBM loads / stores, for the launches that take them as flags (the
+ * one-pass 2D kernels) */
+#define OFX_FLAG_NT_LOAD 4
+#define OFX_FLAG_NT_STORE 8
 
 /* Device scratch area of the allocation-free execution entry point (aclfftExtExecC2CNoAlloc):
  * OFX_SCRATCH_BYTES on every GPU, a file-scope device array held from the library's load until the

@@ -39,18 +39,23 @@ struct Fused2D {
 };
 
 // column store: element i of column c of the image
-template <class C>
+template <class C, bool NTS>
 struct Store2D {
     using V = typename C::V;
     V* __restrict__ p;  // element (0, c) of the output image
     int stride;
     float sg, scale;
-    __device__ __forceinline__ void operator()(int i, V v) const { p[i * stride] = mkv(v.x * scale, v.y * (sg * scale)); }
+    __device__ __forceinline__ void operator()(int i, V v) const {
+        if constexpr (NTS)
+            st_nt(p + i * stride, mkv(v.x * scale, v.y * (sg * scale)));
+        else
+            p[i * stride] = mkv(v.x * scale, v.y * (sg * scale));
+    }
 };
 
 // args: in/out images of NX*NY complex64 at idist/odist apart (batch images), tw = the row passes'
 // twiddle table, tw_aux = the column passes' table
-template <class F>
+template <class F, bool NTL, bool NTS>
 __global__ __launch_bounds__(F::BLOCK) void k_c2c_2d_lds(ofx_args a) {
     using RowC = typename F::RowC;
     using ColC = typename F::ColC;
@@ -65,7 +70,11 @@ __global__ __launch_bounds__(F::BLOCK) void k_c2c_2d_lds(ofx_args a) {
         if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
             const float4* s4 = reinterpret_cast<const float4*>(src);
             for (int l = threadIdx.x; l < F::NX * F::NY / 2; l += F::BLOCK) {
-                const float4 v = s4[l];
+                float4 v;
+                if constexpr (NTL)
+                    v = ld_nt(s4 + l);
+                else
+                    v = s4[l];
                 const int e = 2 * l;
                 const int r = e / F::NY, j = e - r * F::NY;
                 float2* d = lds + F::pos(r, j);
@@ -97,7 +106,7 @@ __global__ __launch_bounds__(F::BLOCK) void k_c2c_2d_lds(ofx_args a) {
         const int tid = threadIdx.x / F::NY;
         float2* col = lds + c + (c >> F::PADSH);
         const LdsRead<ColC> rd{col};
-        const Store2D<ColC> st{dst + c, F::NY, a.out_sign, a.scale};
+        const Store2D<ColC, NTS> st{dst + c, F::NY, a.out_sign, a.scale};
         stockham_passes<ColC, true, false>(col, tid, static_cast<const float2*>(a.tw_aux), rd, st);
     }
 }
@@ -106,7 +115,15 @@ template <class F>
 hipError_t launch_2d(const ofx_args* a, hipStream_t s) {
     if (a->batch <= 0) return hipSuccess;
     if (a->batch > 0x7fffffffLL) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL(k_c2c_2d_lds<F>, dim3((unsigned)a->batch), dim3(F::BLOCK), 0, s, *a);
+    const bool ntl = (a->flags & OFX_FLAG_NT_LOAD) != 0, nts = (a->flags & OFX_FLAG_NT_STORE) != 0;
+    if (ntl && nts)
+        hipLaunchKernelGGL((k_c2c_2d_lds<F, true, true>), dim3((unsigned)a->batch), dim3(F::BLOCK), 0, s, *a);
+    else if (nts)
+        hipLaunchKernelGGL((k_c2c_2d_lds<F, false, true>), dim3((unsigned)a->batch), dim3(F::BLOCK), 0, s, *a);
+    else if (ntl)
+        hipLaunchKernelGGL((k_c2c_2d_lds<F, true, false>), dim3((unsigned)a->batch), dim3(F::BLOCK), 0, s, *a);
+    else
+        hipLaunchKernelGGL((k_c2c_2d_lds<F, false, false>), dim3((unsigned)a->batch), dim3(F::BLOCK), 0, s, *a);
     return hipGetLastError();
 }
 

@@ -275,6 +275,20 @@ struct Builder {
             }
     }
 
+    // The same rule for 2D plans that run as the one-pass kernel: streaming stores for >= 16 MiB
+    // of output, and streaming loads as well when input plus output is >= 384 MiB (as the 1D
+    // kernels do).  Measured at 2^26 points: 32^2 173.5 vs 181.3 us, 64^2 200.0 vs 205.3 us,
+    // 128^2 259.8 vs 286.0 us (2^22 points of 128^2: 14.9 vs 17.8 us).  Column passes keep cached
+    // stores: their groups write 64-128-byte runs of a row, and streaming them measured slower
+    // (2048^2 x 4 102 vs 92 us, 1024^2 x 64 410 vs 389 us).
+    void stream_final_multi(size_t in_bytes, size_t out_bytes) {
+        if (e->steps_.size() != 1 || env_int("ACLFFT_STREAM_STORES", 1) == 0) return;
+        Step& s = e->steps_.back();
+        if (s.type != Step::FUSED2D || s.dst != BUF_OUT) return;
+        if (out_bytes >= kStoreStreamBytes) s.args.flags |= OFX_FLAG_NT_STORE;
+        if (in_bytes + out_bytes >= kStreamBytes) s.args.flags |= OFX_FLAG_NT_LOAD;
+    }
+
     // A runtime-radix step of length n; false if n has an unsupported prime factor or is too long.
     static long long generic_max(bool z) { return z ? OFX_GENERIC_MAX_ELEMS_Z : OFX_GENERIC_MAX_ELEMS; }
     bool generic_step(long long n, Step* out, bool z = false) {
@@ -1391,6 +1405,7 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
             inner *= dims[i];
         }
         }
+        if (r == ACLFFT_SUCCESS && !z) b.stream_final_multi(e->in_bytes_, e->out_bytes_);
     }
     if (r != ACLFFT_SUCCESS) {
         delete e;
@@ -1533,6 +1548,8 @@ std::string Executor::describe() const {
             for (int p = 0; p < k.row_npass; ++p) os << (p ? "x" : "") << k.row_radices[p];
             os << " cols=";
             for (int p = 0; p < k.col_npass; ++p) os << (p ? "x" : "") << k.col_radices[p];
+            if (s.args.flags & OFX_FLAG_NT_LOAD) os << " ntload";
+            if (s.args.flags & OFX_FLAG_NT_STORE) os << " ntstore";
             continue;
         }
         if (s.type == Step::COLUMN) {

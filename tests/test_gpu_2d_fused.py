@@ -96,3 +96,31 @@ def test_fused_2d_unaligned_scaled_and_3d(cuda):
     torch.cuda.synchronize()
     assert oracle.rel_l2(vt.cpu().numpy(), oracle.c2c_2d(v.astype(np.complex128))) <= fp32_tol(nx * ny)
     q.destroy()
+
+
+@pytest.mark.parametrize("nx,ny,batch,flags", [(128, 128, 1024, "ntstore"), (128, 128, 6144, "ntload ntstore"),
+                                               (32, 32, 1024, "")])
+def test_fused_2d_streaming(cuda, monkeypatch, nx, ny, batch, flags):
+    """Streaming stores for >= 16 MiB of output, streaming loads too for >= 384 MiB of input plus
+    output (the plan's description names them): the first and last images against the FP64
+    oracle, and the whole output bit-identical to cached accesses (ACLFFT_STREAM_STORES=0)."""
+    import torch
+    g = torch.Generator(device=cuda).manual_seed(nx + batch)
+    x = torch.randn((batch, nx, ny), dtype=torch.complex64, device=cuda, generator=g)
+    outs = []
+    for rule in ("1", "0"):
+        monkeypatch.setenv("ACLFFT_STREAM_STORES", rule)
+        p = aclfft.Plan.plan2d(batch, nx, ny, aclfft.C2C)
+        d = p.describe()
+        want = flags if rule == "1" else ""
+        assert d.endswith(want), d
+        assert ("ntload" in d) == ("ntload" in want) and ("ntstore" in d) == ("ntstore" in want), d
+        y = torch.empty_like(x)
+        p.exec_c2c(x, y, aclfft.FORWARD)
+        torch.cuda.synchronize()
+        p.destroy()
+        outs.append(y)
+    assert torch.equal(outs[0], outs[1])
+    for sl in (slice(0, 4), slice(batch - 4, batch)):
+        ref = oracle.c2c_2d(x[sl].cpu().numpy().astype(np.complex128))
+        assert oracle.rel_l2(outs[0][sl].cpu().numpy(), ref) <= fp32_tol(nx * ny)
