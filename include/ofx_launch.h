@@ -250,15 +250,19 @@ typedef struct ofx_batch_args {
     unsigned int* counter;          /* device work counter (reset by the launch; dynamic only)  */
     int schedule;                   /* 0 dynamic: after its first tile (blockIdx), a workgroup
                                        takes the next index from the counter, fetched one tile
-                                       ahead; 1 static: the k-th tile of workgroup b is entry
-                                       k*blocks + (k even ? b : blocks-1-b) of the list (the list
+                                       ahead; 1 static: the k-th tile of workgroup b is walk
+                                       position k*blocks + (k even ? b : blocks-1-b) (the list
                                        is sorted by cost, so the rounds alternate direction),
-                                       its descriptor loaded one tile ahead; no counter        */
+                                       its descriptor loaded one tile ahead; no counter.  Walk
+                                       positions map to list entries as prologue_rounds says   */
     float scale;                    /* multiplies every output value (0 is taken as 1)         */
     int class_set;                  /* compiled class set (ofx_batch_class_set_count; 0 default)  */
     int stream;                     /* 1: streaming (non-temporal) HBM accesses for the classes
                                        whose transforms are whole 128-byte lines (n % 16 == 0),
                                        for working sets well beyond the last-level cache      */
+    int prologue_rounds;            /* the walk's first min(prologue_rounds * blocks, ntiles / 2)
+                                       positions take the smallest tiles, in increasing cost (the
+                                       list's end, backwards); the rest the list from its start  */
 } ofx_batch_args;
 OFX_API int ofx_batch_class_count(void);
 OFX_API int ofx_batch_class_info_get(int cls, ofx_batch_class_info* info);   /* class set 0 */

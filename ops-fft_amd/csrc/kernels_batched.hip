@@ -188,6 +188,12 @@ __device__ __forceinline__ ofx_batch_tile load_tile(const ofx_batch_tile* tiles,
     return t;
 }
 
+// Position q of a workgroup's walk -> tile index.  The list is sorted by decreasing cost; the first
+// P positions take the P smallest tiles in increasing cost (the list's end, backwards), the rest
+// the list from its start.  A prologue of small, uneven tiles lets the workgroups drift apart
+// before the long tiles come, so the long tiles do not all load, compute and store in lock step.
+__device__ __forceinline__ int tile_of(int q, int ntiles, int P) { return q < P ? ntiles - 1 - q : q - P; }
+
 // Every loop below is uniform over the workgroup (tile indices come from blockIdx, the tile list
 // or LDS), so every wave leaves together once the index passes ntiles.
 template <class CS, int SCHED, bool STREAM>
@@ -196,14 +202,16 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args
     __shared__ int s_next[2];
     const int G = (int)gridDim.x;
     const int b = (int)blockIdx.x;
+    // prologue: prologue_rounds tiles per workgroup, at most half the list
+    const int P = (int)min((long long)a.prologue_rounds * G, (long long)(a.ntiles / 2));
     if constexpr (SCHED == 1) {
         int t = b;
         ofx_batch_tile cur{};
-        if (t < a.ntiles) cur = load_tile(a.tiles, t);
+        if (t < a.ntiles) cur = load_tile(a.tiles, tile_of(t, a.ntiles, P));
         for (int k = 1; t < a.ntiles; ++k) {
             const int tn = k * G + ((k & 1) ? G - 1 - b : b);
             ofx_batch_tile nxt{};
-            if (tn < a.ntiles) nxt = load_tile(a.tiles, tn);  // in flight while this tile runs
+            if (tn < a.ntiles) nxt = load_tile(a.tiles, tile_of(tn, a.ntiles, P));  // in flight while this tile runs
             __asm__ volatile("" ::: "memory");      // no load of the tile bodies moves out of the loop
             run_class<CS, STREAM>(lds, cur, a);
             lds_barrier();  // LDS is reused by the next tile
@@ -216,7 +224,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(ofx_batch_args
         while (t < a.ntiles) {
             unsigned int nx = 0;
             if (threadIdx.x == 0) nx = atomicAdd(a.counter, 1u);  // the next index, used after this tile
-            const ofx_batch_tile cur = load_tile(a.tiles, t);
+            const ofx_batch_tile cur = load_tile(a.tiles, tile_of(t, a.ntiles, P));
             __asm__ volatile("" ::: "memory");
             run_class<CS, STREAM>(lds, cur, a);
             // two slots: a wave may still read this tile's slot while thread 0 writes the next one

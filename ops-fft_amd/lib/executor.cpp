@@ -1045,6 +1045,13 @@ struct Builder {
                                            : (env_int("ACLFFT_PERSISTENT_SCHEDULE", 1) == 0 ? 0 : 1);
         s.blocks_per_cu = e->desc_.blocks_per_cu > 0 ? e->desc_.blocks_per_cu
                                                      : std::max(0, env_int("ACLFFT_PERSISTENT_BLOCKS_PER_CU", 0));
+        // small-tile prologue: the walk's first 3 tiles per workgroup are the smallest ones, so the
+        // workgroups drift apart before the long tiles (512-group batch: 83.0-83.8 vs 90.3 us;
+        // 1 per workgroup 88.5, 2 88.9-89.2, 4 84.7 us, the whole list reversed 87.0-87.3 us)
+        {
+            const int env_pro = env_int("ACLFFT_PERSISTENT_PROLOGUE", -1);
+            s.b.prologue_rounds = e->desc_.prologue >= 0 ? e->desc_.prologue : env_pro >= 0 ? env_pro : 3;
+        }
         {
             const int env_stream = env_int("ACLFFT_PERSISTENT_STREAM", -1);
             s.b.stream = e->desc_.stream >= 0 ? (e->desc_.stream ? 1 : 0)
@@ -1562,7 +1569,7 @@ std::string Executor::describe() const {
         }
         if (s.type == Step::PERSISTENT) {
             os << "persistent tiles=" << s.b.ntiles << " set=" << s.b.class_set << " sched=" << s.b.schedule
-               << (s.b.stream ? " stream" : "");
+               << (s.b.stream ? " stream" : "") << " prologue=" << s.b.prologue_rounds;
             if (s.blocks_per_cu > 0) os << " bpc=" << s.blocks_per_cu;
             os << " classes=";
             bool first = true;
@@ -2078,6 +2085,7 @@ aclfftResult Executor::tune_plan(hipStream_t s) {
         c.blocks_per_cu = 0;
         c.class_set = 0;
         c.stream = -1;
+        c.prologue = -1;
         if (!desc_.uneven) {
             cands.push_back(c);
             continue;
@@ -2094,18 +2102,20 @@ aclfftResult Executor::tune_plan(hipStream_t s) {
             for (int sched = 0; sched <= 1; ++sched) {
                 const int maxb = std::max(1, ofx_batch_blocks_per_cu_set(set, sched));
                 for (int bpc = 1; bpc <= maxb; ++bpc)
-                    for (int stream = 0; stream <= 1; ++stream) {
-                        // MFMA precisions: the FP32 part (lengths without MFMA kernels) at its
-                        // defaults
-                        if (prec != 0 && !(sched == 1 && bpc == maxb && stream == 0)) continue;
-                        PlanDesc t = c;
-                        t.uneven_form = 1;
-                        t.class_set = set;
-                        t.sched = sched;
-                        t.blocks_per_cu = bpc;
-                        t.stream = prec != 0 ? -1 : stream;
-                        cands.push_back(t);
-                    }
+                    for (int stream = 0; stream <= 1; ++stream)
+                        for (int pro : {3, 0}) {
+                            // MFMA precisions: the FP32 part (lengths without MFMA kernels) at
+                            // its defaults
+                            if (prec != 0 && !(sched == 1 && bpc == maxb && stream == 0 && pro == 3)) continue;
+                            PlanDesc t = c;
+                            t.uneven_form = 1;
+                            t.class_set = set;
+                            t.sched = sched;
+                            t.blocks_per_cu = bpc;
+                            t.stream = prec != 0 ? -1 : stream;
+                            t.prologue = prec != 0 ? -1 : pro;
+                            cands.push_back(t);
+                        }
             }
     }
     // scratch input (pseudo-random) and output, sized for the plan's buffers

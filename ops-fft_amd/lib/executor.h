@@ -85,6 +85,9 @@ struct PlanDesc {
     int stream = -1;               // persistent kernel: 1 streaming accesses for its whole-line
                                    // classes, 0 cached; negative: ACLFFT_PERSISTENT_STREAM, else
                                    // streaming when the plan's input plus output is >= 384 MiB
+    int prologue = -1;             // persistent kernel: small-tile prologue in tiles per
+                                   // workgroup (ofx_batch_args::prologue_rounds); negative:
+                                   // ACLFFT_PERSISTENT_PROLOGUE, else 3
 };
 
 class Executor {

@@ -176,10 +176,11 @@ bool load_plan_choice(const PlanDesc& d, PlanDesc* out) {
     auto m = load();
     auto it = m.find(plan_key(d));
     if (it == m.end()) return false;
-    // "form:sched:bpc:set:precision:stream" (entries without the last field: stream by default)
-    int v[6] = {0, -1, 0, 0, 0, -1};
-    const int got = std::sscanf(it->second.c_str(), "%d:%d:%d:%d:%d:%d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5]);
-    if (got != 5 && got != 6) return false;
+    // "form:sched:bpc:set:precision:stream:prologue" (older entries end early: the defaults)
+    int v[7] = {0, -1, 0, 0, 0, -1, -1};
+    const int got =
+        std::sscanf(it->second.c_str(), "%d:%d:%d:%d:%d:%d:%d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6]);
+    if (got < 5) return false;
     *out = d;
     out->uneven_form = v[0];
     out->sched = v[1];
@@ -187,6 +188,7 @@ bool load_plan_choice(const PlanDesc& d, PlanDesc* out) {
     out->class_set = v[3];
     out->precision = v[4];
     out->stream = v[5];
+    out->prologue = v[6];
     return true;
 }
 
@@ -195,7 +197,7 @@ void store_plan_choice(const PlanDesc& c) {
     auto m = load();
     m[plan_key(c)] = std::to_string(c.uneven_form) + ":" + std::to_string(c.sched) + ":" +
                      std::to_string(c.blocks_per_cu) + ":" + std::to_string(c.class_set) + ":" +
-                     std::to_string(c.precision) + ":" + std::to_string(c.stream);
+                     std::to_string(c.precision) + ":" + std::to_string(c.stream) + ":" + std::to_string(c.prologue);
     write_cache(m);
 }
 

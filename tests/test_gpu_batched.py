@@ -192,3 +192,31 @@ def test_default_form(cuda, monkeypatch):
             assert d.startswith("persistent"), (ln, bt, d)
         else:
             assert "persistent" not in d and "kind7" in d and "[lane" not in d, (ln, bt, d)
+
+
+@pytest.mark.parametrize("sched", [0, 1])
+@pytest.mark.parametrize("prologue", [1, 3, 1000])
+def test_persistent_prologue(cuda, monkeypatch, sched, prologue):
+    """The small-tile prologue (ACLFFT_PERSISTENT_PROLOGUE tiles per workgroup, capped at half the
+    list; 1000 exceeds every grid here) only reorders the walk: each group against the FP64
+    oracle and the output bit-identical to the walk without it."""
+    import torch
+    monkeypatch.setenv("ACLFFT_UNEVEN_MODE", "1")
+    monkeypatch.setenv("ACLFFT_PERSISTENT_SCHEDULE", str(sched))
+    ln, bt = uneven_workload(64, 13)
+    groups = list(zip(ln, bt))
+    total = sum(n * b for n, b in groups)
+    x = crandn(total, 6).astype(np.complex64)
+    xt = torch.from_numpy(x).cuda()
+    outs = []
+    for pro in (prologue, 0):
+        monkeypatch.setenv("ACLFFT_PERSISTENT_PROLOGUE", str(pro))
+        p = aclfft.Plan.plan_many_uneven(ln, bt)
+        assert "prologue=%d" % pro in p.describe()
+        yt = torch.empty_like(xt)
+        p.exec_c2c(xt, yt, aclfft.FORWARD)
+        torch.cuda.synchronize()
+        outs.append(yt.cpu().numpy())
+        p.destroy()
+    check_groups(outs[0], x, groups, aclfft.FORWARD)
+    assert np.array_equal(outs[0], outs[1])
