@@ -109,12 +109,15 @@ TILES = [
 
 # real transforms with the Hermitian partner butterflies in one thread (IOV = 3): the R2C unpack
 # runs in the last pass's registers, the C2R pre-pack in the first pass's, so the spectrum makes
-# one trip through LDS fewer: (kind, N, T, TPB, radices).  HPAIR_FIRST lead their length's
+# one trip through LDS fewer: (kind, N, T, TPB, radices[, pad shift]).  HPAIR_FIRST lead their length's
 # candidates (real 4096 x 8192: R2C 45.5 vs 46.4 us, C2R 45.3 vs 48.4 us for the LDS unpack /
 # pre-pack kernels of the same radices).  At n = 8192 (N = 4096) they measured no faster (R2C x 1024
-# 15.2 us either way, x 8192 117.9 vs 113.9 us; C2R 16.9 vs 15.6 us) and are not built.
+# 15.2 us either way, x 8192 117.9 vs 113.9 us; C2R 16.9 vs 15.6 us) and are not built.  Real 3000
+# (N = 1500): the C2R pair kernel at 64 lanes (unit count 63) with streaming stores is the store
+# rule's choice (36.0 vs 37.3-38.4 us); the R2C one measured slower (36.3 vs 33.5 us).
 HPAIR_FIRST = [("R2C", 2048, 256, 1, (8, 8, 8, 4)), ("C2R", 2048, 256, 1, (4, 8, 8, 8))]
-HPAIR = [("R2C", 2048, 128, 1, (16, 16, 8)), ("C2R", 2048, 128, 1, (8, 16, 16))]
+HPAIR = [("R2C", 2048, 128, 1, (16, 16, 8)), ("C2R", 2048, 128, 1, (8, 16, 16)),
+         ("C2R", 1500, 64, 2, (12, 5, 25), 30)]
 
 # streaming variants by access (mode 1 loads and stores, 2 stores only, 3 loads only):
 # (kind, N, T, TPB, radices, iov, mode[, pad shift]).  The mode-2 twins of the default kernels are
@@ -125,7 +128,8 @@ NTVAR = [
     ("C2R", 2048, 256, 1, (4, 8, 8, 8), 3, 2),
     ("R2C", 4096, 512, 1, (8, 8, 8, 8), 1, 2), ("R2C", 4096, 512, 1, (8, 8, 8, 8), 1, 3),
     ("R2C", 4096, 512, 1, (8, 8, 8, 8), 1, 1), ("C2R", 4096, 512, 1, (8, 8, 8, 8), 1, 2),
-    ("R2C", 1500, 256, 1, (10, 15, 10), 1, 2, 30), ("C2R", 1500, 128, 2, (12, 5, 25), 1, 2, 30),
+    ("R2C", 1500, 256, 1, (10, 15, 10), 1, 2, 30), ("C2R", 1500, 64, 2, (12, 5, 25), 3, 2, 30),
+    ("C2R", 1500, 128, 2, (12, 5, 25), 1, 2, 30),
     ("R2C", 1024, 64, 4, (16, 16, 4), 1, 2), ("C2R", 1024, 64, 4, (16, 16, 4), 1, 2),
     ("C2C", 2048, 128, 2, (16, 16, 8), 1, 2), ("C2C", 4096, 256, 1, (16, 16, 16), 1, 2),
     ("C2C", 1024, 64, 2, (8, 16, 8), 2, 2), ("C2C", 1024, 64, 2, (8, 8, 16), 2, 2),
@@ -304,9 +308,10 @@ def main():
         for kind, n, t, tpb, rad, psh in PADX:
             assert t * tpb <= 1024
             f.write("    OFX_E(%s, %d, %d, %d, %d, false, %s),\n" % (KINDS[kind], n, t, tpb, psh, ", ".join(str(r) for r in rad)))
-        for kind, n, t, tpb, rad in HPAIR:
+        for kind, n, t, tpb, rad, *psh in HPAIR:
             assert t * tpb <= 1024
-            f.write("    OFX_EH(%s, %d, %d, %d, %d, %s),\n" % (KINDS[kind], n, t, tpb, padsh(n), ", ".join(str(r) for r in rad)))
+            f.write("    OFX_EH(%s, %d, %d, %d, %d, %s),\n" % (KINDS[kind], n, t, tpb, psh[0] if psh else padsh(n),
+                                                       ", ".join(str(r) for r in rad)))
         for kind, n, t, tpb, rad, iov, mode, *psh in NTVAR:
             assert t * tpb <= 1024
             f.write("    OFX_EM(%s, %d, %d, %d, %d, %d, %d, %s),\n" % (KINDS[kind], n, t, tpb, psh[0] if psh else padsh(n),
