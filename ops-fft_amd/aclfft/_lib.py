@@ -172,7 +172,7 @@ def kernel_registry():
         lib.ofx_kernel_info_get(i, ctypes.byref(k))
         out.append(dict(id=i, kind=k.kind, n=k.n, T=k.threads_per_transform, TPB=k.transforms_per_block,
                         block=k.block_threads, radices=list(k.radices[:k.npass]), staged=bool(k.staged),
-                        lds_bytes=k.lds_bytes, tw_len=k.tw_len, iov=k.iov, stream=bool(k.stream)))
+                        lds_bytes=k.lds_bytes, tw_len=k.tw_len, iov=k.iov, stream=int(k.stream)))
     return out
 
 
