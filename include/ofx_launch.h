@@ -291,6 +291,9 @@ OFX_API int ofx_batch_blocks_per_cu(int schedule);
  * one-pass 2D kernels) */
 #define OFX_FLAG_NT_LOAD 4
 #define OFX_FLAG_NT_STORE 8
+/* One-pass 2D kernels: run the non-pipelined form (one image per workgroup) even where the
+ * pipelined one applies */
+#define OFX_FLAG_NO_PIPELINE 16
 
 /* Device scratch area of the allocation-free execution entry point (aclfftExtExecC2CNoAlloc):
  * OFX_SCRATCH_BYTES on every GPU, a file-scope device array held from the library's load until the

@@ -15,6 +15,8 @@
 // RS apart) spread over the banks.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "ofx_launch.h"
 #include "stockham.h"
 
@@ -32,6 +34,16 @@ struct Fused2D {
     using RowC = StockhamCfg<NY, T1_, NX, PADSH, false, RowRadices>;
     // columns: element i of a column at i * RS from the column's padded position in a row
     using ColC = StockhamCfg<NX, T2, 1, PADSH, false, ColRadices, 1, NY, RS - NY>;
+    // the same with LDS-only barriers, for the pipelined kernel (its next image's loads stay in
+    // flight through the column passes)
+    using ColCP = StockhamCfg<NX, T2, 1, PADSH, false, ColRadices, 1, NY, RS - NY, float2, 0, false, true>;
+    // float4 (element pair) loads per lane for one image
+    static constexpr int NV = NX * NY / 2 / BLOCK;
+    static constexpr bool PIPE_OK = (NX * NY / 2) % BLOCK == 0;
+    // waves per SIMD the pipelined kernel's registers must allow: all the workgroups its LDS tile
+    // lets a CU hold (160 KiB)
+    static constexpr int PIPE_WGS = 163840 / (NX * RS * 8) > 0 ? 163840 / (NX * RS * 8) : 1;
+    static constexpr int PIPE_MINW = PIPE_WGS * (BLOCK / 64) / 4 > 8 ? 8 : PIPE_WGS * (BLOCK / 64) / 4;
     static_assert(T2 * NY == BLOCK && BLOCK <= 1024, "row and column phases share the workgroup");
     static_assert(RowC::WAVE_SYNC, "a row's lanes must stay inside one wave");
     static constexpr int LDS = NX * RS;
@@ -111,11 +123,121 @@ __global__ __launch_bounds__(F::BLOCK) void k_c2c_2d_lds(ofx_args a) {
     }
 }
 
+// Pipelined form for the sizes whose LDS tile allows at most two workgroups per CU (an image's
+// load, passes and stores would otherwise run one after the other on a CU): a grid of resident
+// workgroups, each taking images b, b + G, ...; the next image's 16-byte loads are issued into
+// registers right after the current image is in LDS and stay in flight through its row and column
+// passes (whose barriers order LDS only), and its column stores drain while the next image is
+// written to LDS.  Images must start 16-byte aligned (checked by the launcher).
+template <class F, bool NTL, bool NTS>
+__global__ __launch_bounds__(F::BLOCK, F::PIPE_MINW) void k_c2c_2d_lds_pipe(ofx_args a) {
+    using RowC = typename F::RowC;
+    using ColC = typename F::ColCP;
+    static_assert(F::PIPE_OK, "whole element pairs per lane");
+    __shared__ float2 lds[F::LDS];
+    const long long G = gridDim.x;
+    long long img = blockIdx.x;
+    const float sg = a.in_sign;
+    float4 buf[F::NV];
+    auto load = [&](long long m) {
+        const float4* s4 = reinterpret_cast<const float4*>(static_cast<const float2*>(a.in) + m * a.idist);
+#pragma unroll
+        for (int v = 0; v < F::NV; ++v) {
+            if constexpr (NTL)
+                buf[v] = ld_nt(s4 + threadIdx.x + v * F::BLOCK);
+            else
+                buf[v] = s4[threadIdx.x + v * F::BLOCK];
+        }
+    };
+    if (img < a.batch) load(img);
+    for (; img < a.batch; img += G) {
+#pragma unroll
+        for (int v = 0; v < F::NV; ++v) {
+            const int e = 2 * (int)(threadIdx.x + v * F::BLOCK);
+            const int r = e / F::NY, j = e - r * F::NY;
+            float2* d = lds + F::pos(r, j);
+            d[0] = make_float2(buf[v].x, buf[v].y * sg);
+            d[1] = make_float2(buf[v].z, buf[v].w * sg);
+        }
+        lds_barrier();
+        if (img + G < a.batch) load(img + G);  // in flight through this image's passes
+        {
+            const int r = threadIdx.x / RowC::T;
+            const int tid = threadIdx.x - r * RowC::T;
+            float2* row = lds + r * F::RS;
+            const LdsRead<RowC> rd{row};
+            const LdsWrite<RowC> wr{row};
+            stockham_passes<RowC, true, true>(row, tid, static_cast<const float2*>(a.tw), rd, wr);
+        }
+        lds_barrier();
+        {
+            const int c = threadIdx.x % F::NY;
+            const int tid = threadIdx.x / F::NY;
+            float2* col = lds + c + (c >> F::PADSH);
+            const LdsRead<ColC> rd{col};
+            const Store2D<ColC, NTS> st{static_cast<float2*>(a.out) + img * a.odist + c, F::NY, a.out_sign, a.scale};
+            stockham_passes<ColC, true, false>(col, tid, static_cast<const float2*>(a.tw_aux), rd, st);
+        }
+        lds_barrier();  // every column read of LDS precedes the next image's writes
+    }
+}
+
+// resident workgroups of the pipelined kernel on the current device (cached per device)
+template <class F, bool NTL, bool NTS>
+static long long pipe_slots() {
+    static std::atomic<int> cache[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        dev = -1;
+    }
+    int slots = dev >= 0 ? cache[dev].load(std::memory_order_relaxed) : 0;
+    if (slots <= 0) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_c2c_2d_lds_pipe<F, NTL, NTS>, F::BLOCK, 0) !=
+                hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev < 0 ? 0 : dev) != hipSuccess ||
+            per_cu <= 0 || cus <= 0) {
+            (void)hipGetLastError();
+            per_cu = 1;
+            cus = 256;
+        }
+        slots = per_cu * cus;
+        if (dev >= 0) cache[dev].store(slots, std::memory_order_relaxed);
+    }
+    return slots;
+}
+
+// The pipelined form pays for its prefetch registers (at the occupancy its LDS tile allows it
+// spills 48-64 bytes per lane): with one image per workgroup it is slower (128^2 x 256 images
+// 22.0 vs 15.0 us, x 1024: 67.3 vs 62.3 us), with 16 per workgroup faster (x 4096: 234 vs 260 us;
+// 64x128 / 128x64 x 8192: 215 / 212 vs 224 / 226 us).  It runs when every resident workgroup gets
+// at least 8 images.
+template <class F, bool NTL, bool NTS>
+static bool launch_2d_pipe(const ofx_args* a, hipStream_t s) {
+    const long long slots = pipe_slots<F, NTL, NTS>();
+    if (a->batch < 8 * slots) return false;
+    hipLaunchKernelGGL((k_c2c_2d_lds_pipe<F, NTL, NTS>), dim3((unsigned)slots), dim3(F::BLOCK), 0, s, *a);
+    return true;
+}
+
 template <class F>
 hipError_t launch_2d(const ofx_args* a, hipStream_t s) {
     if (a->batch <= 0) return hipSuccess;
     if (a->batch > 0x7fffffffLL) return hipErrorInvalidConfiguration;
     const bool ntl = (a->flags & OFX_FLAG_NT_LOAD) != 0, nts = (a->flags & OFX_FLAG_NT_STORE) != 0;
+    if constexpr (F::PIPE_OK && F::LDS * sizeof(float2) > 40 * 1024) {
+        // at most two workgroups per CU: the pipelined form, for 16-byte aligned images
+        const bool aligned = ((reinterpret_cast<uintptr_t>(a->in) | reinterpret_cast<uintptr_t>(a->out)) & 15) == 0 &&
+                             ((a->idist | a->odist) & 1) == 0;
+        if (aligned && (a->flags & OFX_FLAG_NO_PIPELINE) == 0) {
+            const bool ran = ntl && nts ? launch_2d_pipe<F, true, true>(a, s)
+                             : nts      ? launch_2d_pipe<F, false, true>(a, s)
+                             : ntl      ? launch_2d_pipe<F, true, false>(a, s)
+                                        : launch_2d_pipe<F, false, false>(a, s);
+            if (ran) return hipGetLastError();
+        }
+    }
     if (ntl && nts)
         hipLaunchKernelGGL((k_c2c_2d_lds<F, true, true>), dim3((unsigned)a->batch), dim3(F::BLOCK), 0, s, *a);
     else if (nts)

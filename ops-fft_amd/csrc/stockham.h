@@ -54,8 +54,10 @@ struct Radices {
 // where measured faster.
 // SPLITX_: the LDS tile holds one scalar per element and each exchange between passes moves the
 // real parts, then the imaginary parts (split_fft below).
+// LDSBAR_: the passes' workgroup barriers order LDS only (lds_barrier), so that global loads a
+// kernel keeps in flight across the passes (the next tile's data) are not waited for.
 template <int N_, int T_, int TPB_, int PADSH_, bool STAGED_, class RL, int IOV_ = 1, int COLS_ = 0, int COLPAD_ = 1,
-          class V_ = float2, int STREAM_ = 0, bool SPLITX_ = false>
+          class V_ = float2, int STREAM_ = 0, bool SPLITX_ = false, bool LDSBAR_ = false>
 struct StockhamCfg {
     using V = V_;
     static constexpr int STREAM = STREAM_;                       // 0 none, 1 both, 2 stores, 3 loads
@@ -122,6 +124,8 @@ struct StockhamCfg {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        } else if constexpr (LDSBAR_) {
+            lds_barrier();
         } else {
             __syncthreads();
         }

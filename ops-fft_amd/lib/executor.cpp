@@ -148,6 +148,9 @@ struct Builder {
             s.args.batch = images;
             s.args.idist = s.args.odist = nx * ny;
             s.args.in_sign = s.args.out_sign = s.args.scale = 1.f;
+            // the pipelined form (next image's loads in flight) for the sizes it covers, unless
+            // ACLFFT_2D_PIPE=0
+            if (env_int("ACLFFT_2D_PIPE", 1) == 0) s.args.flags |= OFX_FLAG_NO_PIPELINE;
             s.conj_in = s.conj_out = true;
             e->steps_.push_back(s);
             return true;

@@ -124,3 +124,28 @@ def test_fused_2d_streaming(cuda, monkeypatch, nx, ny, batch, flags):
     for sl in (slice(0, 4), slice(batch - 4, batch)):
         ref = oracle.c2c_2d(x[sl].cpu().numpy().astype(np.complex128))
         assert oracle.rel_l2(outs[0][sl].cpu().numpy(), ref) <= fp32_tol(nx * ny)
+
+
+@pytest.mark.parametrize("nx,ny,batch", [(128, 128, 2048 + 3), (64, 128, 4096 + 3), (128, 64, 4096 + 3)])
+def test_fused_2d_pipelined(cuda, monkeypatch, nx, ny, batch):
+    """The pipelined one-pass kernel (resident workgroups walking several images each, the next
+    image's loads in flight through the passes; it runs from 8 images per resident workgroup: 256
+    of 128^2, 512 of 64 x 128 on this GPU) against the one-image-per-workgroup form
+    (ACLFFT_2D_PIPE=0): bit-identical, with a ragged last round; a sample of images against the
+    FP64 oracle."""
+    import torch
+    g = torch.Generator(device=cuda).manual_seed(nx * ny)
+    x = torch.randn((batch, nx, ny), dtype=torch.complex64, device=cuda, generator=g)
+    outs = []
+    for pipe in ("1", "0"):
+        monkeypatch.setenv("ACLFFT_2D_PIPE", pipe)
+        p = aclfft.Plan.plan2d(batch, nx, ny, aclfft.C2C)
+        y = torch.empty_like(x)
+        p.exec_c2c(x, y, aclfft.BACKWARD)
+        torch.cuda.synchronize()
+        p.destroy()
+        outs.append(y)
+    assert torch.equal(outs[0], outs[1])
+    for sl in (slice(0, 3), slice(1000, 1003), slice(batch - 3, batch)):
+        ref = oracle.c2c_2d(x[sl].cpu().numpy().astype(np.complex128), aclfft.BACKWARD)
+        assert oracle.rel_l2(outs[0][sl].cpu().numpy(), ref) <= fp32_tol(nx * ny)
