@@ -100,6 +100,7 @@ struct StockhamCfg {
         const int by_waves = 32 / ((BLOCK + 63) / 64);
         return by_lds < by_waves ? by_lds : by_waves;
     }
+    static constexpr bool LDSBAR = LDSBAR_;
     static constexpr int MINW = IOV_ != 3 ? 1
                                 : (wgs_per_cu() * ((BLOCK + 63) / 64) / 4 > 8 ? 8
                                    : wgs_per_cu() * ((BLOCK + 63) / 64) / 4 < 1 ? 1
