@@ -39,11 +39,15 @@ struct Fused2D {
     using ColCP = StockhamCfg<NX, T2, 1, PADSH, false, ColRadices, 1, NY, RS - NY, float2, 0, false, true>;
     // float4 (element pair) loads per lane for one image
     static constexpr int NV = NX * NY / 2 / BLOCK;
-    static constexpr bool PIPE_OK = (NX * NY / 2) % BLOCK == 0;
+    // the pipelined form pays off where the one-image kernel's row passes (radix-16 rows, ny >= 64)
+    // already hold it to 4-5 waves per SIMD; the 32-point rows run at 8 waves per SIMD one image
+    // per workgroup, and the prefetch registers would halve that (2^26 points: 32^2 189 vs 173 us,
+    // 64x32 185 vs 172 us, 128x32 188 vs 183 us pipelined)
+    static constexpr bool PIPE_OK = (NX * NY / 2) % BLOCK == 0 && NY >= 64;
     // waves per SIMD the pipelined kernel's registers must allow: all the workgroups its LDS tile
-    // lets a CU hold (160 KiB)
+    // lets a CU hold (160 KiB), at most 4 (128 VGPRs: the prefetch and the passes without spills)
     static constexpr int PIPE_WGS = 163840 / (NX * RS * 8) > 0 ? 163840 / (NX * RS * 8) : 1;
-    static constexpr int PIPE_MINW = PIPE_WGS * (BLOCK / 64) / 4 > 8 ? 8 : PIPE_WGS * (BLOCK / 64) / 4;
+    static constexpr int PIPE_MINW = PIPE_WGS * (BLOCK / 64) / 4 > 4 ? 4 : PIPE_WGS * (BLOCK / 64) / 4;
     static_assert(T2 * NY == BLOCK && BLOCK <= 1024, "row and column phases share the workgroup");
     static_assert(RowC::WAVE_SYNC, "a row's lanes must stay inside one wave");
     static constexpr int LDS = NX * RS;
@@ -123,9 +127,9 @@ __global__ __launch_bounds__(F::BLOCK) void k_c2c_2d_lds(ofx_args a) {
     }
 }
 
-// Pipelined form for the sizes whose LDS tile allows at most two workgroups per CU (an image's
-// load, passes and stores would otherwise run one after the other on a CU): a grid of resident
-// workgroups, each taking images b, b + G, ...; the next image's 16-byte loads are issued into
+// Pipelined form (an image's load, passes and stores would otherwise run one after the other in a
+// workgroup, and these sizes hold few workgroups per CU): a grid of resident workgroups, each
+// taking images b, b + G, ...; the next image's 16-byte loads are issued into
 // registers right after the current image is in LDS and stay in flight through its row and column
 // passes (whose barriers order LDS only), and its column stores drain while the next image is
 // written to LDS.  Images must start 16-byte aligned (checked by the launcher).
@@ -139,15 +143,16 @@ __global__ __launch_bounds__(F::BLOCK, F::PIPE_MINW) void k_c2c_2d_lds_pipe(ofx_
     long long img = blockIdx.x;
     const float sg = a.in_sign;
     float4 buf[F::NV];
+    // buffer loads: the image base in a scalar resource, one lane offset for all NV loads and the
+    // load index in the scalar offset, so the prefetch costs its data registers and one address
     auto load = [&](long long m) {
-        const float4* s4 = reinterpret_cast<const float4*>(static_cast<const float2*>(a.in) + m * a.idist);
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<float2*>(static_cast<const float2*>(a.in) + m * a.idist), (short)0, F::NX * F::NY * 8,
+            0x00020000);
+        const int vo = (int)threadIdx.x * 16;
 #pragma unroll
-        for (int v = 0; v < F::NV; ++v) {
-            if constexpr (NTL)
-                buf[v] = ld_nt(s4 + threadIdx.x + v * F::BLOCK);
-            else
-                buf[v] = s4[threadIdx.x + v * F::BLOCK];
-        }
+        for (int v = 0; v < F::NV; ++v)
+            buf[v] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, vo, v * F::BLOCK * 16, NTL ? 2 : 0));
     };
     if (img < a.batch) load(img);
     for (; img < a.batch; img += G) {
@@ -208,15 +213,14 @@ static long long pipe_slots() {
     return slots;
 }
 
-// The pipelined form pays for its prefetch registers (at the occupancy its LDS tile allows it
-// spills 48-64 bytes per lane): with one image per workgroup it is slower (128^2 x 256 images
-// 22.0 vs 15.0 us, x 1024: 67.3 vs 62.3 us), with 16 per workgroup faster (x 4096: 234 vs 260 us;
-// 64x128 / 128x64 x 8192: 215 / 212 vs 224 / 226 us).  It runs when every resident workgroup gets
-// at least 8 images.
+// The pipelined form runs when every resident workgroup gets at least 2 images: 128^2 x 512 images
+// 28.4 vs 32.6 us, x 1024 51.3 vs 62.8 us, x 4096 201-203 vs 259-260 us; 64x64 x 16384 188 vs 201
+// us; 64x128 / 128x64 x 8192 187 / 186 vs 223 / 226 us; one image per workgroup about the same
+// (128^2 x 256: 15.4 vs 15.0 us).
 template <class F, bool NTL, bool NTS>
 static bool launch_2d_pipe(const ofx_args* a, hipStream_t s) {
     const long long slots = pipe_slots<F, NTL, NTS>();
-    if (a->batch < 8 * slots) return false;
+    if (a->batch < 2 * slots) return false;
     hipLaunchKernelGGL((k_c2c_2d_lds_pipe<F, NTL, NTS>), dim3((unsigned)slots), dim3(F::BLOCK), 0, s, *a);
     return true;
 }
@@ -226,8 +230,8 @@ hipError_t launch_2d(const ofx_args* a, hipStream_t s) {
     if (a->batch <= 0) return hipSuccess;
     if (a->batch > 0x7fffffffLL) return hipErrorInvalidConfiguration;
     const bool ntl = (a->flags & OFX_FLAG_NT_LOAD) != 0, nts = (a->flags & OFX_FLAG_NT_STORE) != 0;
-    if constexpr (F::PIPE_OK && F::LDS * sizeof(float2) > 40 * 1024) {
-        // at most two workgroups per CU: the pipelined form, for 16-byte aligned images
+    if constexpr (F::PIPE_OK) {
+        // the pipelined form, for 16-byte aligned images
         const bool aligned = ((reinterpret_cast<uintptr_t>(a->in) | reinterpret_cast<uintptr_t>(a->out)) & 15) == 0 &&
                              ((a->idist | a->odist) & 1) == 0;
         if (aligned && (a->flags & OFX_FLAG_NO_PIPELINE) == 0) {

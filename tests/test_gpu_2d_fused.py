@@ -126,13 +126,14 @@ def test_fused_2d_streaming(cuda, monkeypatch, nx, ny, batch, flags):
         assert oracle.rel_l2(outs[0][sl].cpu().numpy(), ref) <= fp32_tol(nx * ny)
 
 
-@pytest.mark.parametrize("nx,ny,batch", [(128, 128, 2048 + 3), (64, 128, 4096 + 3), (128, 64, 4096 + 3)])
+@pytest.mark.parametrize("nx,ny,batch", [(128, 128, 2048 + 3), (64, 128, 4096 + 3), (128, 64, 4096 + 3),
+                                         (64, 64, 2048 + 3), (32, 128, 2048 + 3), (32, 64, 4096 + 3)])
 def test_fused_2d_pipelined(cuda, monkeypatch, nx, ny, batch):
     """The pipelined one-pass kernel (resident workgroups walking several images each, the next
-    image's loads in flight through the passes; it runs from 8 images per resident workgroup: 256
-    of 128^2, 512 of 64 x 128 on this GPU) against the one-image-per-workgroup form
-    (ACLFFT_2D_PIPE=0): bit-identical, with a ragged last round; a sample of images against the
-    FP64 oracle."""
+    image's loads in flight through the passes; sizes with rows of 64 or more, from 2 images per
+    resident workgroup: 256 of 128^2, 512 of 64 x 128, 1024 of 64^2 on this GPU) against the
+    one-image-per-workgroup form (ACLFFT_2D_PIPE=0): bit-identical, with a ragged last round; a
+    sample of images against the FP64 oracle."""
     import torch
     g = torch.Generator(device=cuda).manual_seed(nx * ny)
     x = torch.randn((batch, nx, ny), dtype=torch.complex64, device=cuda, generator=g)
