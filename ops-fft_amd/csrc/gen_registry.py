@@ -138,6 +138,15 @@ NTVAR = [
 ]
 
 
+# streaming-store twins (mode 2) of the default complex128 kernels (the plan's store rule)
+ZNT = [
+    ("Z2Z", 1024, 128, 2, (8, 8, 16)), ("Z2Z", 2048, 256, 1, (8, 8, 8, 4)), ("Z2Z", 4096, 512, 1, (8, 8, 8, 8)),
+    ("D2Z", 1024, 128, 1, (8, 8, 16)), ("Z2D", 1024, 128, 1, (8, 8, 16)),
+    ("D2Z", 2048, 256, 1, (8, 8, 8, 4)), ("Z2D", 2048, 256, 1, (8, 8, 8, 4)),
+    ("D2Z", 4096, 512, 1, (8, 8, 8, 8)), ("Z2D", 4096, 512, 1, (8, 8, 8, 8)),
+]
+
+
 # LDS pad variants (pad shift other than the default): (kind, N, T, TPB, radices, pad shift)
 PADX = [
     ("R2C", 1500, 256, 1, (15, 10, 10), 30), ("R2C", 1500, 256, 1, (25, 15, 4), 30),
@@ -312,13 +321,16 @@ def main():
             assert t * tpb <= 1024
             f.write("    OFX_EH(%s, %d, %d, %d, %d, %s),\n" % (KINDS[kind], n, t, tpb, psh[0] if psh else padsh(n),
                                                        ", ".join(str(r) for r in rad)))
+        for kind, n, t, tpb, rad in ZNT:
+            assert t * tpb <= 1024
+            f.write("    OFX_EZM(%s, %d, %d, %d, %d, 2, %s),\n" % (KINDS[kind], n, t, tpb, padsh(n), ", ".join(str(r) for r in rad)))
         for kind, n, t, tpb, rad, iov, mode, *psh in NTVAR:
             assert t * tpb <= 1024
             f.write("    OFX_EM(%s, %d, %d, %d, %d, %d, %d, %s),\n" % (KINDS[kind], n, t, tpb, psh[0] if psh else padsh(n),
                                                                 iov, mode, ", ".join(str(r) for r in rad)))
         f.write("};\n")
     print("wrote %d registry entries" % (len(FIRST) + len(rows) + len(TILES) + len(EXTRA) + len(PAIRED) +
-                                         len(z_lines()) + len(STREAM) + len(PADX) + len(SPLIT) + len(SPLIT_FIRST) + len(HPAIR) + len(HPAIR_FIRST) + len(NTVAR)))
+                                         len(z_lines()) + len(STREAM) + len(PADX) + len(SPLIT) + len(SPLIT_FIRST) + len(HPAIR) + len(HPAIR_FIRST) + len(NTVAR) + len(ZNT)))
 
 
 if __name__ == "__main__":

@@ -858,6 +858,9 @@ Entry make_entry() {
 // a configuration with I/O form IOV and streaming mode MODE (StockhamCfg::STREAM)
 #define OFX_EM(KIND, N, T, TPB, PADSH, IOV, MODE, ...) \
     make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, IOV, 0, 1, float2, MODE>, KIND>()
+// the same for complex128
+#define OFX_EZM(KIND, N, T, TPB, PADSH, MODE, ...) \
+    make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, double2, MODE>, KIND>()
 #define OFX_EX(KIND, N, T, TPB, PADSH, ...) \
     make_entry<StockhamCfg<N, T, TPB, PADSH, false, Radices<__VA_ARGS__>, 1, 0, 1, float2, false, true>, KIND>()
 // streaming variants (STREAM_ = true) of a direct, a paired and a complex128 configuration
