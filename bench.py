@@ -293,7 +293,9 @@ def extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y, steps, stream, e32):
 def extra_mfma_lengths(torch, aclfft, oracle, np, x, y, steps, stream):
     """The MFMA split path at the other lengths of its registry (16-, 32- and 64-point dense DFT
     stages), each at 64 Mi points (1 GiB moved) against the FP32 Stockham kernel: time, HBM
-    fraction and the error of 64 transforms against the FP64 oracle, per operand precision."""
+    fraction and the error of 64 transforms against the FP64 oracle, per operand precision; then
+    2D images of 32 and 64 points per axis (the stages as F X F) against the one-pass FP32 2D
+    kernel, the same way."""
     out = {}
     pts = x.numel()
     for n in (256, 512, 2048, 4096):
@@ -315,6 +317,26 @@ def extra_mfma_lengths(torch, aclfft, oracle, np, x, y, steps, stream):
         for name in ("bf16", "fp16x2", "bf16x3"):
             rec[name]["vs_fp32"] = round(rec["fp32"]["ms"] / rec[name]["ms"], 3)
         out["n%d_b%d" % (n, b)] = rec
+    # 2D images (the reference's 2D sizes with 32- and 64-point axes) as F X F on the same stages
+    for nx, ny in ((32, 32), (32, 64), (64, 64)):
+        b = pts // (nx * ny)
+        xv, yv = x.view(-1)[:b * nx * ny].view(b, nx, ny), y.view(-1)[:b * nx * ny].view(b, nx, ny)
+        ref = oracle.c2c_2d(xv[:64].cpu().numpy().astype(np.complex128))
+        rec = {}
+        for mode, name in ((0, "fp32"), (aclfft.PRECISION_BF16, "bf16"), (aclfft.PRECISION_FP16X2, "fp16x2"),
+                           (aclfft.PRECISION_BF16X3, "bf16x3")):
+            p = aclfft.Plan.plan2d(b, nx, ny, aclfft.C2C)
+            if mode:
+                p.set_precision(mode)
+            p.set_stream(stream)
+            ms = time_steps(torch, lambda: p.exec_c2c(xv, yv, aclfft.FORWARD), steps, 3, stream)
+            el2, emax = errs(oracle, np, yv[:64].cpu().numpy(), ref)
+            rec[name] = {"ms": round(ms, 5), "hbm_frac": frac(16.0 * pts, ms), "rel_l2_vs_fp64": el2,
+                         "max_abs_rel_vs_fp64": emax, "plan": p.describe()}
+            p.destroy()
+        for name in ("bf16", "fp16x2", "bf16x3"):
+            rec[name]["vs_fp32"] = round(rec["fp32"]["ms"] / rec[name]["ms"], 3)
+        out["2d_%dx%d_b%d" % (nx, ny, b)] = rec
     return out
 
 
@@ -812,7 +834,8 @@ def profile_reflen(torch, aclfft, reps):
 
 def profile_mfma(torch, aclfft, reps):
     """The MFMA split-precision path in each mode (BF16, FP16 hi/lo, BF16 hi/mid/lo): C2C at
-    n = 256, 512, 1024, 2048, 4096 over 2^26 points, R2C and C2R at n = 8192 batch 1024, and the
+    n = 256, 512, 1024, 2048, 4096 over 2^26 points, 2D 32^2 and 64^2 images over 2^26 points,
+    R2C and C2R at n = 8192 batch 1024, and the
     512-group uneven batch; then the streaming copy of 512 MiB."""
     s = torch.cuda.current_stream()
     modes = ((aclfft.PRECISION_BF16, "bf16"), (aclfft.PRECISION_FP16X2, "fp16x2"),
@@ -827,6 +850,14 @@ def profile_mfma(torch, aclfft, reps):
             p.set_stream(s)
             _profile_rep(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), reps)
             log("c2c n=%d %s: %s" % (n, name, p.describe()))
+            p.destroy()
+    for nx, ny in ((32, 32), (64, 64)):
+        for mode, name in modes:
+            p = aclfft.Plan.plan2d(pts // (nx * ny), nx, ny, aclfft.C2C)
+            p.set_precision(mode)
+            p.set_stream(s)
+            _profile_rep(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), reps)
+            log("2d %dx%d %s: %s" % (nx, ny, name, p.describe()))
             p.destroy()
     del x, y
     n, b = 8192, 1024

@@ -204,7 +204,9 @@ OFX_API int ofx_launch_mfma_c2c(int mode, int n, const ofx_args* args, int block
  * v_mfma_f32_16x16x32 (shape 16).  Same arguments as ofx_launch_mfma_c2c: contiguous transforms,
  * args->tw = the entry's constant table for `mode` (ofx_gen_mfma_kernel_consts, bytes from
  * ofx_mfma_kernel_const_bytes), args->tw_aux = W_n^k for k < n (ofx_gen_roots); blocks = grid
- * (0 = default).  No reference counterpart beyond the matrix-DFT stage shapes of its Cube kernels. */
+ * (0 = default); args->flags OFX_FLAG_MFMA_2D (entries >= 1): 2D images of r2 x r1 (the
+ * reference's 2D matrix form P X Q, src/fft2_d/arch32/dd/fft2_d_dd.cpp:31-66).  No other
+ * reference counterpart beyond the matrix-DFT stage shapes of its Cube kernels. */
 typedef struct ofx_mfma_info {
     int n;
     int r1;      /* stage-2 DFT length (n = n1 + r1 * n2)                                         */
@@ -294,6 +296,11 @@ OFX_API int ofx_batch_blocks_per_cu(int schedule);
 /* One-pass 2D kernels: run the non-pipelined form (one image per workgroup) even where the
  * pipelined one applies */
 #define OFX_FLAG_NO_PIPELINE 16
+/* MFMA DFT kernels (ofx_launch_mfma_kernel, ids >= 1): 2D images instead of 1D transforms.  Each
+ * transform is an image of r2 rows and r1 columns (row-major); with args->tw_aux all ones (W_1^k,
+ * ofx_gen_roots with len 1) stage 1 transforms the columns, stage 2 the rows, and the output is
+ * stored in the image layout (element (k_row, k_col) at r1 * k_row + k_col). */
+#define OFX_FLAG_MFMA_2D 32
 
 /* Device scratch area of the allocation-free execution entry point (aclfftExtExecC2CNoAlloc):
  * OFX_SCRATCH_BYTES on every GPU, a file-scope device array held from the library's load until the

@@ -24,8 +24,11 @@ ACLFFT_API aclfftResult aclfftExtDescribe(aclfftHandle plan, char* buf, size_t l
  * length 256, 512, 1024, 2048 or 4096; 1D R2C / C2R plans of length 512, 1024, 2048, 4096 or 8192
  * (the half-length complex transform on MFMA, the Hermitian unpack / pre-pack in FP32); uneven
  * plans (aclfftExtPlanManyUneven) with at least one such length (its groups run on MFMA, the
- * other lengths stay on the FP32 kernels).  Other plans return ACLFFT_NOT_SUPPORTED and keep their
- * current precision.  Rebuilds the plan's launch list. */
+ * other lengths stay on the FP32 kernels); 2D C2C plans whose axes are 16, 32 or 64 points (each
+ * image as F X F, the column and row DFTs as the two dense stages: the matrix form of the
+ * reference's 2D kernel, src/fft2_d/arch32/dd/fft2_d_dd.cpp:31-66; its 128-point axes have no
+ * MFMA stage).  Other plans return ACLFFT_NOT_SUPPORTED and keep their current precision.
+ * Rebuilds the plan's launch list. */
 #define ACLFFT_EXT_PRECISION_FP32 0
 #define ACLFFT_EXT_PRECISION_BF16 1
 #define ACLFFT_EXT_PRECISION_FP16X2 2

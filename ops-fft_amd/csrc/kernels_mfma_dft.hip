@@ -103,10 +103,46 @@ __device__ __forceinline__ float2 ldx(const float2* p) {
         return *p;
 }
 
-template <int MODE, class SH, bool NT>
+// 2D store (OFX_FLAG_MFMA_2D): the input is an image of R2 rows and R1 columns, stage 1 transformed
+// its columns and stage 2 its rows, so accumulator (b, c, q) is output element (k2, k1) = (16 c + r,
+// 16 b + 4 g + q) of the image, at R1 k2 + k1.  A lane's four values q are then 32 contiguous bytes
+// of one row: with a 16-byte aligned output (v16) they go out as two 16-byte stores, and four lanes
+// cover a 128-byte row segment.
+template <class SH, bool NT>
+__device__ __forceinline__ void dft_store_2d(const floatx4 (&acc2)[SH::S2][SH::S1][2], float2* __restrict__ y,
+                                             int r, int g, float inv, float invi, bool v16) {
+#pragma unroll
+    for (int b = 0; b < SH::S2; ++b)
+#pragma unroll
+        for (int c = 0; c < SH::S1; ++c) {
+            float2* row = y + SH::R1 * (16 * c + r) + 16 * b + 4 * g;
+            if (v16) {
+#pragma unroll
+                for (int q = 0; q < 4; q += 2) {
+                    const float4 val = make_float4(acc2[b][c][0][q] * inv, acc2[b][c][1][q] * invi,
+                                                   acc2[b][c][0][q + 1] * inv, acc2[b][c][1][q + 1] * invi);
+                    if constexpr (NT)
+                        st_nt(reinterpret_cast<float4*>(row + q), val);
+                    else
+                        *reinterpret_cast<float4*>(row + q) = val;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float2 val = make_float2(acc2[b][c][0][q] * inv, acc2[b][c][1][q] * invi);
+                    if constexpr (NT)
+                        st_nt(row + q, val);
+                    else
+                        row[q] = val;
+                }
+            }
+        }
+}
+
+template <int MODE, class SH, bool NT, bool T2D>
 __device__ __forceinline__ void dft_transform(const float2 (&v)[SH::S2][SH::S1][4], float2* __restrict__ y,
                                               const typename SplitTraits<MODE>::vec* cst, const float2* tw, int lane,
-                                              int r, int g, float in_sign, float out_sign, float scale) {
+                                              int r, int g, float in_sign, float out_sign, float scale, bool v16) {
     typedef typename SplitTraits<MODE>::vec vec;
     constexpr int P = SplitTraits<MODE>::P;
     constexpr int S1 = SH::S1, S2 = SH::S2, R2 = SH::R2;
@@ -194,9 +230,13 @@ __device__ __forceinline__ void dft_transform(const float2 (&v)[SH::S2][SH::S1][
         }
     }
 
-    // ---- X[R2 k1 + k2], k1 = 16 b + 4 g + q, k2 = 16 c + r -------------------------------------
+    // ---- X[R2 k1 + k2], k1 = 16 b + 4 g + q, k2 = 16 c + r (2D: dft_store_2d) ----------------
     const float inv = (2.f * R2) / s1 * scale;
     const float invi = inv * out_sign;
+    if constexpr (T2D) {
+        dft_store_2d<SH, NT>(acc2, y, r, g, inv, invi, v16);
+        return;
+    }
 #pragma unroll
     for (int b = 0; b < S2; ++b)
 #pragma unroll
@@ -227,11 +267,11 @@ __device__ __forceinline__ void dft_load_tile(float2 (&v)[SH::S1][4], const floa
         for (int q = 0; q < 4; ++q) v[s][q] = ldx<NT>(x + (16 * mt + r) + SH::R1 * (16 * s + 4 * g + q));
 }
 
-template <int MODE, class SH, bool NT>
+template <int MODE, class SH, bool NT, bool T2D>
 __device__ __forceinline__ void dft_transform_tiled(const float2* __restrict__ x, float2* __restrict__ y,
                                                     const typename SplitTraits<MODE>::vec* cst, const float2* tw,
                                                     int lane, int r, int g, float in_sign, float out_sign,
-                                                    float scale) {
+                                                    float scale, bool v16) {
     typedef typename SplitTraits<MODE>::vec vec;
     constexpr int P = SplitTraits<MODE>::P;
     constexpr int S1 = SH::S1, S2 = SH::S2, R2 = SH::R2;
@@ -343,6 +383,10 @@ __device__ __forceinline__ void dft_transform_tiled(const float2* __restrict__ x
 
     const float inv = (2.f * R2) / srun * scale;
     const float invi = inv * out_sign;
+    if constexpr (T2D) {
+        dft_store_2d<SH, NT>(acc2, y, r, g, inv, invi, v16);
+        return;
+    }
 #pragma unroll
     for (int b = 0; b < S2; ++b)
 #pragma unroll
@@ -369,7 +413,7 @@ __device__ __forceinline__ void dft_load(float2 (&v)[SH::S2][SH::S1][4], const f
 }
 
 // args: tw = constant table (ofx_gen_mfma_dft_consts), tw_aux = W_n^k for k < n (ofx_gen_roots)
-template <int MODE, int R1, int R2, bool NT, bool TILED>
+template <int MODE, int R1, int R2, bool NT, bool TILED, bool T2D>
 __global__ __launch_bounds__(kDftBlock) void k_mfma_dft(ofx_args a) {
     typedef DftShape<R1, R2> SH;
     typedef typename SplitTraits<MODE>::vec vec;
@@ -400,6 +444,9 @@ __global__ __launch_bounds__(kDftBlock) void k_mfma_dft(ofx_args a) {
     // offset `offset`, one wave per transform, so the constant tables are loaded once per
     // workgroup, not once per tile
     const bool tiled_list = a.tiles != nullptr;
+    // T2D (OFX_FLAG_MFMA_2D): a compiled variant of its own, so the 1D kernels keep their registers;
+    // 16-byte stores of the 2D layout need a 16-byte aligned output (transform offsets are even)
+    const bool v16 = T2D && (reinterpret_cast<unsigned long long>(a.out) & 15) == 0 && (a.odist & 1) == 0;
     const long long ntile = tiled_list ? a.batch : 1;
     for (long long ti = tiled_list ? blockIdx.x : 0; ti < ntile; ti += tiled_list ? gridDim.x : 1) {
         const float2* __restrict__ tin = in;
@@ -418,12 +465,13 @@ __global__ __launch_bounds__(kDftBlock) void k_mfma_dft(ofx_args a) {
             // constants are re-read from LDS for every transform rather than held in VGPRs
             __asm__ volatile("" ::: "memory");
             if constexpr (TILED) {
-                dft_transform_tiled<MODE, SH, NT>(tin + t * a.idist, tout + t * a.odist, cst, tw, lane, r, g, a.in_sign,
-                                                  a.out_sign, a.scale);
+                dft_transform_tiled<MODE, SH, NT, T2D>(tin + t * a.idist, tout + t * a.odist, cst, tw, lane, r, g,
+                                                       a.in_sign, a.out_sign, a.scale, v16);
             } else {
                 float2 v[SH::S2][SH::S1][4];
                 dft_load<SH, NT>(v, tin + t * a.idist, r, g);
-                dft_transform<MODE, SH, NT>(v, tout + t * a.odist, cst, tw, lane, r, g, a.in_sign, a.out_sign, a.scale);
+                dft_transform<MODE, SH, NT, T2D>(v, tout + t * a.odist, cst, tw, lane, r, g, a.in_sign, a.out_sign,
+                                                 a.scale, v16);
             }
         }
     }
@@ -491,9 +539,11 @@ int parts_of(int mode) {
 typedef void (*dft_kernel_t)(ofx_args);
 
 template <int MODE>
-dft_kernel_t dft_kernel_for(int id, bool nt) {
-#define OFX_DFT_CASE(I, A, B, T) \
-    case I: return nt ? k_mfma_dft<MODE, A, B, true, T> : k_mfma_dft<MODE, A, B, false, T>
+dft_kernel_t dft_kernel_for(int id, bool nt, bool t2d) {
+#define OFX_DFT_CASE(I, A, B, T)                                                                      \
+    case I:                                                                                           \
+        if (t2d) return nt ? k_mfma_dft<MODE, A, B, true, T, true> : k_mfma_dft<MODE, A, B, false, T, true>; \
+        return nt ? k_mfma_dft<MODE, A, B, true, T, false> : k_mfma_dft<MODE, A, B, false, T, false>
     switch (id) {
         OFX_DFT_CASE(1, 16, 16, false);
         OFX_DFT_CASE(2, 16, 32, false);
@@ -560,7 +610,9 @@ extern "C" int ofx_gen_mfma_kernel_consts(int id, int mode, void* table, hipStre
 
 extern "C" int ofx_launch_mfma_kernel(int id, int mode, const ofx_args* a, int blocks, hipStream_t stream) {
     if (a == nullptr || id < 0 || id >= kDftCount || parts_of(mode) == 0) return (int)hipErrorInvalidValue;
-    if (id == 0) return a->tiles != nullptr ? (int)hipErrorInvalidValue : ofx_launch_mfma_c2c(mode, 1024, a, blocks, stream);
+    if (id == 0)
+        return (a->tiles != nullptr || (a->flags & OFX_FLAG_MFMA_2D) != 0) ? (int)hipErrorInvalidValue
+                                                                           : ofx_launch_mfma_c2c(mode, 1024, a, blocks, stream);
     if (a->batch <= 0) return 0;
     const int n = kDft[id].n;
     const long long need = (a->batch + 3) / 4;
@@ -572,11 +624,13 @@ extern "C" int ofx_launch_mfma_kernel(int id, int mode, const ofx_args* a, int b
     // streaming HBM access when input and output together exceed the 256 MB last-level cache by
     // half again (the threshold of the FP32 kernels' streaming variants; tile lists never stream)
     const bool nt = a->tiles == nullptr && a->batch * (long long)n * 16 >= (384LL << 20);
+    const bool t2d = (a->flags & OFX_FLAG_MFMA_2D) != 0;
+    if (t2d && a->tiles != nullptr) return (int)hipErrorInvalidValue;
     dft_kernel_t k = nullptr;
     switch (mode) {
-        case OFX_MFMA_BF16: k = dft_kernel_for<OFX_MFMA_BF16>(id, nt); break;
-        case OFX_MFMA_FP16X2: k = dft_kernel_for<OFX_MFMA_FP16X2>(id, nt); break;
-        default: k = dft_kernel_for<OFX_MFMA_BF16X3>(id, nt); break;
+        case OFX_MFMA_BF16: k = dft_kernel_for<OFX_MFMA_BF16>(id, nt, t2d); break;
+        case OFX_MFMA_FP16X2: k = dft_kernel_for<OFX_MFMA_FP16X2>(id, nt, t2d); break;
+        default: k = dft_kernel_for<OFX_MFMA_BF16X3>(id, nt, t2d); break;
     }
     if (k == nullptr) return (int)hipErrorInvalidValue;
     hipLaunchKernelGGL(k, dim3(grid), dim3(kDftBlock), 0, stream, *a);

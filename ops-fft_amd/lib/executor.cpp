@@ -750,6 +750,55 @@ struct Builder {
         return ACLFFT_SUCCESS;
     }
 
+    // 2D C2C in an MFMA precision: images of nx rows and ny columns with nx, ny in {16, 32, 64}, each
+    // as the two dense DFT stages of an MFMA kernel with r1 = ny, r2 = nx, no inter-stage twiddle
+    // (a table of ones) and the output stored in the image layout (OFX_FLAG_MFMA_2D): the column
+    // DFTs are F_nx X, the row DFTs (F_nx X) F_ny, the reference's P X Q
+    // (src/fft2_d/arch32/dd/fft2_d_dd.cpp:31-66) on the matrix cores.
+    static std::vector<int> mfma_2d_kernels_for(long long nx, long long ny) {
+        std::vector<int> ids, tiled;
+        for (int i = 1; i < ofx_mfma_kernel_count(); ++i) {
+            ofx_mfma_info mi;
+            if (ofx_mfma_kernel_info_get(i, &mi) == 0 && mi.r1 == ny && mi.r2 == nx) {
+                ids.push_back(i);
+                tiled.push_back(mi.tiled);
+            }
+        }
+        if (nx * ny > 2048) {  // 64 x 64: the tiled form first, as for the 1D 4096-point transform
+            std::vector<int> sorted;
+            for (int want = 1; want >= 0; --want)
+                for (size_t k = 0; k < ids.size(); ++k)
+                    if (tiled[k] == want) sorted.push_back(ids[k]);
+            ids = sorted;
+        }
+        return ids;
+    }
+    aclfftResult c2c_2d_mfma(long long nx, long long ny, long long images, int mode) {
+        const std::vector<int> ids = mfma_2d_kernels_for(nx, ny);
+        if (ids.empty() || ofx_mfma_kernel_const_bytes(ids[0], mode) <= 0) return ACLFFT_NOT_SUPPORTED;
+        const long long n = nx * ny;
+        Step s;
+        s.type = Step::MFMA;
+        s.n = (int)n;
+        s.mfma_mode = mode;
+        s.candidates = ids;
+        for (int id : ids) s.candidate_tables.push_back(mfma_table(id, mode));
+        const int pick = std::min<int>(std::max(0, env_int("ACLFFT_MFMA_VARIANT", 0)), (int)ids.size() - 1);
+        s.kernel = ids[pick];
+        s.tw_table = s.candidate_tables[pick];
+        s.aux_table = roots_table(1, n);  // W_1^k = 1: no inter-stage twiddle
+        s.src = BUF_IN;
+        s.dst = BUF_OUT;
+        s.args.batch = images;
+        s.args.idist = n;
+        s.args.odist = n;
+        s.args.flags = OFX_FLAG_MFMA_2D;
+        s.args.in_sign = s.args.out_sign = s.args.scale = 1.f;
+        s.conj_in = s.conj_out = true;
+        e->steps_.push_back(s);
+        return ACLFFT_SUCCESS;
+    }
+
     // Real transforms in an MFMA precision (even n, n/2 with an MFMA kernel): R2C as the n/2-point
     // MFMA transform of the real samples viewed as complex, into the workspace, then the Hermitian
     // unpack; C2R as the pre-pack into the workspace (conjugated), then the forward MFMA transform
@@ -1327,7 +1376,8 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
     Builder b{e};
     aclfftResult r = ACLFFT_NOT_IMPLEMENTED;
     if (d.precision != 0 && !(d.rank == 1 && !d.vertical &&
-                              (d.type == ACLFFT_C2C || d.type == ACLFFT_R2C || d.type == ACLFFT_C2R))) {
+                              (d.type == ACLFFT_C2C || d.type == ACLFFT_R2C || d.type == ACLFFT_C2R)) &&
+        !(d.rank == 2 && d.type == ACLFFT_C2C)) {
         delete e;
         return ACLFFT_NOT_SUPPORTED;
     }
@@ -1404,7 +1454,11 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
         // outer = batch * product of the slower axes images
         const int k = (int)dims.size();
         long long inner = dims[k - 1];
-        if (k == 2 && !z && b.fused_2d(dims[0], dims[1], total / (dims[0] * dims[1]) * d.batch)) {
+        if (d.precision != 0) {
+            // MFMA precision: 2D C2C images of 16, 32 or 64 points per axis
+            r = (k == 2 && !z) ? b.c2c_2d_mfma(dims[0], dims[1], total / (dims[0] * dims[1]) * d.batch, d.precision)
+                               : ACLFFT_NOT_SUPPORTED;
+        } else if (k == 2 && !z && b.fused_2d(dims[0], dims[1], total / (dims[0] * dims[1]) * d.batch)) {
             // images that fit in LDS: rows and columns in one pass over HBM
             r = ACLFFT_SUCCESS;
         } else {
@@ -1591,6 +1645,7 @@ std::string Executor::describe() const {
             os << "mfma n=" << s.n << " " << mi.r1 << "x" << mi.r2 << " shape=" << mi.shape << (mi.tiled ? " tiled" : "")
                << " mode=" << (s.mfma_mode == OFX_MFMA_BF16 ? "bf16" : s.mfma_mode == OFX_MFMA_FP16X2 ? "fp16x2" : "bf16x3");
             if (s.tiles_table >= 0) os << " tiles=" << s.args.batch;
+            if (s.args.flags & OFX_FLAG_MFMA_2D) os << " 2d=" << mi.r2 << "x" << mi.r1;
             continue;
         }
         if (s.type == Step::GENERIC) {

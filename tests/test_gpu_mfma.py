@@ -173,6 +173,105 @@ def test_mfma_real_transforms(cuda, n):
         pc.destroy()
 
 
+def mfma_2d_count(nx, ny):
+    """Number of registry kernels (entries >= 1) that run nx x ny images: r2 = nx, r1 = ny."""
+    lib = aclfft.lib
+    c = 0
+    for i in range(1, lib.ofx_mfma_kernel_count()):
+        mi = MfmaInfo()
+        assert lib.ofx_mfma_kernel_info_get(i, ctypes.byref(mi)) == 0
+        c += mi.r1 == ny and mi.r2 == nx
+    return c
+
+
+def run_2d(torch, x, mode, direction=aclfft.FORWARD, in_place=False):
+    xt = torch.from_numpy(x.astype(np.complex64)).cuda()
+    yt = xt if in_place else torch.empty_like(xt)
+    p = aclfft.Plan.plan2d(x.shape[0], x.shape[1], x.shape[2], aclfft.C2C)
+    if mode:
+        p.set_precision(mode)
+    p.exec_c2c(xt, yt, direction)
+    torch.cuda.synchronize()
+    d = p.describe()
+    p.destroy()
+    return yt.cpu().numpy(), d
+
+
+@pytest.mark.parametrize("nx", [16, 32, 64])
+@pytest.mark.parametrize("ny", [16, 32, 64])
+@pytest.mark.parametrize("direction", [aclfft.FORWARD, aclfft.BACKWARD])
+def test_mfma_2d_vs_fp64(cuda, monkeypatch, nx, ny, direction):
+    """2D C2C in an MFMA precision: each nx x ny image as F_nx X F_ny, the column and row DFTs as
+    the two dense stages of an MFMA kernel (the reference's P X Q form of its 2D kernel,
+    src/fft2_d/arch32/dd/fft2_d_dd.cpp:31-66), every kernel that runs the shape, every operand
+    precision, against the FP64 oracle with the 1D bars at n = nx * ny."""
+    import torch
+    x = crandn((7, nx, ny), 5 * nx + ny + direction).astype(np.complex64)
+    ref = oracle.c2c_2d(x.astype(np.complex128), direction)
+    e32 = oracle.rel_l2(run_2d(torch, x, 0, direction)[0], ref)
+    count = mfma_2d_count(nx, ny)
+    assert count >= 1
+    for variant in range(count):
+        monkeypatch.setenv("ACLFFT_MFMA_VARIANT", str(variant))
+        errs = {}
+        for mode in MODES:
+            y, d = run_2d(torch, x, mode, direction)
+            assert d.startswith("mfma n=%d %dx%d" % (nx * ny, ny, nx)) and "2d=%dx%d" % (nx, ny) in d, d
+            errs[mode] = oracle.rel_l2(y, ref)
+        print("2D %dx%d variant %d rel_l2 vs FP64: fp32 %.3e  bf16 %.3e  fp16x2 %.3e  bf16x3 %.3e" % (
+            nx, ny, variant, e32, errs[1], errs[2], errs[3]))
+        assert errs[aclfft.PRECISION_BF16] < 2e-2
+        for mode in (aclfft.PRECISION_FP16X2, aclfft.PRECISION_BF16X3):
+            assert errs[mode] <= max(2 * e32, 1e-7), (mode, errs[mode], e32)
+            assert errs[mode] <= fp32_tol(nx * ny)
+
+
+def test_mfma_2d_batches_in_place_and_sixty_db(cuda):
+    """Ragged image counts, in-place execution, an output that is not 16-byte aligned, and the
+    60 dB tone case laid out as 32 x 32 images (a 1024-point tone signal per image), FP16 hi/lo
+    and BF16 hi/mid/lo."""
+    import torch
+    for batch in (1, 3, 4099):
+        x = crandn((batch, 32, 64), batch).astype(np.complex64)
+        y, _ = run_2d(torch, x, aclfft.PRECISION_FP16X2, in_place=True)
+        assert oracle.rel_l2(y, oracle.c2c_2d(x.astype(np.complex128))) <= fp32_tol(2048)
+    # an output 8 bytes past a 16-byte boundary (the 2D stores fall back to 8-byte accesses)
+    x = crandn((5, 64, 32), 9).astype(np.complex64)
+    xt = torch.from_numpy(x).cuda()
+    buf = torch.zeros(x.size + 2, dtype=torch.complex64, device="cuda")
+    yt = buf[1:1 + x.size].view(5, 64, 32)
+    p = aclfft.Plan.plan2d(5, 64, 32, aclfft.C2C)
+    p.set_precision(aclfft.PRECISION_BF16X3)
+    p.exec_c2c(xt, yt, aclfft.FORWARD)
+    torch.cuda.synchronize()
+    p.destroy()
+    assert oracle.rel_l2(yt.cpu().numpy(), oracle.c2c_2d(x.astype(np.complex128))) <= fp32_tol(2048)
+    assert buf[0].item() == 0 and buf[-1].item() == 0
+    x = sixty_db(16, 1024, 7).reshape(16, 32, 32).astype(np.complex64)
+    ref = oracle.c2c_2d(x.astype(np.complex128))
+    e32 = oracle.rel_l2(run_2d(torch, x, 0)[0], ref)
+    for mode in (aclfft.PRECISION_FP16X2, aclfft.PRECISION_BF16X3):
+        e = oracle.rel_l2(run_2d(torch, x, mode)[0], ref)
+        print("2D 60 dB 32x32: mode %d rel_l2 %.3e (fp32 %.3e)" % (mode, e, e32))
+        assert e <= max(2 * e32, 1e-7)
+
+
+@pytest.mark.parametrize("rank,shape", [(2, (2, 128, 128)), (2, (2, 128, 32)), (2, (2, 48, 32)), (3, (32, 32, 32))])
+def test_mfma_2d_gate(cuda, rank, shape):
+    """Shapes without an MFMA form (an axis of 128 or of a length other than 16, 32, 64; 3D)
+    refuse an MFMA precision and keep their FP32 launch list."""
+    if rank == 2:
+        p = aclfft.Plan.plan2d(*shape, aclfft.C2C)
+    else:
+        p = aclfft.Plan.plan3d(*shape, aclfft.C2C)
+    before = p.describe()
+    with pytest.raises(aclfft.AclfftError) as e:
+        p.set_precision(aclfft.PRECISION_FP16X2)
+    assert e.value.code == aclfft.NOT_SUPPORTED
+    assert p.describe() == before
+    p.destroy()
+
+
 def test_mfma_precision_gate(cuda):
     p = aclfft.Plan.plan1d(1000, aclfft.C2C, 4)
     with pytest.raises(aclfft.AclfftError) as e:
