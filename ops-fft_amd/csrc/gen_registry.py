@@ -133,8 +133,8 @@ NTVAR = [
     ("R2C", 1024, 64, 4, (16, 16, 4), 1, 2), ("C2R", 1024, 64, 4, (16, 16, 4), 1, 2),
     ("C2C", 2048, 128, 2, (16, 16, 8), 1, 2), ("C2C", 4096, 256, 1, (16, 16, 16), 1, 2),
     ("C2C", 1024, 64, 2, (8, 16, 8), 2, 2), ("C2C", 1024, 64, 2, (8, 8, 16), 2, 2),
-    ("C2C", 256, 16, 16, (16, 16), 1, 2), ("C2C", 128, 8, 32, (8, 16), 1, 2), ("C2C", 512, 32, 8, (8, 8, 8), 1, 2),
-    ("C2C", 64, 4, 64, (8, 8), 1, 2), ("C2C", 8192, 512, 1, (16, 16, 16, 2), 1, 2),
+    ("C2C", 256, 16, 16, (16, 16), 1, 2), ("C2C", 512, 32, 8, (8, 8, 8), 1, 2),
+    ("C2C", 8192, 512, 1, (16, 16, 16, 2), 1, 2),
 ]
 
 
@@ -206,8 +206,13 @@ def z_lines():
 
 # streaming variants (non-temporal HBM input/output) of the default C2C / Z2Z kernels, chosen by
 # the plan for working sets well beyond the last-level cache: (kind, N, T, TPB, radices, iov)
+# A streaming store pays only where each store instruction writes whole 128-byte lines: the
+# transform's lanes must span at least 128 bytes (T >= 16 for complex64).  At 128 points (T = 8,
+# 64-byte runs) the streaming kernels measured slower than the cached one over 2^26 points (loads
+# and stores 208 us, stores 197 us, cached 185 us), at 64 points (T = 4) streaming stores took 4x
+# as long (779 vs 216 us), so those lengths have none.
 STREAM = [
-    ("C2C", 128, 8, 32, (8, 16), 1), ("C2C", 256, 16, 16, (16, 16), 1), ("C2C", 512, 32, 8, (8, 8, 8), 1),
+    ("C2C", 256, 16, 16, (16, 16), 1), ("C2C", 512, 32, 8, (8, 8, 8), 1),
     # 1024: radix 8 x 8 x 16 first (measured 171.8 vs 173.0 us for 8 x 16 x 8 at batch 65536)
     ("C2C", 1024, 64, 2, (8, 8, 16), 2), ("C2C", 1024, 64, 2, (8, 16, 8), 2),
     ("C2C", 2048, 128, 2, (16, 16, 8), 1),
@@ -248,7 +253,12 @@ def lines():
             if n == 1024 and rad == (32, 32):
                 t = 32
             tpb = tpb_for(n, t)
-            for staged in ((False, True) if pi == 0 else (False,)):
+            # up to 64 points (at most 4 lanes per transform) the staged form leads: its loads and
+            # stores go through LDS as whole rows of the batch, where the direct form's lanes touch
+            # 8-32-byte pieces of many lines (2^26 points: 16 points 191 vs 288 us, 32: 199 vs 266,
+            # 64: 199 vs 216 us)
+            order = (True, False) if n <= 64 else (False, True)
+            for staged in (order if pi == 0 else (False,)):
                 out.append(("C2C", n, t, tpb, padsh(n), staged, rad))
     # four-step row step (rows in, transposed out): as many rows per workgroup as LDS allows,
     # so that the transposed stores run over >= 32-byte segments
