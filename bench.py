@@ -292,13 +292,13 @@ def extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y, steps, stream, e32):
 
 def extra_mfma_lengths(torch, aclfft, oracle, np, x, y, steps, stream):
     """The MFMA split path at the other lengths of its registry (16-, 32- and 64-point dense DFT
-    stages), each at 64 Mi points (1 GiB moved) against the FP32 Stockham kernel: time, HBM
+    stages; n = 16, 32, 64 as one dense stage), each at 64 Mi points (1 GiB moved) against the FP32 Stockham kernel: time, HBM
     fraction and the error of 64 transforms against the FP64 oracle, per operand precision; then
     2D images of 32 and 64 points per axis (the stages as F X F) against the one-pass FP32 2D
     kernel, the same way."""
     out = {}
     pts = x.numel()
-    for n in (256, 512, 2048, 4096):
+    for n in (16, 32, 64, 256, 512, 2048, 4096):
         b = pts // n
         xv, yv = x.view(-1)[:b * n].view(b, n), y.view(-1)[:b * n].view(b, n)
         ref = oracle.c2c(xv[:64].cpu().numpy().astype(np.complex128))

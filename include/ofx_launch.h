@@ -199,7 +199,9 @@ OFX_API int ofx_gen_mfma_consts(int mode, void* table, hipStream_t stream);
  * of the current one). */
 OFX_API int ofx_launch_mfma_c2c(int mode, int n, const ofx_args* args, int blocks, hipStream_t stream);
 
-/* Registry of the MFMA C2C kernels: n = r1 * r2 as two dense DFT stages (r1, r2 in {16, 32, 64}).
+/* Registry of the MFMA C2C kernels: n = r1 * r2 as two dense DFT stages (r1, r2 in {16, 32, 64}),
+ * or, with r1 = 1, n = r2 in {16, 32, 64} as one dense stage for 16 transforms per wave (no tile
+ * lists, no 2D form).
  * Entry 0 is the n = 1024 kernel above (shape 32: v_mfma_f32_32x32x16); the others run on
  * v_mfma_f32_16x16x32 (shape 16).  Same arguments as ofx_launch_mfma_c2c: contiguous transforms,
  * args->tw = the entry's constant table for `mode` (ofx_gen_mfma_kernel_consts, bytes from
@@ -209,7 +211,7 @@ OFX_API int ofx_launch_mfma_c2c(int mode, int n, const ofx_args* args, int block
  * reference counterpart beyond the matrix-DFT stage shapes of its Cube kernels. */
 typedef struct ofx_mfma_info {
     int n;
-    int r1;      /* stage-2 DFT length (n = n1 + r1 * n2)                                         */
+    int r1;      /* stage-2 DFT length (n = n1 + r1 * n2); 1: the single-stage dense kernels      */
     int r2;      /* stage-1 DFT length                                                            */
     int shape;   /* 32 or 16: the MFMA tile                                                       */
     int tiled;   /* 1: the input is read one stage-1 row tile at a time (fewer registers)          */

@@ -20,8 +20,10 @@ ACLFFT_API aclfftResult aclfftExtDescribe(aclfftHandle plan, char* buf, size_t l
  *   ACLFFT_EXT_PRECISION_BF16     (1) MFMA dense-DFT stages, one BF16 part per operand
  *   ACLFFT_EXT_PRECISION_FP16X2   (2) MFMA, FP16 hi + lo split operands (about FP32 accuracy)
  *   ACLFFT_EXT_PRECISION_BF16X3   (3) MFMA, BF16 hi + mid + lo split operands
- * The MFMA modes run n = r1 * r2 as two dense DFT stages of 16, 32 or 64 points: 1D C2C plans of
- * length 256, 512, 1024, 2048 or 4096; 1D R2C / C2R plans of length 512, 1024, 2048, 4096 or 8192
+ * The MFMA modes run n = r1 * r2 as two dense DFT stages of 16, 32 or 64 points, and n = 16, 32
+ * or 64 as one dense stage (the reference's dense small-n DFT, src/fft1_d/arch32/dft/
+ * fft1_d_dft.cpp:23-77): 1D C2C plans of length 16, 32, 64, 256, 512, 1024, 2048 or 4096; 1D R2C /
+ * C2R plans of length 32, 64, 128, 512, 1024, 2048, 4096 or 8192
  * (the half-length complex transform on MFMA, the Hermitian unpack / pre-pack in FP32); uneven
  * plans (aclfftExtPlanManyUneven) with at least one such length (its groups run on MFMA, the
  * other lengths stay on the FP32 kernels); 2D C2C plans whose axes are 16, 32 or 64 points (each

@@ -478,6 +478,126 @@ __global__ __launch_bounds__(kDftBlock) void k_mfma_dft(ofx_args a) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Dense single-stage DFT for n = 16, 32, 64 (the reference's dense small-n matrix DFT,
+// src/fft1_d/arch32/dft/fft1_d_dft.cpp:23-77): Y = X F_n for 16 transforms at a time per wave, the
+// transforms as the MFMA's M dimension.  Lane (r, g) holds the A-operand row of transform r
+// (inputs 16 s + 4 g + q, four contiguous complex values per k-step), the B operands are the
+// stage-1 constant tiles of F_n (the same table as a two-stage kernel's stage 1 with R2 = n), and
+// accumulator v of column block c is output 16 c + r of transform 4 g + v (16 lanes store 128
+// contiguous bytes).  FP16 operands take a power-of-two scale per transform (the four lanes of a
+// row agree on it through two lane exchanges), undone exactly on the output.
+// ---------------------------------------------------------------------------------------------
+template <int MODE, int N, bool NT>
+__global__ __launch_bounds__(kDftBlock) void k_mfma_dense(ofx_args a) {
+    typedef typename SplitTraits<MODE>::vec vec;
+    constexpr int P = SplitTraits<MODE>::P;
+    constexpr int S = N / 16;
+    constexpr int NC = S * S * P * 64;
+    static_assert(N % 16 == 0 && N <= 64, "dense stages of 16, 32 or 64 points");
+    __shared__ vec cst[NC];
+    {
+        const vec* gc = static_cast<const vec*>(a.tw);
+        for (int i = threadIdx.x; i < NC; i += kDftBlock) cst[i] = gc[i];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int r = lane & 15;
+    const int g = lane >> 4;
+    const float2* __restrict__ in = static_cast<const float2*>(a.in);
+    float2* __restrict__ out = static_cast<float2*>(a.out);
+    // 16-byte loads of input pairs when every transform starts 16-byte aligned
+    const bool v16 = (reinterpret_cast<unsigned long long>(a.in) & 15) == 0 && (a.idist & 1) == 0;
+    const long long groups = (a.batch + 15) / 16;
+    for (long long grp = (long long)blockIdx.x * (kDftBlock / 64) + wave; grp < groups;
+         grp += (long long)gridDim.x * (kDftBlock / 64)) {
+        __asm__ volatile("" ::: "memory");  // constants re-read from LDS for every group
+        const long long t0 = grp * 16;
+        const bool okr = t0 + r < a.batch;
+        float2 v[S][4];
+        const float2* src = in + (okr ? (t0 + r) : 0) * a.idist + 4 * g;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (!okr) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[s][q] = make_float2(0.f, 0.f);
+            } else if (v16) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const float4* p4 = reinterpret_cast<const float4*>(src + 16 * s + 2 * h);
+                    float4 w;
+                    if constexpr (NT)
+                        w = ld_nt(p4);
+                    else
+                        w = *p4;
+                    v[s][2 * h] = make_float2(w.x, w.y);
+                    v[s][2 * h + 1] = make_float2(w.z, w.w);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[s][q] = ldx<NT>(src + 16 * s + q);
+            }
+        }
+        float sc = 1.f;
+        if constexpr (MODE == OFX_MFMA_FP16X2) {
+            float m = 0.f;
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) m = fmaxf(m, fmaxf(fabsf(v[s][q].x), fabsf(v[s][q].y)));
+            m = fmaxf(m, __shfl_xor(m, 16, 64));
+            m = fmaxf(m, __shfl_xor(m, 32, 64));
+            sc = fp16_scale<MODE>(m);
+        }
+        const float si = sc * a.in_sign;
+        floatx4 acc[S][2];
+#pragma unroll
+        for (int c = 0; c < S; ++c) acc[c][0] = acc[c][1] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            float av[8];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                av[q] = v[s][q].x * sc;
+                av[4 + q] = v[s][q].y * si;
+            }
+            vec ap[P];
+            split8<MODE>(av, ap);
+#pragma unroll
+            for (int c = 0; c < S; ++c) {
+                vec bre[P], bim[P];
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    bre[p] = cst[((c * S + s) * P + p) * 64 + lane];
+                    bim[p] = im_tile(bre[p]);
+                }
+                acc[c][0] = mfma16_split<MODE>(ap, bre, acc[c][0]);
+                acc[c][1] = mfma16_split<MODE>(ap, bim, acc[c][1]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long long t = t0 + 4 * g + q;
+            // the scale of transform 4 g + q is held by lane 4 g + q (row r = 4 g + q)
+            const float st = __shfl(sc, 4 * g + q, 64);
+            const float inv = a.scale / st;
+            const float invi = inv * a.out_sign;
+            if (t < a.batch) {
+                float2* o = out + t * a.odist + r;
+#pragma unroll
+                for (int c = 0; c < S; ++c) {
+                    const float2 val = make_float2(acc[c][0][q] * inv, acc[c][1][q] * invi);
+                    if constexpr (NT)
+                        st_nt(o + 16 * c, val);
+                    else
+                        o[16 * c] = val;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // constant table: FP64 sincospi on the GPU, split into narrow parts
 //   stage 1 (R = R2): entry ((c * S1 + s) * P + p) * 64 + lane, column k2 = 16c + r, slot j of
 //     input n2 = 16s + 4g + (j & 3): j < 4 -> cos(2 pi n2 k2 / R2), j >= 4 -> sin(...)
@@ -529,6 +649,8 @@ constexpr DftEntry kDft[] = {
     {256, 16, 16, 0},   {512, 16, 32, 0},   {512, 32, 16, 0},   {1024, 32, 32, 0},  {1024, 16, 64, 0},
     {1024, 64, 16, 0},  {2048, 32, 64, 0},  {2048, 64, 32, 0},  {4096, 64, 64, 0},  {1024, 32, 32, 1},
     {1024, 64, 16, 1},  {2048, 32, 64, 1},  {2048, 64, 32, 1},  {4096, 64, 64, 1},
+    // dense single stage (r1 = 1): k_mfma_dense
+    {16, 1, 16, 0},     {32, 1, 32, 0},     {64, 1, 64, 0},
 };
 constexpr int kDftCount = (int)(sizeof(kDft) / sizeof(kDft[0]));
 
@@ -559,6 +681,9 @@ dft_kernel_t dft_kernel_for(int id, bool nt, bool t2d) {
         OFX_DFT_CASE(12, 32, 64, true);
         OFX_DFT_CASE(13, 64, 32, true);
         OFX_DFT_CASE(14, 64, 64, true);
+        case 15: return nt ? k_mfma_dense<MODE, 16, true> : k_mfma_dense<MODE, 16, false>;
+        case 16: return nt ? k_mfma_dense<MODE, 32, true> : k_mfma_dense<MODE, 32, false>;
+        case 17: return nt ? k_mfma_dense<MODE, 64, true> : k_mfma_dense<MODE, 64, false>;
         default: return nullptr;
     }
 #undef OFX_DFT_CASE
@@ -615,7 +740,10 @@ extern "C" int ofx_launch_mfma_kernel(int id, int mode, const ofx_args* a, int b
                                                                            : ofx_launch_mfma_c2c(mode, 1024, a, blocks, stream);
     if (a->batch <= 0) return 0;
     const int n = kDft[id].n;
-    const long long need = (a->batch + 3) / 4;
+    const bool dense = kDft[id].r1 == 1;
+    // dense kernels: 16 transforms per wave, no tile lists, no 2D form
+    if (dense && (a->tiles != nullptr || (a->flags & OFX_FLAG_MFMA_2D) != 0)) return (int)hipErrorInvalidValue;
+    const long long need = dense ? (a->batch + 63) / 64 : (a->batch + 3) / 4;
     // one wave per transform; by default a grid of 2048 workgroups strides over the batch (with a
     // tile list: one workgroup per tile, `batch` tiles)
     if (a->tiles != nullptr && a->batch > 0x7fffffffLL) return (int)hipErrorInvalidConfiguration;

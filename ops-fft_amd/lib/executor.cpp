@@ -693,7 +693,7 @@ struct Builder {
     }
 
     // MFMA split-precision C2C: n = r1 * r2 as two dense DFT stages (r1, r2 in {16, 32, 64}; the
-    // registry's lengths 256 ... 4096).  Every registry entry of length n is a candidate (the
+    // registry's lengths 256 ... 4096), or n = 16, 32, 64 as one dense stage (r1 = 1).  Every registry entry of length n is a candidate (the
     // autotuner times them); the first one is the default (for 1024 the 32 x 32 kernel on
     // v_mfma_f32_32x32x16).  ACLFFT_MFMA_VARIANT picks another candidate.
     static std::vector<int> mfma_kernels_for(long long n) {
@@ -972,8 +972,11 @@ struct Builder {
             off += (long long)len[g] * bat[g];
             if (ids_of.count(len[g]) == 0) {
                 std::vector<int> ids;
-                for (int id : mfma_kernels_for(len[g]))
-                    if (id != 0) ids.push_back(id);
+                for (int id : mfma_kernels_for(len[g])) {
+                    // tile lists: the two-stage kernels (not entry 0, not the dense small-n ones)
+                    ofx_mfma_info mi;
+                    if (id != 0 && ofx_mfma_kernel_info_get(id, &mi) == 0 && mi.r1 >= 16) ids.push_back(id);
+                }
                 ids_of[len[g]] = ids;
             }
             on_mfma[g] = ids_of[len[g]].empty() ? 0 : 1;

@@ -2,7 +2,8 @@
 
 Every MFMA kernel of the registry (n = r1 * r2 with 16-, 32- and 64-point dense DFT stages:
 256 = 16x16, 512 = 16x32 / 32x16, 1024 = 32x32 on both MFMA tiles / 16x64 / 64x16, 2048 = 32x64 /
-64x32, 4096 = 64x64), forced one by one with ACLFFT_MFMA_VARIANT, in every operand precision.
+64x32, 4096 = 64x64; and n = 16, 32, 64 as one dense stage), forced one by one with
+ACLFFT_MFMA_VARIANT, in every operand precision; 2D images as F X F.
 
 Bars (the precision workload): the FP16 hi/lo and BF16 hi/mid/lo paths must be no worse than twice
 the FP32 Stockham path's error against FP64 and within rel-L2 1e-6 * log2(n); single-BF16 is
@@ -112,7 +113,7 @@ def test_mfma_sixty_db(cuda, n):
         assert e <= max(2 * e32, 1e-7)
 
 
-@pytest.mark.parametrize("n", [256, 1024, 4096])
+@pytest.mark.parametrize("n", [32, 256, 1024, 4096])
 def test_mfma_ragged_batch_and_in_place(cuda, n):
     import torch
     for batch in (1, 3, 5, 4099):
@@ -126,11 +127,12 @@ def test_mfma_ragged_batch_and_in_place(cuda, n):
         p.destroy()
 
 
-def test_mfma_extreme_magnitudes(cuda):
+@pytest.mark.parametrize("n", [2048, 64])
+def test_mfma_extreme_magnitudes(cuda, n):
     """FP16 operands are formed from data scaled per transform by a power of two: transforms of
-    magnitude 1e-30 and 1e30 next to ordinary ones keep their accuracy; zero transforms stay zero."""
+    magnitude 1e-30 and 1e30 next to ordinary ones keep their accuracy; zero transforms stay zero
+    (n = 64: the dense kernel, whose 16 transforms per wave each take their own scale)."""
     import torch
-    n = 2048
     x = crandn((6, n), 41)
     x[1] *= 1e-30
     x[2] *= 1e30
@@ -145,7 +147,7 @@ def test_mfma_extreme_magnitudes(cuda):
             assert oracle.rel_l2(y[b], ref[b]) <= fp32_tol(n), (mode, b)
 
 
-@pytest.mark.parametrize("n", [512, 2048, 4096, 8192])
+@pytest.mark.parametrize("n", [32, 128, 512, 2048, 4096, 8192])
 def test_mfma_real_transforms(cuda, n):
     """R2C / C2R in an MFMA precision: the n/2-point MFMA transform with the Hermitian unpack /
     pre-pack kernels."""
@@ -269,6 +271,28 @@ def test_mfma_2d_gate(cuda, rank, shape):
         p.set_precision(aclfft.PRECISION_FP16X2)
     assert e.value.code == aclfft.NOT_SUPPORTED
     assert p.describe() == before
+    p.destroy()
+
+
+def test_mfma_dense_lengths_in_uneven_plan(cuda):
+    """Uneven plans in an MFMA precision keep the dense small-n lengths on the FP32 kernels (the
+    dense kernels take no tile lists) and run the rest on MFMA; every group against the oracle."""
+    import torch
+    ln, bt = [16, 1024, 64, 32, 1024, 16], [5, 3, 40, 17, 1, 2]
+    x = crandn((sum(a * b for a, b in zip(ln, bt)),), 77).astype(np.complex64)
+    xt = torch.from_numpy(x).cuda()
+    yt = torch.empty_like(xt)
+    p = aclfft.Plan.plan_many_uneven(ln, bt)
+    p.set_precision(aclfft.PRECISION_FP16X2)
+    assert "mfma n=1024" in p.describe()
+    p.exec_c2c(xt, yt, aclfft.FORWARD)
+    torch.cuda.synchronize()
+    y = yt.cpu().numpy()
+    off = 0
+    for a, b in zip(ln, bt):
+        seg = x[off:off + a * b].reshape(b, a).astype(np.complex128)
+        assert oracle.rel_l2(y[off:off + a * b].reshape(b, a), oracle.c2c(seg)) <= fp32_tol(a), (a, b)
+        off += a * b
     p.destroy()
 
 
