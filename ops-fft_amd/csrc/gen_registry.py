@@ -31,7 +31,7 @@ PLANS = {
     3000: [(8, 15, 25)],
     # the reference's own case lengths (its mixed-radix and real-transform tests) and the complex
     # halves of its real lengths 1536, 2100, 5040 and 25200
-    6: [(6,)], 105: [(15, 7)], 210: [(14, 15)], 420: [(20, 21)], 768: [(16, 16, 3)], 1050: [(10, 15, 7), (6, 25, 7)],
+    6: [(6,)], 105: [(15, 7)], 210: [(14, 15)], 420: [(20, 21)], 768: [(16, 16, 3)], 1050: [(10, 15, 7)],
     1536: [(16, 16, 6)], 2100: [(12, 25, 7)], 2310: [(10, 21, 11)], 2520: [(12, 14, 15)], 5040: [(12, 20, 21)],
     12600: [(8, 9, 25, 7)],
 }
@@ -240,6 +240,8 @@ FIRST = [
     ("R2C", 4096, 512, 1, (8, 8, 8, 8), 1), ("C2R", 4096, 512, 1, (8, 8, 8, 8), 1),
     # four-step row step of 1024 points: two radix-32 passes (2^20 x 64: 472 vs 482 us); pad one
     # element per 32, so the first pass's stride-32 writes spread over all banks (463.5 vs 466.6 us)
+    # 1050 points: radix 6 x 25 x 7 at 64 lanes (2^26 points 208 vs 219 us for 10 x 15 x 7 at 128)
+    ("C2C", 1050, 64, 4, (6, 25, 7), 1),
     ("ROWT", 1024, 32, 16, (32, 32), 1, 5),
     ("ROWT", 1024, 32, 16, (32, 32), 1),
 ]
@@ -257,7 +259,10 @@ def lines():
             # stores go through LDS as whole rows of the batch, where the direct form's lanes touch
             # 8-32-byte pieces of many lines (2^26 points: 16 points 191 vs 288 us, 32: 199 vs 266,
             # 64: 199 vs 216 us)
-            order = (True, False) if n <= 64 else (False, True)
+            # (at 1000 and 60 points the staged form is faster at 2^26 points but 20-45% slower at
+            # 2^22: 1000 points 197 vs 211 and 16.8 vs 13.7 us, 60 points 224 vs 251 and 20.8 vs
+            # 14.4 us; they keep the direct form first)
+            order = (True, False) if n <= 64 and n & (n - 1) == 0 else (False, True)
             for staged in (order if pi == 0 else (False,)):
                 out.append(("C2C", n, t, tpb, padsh(n), staged, rad))
     # four-step row step (rows in, transposed out): as many rows per workgroup as LDS allows,
@@ -275,6 +280,11 @@ def lines():
         rad = PLANS[n][0]
         t = choose_t(n, rad)
         tpb = tpb_for(n, t)
+        if n <= 16:
+            # C2R of up to 32 points: the staged form leads (2^26 points: 32 points 102 vs 139 us,
+            # 16 points 99 vs 121 us); staged R2C measured no faster at any short length, and from
+            # 64 real points on the direct C2R is faster as well
+            out.append(("C2R", n, t, tpb, 4, True, rad))
         out.append(("R2C", n, t, tpb, 4, False, rad))
         out.append(("C2R", n, t, tpb, 4, False, rad))
         if n in (2048, 1500, 4096):
