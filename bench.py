@@ -834,7 +834,8 @@ def profile_reflen(torch, aclfft, reps):
 
 def profile_mfma(torch, aclfft, reps):
     """The MFMA split-precision path in each mode (BF16, FP16 hi/lo, BF16 hi/mid/lo): C2C at
-    n = 256, 512, 1024, 2048, 4096 over 2^26 points, 2D 32^2 and 64^2 images over 2^26 points,
+    n = 16, 32, 64 (dense single stage) and 256, 512, 1024, 2048, 4096 over 2^26 points, 2D 32^2
+    and 64^2 images over 2^26 points,
     R2C and C2R at n = 8192 batch 1024, and the
     512-group uneven batch; then the streaming copy of 512 MiB."""
     s = torch.cuda.current_stream()
@@ -843,7 +844,7 @@ def profile_mfma(torch, aclfft, reps):
     pts = 1 << 26
     x = torch.randn(pts, dtype=torch.complex64, device="cuda")
     y = torch.empty_like(x)
-    for n in (256, 512, 1024, 2048, 4096):
+    for n in (16, 32, 64, 256, 512, 1024, 2048, 4096):
         for mode, name in modes:
             p = aclfft.Plan.plan1d(n, aclfft.C2C, pts // n)
             p.set_precision(mode)
