@@ -717,6 +717,16 @@ struct Builder {
         }
         return ids;
     }
+    // the MFMA kernels of length n that run tile lists (uneven plans): the two-stage ones other
+    // than entry 0 (the dense single-stage kernels take no tile lists)
+    static std::vector<int> mfma_tile_kernels_for(long long n) {
+        std::vector<int> ids;
+        for (int id : mfma_kernels_for(n)) {
+            ofx_mfma_info mi;
+            if (id != 0 && ofx_mfma_kernel_info_get(id, &mi) == 0 && mi.r1 >= 16) ids.push_back(id);
+        }
+        return ids;
+    }
     int mfma_table(int id, int mode) {
         TableSpec t;
         t.kind = TableSpec::MFMA_CONST;
@@ -971,13 +981,7 @@ struct Builder {
             offs[g] = off;
             off += (long long)len[g] * bat[g];
             if (ids_of.count(len[g]) == 0) {
-                std::vector<int> ids;
-                for (int id : mfma_kernels_for(len[g])) {
-                    // tile lists: the two-stage kernels (not entry 0, not the dense small-n ones)
-                    ofx_mfma_info mi;
-                    if (id != 0 && ofx_mfma_kernel_info_get(id, &mi) == 0 && mi.r1 >= 16) ids.push_back(id);
-                }
-                ids_of[len[g]] = ids;
+                ids_of[len[g]] = mfma_tile_kernels_for(len[g]);
             }
             on_mfma[g] = ids_of[len[g]].empty() ? 0 : 1;
         }
@@ -1388,9 +1392,9 @@ aclfftResult Executor::build(const PlanDesc& d, Executor** out) {
         if (d.type != ACLFFT_C2C) {
             r = ACLFFT_NOT_SUPPORTED;
         } else if (d.precision != 0) {
-            // an MFMA precision needs at least one length with an MFMA kernel
+            // an MFMA precision needs at least one length with a tile-list MFMA kernel
             bool any = false;
-            for (int n : d.ulen) any = any || !Builder::mfma_kernels_for(n).empty();
+            for (int n : d.ulen) any = any || !Builder::mfma_tile_kernels_for(n).empty();
             r = any ? b.uneven_mfma(d.ulen, d.ubatch, d.precision) : ACLFFT_NOT_SUPPORTED;
         } else {
             r = b.uneven(d.ulen, d.ubatch);

@@ -297,7 +297,38 @@ def test_uneven_plan_argument_gates():
     assert lib.aclfftExtPlanManyUneven(ctypes.byref(h3), 2, L3, B, aclfft.C2C) == aclfft.SUCCESS
     assert lib.aclfftExtSetPrecision(h3, aclfft.PRECISION_FP16X2) == aclfft.NOT_SUPPORTED
     assert lib.aclfftDestroy(h3) == aclfft.SUCCESS
+    # nor one whose only MFMA lengths are the dense single-stage ones (no tile lists): 64 and 128
+    assert lib.aclfftExtSetPrecision(h, aclfft.PRECISION_FP16X2) == aclfft.NOT_SUPPORTED
     assert lib.aclfftDestroy(h) == aclfft.SUCCESS
+
+
+@pytest.mark.parametrize("shape,ok", [((4, 32, 32), True), ((4, 16, 64), True), ((4, 64, 64), True),
+                                      ((4, 128, 128), False), ((4, 32, 128), False), ((4, 48, 32), False)])
+def test_mfma_precision_on_2d_plans(shape, ok):
+    """2D C2C plans take an MFMA precision when both axes are 16, 32 or 64 points (the launch
+    list is chosen on the host, so no GPU is needed); other shapes refuse and keep FP32."""
+    p = aclfft.Plan.plan2d(*shape, aclfft.C2C)
+    before = p.describe()
+    if ok:
+        p.set_precision(aclfft.PRECISION_BF16X3)
+        d = p.describe()
+        assert d.startswith("mfma n=%d %dx%d" % (shape[1] * shape[2], shape[2], shape[1])), d
+        assert "2d=%dx%d" % (shape[1], shape[2]) in d
+    else:
+        with pytest.raises(aclfft.AclfftError) as e:
+            p.set_precision(aclfft.PRECISION_BF16X3)
+        assert e.value.code == aclfft.NOT_SUPPORTED
+        assert p.describe() == before
+    p.destroy()
+
+
+@pytest.mark.parametrize("n", [16, 32, 64])
+def test_mfma_precision_dense_lengths(n):
+    """1D C2C of 16, 32 or 64 points takes the dense single-stage MFMA kernel."""
+    p = aclfft.Plan.plan1d(n, aclfft.C2C, 8)
+    p.set_precision(aclfft.PRECISION_FP16X2)
+    assert p.describe().startswith("mfma n=%d 1x%d" % (n, n)), p.describe()
+    p.destroy()
 
 
 def test_python_plan_refuses_short_buffers():
