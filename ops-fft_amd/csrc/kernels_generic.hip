@@ -86,6 +86,14 @@ __device__ __forceinline__ bool run_pass(int r, const V* src, V* dst, int n, int
             case 13: gen_pass<13, V>(src, dst, n, ln, ns, tpb, tw); return true;
             case 15: gen_pass<15, V>(src, dst, n, ln, ns, tpb, tw); return true;
             case 16: gen_pass<16, V>(src, dst, n, ln, ns, tpb, tw); return true;
+            case 25:
+                // complex64 only: the radix-25 codelet keeps the small form at 103 VGPRs (4 waves
+                // per SIMD), where it would take complex128 from 131 to 171
+                if constexpr (sizeof(V) == 8) {
+                    gen_pass<25, V>(src, dst, n, ln, ns, tpb, tw);
+                    return true;
+                }
+                return false;
             default: return false;
         }
     }
@@ -429,7 +437,7 @@ extern "C" int ofx_launch_generic(const ofx_args* args, const ofx_generic_desc* 
         attr_set.fetch_or(1ULL << dev);
     }
     bool big = false;
-    for (int q = 0; q < d.npass; ++q) big = big || d.radices[q] > 16;
+    for (int q = 0; q < d.npass; ++q) big = big || (d.radices[q] > 16 && (z || d.radices[q] != 25));
     GenParams p{*args, d};
     if (z) {
         if (big)
