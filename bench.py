@@ -19,7 +19,7 @@ Prints ONE JSON line:
                  double precision, latency), each with its parity figure, and the CPU golden path
                  timed on every workload
 
---profile-workloads [secondary|reflen|mfma] runs a set of workloads back to back for a profiler
+--profile-workloads [secondary|reflen|mfma|generic] runs a set of workloads back to back for a profiler
 instead (each plan is called until 0.1 s of device time has passed, then --reps more times) and
 prints no JSON line: the secondary workloads, the reference's own test-case lengths at batch
 >= 1024, or the MFMA split-precision kernels in every mode.
@@ -832,6 +832,25 @@ def profile_reflen(torch, aclfft, reps):
     _profile_copy(torch, reps)
 
 
+def profile_generic(torch, aclfft, reps):
+    """The runtime-radix kernel (lengths without a compiled kernel): C2C at 7, 100, 360, 1200, 2000,
+    4000, 4913 and 9000 points over 2^26 points; then the streaming copy of 512 MiB (counter
+    calibration)."""
+    s = torch.cuda.current_stream()
+    pts = 1 << 26
+    x = torch.randn(pts, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    for n in (7, 100, 360, 1200, 2000, 4000, 4913, 9000):
+        b = pts // n
+        p = aclfft.Plan.plan1d(n, aclfft.C2C, b)
+        p.set_stream(s)
+        _profile_rep(torch, lambda: p.exec_c2c(x[:n * b], y[:n * b], aclfft.FORWARD), reps)
+        log("c2c n=%d b=%d: %s" % (n, b, p.describe()))
+        p.destroy()
+    del x, y
+    _profile_copy(torch, reps)
+
+
 def profile_mfma(torch, aclfft, reps):
     """The MFMA split-precision path in each mode (BF16, FP16 hi/lo, BF16 hi/mid/lo): C2C at
     n = 16, 32, 64 (dense single stage) and 256, 512, 1024, 2048, 4096 over 2^26 points, 2D 32^2
@@ -973,7 +992,7 @@ def main():
     ap.add_argument("--with-copy", action="store_true",
                     help="also run one device copy of the 1 GiB input (counter calibration for profiles)")
     ap.add_argument("--profile-workloads", nargs="?", const="secondary", default=None,
-                    choices=["secondary", "reflen", "mfma"],
+                    choices=["secondary", "reflen", "mfma", "generic"],
                     help="run a set of workloads for a profiler, print no JSON line")
     ap.add_argument("--reps", type=int, default=20, help="--profile-workloads: measured calls per plan")
     args = ap.parse_args()
@@ -988,7 +1007,7 @@ def main():
 
     if args.profile_workloads:
         {"secondary": profile_workloads, "reflen": profile_reflen,
-         "mfma": profile_mfma}[args.profile_workloads](torch, aclfft, args.reps)
+         "mfma": profile_mfma, "generic": profile_generic}[args.profile_workloads](torch, aclfft, args.reps)
         return
 
     from oracle import oracle
