@@ -205,6 +205,44 @@ def time_steps(torch, fn, steps, warmup, stream, min_seconds=0.1):
     return e0.elapsed_time(e1) / steps
 
 
+def time_cold(torch, fn, stream, reps=10):
+    """Median device time of single calls, each issued right after reading 1 GiB (four times the
+    256 MB last-level cache) on the same stream, so the call finds none of its data cached.  The
+    flush reads rather than writes: a written flush would leave the caches full of dirty lines
+    whose write-back the timed call would pay for.  The flush is outside the timed events."""
+    buf = torch.zeros(1 << 28, dtype=torch.float32, device="cuda")
+    times = []
+    for _ in range(reps + 1):
+        buf.sum()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    del buf
+    times = sorted(times[1:])
+    return times[len(times) // 2]
+
+
+def time_single(torch, fn, stream, reps=10):
+    """Median device time of single calls, each after the previous one finished (no flush): the
+    cold figure's counterpart with a warm cache, including the same per-call launch cost."""
+    times = []
+    for _ in range(reps + 1):
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1))
+    times = sorted(times[1:])
+    return times[len(times) // 2]
+
+
 def frac(nbytes, ms):
     return round(nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)
 
@@ -353,6 +391,9 @@ def extra_real(torch, aclfft, oracle, np, steps, stream):
         pc.set_stream(stream)
         ms_r = time_steps(torch, lambda: pr.exec_r2c(xr, X), steps, 3, stream)
         ms_c = time_steps(torch, lambda: pc.exec_c2r(X, xb), steps, 3, stream)
+        cold_r = time_cold(torch, lambda: pr.exec_r2c(xr, X), stream)
+        cold_c = time_cold(torch, lambda: pc.exec_c2r(X, xb), stream)
+        single_r = time_single(torch, lambda: pr.exec_r2c(xr, X), stream)
         byts = (4.0 * n + 8.0 * (n // 2 + 1)) * b
         rt = float(((xb / n - xr).norm() / xr.norm()).item())
         err = oracle.rel_l2(X[:64].cpu().numpy(), oracle.r2c(xr[:64].cpu().numpy().astype(np.float64)))
@@ -379,6 +420,9 @@ def extra_real(torch, aclfft, oracle, np, steps, stream):
         out["n%d_b%d" % (n, b)] = {
             "r2c_ms": round(ms_r, 5), "c2r_ms": round(ms_c, 5),
             "r2c_hbm_frac": frac(byts, ms_r), "c2r_hbm_frac": frac(byts, ms_c),
+            "r2c_cold_llc_ms": round(cold_r, 5), "c2r_cold_llc_ms": round(cold_c, 5),
+            "r2c_single_call_warm_ms": round(single_r, 5),
+            "r2c_cold_llc_hbm_frac": frac(byts, cold_r), "c2r_cold_llc_hbm_frac": frac(byts, cold_c),
             "r2c_GFLOP/s_2.5nlogn": round(2.5 * n * math.log2(n) * b / ms_r / 1e6, 1),
             "roundtrip_rel_l2": rt, "r2c_rel_l2_vs_fp64": err, "plan_r2c": pr.describe(),
             "plan_c2r": pc.describe()}
@@ -519,9 +563,13 @@ def extra_2d(torch, aclfft, oracle, np, steps, stream):
     p = aclfft.Plan.plan2d(B, NX, NY, aclfft.C2C)
     p.set_stream(stream)
     ms = time_steps(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
+    cold = time_cold(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), stream)
+    single = time_single(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), stream)
     err = oracle.rel_l2(y[:1].cpu().numpy(), oracle.c2c_2d(x[:1].cpu().numpy().astype(np.complex128)))
     pts = B * NX * NY
-    out = {"ms": round(ms, 5), "GFLOP/s": round(5.0 * pts * math.log2(NX * NY) / ms / 1e6, 1),
+    out = {"ms": round(ms, 5), "cold_llc_ms": round(cold, 5), "cold_llc_hbm_frac_two_pass": frac(32.0 * pts, cold),
+           "single_call_warm_ms": round(single, 5),
+           "GFLOP/s": round(5.0 * pts * math.log2(NX * NY) / ms / 1e6, 1),
            "hbm_frac_compulsory": frac(16.0 * pts, ms), "hbm_frac_two_pass": frac(32.0 * pts, ms),
            "rel_l2_vs_fp64": err, "plan": p.describe()}
     p.destroy()
@@ -624,6 +672,8 @@ def extra_batched(torch, aclfft, oracle, np, steps, stream):
     p = aclfft.Plan.plan_many_uneven(ln, bt)
     p.set_stream(stream)
     ms = time_steps(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), steps, 3, stream)
+    ms_cold = time_cold(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), stream)
+    ms_single = time_single(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), stream)
     pg = aclfft.Plan.plan_many_uneven(ln, bt)
     pg.set_stream(stream)
     pg.set_graph_mode(True)
@@ -653,7 +703,10 @@ def extra_batched(torch, aclfft, oracle, np, steps, stream):
         q.set_stream(stream)
     ms_naive = time_steps(torch, per_group, max(3, steps // 5), 2, stream)
     flops = sum(5.0 * n * math.log2(n) * b for n, b in zip(ln, bt))
-    out = {"groups": 512, "points": int(total), "persistent_ms": round(ms, 5), "persistent_hipgraph_ms": round(ms_graph, 5),
+    out = {"groups": 512, "points": int(total), "persistent_ms": round(ms, 5),
+           "persistent_cold_llc_ms": round(ms_cold, 5), "cold_llc_hbm_frac": frac(16.0 * total, ms_cold),
+           "persistent_single_call_warm_ms": round(ms_single, 5),
+           "persistent_hipgraph_ms": round(ms_graph, 5),
            "per_group_launches_ms": round(ms_naive, 5), "speedup": round(ms_naive / ms, 2),
            "GFLOP/s": round(flops / ms / 1e6, 1), "hbm_frac": frac(16.0 * total, ms),
            "worst_rel_l2_vs_fp64_all_groups": worst, "plan": p.describe()}
@@ -1079,7 +1132,14 @@ def main():
     }
     if not args.no_extras:
         extras = {}
-        for name, fn_ in (("copy_1GiB", lambda: extra_copy(torch, aclfft, x, 20, stream, ms)),
+        def headline_cold():
+            cms = time_cold(torch, lambda: plan.exec_c2c(x, y, aclfft.FORWARD), stream)
+            sms = time_single(torch, lambda: plan.exec_c2c(x, y, aclfft.FORWARD), stream)
+            return {"ms": round(cms, 5), "hbm_frac": frac(algo_bytes, cms), "single_call_warm_ms": round(sms, 5),
+                    "note": "single calls after reading 1 GiB (4x the last-level cache), median of 10"}
+
+        for name, fn_ in (("headline_cold_llc", headline_cold),
+                          ("copy_1GiB", lambda: extra_copy(torch, aclfft, x, 20, stream, ms)),
                           ("mfma_split_n1024_b65536", lambda: extra_mfma(torch, aclfft, oracle, np, x, x_cpu, y,
                                                                          args.steps, stream, err)),
                           ("mfma_split_other_lengths", lambda: extra_mfma_lengths(torch, aclfft, oracle, np, x, y,
