@@ -264,12 +264,19 @@ struct Builder {
     // 46.2 us, R2C / C2R 8192 x 8192 94.0 / 98.1 vs 115.7 / 124.1 us.  A consumer that reads the
     // output right after the call then reads it from HBM, not from the cache
     // (ACLFFT_STREAM_STORES=0 keeps cached stores).
+    //
+    // R2C plans stream their stores only below 64 MiB of output: from a cold last-level cache the
+    // larger ones run faster with cached stores (4096 x 8192: 56.8 vs 62.4 us, 3000 x 8192: 44.3 vs
+    // 45.6 us), and back-to-back calls on one input tie (43.7-45.2 vs 44.1-44.7 us, 33.0-34.4 vs
+    // 33.6 us); 8192 x 1024 (34 MB) gains in both (cold 18.5 vs 19.0 us).
     static constexpr size_t kStoreStreamBytes = size_t(16) << 20;
+    static constexpr size_t kR2CStoreStreamMax = size_t(64) << 20;
     void stream_stores(size_t out_bytes) {
         if (e->steps_.size() != 1 || out_bytes < kStoreStreamBytes) return;
         if (std::getenv("ACLFFT_KERNEL_VARIANT") != nullptr || env_int("ACLFFT_STREAM_STORES", 1) == 0) return;
         Step& s = e->steps_[0];
         if (s.type != Step::KERNEL || s.dst != BUF_OUT || registry()[s.kernel].stream != 0) return;
+        if (registry()[s.kernel].kind == OFX_KIND_R2C && out_bytes >= kR2CStoreStreamMax) return;
         for (size_t c = 0; c < s.candidates.size(); ++c)
             if (registry()[s.candidates[c]].stream == 2) {
                 s.kernel = s.candidates[c];

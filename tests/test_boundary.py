@@ -452,3 +452,13 @@ def test_tuning_signatures_tell_every_kernel_apart():
             s = sig(k)
             assert s not in seen, (i, seen.get(s), s)
             seen[s] = i
+
+
+@pytest.mark.parametrize("n,batch,streamed", [(8192, 1024, True), (4096, 2048, True), (4096, 8192, False),
+                                              (3000, 8192, False)])
+def test_r2c_store_rule_by_output_size(n, batch, streamed):
+    """R2C plans of one launch stream their stores from 16 MiB up to 64 MiB of output; larger ones
+    keep cached stores (the launch list is chosen on the host, so no GPU is needed)."""
+    p = aclfft.Plan.plan1d(n, aclfft.R2C, batch)
+    assert ("ntstore" in p.describe()) == streamed, p.describe()
+    p.destroy()
