@@ -23,6 +23,7 @@ namespace ofx {
 
 constexpr int kGenBlock = 256;
 constexpr int kGenMaxTpb = 64;  // transforms per workgroup (ofx_generic_desc::tpb)
+constexpr int kGenLoadU = 4;    // complex-input loads (and output elements) in flight per lane
 
 struct GenParams {
     ofx_args a;
@@ -161,7 +162,44 @@ __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
 
     // ---- load into buf0 ----------------------------------------------------------------------
     const int h = n / 2;  // used by the real-data modes
-    for (int f = threadIdx.x; f < nvalid * n; f += blockDim.x) {
+    // complex input (the common mode): kGenLoadU loads per thread issued before any is used, so
+    // each lane keeps several HBM requests in flight (one at a time left the kernel bound by
+    // memory latency at a quarter of the bandwidth)
+    const bool cplx_in = d.in_mode == OFX_IN_COMPLEX;
+    if (cplx_in) {
+        const int total = nvalid * n;
+        const int B = (int)blockDim.x;
+        for (int f0 = threadIdx.x; f0 < total; f0 += B * kGenLoadU) {
+            V vals[kGenLoadU];
+            int dst_at[kGenLoadU];
+#pragma unroll
+            for (int u = 0; u < kGenLoadU; ++u) {
+                const int f = f0 + u * B;
+                dst_at[u] = -1;
+                if (f < total) {
+                    int t, i;
+                    if (d.in_tmajor) {
+                        t = f % nvalid;
+                        i = f / nvalid;
+                    } else {
+                        t = f / n;
+                        i = f - t * n;
+                    }
+                    vals[u] = static_cast<const V*>(a.in)[s_ibase[t] + i * a.istride];
+                    dst_at[u] = t * ln + gpad(i);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kGenLoadU; ++u) {
+                if (dst_at[u] >= 0) {
+                    V v = vals[u];
+                    v.y *= S(a.in_sign);
+                    buf0[dst_at[u]] = v;
+                }
+            }
+        }
+    }
+    for (int f = cplx_in ? nvalid * n : threadIdx.x; f < nvalid * n; f += blockDim.x) {
         int t, i;
         if (d.in_tmajor) {
             t = f % nvalid;
@@ -228,7 +266,49 @@ __global__ __launch_bounds__(kGenBlock) void k_generic(GenParams p) {
     // ---- store from src ------------------------------------------------------------------------
     const double2* ftab = static_cast<const double2*>(a.tw_aux);
     const int nout = (d.out_mode == OFX_OUT_HALF) ? h + 1 : (d.out_mode == OFX_OUT_R2C_UNPACK ? n + 1 : n);
-    for (int f = threadIdx.x; f < nvalid * nout; f += blockDim.x) {
+    // complex output (the common mode): kGenLoadU elements per lane, their LDS reads and four-step
+    // twiddle loads issued before any is used
+    const bool cplx_out = d.out_mode == OFX_OUT_COMPLEX;
+    if (cplx_out) {
+        const int total = nvalid * n;
+        const int B = (int)blockDim.x;
+        for (int f0 = threadIdx.x; f0 < total; f0 += B * kGenLoadU) {
+            V vals[kGenLoadU], tws[kGenLoadU];
+            long long at[kGenLoadU];
+#pragma unroll
+            for (int u = 0; u < kGenLoadU; ++u) {
+                const int f = f0 + u * B;
+                at[u] = -1;
+                if (f < total) {
+                    int t, k;
+                    if (d.out_tmajor) {
+                        t = f % nvalid;
+                        k = f / nvalid;
+                    } else {
+                        t = f / n;
+                        k = f - t * n;
+                    }
+                    vals[u] = src[t * ln + gpad(k)];
+                    if (d.ft_len > 0) {
+                        long long e = (s_tlo[t] + d.ft_base) * (long long)k;
+                        if (e >= d.ft_len) e %= d.ft_len;  // never for the four-step's own steps
+                        tws[u] = ft_twiddle_v((V*)nullptr, ftab, e, d.ft_len);
+                    }
+                    at[u] = s_obase[t] + k * a.ostride;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kGenLoadU; ++u) {
+                if (at[u] >= 0) {
+                    V v = vals[u];
+                    v.y *= S(a.out_sign);
+                    if (d.ft_len > 0) v = cmul(v, tws[u]);
+                    static_cast<V*>(a.out)[at[u]] = mkv(v.x * scale, v.y * scale);
+                }
+            }
+        }
+    }
+    for (int f = cplx_out ? nvalid * nout : threadIdx.x; f < nvalid * nout; f += blockDim.x) {
         int t, k;
         if (d.out_tmajor) {
             t = f % nvalid;
