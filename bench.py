@@ -904,6 +904,24 @@ def profile_generic(torch, aclfft, reps):
     _profile_copy(torch, reps)
 
 
+def profile_pow2(torch, aclfft, reps):
+    """C2C at 1024, 2048 and 4096 points over 2^26 points (the default plans), then the streaming
+    copy of 512 MiB (counter calibration)."""
+    s = torch.cuda.current_stream()
+    pts = 1 << 26
+    x = torch.randn(pts, dtype=torch.complex64, device="cuda")
+    y = torch.empty_like(x)
+    for n in (1024, 2048, 4096):
+        b = pts // n
+        p = aclfft.Plan.plan1d(n, aclfft.C2C, b)
+        p.set_stream(s)
+        _profile_rep(torch, lambda: p.exec_c2c(x, y, aclfft.FORWARD), reps)
+        log("c2c n=%d b=%d: %s" % (n, b, p.describe()))
+        p.destroy()
+    del x, y
+    _profile_copy(torch, reps)
+
+
 def profile_mfma(torch, aclfft, reps):
     """The MFMA split-precision path in each mode (BF16, FP16 hi/lo, BF16 hi/mid/lo): C2C at
     n = 16, 32, 64 (dense single stage) and 256, 512, 1024, 2048, 4096 over 2^26 points, 2D 32^2
@@ -1045,7 +1063,7 @@ def main():
     ap.add_argument("--with-copy", action="store_true",
                     help="also run one device copy of the 1 GiB input (counter calibration for profiles)")
     ap.add_argument("--profile-workloads", nargs="?", const="secondary", default=None,
-                    choices=["secondary", "reflen", "mfma", "generic"],
+                    choices=["secondary", "reflen", "mfma", "generic", "pow2"],
                     help="run a set of workloads for a profiler, print no JSON line")
     ap.add_argument("--reps", type=int, default=20, help="--profile-workloads: measured calls per plan")
     args = ap.parse_args()
@@ -1060,7 +1078,7 @@ def main():
 
     if args.profile_workloads:
         {"secondary": profile_workloads, "reflen": profile_reflen,
-         "mfma": profile_mfma, "generic": profile_generic}[args.profile_workloads](torch, aclfft, args.reps)
+         "mfma": profile_mfma, "generic": profile_generic, "pow2": profile_pow2}[args.profile_workloads](torch, aclfft, args.reps)
         return
 
     from oracle import oracle
