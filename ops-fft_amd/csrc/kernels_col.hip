@@ -208,8 +208,10 @@ static const ColEntry kCols[] = {
     OFX_COL(1024, 128, 8, 1, 8, 8, 16),
     // split exchange, two workgroups per CU (58% vs 55% of HBM for 2048 x 8192 VERTICAL); radix
     // 8 x 16 x 16 first (2D 2048 x 2048 x 4: 93.1-94.2 us vs 95.6 us for 16 x 16 x 8); no padding
-    // between the interleaved rows of 8 columns (2D 2048 x 2048 x 4: 89.1-89.2 vs 92.3-92.4 us with
-    // one spare word, 2048 x 8192 VERTICAL 52.8-53.0 vs 53.5-54.0 us, 2048 x 2048 x 8 a tie)
+    // between the interleaved rows of 8 columns: never slower than one spare word, faster in some
+    // processes (2D 2048 x 2048 x 4 89.1 vs 92.3 us in a process holding only its own buffers, a
+    // tie at 91.0-91.6 us behind 1 GiB of other allocations; 2048 x 8192 VERTICAL 52.8-53.0 vs
+    // 53.5-54.0 us; 2048 x 2048 x 8 a tie)
     OFX_COLX(2048, 64, 8, 0, 8, 16, 16),
     OFX_COLX(2048, 64, 8, 1, 32, 8, 8),
     OFX_COLX(2048, 64, 8, 1, 16, 16, 8),
